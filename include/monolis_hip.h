@@ -1,0 +1,123 @@
+/*
+ * monolis_hip.h -- C-ABI of libmonolis_hip.so, the MI355X (gfx950) Krylov path
+ * of monolis: block-CSR SpMV, CG / BiCGSTAB and the diagonal (block-Jacobi)
+ * preconditioner, on one GPU.
+ *
+ * Part 1 are monolis's own flat entry points, with identical names and
+ * signatures, so a monolis C or Fortran caller links against this library
+ * instead of libmonolis_solver.  Part 2 adds a device-resident interface (the
+ * matrix is uploaded once and stays in HBM) and the precision, batching and
+ * adjoint extensions.
+ *
+ * Conventions of part 1 (as in monolis):
+ *  - index[NP+1] holds 0-based row offsets, item[NZ] 1-based block columns
+ *    (diagonal included, ascending per row), A[NZ*ndof*ndof] row-major blocks;
+ *  - Iarray[100] / Rarray[100]: slot k here is Fortran slot k (1-based names,
+ *    e.g. Iarray[1] = method, Iarray[4] = cur_iter, Rarray[1] = tol);
+ *  - pointers may be host or device memory (device memory is used in place).
+ * One rank only: comm_size must be 1 (the halo tables are then empty).
+ */
+#ifndef MONOLIS_HIP_H
+#define MONOLIS_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================== */
+/* Part 1: monolis flat C-ABI                                               */
+/* ======================================================================== */
+
+/* Replaces monolis_solve_R_c_main, wrapper/solver/monolis_solver_c.h:51-80,
+ * defined as the Fortran bind(c) routine wrapper/solver/solver_wrapper.f90:11-92.
+ * Runs monolis_solve_main_R (src/solver/solver.f90:77-185) with the method in
+ * Iarray[1] (CG = 1, BiCGSTAB = 5) and preconditioner in Iarray[2] (NONE = 0,
+ * DIAG = 1) on the GPU.  Fills Iarray[4] (cur_iter), Iarray[5] (status: 0 ok,
+ * 1 not converged, < 0 error), Rarray[2] (cur_resid) and Rarray[3..4]
+ * (solution / preparation time, s).  Non-convergence stops the process when
+ * Iarray[16] (error_abort) is set and Iarray[13] (measurement) is not,
+ * as src/linalg/mat_converge.f90:134-140 does. */
+void monolis_solve_R_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZNDOF2, int* n_dof_list,
+                            int* n_dof_index, int* n_dof_index2, double* A, double* x, double* b, int* index,
+                            int* item, int my_rank, int comm, int comm_size, int recv_n_neib, int recv_nitem,
+                            int* recv_neib_pe, int* recv_index, int* recv_item, int send_n_neib, int send_nitem,
+                            int* send_neib_pe, int* send_index, int* send_item, int* Iarray, double* Rarray);
+
+/* Replaces monolis_matvec_product_R_c_main, wrapper/linalg/monolis_matvec_c.h:39-66,
+ * defined at wrapper/linalg/matvec_wrap.f90:13-108: y[0 .. n*n_dof) = A x. */
+void monolis_matvec_product_R_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZNDOF2, int* n_dof_list,
+                                     int* n_dof_index, int* n_dof_index2, double* A, double* x, double* y,
+                                     int* index, int* item, int my_rank, int comm, int comm_size,
+                                     int recv_n_neib, int recv_nitem, int* recv_neib_pe, int* recv_index,
+                                     int* recv_item, int send_n_neib, int send_nitem, int* send_neib_pe,
+                                     int* send_index, int* send_item);
+
+/* Replaces monolis_inner_product_R_c_main, wrapper/linalg/monolis_inner_product_c.h:139,
+ * defined at wrapper/linalg/inner_product_wrap.f90:35-55: *sum = sum_i x_i y_i. */
+void monolis_inner_product_R_c_main(int m, double* x, double* y, double* sum, int comm);
+
+/* ======================================================================== */
+/* Part 2: device-resident interface and extensions (no monolis counterpart  */
+/* except where cited).  Every function returns 0 on success, < 0 on error;  */
+/* monolis_hip_last_error() describes the last failure of the calling thread.*/
+/* ======================================================================== */
+
+typedef struct monolis_hip_matrix monolis_hip_matrix;
+
+const char* monolis_hip_last_error(void);
+/* Selects the GPU, creates the library stream (default priority). */
+int monolis_hip_init(void);
+/* Library stream as a hipStream_t (void* here to keep HIP out of this header). */
+void* monolis_hip_stream(void);
+int monolis_hip_synchronize(void);
+int monolis_hip_malloc(void** ptr, size_t bytes);
+int monolis_hip_free(void* ptr);
+/* Copy in any direction (host/device detected from the pointers), synchronous. */
+int monolis_hip_memcpy(void* dst, const void* src, size_t bytes);
+int monolis_hip_memset(void* dst, int value, size_t bytes);
+
+/* Uploads a monolis BCSR matrix (host or device arrays, same layout as part 1)
+ * and builds the GPU layout.  The counterpart of monolis's device staging
+ * (src/solver/solver.f90:102-156) done once instead of per solve. */
+int monolis_hip_matrix_create(monolis_hip_matrix** out, int n, int np, int nz, int n_dof, const int* index,
+                              const int* item, const double* A);
+int monolis_hip_matrix_destroy(monolis_hip_matrix* M);
+/* New values, same pattern (A(theta) updates). */
+int monolis_hip_matrix_set_values(monolis_hip_matrix* M, const double* A);
+/* info[0..7] = n, np, nz, n_dof, slices, slots, bytes of FP64 layout, bytes of FP32 layout */
+int monolis_hip_matrix_info(const monolis_hip_matrix* M, long long* info);
+
+/* y = A x on device vectors (FP64 / FP32 values and vectors). */
+int monolis_hip_matvec_R(monolis_hip_matrix* M, const double* x, double* y);
+int monolis_hip_matvec_S(monolis_hip_matrix* M, const float* x, float* y);
+/* Deterministic device dot product; *sum is host memory. */
+int monolis_hip_inner_product_R(int m, const double* x, const double* y, double* sum);
+
+/* monolis_solve_main_R on device-resident b and x (FP64), Iarray/Rarray host,
+ * same slots and semantics as monolis_solve_R_c_main except that failures are
+ * returned, never turned into a process stop.  If hist != NULL and
+ * hist_len > 0, hist[k] = relative residual after iteration k (k >= 1). */
+int monolis_hip_solve_R(monolis_hip_matrix* M, const double* b, double* x, int* Iarray, double* Rarray,
+                        double* hist, int hist_len);
+
+/* FP32-inner / FP64-refined solve.  The inner CG (Iarray[1] = 1) or BiCGSTAB
+ * (5) runs its SpMV, vectors and preconditioner in FP32 to relative residual
+ * inner_tol; FP64 is used for r = b - A x, the update x += d and the norms.
+ * Stops when ||b - A x|| / ||b - A x0|| < Rarray[1].  stats[0..3] = outer
+ * iterations, total inner iterations, final relative residual, converged. */
+int monolis_hip_solve_refined_R(monolis_hip_matrix* M, const double* b, double* x, int* Iarray, double* Rarray,
+                                double inner_tol, int max_outer, double* stats);
+
+/* Kernel timing with HIP events on the library stream: `warmup` untimed and
+ * `reps` timed SpMVs; *ms = average milliseconds per SpMV.  fp32 selects the
+ * FP32 layout (x, y are float* then). */
+int monolis_hip_time_matvec(monolis_hip_matrix* M, int fp32, const void* x, void* y, int warmup, int reps,
+                            double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MONOLIS_HIP_H */

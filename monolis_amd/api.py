@@ -1,0 +1,262 @@
+"""Python mirror of monolis's solver interface over libmonolis_hip.so.
+
+Names, argument meaning and defaults follow monolis's C API
+(wrapper/solver/monolis_solver_c.c:8-60, wrapper/linalg/monolis_matvec_c.c:8-58,
+wrapper/linalg/monolis_inner_product_c.c:32-44,
+wrapper/define/monolis_def_solver_prm_c.c:7-45,
+wrapper/define/monolis_def_solver_prm_util_c.c:7-194), so tests read like the
+reference's own.  Host numpy arrays go through the flat C-ABI exactly as a C
+caller's arrays would; ``DeviceMatrix`` / ``DeviceBuffer`` use the
+device-resident extension.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .matrix import Matrix
+
+# ids, src/define/def_solver_prm.f90:7-58 / wrapper/define/monolis_def_solver_prm_c.h
+MONOLIS_ITER_CG, MONOLIS_ITER_BICGSTAB = 1, 5
+MONOLIS_PREC_NONE, MONOLIS_PREC_DIAG, MONOLIS_PREC_SOR, MONOLIS_PREC_LU = 0, 1, 2, 3
+MONOLIS_PRM_I_METHOD, MONOLIS_PRM_I_PRECOND, MONOLIS_PRM_I_MAX_ITER = 1, 2, 3
+MONOLIS_PRM_I_CUR_ITER, MONOLIS_PRM_I_IERR = 4, 5
+MONOLIS_PRM_I_IS_INIT_X, MONOLIS_PRM_I_IS_SYM_MATRIX, MONOLIS_PRM_I_IS_DEBUG = 10, 11, 12
+MONOLIS_PRM_I_IS_MEASUREMENT, MONOLIS_PRM_I_IS_CHECK_DIAG = 13, 14
+MONOLIS_PRM_I_IS_PREC_STORED, MONOLIS_PRM_I_IS_ERROR_ABORT = 15, 16
+MONOLIS_PRM_I_SHOW_ITERLOG, MONOLIS_PRM_I_SHOW_TIME, MONOLIS_PRM_I_SHOW_SUMMARY = 20, 21, 22
+MONOLIS_PRM_I_SHOW_TIME_STATISTICS, MONOLIS_PRM_I_IDRS_DIM, MONOLIS_PRM_I_SPMV_FORMAT = 23, 24, 25
+MONOLIS_PRM_R_TOL, MONOLIS_PRM_R_CUR_RESID = 1, 2
+MONOLIS_R_TIME_SOL, MONOLIS_R_TIME_PREP, MONOLIS_R_TIME_SPMV = 3, 4, 5
+MONOLIS_R_TIME_DOTP, MONOLIS_R_TIME_PREC = 6, 7
+MONOLIS_R_TIME_COMM_DOTP, MONOLIS_R_TIME_COMM_SPMV = 8, 9
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Prm:
+    """MONOLIS_PRM (monolis_def_solver_prm_c.h:136-142) with monolis_prm_initialize defaults."""
+
+    def __init__(self):
+        self.Iarray = np.zeros(100, np.int32)
+        self.Rarray = np.zeros(100, np.float64)
+        I = self.Iarray
+        I[MONOLIS_PRM_I_METHOD] = 1
+        I[MONOLIS_PRM_I_PRECOND] = 1
+        I[MONOLIS_PRM_I_MAX_ITER] = 1000
+        I[MONOLIS_PRM_I_CUR_ITER] = 0
+        I[MONOLIS_PRM_I_IERR] = -1
+        I[MONOLIS_PRM_I_IS_INIT_X] = 1
+        I[MONOLIS_PRM_I_IS_ERROR_ABORT] = 1
+        I[MONOLIS_PRM_I_SHOW_ITERLOG] = 1
+        I[MONOLIS_PRM_I_SHOW_TIME] = 1
+        I[MONOLIS_PRM_I_SHOW_SUMMARY] = 1
+        I[MONOLIS_PRM_I_IDRS_DIM] = 4
+        self.Rarray[MONOLIS_PRM_R_TOL] = 1.0e-8
+
+    def quiet(self) -> "Prm":
+        self.Iarray[MONOLIS_PRM_I_SHOW_ITERLOG] = 0
+        self.Iarray[MONOLIS_PRM_I_SHOW_TIME] = 0
+        self.Iarray[MONOLIS_PRM_I_SHOW_SUMMARY] = 0
+        return self
+
+
+class Monolis:
+    """MONOLIS {prm, mat, prec} (monolis_def_struc_c.h:13-17)."""
+
+    def __init__(self, mat: Matrix, prm: Optional[Prm] = None):
+        self.mat = mat.ensure_dof_index()
+        self.prm = prm if prm is not None else Prm()
+
+
+# setters / getters, monolis_def_solver_prm_util_c.c:7-194
+def monolis_set_method(m: Monolis, v: int): m.prm.Iarray[MONOLIS_PRM_I_METHOD] = v
+def monolis_set_precond(m: Monolis, v: int): m.prm.Iarray[MONOLIS_PRM_I_PRECOND] = v
+def monolis_set_maxiter(m: Monolis, v: int): m.prm.Iarray[MONOLIS_PRM_I_MAX_ITER] = v
+def monolis_set_tolerance(m: Monolis, v: float): m.prm.Rarray[MONOLIS_PRM_R_TOL] = v
+def monolis_set_init_x(m: Monolis, v: bool): m.prm.Iarray[MONOLIS_PRM_I_IS_INIT_X] = int(v)
+def monolis_set_error_abort(m: Monolis, v: bool): m.prm.Iarray[MONOLIS_PRM_I_IS_ERROR_ABORT] = int(v)
+def monolis_set_performance_measurement(m: Monolis, v: bool): m.prm.Iarray[MONOLIS_PRM_I_IS_MEASUREMENT] = int(v)
+def monolis_show_iterlog(m: Monolis, v: bool): m.prm.Iarray[MONOLIS_PRM_I_SHOW_ITERLOG] = int(v)
+def monolis_show_timelog(m: Monolis, v: bool): m.prm.Iarray[MONOLIS_PRM_I_SHOW_TIME] = int(v)
+def monolis_show_summary(m: Monolis, v: bool): m.prm.Iarray[MONOLIS_PRM_I_SHOW_SUMMARY] = int(v)
+def monolis_get_converge_iter(m: Monolis) -> int: return int(m.prm.Iarray[MONOLIS_PRM_I_CUR_ITER])
+def monolis_get_error_tag(m: Monolis) -> int: return int(m.prm.Iarray[MONOLIS_PRM_I_IERR])
+def monolis_get_converge_residual(m: Monolis) -> float: return float(m.prm.Rarray[MONOLIS_PRM_R_CUR_RESID])
+def monolis_get_time_solver(m: Monolis) -> float: return float(m.prm.Rarray[MONOLIS_R_TIME_SOL])
+def monolis_get_time_preparing(m: Monolis) -> float: return float(m.prm.Rarray[MONOLIS_R_TIME_PREP])
+
+
+def _flat_mat_args(mat: Matrix):
+    """The 14 matrix arguments of the flat entry points (monolis_solver_c.c:27-41)."""
+    mat.ensure_dof_index()
+    return [mat.n, mat.np_, mat.nz, mat.ndof, mat.npndof, mat.nzndof2,
+            _p(mat.n_dof_list), _p(mat.n_dof_index), _p(mat.n_dof_index2), _p(mat.A)]
+
+
+_EMPTY = np.zeros(1, np.int32)
+
+
+def _self_comm_args():
+    """monolis_com_initialize_by_self: one rank, empty halo tables."""
+    e = _p(_EMPTY)
+    return [0, 0, 1, 0, 0, e, e, e, 0, 0, e, e, e]
+
+
+def monolis_solve_R(monolis: Monolis, b: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """C monolis_solve_R (monolis_solver_c.c:8-60): x is overwritten in place."""
+    L = _lib.load()
+    mat = monolis.mat
+    assert x.dtype == np.float64 and x.flags.c_contiguous and len(x) >= mat.npndof
+    b = np.ascontiguousarray(b, np.float64)
+    args = _flat_mat_args(mat)
+    L.monolis_solve_R_c_main(*args[:10], _p(x), _p(b), _p(mat.index), _p(mat.item),
+                             *_self_comm_args(), _p(monolis.prm.Iarray), _p(monolis.prm.Rarray))
+    return x
+
+
+def monolis_matvec_product_R(monolis: Monolis, x: np.ndarray, y: Optional[np.ndarray] = None) -> np.ndarray:
+    """C monolis_matvec_product_R (monolis_matvec_c.c:8-58)."""
+    L = _lib.load()
+    mat = monolis.mat
+    x = np.ascontiguousarray(x, np.float64)
+    if y is None:
+        y = np.zeros(mat.npndof, np.float64)
+    args = _flat_mat_args(mat)
+    L.monolis_matvec_product_R_c_main(*args[:10], _p(x), _p(y), _p(mat.index), _p(mat.item),
+                                      *_self_comm_args())
+    return y
+
+
+def monolis_inner_product_R(monolis: Monolis, n_dof: int, x: np.ndarray, y: np.ndarray) -> float:
+    """C monolis_inner_product_R (monolis_inner_product_c.c:32-44): over N*n_dof entries."""
+    L = _lib.load()
+    m = monolis.mat.n * n_dof
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    s = np.zeros(1, np.float64)
+    L.monolis_inner_product_R_c_main(m, _p(x), _p(y), _p(s), 0)
+    return float(s[0])
+
+
+# ---------------------------------------------------------------------------
+# device-resident interface
+# ---------------------------------------------------------------------------
+class DeviceBuffer:
+    """A device allocation owned by Python (freed with the object)."""
+
+    def __init__(self, count: int, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = max(1, self.count * self.dtype.itemsize)
+        L = _lib.load()
+        p = ctypes.c_void_p()
+        _lib.check(L.monolis_hip_malloc(ctypes.byref(p), self.nbytes), "monolis_hip_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.size <= self.count
+        _lib.check(_lib.load().monolis_hip_memcpy(ctypes.c_void_p(self.ptr), _p(a), a.nbytes), "upload")
+
+    def numpy(self, count: Optional[int] = None) -> np.ndarray:
+        n = self.count if count is None else count
+        out = np.empty(n, self.dtype)
+        _lib.check(_lib.load().monolis_hip_memcpy(_p(out), ctypes.c_void_p(self.ptr), out.nbytes), "download")
+        return out
+
+    def zero(self) -> None:
+        _lib.check(_lib.load().monolis_hip_memset(ctypes.c_void_p(self.ptr), 0, self.nbytes), "memset")
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                _lib.load().monolis_hip_free(ctypes.c_void_p(self.ptr))
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class DeviceMatrix:
+    """A monolis BCSR matrix resident in HBM in the GPU layout."""
+
+    def __init__(self, mat: Matrix):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        mat.ensure_dof_index()
+        _lib.check(L.monolis_hip_matrix_create(ctypes.byref(h), mat.n, mat.np_, mat.nz, mat.ndof,
+                                               _p(mat.index), _p(mat.item), _p(mat.A)),
+                   "monolis_hip_matrix_create")
+        self.h = h.value
+        self.mat = mat
+        self.m = mat.nndof
+        self.vec = mat.npndof
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib.load().monolis_hip_matrix_destroy(ctypes.c_void_p(self.h))
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        a = (ctypes.c_longlong * 8)()
+        _lib.check(_lib.load().monolis_hip_matrix_info(ctypes.c_void_p(self.h), a), "info")
+        keys = ["n", "np", "nz", "ndof", "slices", "slots", "bytes_fp64", "bytes_fp32"]
+        return dict(zip(keys, list(a)))
+
+    def set_values(self, A) -> None:
+        ptr = A.ptr if isinstance(A, DeviceBuffer) else _p(np.ascontiguousarray(A, np.float64))
+        _lib.check(_lib.load().monolis_hip_matrix_set_values(ctypes.c_void_p(self.h), ctypes.c_void_p(ptr) if isinstance(A, DeviceBuffer) else ptr), "set_values")
+
+    def matvec(self, x: DeviceBuffer, y: DeviceBuffer) -> None:
+        L = _lib.load()
+        if x.dtype == np.float32:
+            _lib.check(L.monolis_hip_matvec_S(ctypes.c_void_p(self.h), ctypes.c_void_p(x.ptr), ctypes.c_void_p(y.ptr)), "matvec_S")
+        else:
+            _lib.check(L.monolis_hip_matvec_R(ctypes.c_void_p(self.h), ctypes.c_void_p(x.ptr), ctypes.c_void_p(y.ptr)), "matvec_R")
+
+    def solve(self, b: DeviceBuffer, x: DeviceBuffer, prm: Prm, hist_len: int = 0):
+        hist = np.zeros(max(hist_len, 1), np.float64)
+        rc = _lib.load().monolis_hip_solve_R(ctypes.c_void_p(self.h), ctypes.c_void_p(b.ptr), ctypes.c_void_p(x.ptr),
+                                             _p(prm.Iarray), _p(prm.Rarray), _p(hist), hist_len)
+        _lib.check(rc, "monolis_hip_solve_R")
+        return hist[:hist_len] if hist_len else None
+
+    def solve_refined(self, b: DeviceBuffer, x: DeviceBuffer, prm: Prm, inner_tol: float = 1e-4,
+                      max_outer: int = 50) -> dict:
+        st = np.zeros(4, np.float64)
+        rc = _lib.load().monolis_hip_solve_refined_R(ctypes.c_void_p(self.h), ctypes.c_void_p(b.ptr),
+                                                     ctypes.c_void_p(x.ptr), _p(prm.Iarray), _p(prm.Rarray),
+                                                     float(inner_tol), int(max_outer), _p(st))
+        _lib.check(rc, "monolis_hip_solve_refined_R")
+        return {"outer": int(st[0]), "inner_total": int(st[1]), "resid": float(st[2]), "converged": bool(st[3])}
+
+    def time_matvec(self, x: DeviceBuffer, y: DeviceBuffer, warmup: int = 5, reps: int = 50) -> float:
+        ms = ctypes.c_double()
+        fp32 = 1 if x.dtype == np.float32 else 0
+        _lib.check(_lib.load().monolis_hip_time_matvec(ctypes.c_void_p(self.h), fp32, ctypes.c_void_p(x.ptr),
+                                                       ctypes.c_void_p(y.ptr), warmup, reps, ctypes.byref(ms)),
+                   "time_matvec")
+        return ms.value
+
+
+def inner_product_device(x: DeviceBuffer, y: DeviceBuffer, m: Optional[int] = None) -> float:
+    s = ctypes.c_double()
+    n = x.count if m is None else m
+    _lib.check(_lib.load().monolis_hip_inner_product_R(n, ctypes.c_void_p(x.ptr), ctypes.c_void_p(y.ptr),
+                                                       ctypes.byref(s)), "inner_product")
+    return s.value

@@ -1,0 +1,494 @@
+// extern "C" entry points of libmonolis_hip.so (declared in include/monolis_hip.h).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/monolis_hip.h"
+#include "mh_internal.hpp"
+
+using mh::Context;
+using mh::Matrix;
+
+struct monolis_hip_matrix {
+  Matrix* M = nullptr;
+  mh::Workspace W;
+  ~monolis_hip_matrix() { delete M; }
+};
+
+namespace {
+
+// Iarray / Rarray slots (src/define/def_solver_prm.f90:90-163)
+enum {
+  I_METHOD = 1, I_PRECOND = 2, I_MAX_ITER = 3, I_CUR_ITER = 4, I_IERR = 5, I_IS_INIT_X = 10,
+  I_IS_MEASUREMENT = 13, I_IS_ERROR_ABORT = 16, I_SHOW_ITERLOG = 20, I_SHOW_TIME = 21, I_SHOW_SUMMARY = 22
+};
+enum { R_TOL = 1, R_CUR_RESID = 2, R_TIME_SOL = 3, R_TIME_PREP = 4, R_TIME_SPMV = 5, R_TIME_DOTP = 6,
+       R_TIME_PREC = 7, R_TIME_COMM_DOTP = 8, R_TIME_COMM_SPMV = 9 };
+
+// Status codes written to Iarray[5].
+enum { ST_OK = 0, ST_NOT_CONVERGED = 1, ST_HIP = -1, ST_ARGS = -2, ST_PRECOND = -3, ST_METHOD = -4,
+       ST_MULTI_RANK = -5 };
+
+const char* str_iter(int m) {
+  static const char* s[] = {"CG", "GropCG", "PipeCG", "PipeCR", "BiCGSTAB", "PipeBiCGSTAB", "BiCGSTAB_noprec",
+                            "PipeBiCGSTAB_noprec", "BiCGSTAB_N128", "SOR", "BiCGSAFE", "IDR(s)", "COCG"};
+  return (m >= 1 && m <= 13) ? s[m - 1] : "unknown";
+}
+const char* str_prec(int p) {
+  static const char* s[] = {"None", "Diag", "SOR", "LU", "MUMPS", "MUMPSL"};
+  return (p >= 0 && p <= 5) ? s[p] : "unknown";
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+mh::SolveParams params_from(const int* Iarray, const double* Rarray) {
+  mh::SolveParams P;
+  P.method = Iarray[I_METHOD];
+  P.precond = Iarray[I_PRECOND];
+  P.max_iter = Iarray[I_MAX_ITER];
+  P.init_x = Iarray[I_IS_INIT_X] != 0;
+  P.tol = Rarray[R_TOL];
+  return P;
+}
+
+int check_method(int method, int precond) {
+  if (method != 1 && method != 5) {
+    mh::set_error("method %d (%s) is not implemented on the GPU path (CG = 1, BiCGSTAB = 5)", method,
+                  str_iter(method));
+    return ST_METHOD;
+  }
+  if (precond != 0 && precond != 1) {
+    mh::set_error("preconditioner %d (%s) is not implemented on the GPU path (NONE = 0, DIAG = 1)", precond,
+                  str_prec(precond));
+    return ST_PRECOND;
+  }
+  return 0;
+}
+
+void print_banner(const int* Iarray, int my_rank) {
+  // src/solver/solver.f90:230-234
+  if (Iarray[I_SHOW_SUMMARY] && my_rank == 0)
+    printf("** MONOLIS solver: %s, prec: %s\n", str_iter(Iarray[I_METHOD]), str_prec(Iarray[I_PRECOND]));
+}
+
+void print_after(const int* Iarray, const double* Rarray, const std::vector<double>& hist, int my_rank) {
+  if (my_rank != 0) return;
+  // per-iteration log of src/linalg/mat_converge.f90:125-127, printed after the solve
+  if (Iarray[I_SHOW_ITERLOG])
+    for (size_t k = 1; k < hist.size(); ++k) printf("%7d%16.6E\n", (int)k, hist[k]);
+  // src/define/def_solver_prm.f90:272-303
+  if (Iarray[I_SHOW_SUMMARY]) {
+    printf("** MONOLIS converge iter:%10d\n", Iarray[I_CUR_ITER]);
+    printf("** MONOLIS rel. residual:%10.3E\n", Rarray[R_CUR_RESID]);
+  }
+  if (Iarray[I_SHOW_SUMMARY] || Iarray[I_SHOW_TIME]) printf("** MONOLIS solution time:%10.3E\n", Rarray[R_TIME_SOL]);
+  if (Iarray[I_SHOW_TIME]) {
+    printf(" - solution/prepost time:%10.3E\n", Rarray[R_TIME_PREP]);
+    printf(" - solution/SpMV    time:%10.3E\n", Rarray[R_TIME_SPMV]);
+    printf(" - solution/inner p time:%10.3E\n", Rarray[R_TIME_DOTP]);
+    printf(" - solution/precond time:%10.3E\n", Rarray[R_TIME_PREC]);
+    printf(" - (comm time/inner p)  :%10.3E\n", Rarray[R_TIME_COMM_DOTP]);
+    printf(" - (comm time/spmv)     :%10.3E\n", Rarray[R_TIME_COMM_SPMV]);
+  }
+  fflush(stdout);
+}
+
+[[noreturn]] void error_stop(const char* msg) {
+  fprintf(stderr, "** monolis error: %s\n", msg);
+  fflush(stderr);
+  exit(EXIT_FAILURE);
+}
+
+// Common device-resident solve.  Fills Iarray/Rarray; returns a status code.
+int solve_device(monolis_hip_matrix* H, const double* b, double* x, int* Iarray, double* Rarray,
+                 std::vector<double>* hist_out) {
+  Context* c = nullptr;
+  if (mh::context(&c)) return ST_HIP;
+  const mh::SolveParams P = params_from(Iarray, Rarray);
+  int rc = check_method(P.method, P.precond);
+  if (rc) return rc;
+  mh::SolveResult res;
+  rc = mh::krylov_solve<double>(*H->M, P, b, x, H->W, &res, c->stream);
+  if (rc) return rc == -3 ? ST_PRECOND : rc == -4 ? ST_METHOD : rc == -2 ? ST_ARGS : ST_HIP;
+  if (!res.zero_rhs) {
+    Iarray[I_CUR_ITER] = res.iter;
+    Rarray[R_CUR_RESID] = res.resid;
+  } else {
+    fprintf(stderr, "** monolis error: norm of RHS vector B is 0.0\n");  // mat_converge.f90:41-43
+  }
+  if (hist_out) *hist_out = std::move(res.hist);
+  return res.converged ? ST_OK : ST_NOT_CONVERGED;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* monolis_hip_last_error(void) { return mh::last_error(); }
+
+int monolis_hip_init(void) {
+  Context* c = nullptr;
+  return mh::context(&c);
+}
+
+void* monolis_hip_stream(void) {
+  Context* c = nullptr;
+  if (mh::context(&c)) return nullptr;
+  return (void*)c->stream;
+}
+
+int monolis_hip_synchronize(void) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_malloc(void** ptr, size_t bytes) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return 0;
+}
+
+int monolis_hip_free(void* ptr) {
+  if (!ptr) return 0;
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  MH_HIP(hipFree(ptr));
+  return 0;
+}
+
+int monolis_hip_memcpy(void* dst, const void* src, size_t bytes) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (bytes == 0) return 0;
+  MH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_memset(void* dst, int value, size_t bytes) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (bytes == 0) return 0;
+  MH_HIP(hipMemsetAsync(dst, value, bytes, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_matrix_create(monolis_hip_matrix** out, int n, int np, int nz, int n_dof, const int* index,
+                              const int* item, const double* A) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (!out) {
+    mh::set_error("null output handle");
+    return ST_ARGS;
+  }
+  *out = nullptr;
+  if (n_dof == -1) {
+    mh::set_error("variable block size (n_dof = -1) is not supported by the GPU path");
+    return ST_ARGS;
+  }
+  Matrix* M = nullptr;
+  MH_TRY(mh::matrix_create(&M, n, np, nz, n_dof, index, item, A, c->stream));
+  auto* H = new monolis_hip_matrix();
+  H->M = M;
+  *out = H;
+  return 0;
+}
+
+int monolis_hip_matrix_destroy(monolis_hip_matrix* H) {
+  if (!H) return 0;
+  Context* c = nullptr;
+  if (mh::context(&c) == 0) (void)hipStreamSynchronize(c->stream);
+  delete H;
+  return 0;
+}
+
+int monolis_hip_matrix_set_values(monolis_hip_matrix* H, const double* A) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_TRY(mh::matrix_set_values(H->M, A, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_matrix_info(const monolis_hip_matrix* H, long long* info) {
+  const Matrix* M = H->M;
+  info[0] = M->n;
+  info[1] = M->np;
+  info[2] = M->nz;
+  info[3] = M->b;
+  info[4] = M->nslices;
+  info[5] = (long long)M->nslots;
+  info[6] = (long long)(M->nslots * mh::kWave * (size_t)M->b * M->b * 8 + M->nslots * mh::kWave * 4);
+  info[7] = (long long)(M->nslots * mh::kWave * (size_t)M->b * M->b * 4 + M->nslots * mh::kWave * 4);
+  return 0;
+}
+
+int monolis_hip_matvec_R(monolis_hip_matrix* H, const double* x, double* y) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_TRY(mh::spmv<double>(*H->M, x, y, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_matvec_S(monolis_hip_matrix* H, const float* x, float* y) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_TRY(mh::matrix_make_fp32(H->M, c->stream));
+  MH_TRY(mh::spmv<float>(*H->M, x, y, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_inner_product_R(int m, const double* x, const double* y, double* sum) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  mh::DevArray<double> scratch;
+  const size_t np = mh::partials_needed(m);
+  MH_TRY(scratch.alloc(np + 16));
+  unsigned* ticket = (unsigned*)(scratch.p + np + 8);
+  MH_HIP(hipMemsetAsync(ticket, 0, 8 * sizeof(double), c->stream));
+  const double* dx = x;
+  const double* dy = y;
+  mh::DevArray<double> hx, hy;
+  if (!mh::is_device_pointer(x)) {
+    MH_TRY(hx.alloc((size_t)m));
+    MH_TRY(mh::upload(hx.p, x, sizeof(double) * (size_t)m, c->stream));
+    dx = hx.p;
+  }
+  if (!mh::is_device_pointer(y)) {
+    MH_TRY(hy.alloc((size_t)m));
+    MH_TRY(mh::upload(hy.p, y, sizeof(double) * (size_t)m, c->stream));
+    dy = hy.p;
+  }
+  MH_TRY(mh::dot<double>(m, dx, dy, scratch.p + np, scratch.p, ticket, c->stream));
+  MH_HIP(hipMemcpyAsync(sum, scratch.p + np, sizeof(double), hipMemcpyDefault, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_solve_R(monolis_hip_matrix* H, const double* b, double* x, int* Iarray, double* Rarray,
+                        double* hist, int hist_len) {
+  const double t0 = now();
+  std::vector<double> h;
+  const int st = solve_device(H, b, x, Iarray, Rarray, &h);
+  Iarray[I_IERR] = st;
+  Rarray[R_TIME_SOL] = now() - t0;
+  if (hist && hist_len > 0) {
+    for (int k = 0; k < hist_len; ++k) hist[k] = (k < (int)h.size()) ? h[k] : 0.0;
+  }
+  return st < 0 ? st : 0;
+}
+
+int monolis_hip_solve_refined_R(monolis_hip_matrix* H, const double* b, double* x, int* Iarray, double* Rarray,
+                                double inner_tol, int max_outer, double* stats) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  const int rc0 = check_method(Iarray[I_METHOD], Iarray[I_PRECOND]);
+  if (rc0) {
+    Iarray[I_IERR] = rc0;
+    return rc0;
+  }
+  const double t0 = now();
+  mh::RefineParams P;
+  P.tol = Rarray[R_TOL];
+  P.inner_tol = inner_tol;
+  P.max_outer = max_outer;
+  P.inner_max_iter = Iarray[I_MAX_ITER];
+  P.method = Iarray[I_METHOD];
+  P.precond = Iarray[I_PRECOND];
+  P.init_x = Iarray[I_IS_INIT_X] != 0;
+  mh::RefineResult R;
+  const int rc = mh::refine_solve(*H->M, P, b, x, H->W, &R, c->stream);
+  if (rc) {
+    Iarray[I_IERR] = ST_HIP;
+    return rc;
+  }
+  Iarray[I_CUR_ITER] = R.inner_total;
+  Rarray[R_CUR_RESID] = R.resid;
+  Iarray[I_IERR] = R.converged ? ST_OK : ST_NOT_CONVERGED;
+  Rarray[R_TIME_SOL] = now() - t0;
+  if (stats) {
+    stats[0] = R.outer;
+    stats[1] = R.inner_total;
+    stats[2] = R.resid;
+    stats[3] = R.converged;
+  }
+  return 0;
+}
+
+int monolis_hip_time_matvec(monolis_hip_matrix* H, int fp32, const void* x, void* y, int warmup, int reps,
+                            double* ms) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (fp32) MH_TRY(mh::matrix_make_fp32(H->M, c->stream));
+  auto one = [&]() -> int {
+    if (fp32) return mh::spmv<float>(*H->M, (const float*)x, (float*)y, c->stream);
+    return mh::spmv<double>(*H->M, (const double*)x, (double*)y, c->stream);
+  };
+  for (int i = 0; i < warmup; ++i) MH_TRY(one());
+  hipEvent_t e0, e1;
+  MH_HIP(hipEventCreate(&e0));
+  MH_HIP(hipEventCreate(&e1));
+  MH_HIP(hipEventRecord(e0, c->stream));
+  for (int i = 0; i < reps; ++i) MH_TRY(one());
+  MH_HIP(hipEventRecord(e1, c->stream));
+  MH_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  MH_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = reps > 0 ? (double)t / reps : 0.0;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// monolis flat ABI
+// ---------------------------------------------------------------------------
+void monolis_solve_R_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZNDOF2, int* n_dof_list,
+                            int* n_dof_index, int* n_dof_index2, double* A, double* x, double* b, int* index,
+                            int* item, int my_rank, int comm, int comm_size, int recv_n_neib, int recv_nitem,
+                            int* recv_neib_pe, int* recv_index, int* recv_item, int send_n_neib, int send_nitem,
+                            int* send_neib_pe, int* send_index, int* send_item, int* Iarray, double* Rarray) {
+  (void)n_dof_list; (void)n_dof_index; (void)n_dof_index2; (void)NZNDOF2; (void)comm;
+  (void)recv_n_neib; (void)recv_nitem; (void)recv_neib_pe; (void)recv_index; (void)recv_item;
+  (void)send_n_neib; (void)send_nitem; (void)send_neib_pe; (void)send_index; (void)send_item;
+  const double t0 = now();
+  // monolis_timer_initialize (def_solver_prm.f90:237-251)
+  for (int k = R_TIME_SOL; k <= R_TIME_COMM_SPMV; ++k) Rarray[k] = 0.0;
+  const int abort_on_error = Iarray[I_IS_ERROR_ABORT] != 0;
+  auto fail = [&](int code) {
+    Iarray[I_IERR] = code;
+    fprintf(stderr, "** monolis error: %s\n", mh::last_error());
+    fflush(stderr);
+    if (abort_on_error) error_stop("monolis_solve_R (GPU path) failed");
+  };
+  if (comm_size > 1) {
+    mh::set_error("comm_size %d > 1: the GPU path runs on one rank", comm_size);
+    fail(ST_MULTI_RANK);
+    return;
+  }
+  int rc = check_method(Iarray[I_METHOD], Iarray[I_PRECOND]);
+  if (rc) {
+    fail(rc);
+    return;
+  }
+  print_banner(Iarray, my_rank);
+  Context* c = nullptr;
+  if (mh::context(&c)) {
+    fail(ST_HIP);
+    return;
+  }
+  // staging: matrix (SELL build) and vectors, then the preconditioner
+  const double tp0 = now();
+  monolis_hip_matrix* H = nullptr;
+  rc = monolis_hip_matrix_create(&H, n, np, nz, n_dof, index, item, A);
+  if (rc) {
+    fail(rc == -2 ? ST_ARGS : ST_HIP);
+    return;
+  }
+  std::unique_ptr<monolis_hip_matrix> guard(H);
+  const size_t vec = (size_t)NPNDOF;
+  double* dx = x;
+  const double* db = b;
+  mh::DevArray<double> bx, bb;
+  const int x_dev = mh::is_device_pointer(x), b_dev = mh::is_device_pointer(b);
+  if (!x_dev) {
+    if (bx.alloc(vec) || mh::upload(bx.p, x, sizeof(double) * vec, c->stream)) {
+      fail(ST_HIP);
+      return;
+    }
+    dx = bx.p;
+  }
+  if (!b_dev) {
+    if (bb.alloc(vec) || mh::upload(bb.p, b, sizeof(double) * vec, c->stream)) {
+      fail(ST_HIP);
+      return;
+    }
+    db = bb.p;
+  }
+  if (Iarray[I_PRECOND] == 1 && mh::diag_setup<double>(*H->M, c->stream)) {
+    fail(ST_HIP);
+    return;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    mh::set_error("staging failed");
+    fail(ST_HIP);
+    return;
+  }
+  Rarray[R_TIME_PREP] = now() - tp0;
+  std::vector<double> hist;
+  const int st = solve_device(H, db, dx, Iarray, Rarray, &hist);
+  if (st < 0) {
+    fail(st);
+    return;
+  }
+  if (!x_dev) {
+    if (hipMemcpyAsync(x, dx, sizeof(double) * vec, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      mh::set_error("copy of the solution failed");
+      fail(ST_HIP);
+      return;
+    }
+  }
+  Iarray[I_IERR] = st;
+  Rarray[R_TIME_SOL] = now() - t0;
+  print_after(Iarray, Rarray, hist, my_rank);
+  if (st == ST_NOT_CONVERGED && !Iarray[I_IS_MEASUREMENT] && abort_on_error)
+    error_stop("reached the maximum number of iterations");  // mat_converge.f90:134-140
+}
+
+void monolis_matvec_product_R_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZNDOF2, int* n_dof_list,
+                                     int* n_dof_index, int* n_dof_index2, double* A, double* x, double* y,
+                                     int* index, int* item, int my_rank, int comm, int comm_size,
+                                     int recv_n_neib, int recv_nitem, int* recv_neib_pe, int* recv_index,
+                                     int* recv_item, int send_n_neib, int send_nitem, int* send_neib_pe,
+                                     int* send_index, int* send_item) {
+  (void)n_dof_list; (void)n_dof_index; (void)n_dof_index2; (void)NZNDOF2; (void)my_rank; (void)comm;
+  (void)recv_n_neib; (void)recv_nitem; (void)recv_neib_pe; (void)recv_index; (void)recv_item;
+  (void)send_n_neib; (void)send_nitem; (void)send_neib_pe; (void)send_index; (void)send_item;
+  if (comm_size > 1) error_stop("monolis_matvec_product_R (GPU path): comm_size > 1 is not supported");
+  Context* c = nullptr;
+  if (mh::context(&c)) error_stop(mh::last_error());
+  monolis_hip_matrix* H = nullptr;
+  if (monolis_hip_matrix_create(&H, n, np, nz, n_dof, index, item, A)) error_stop(mh::last_error());
+  std::unique_ptr<monolis_hip_matrix> guard(H);
+  const size_t vec = (size_t)NPNDOF;
+  const size_t m = (size_t)n * n_dof;
+  const double* dx = x;
+  double* dy = y;
+  mh::DevArray<double> bx, by;
+  const int x_dev = mh::is_device_pointer(x), y_dev = mh::is_device_pointer(y);
+  if (!x_dev) {
+    if (bx.alloc(vec) || mh::upload(bx.p, x, sizeof(double) * vec, c->stream)) error_stop(mh::last_error());
+    dx = bx.p;
+  }
+  if (!y_dev) {
+    if (by.alloc(vec)) error_stop(mh::last_error());
+    dy = by.p;
+  }
+  if (mh::spmv<double>(*H->M, dx, dy, c->stream)) error_stop(mh::last_error());
+  if (!y_dev && m > 0) {
+    if (hipMemcpyAsync(y, dy, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+      error_stop("copy of y failed");
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) error_stop("monolis_matvec_product_R (GPU path) failed");
+}
+
+void monolis_inner_product_R_c_main(int m, double* x, double* y, double* sum, int comm) {
+  (void)comm;
+  if (monolis_hip_inner_product_R(m, x, y, sum)) error_stop(mh::last_error());
+}
+
+}  // extern "C"

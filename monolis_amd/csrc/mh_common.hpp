@@ -1,0 +1,153 @@
+// Shared device/host helpers for the monolis HIP path (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace mh {
+
+constexpr int kWave = 64;     // CDNA wavefront width
+constexpr int kBlock = 256;   // 4 wavefronts per workgroup
+constexpr int kWpb = kBlock / kWave;
+constexpr int kXcd = 8;       // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// ---------------------------------------------------------------------------
+// Host-side error state.  Every internal entry point returns 0 on success or a
+// negative code; the message of the last failure is kept per thread.
+// ---------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+const char* last_error();
+void clear_error();
+
+#define MH_HIP(expr)                                                                 \
+  do {                                                                               \
+    hipError_t mh_e_ = (expr);                                                       \
+    if (mh_e_ != hipSuccess) {                                                       \
+      ::mh::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr,              \
+                      hipGetErrorString(mh_e_));                                     \
+      return -1;                                                                     \
+    }                                                                                \
+  } while (0)
+
+#define MH_TRY(expr)              \
+  do {                            \
+    int mh_rc_ = (expr);          \
+    if (mh_rc_ != 0) return mh_rc_; \
+  } while (0)
+
+// Launch geometry with XCD-aware remapping: the grid is padded to a multiple
+// of 8 workgroups and logical tile t = (xcd, k) is given to hardware workgroup
+// k*8 + xcd, so each XCD (and its private L2) walks one contiguous eighth of
+// the rows.  Gathered vector entries of neighbouring rows then hit the same L2.
+inline int padded_grid(long long nwg) {
+  long long g = (nwg + kXcd - 1) / kXcd * kXcd;
+  return (int)(g > 0 ? g : kXcd);
+}
+
+__device__ __forceinline__ int xcd_tile(int nwg_logical) {
+  const int g = gridDim.x;            // multiple of kXcd
+  const int per = g / kXcd;
+  const int b = blockIdx.x;
+  const int t = (b % kXcd) * per + b / kXcd;
+  return t < nwg_logical ? t : -1;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions.  All sums are deterministic: fixed butterfly order inside a
+// wavefront, fixed wave order inside a workgroup, and a fixed-order final pass
+// over per-workgroup partials done by the last workgroup to arrive.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    int w = __shfl_xor(v, o, kWave);
+    v = v > w ? v : w;
+  }
+  return v;
+}
+
+// Sum NV values over the workgroup; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+  __shared__ double red[kWpb][NV];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[w][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = red[0][k];
+      for (int i = 1; i < kWpb; ++i) s += red[i][k];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void store_agent(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_agent(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publishes this workgroup's NV partial sums (thread 0 holds them) and returns
+// true in every thread of the workgroup that arrives last.  That workgroup then
+// owns `total[k]` = fixed-order sum over all partials.  The ticket is reset by
+// the last workgroup, so it is ready for the next launch.
+template <int NV>
+__device__ __forceinline__ bool reduce_last(const double (&v)[NV], double* partials, int tile,
+                                            int ntiles, unsigned* ticket, double (&total)[NV]) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) store_agent(&partials[(size_t)tile * NV + k], v[k]);
+    __threadfence();
+    const unsigned t = atomicAdd(ticket, 1u);
+    is_last = (t == (unsigned)ntiles - 1u);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int i = threadIdx.x; i < ntiles; i += kBlock) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] += load_agent(&partials[(size_t)i * NV + k]);
+  }
+  block_sum<NV>(acc);
+  __shared__ double tot[NV];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] = acc[k];
+    *ticket = 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) total[k] = tot[k];
+  return true;
+}
+
+// Streaming loads: matrix values and column ids are read once per SpMV, so
+// they are fetched non-temporally to keep L2 / MALL for the gathered vector.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  return __builtin_nontemporal_load(p);
+}
+
+}  // namespace mh

@@ -1,0 +1,184 @@
+// Host-side structures shared by the translation units of libmonolis_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "mh_common.hpp"
+
+namespace mh {
+
+template <typename TV>
+struct SellView;
+
+// Largest fixed block size with specialised kernels.
+constexpr int kMaxB = 8;
+
+// ---------------------------------------------------------------------------
+// Per-process device context: one HIP stream at default priority, pinned host
+// slots for polling solver flags, and a reusable pool of reduction scratch.
+// ---------------------------------------------------------------------------
+struct Context {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  int* h_flags = nullptr;            // pinned host, kFlagSlots ints
+  hipEvent_t ev[16] = {};
+  bool ok = false;
+};
+constexpr int kFlagSlots = 16;
+
+// Returns 0 and fills *c on success.  Initialises HIP on first use.
+int context(Context** c);
+
+template <typename T>
+struct DevArray {
+  T* p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count);
+  void release();
+  ~DevArray() { release(); }
+  DevArray() = default;
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+};
+
+// Copies `count` elements from a host or device pointer into device memory
+// (the direction is detected from the pointer's attributes).
+int is_device_pointer(const void* p);
+int upload(void* dst, const void* src, size_t bytes, hipStream_t s);
+
+// Device-side solver scalars, written by the last workgroup of a reduction and
+// read by later kernels.  One instance per solve.
+struct KState {
+  double rho;       // (r, z) of the current iteration (CG) / (r, rt) (BiCGSTAB)
+  double rho_prev;
+  double alpha, beta, omega;
+  double pq;        // (p, q) / (rt, v)
+  double rr;        // (r, r)
+  double B2;        // (r0, r0), denominator of the relative residual
+  double resid;     // sqrt(rr / B2)
+  double tol;
+  double ts, tt;    // BiCGSTAB (t, s), (t, t)
+  double aux[4];
+  int iter;         // last iteration whose convergence test ran
+  int max_iter;
+  int converged;
+  int done;
+  int breakdown;    // bit mask of exact-zero dot products seen (BiCGSTAB)
+  int pad[3];
+  unsigned ticket[4];
+};
+
+// ---------------------------------------------------------------------------
+// Device matrix: SELL-64 pattern + FP64 (and optionally FP32) values, plus the
+// CSR arrays kept for the preconditioner setup, transposes and gradients.
+// ---------------------------------------------------------------------------
+struct Matrix {
+  int n = 0, np = 0, nz = 0, b = 0;
+  int nslices = 0;
+  size_t nslots = 0;
+  DevArray<int> slice_ptr, rowptr, col;     // SELL pattern (rowptr = monolis index)
+  DevArray<int> csr_item;                   // 1-based, as given
+  DevArray<double> csr_val;                 // AoS blocks, as given (kept for setup / gradients)
+  DevArray<double> val64;
+  DevArray<float> val32;
+  // block-Jacobi factors, structure of arrays (entry e of row i at D[e*n + i])
+  DevArray<double> D64;
+  DevArray<float> D32;
+  bool diag_ready = false, diag32_ready = false, val32_ready = false;
+  // transpose (for adjoint solves of nonsymmetric systems), built lazily
+  Matrix* transpose = nullptr;
+  ~Matrix();
+};
+
+int matrix_create(Matrix** out, int n, int np, int nz, int b, const int* index, const int* item,
+                  const double* A, hipStream_t s);
+int matrix_set_values(Matrix* M, const double* A, hipStream_t s);
+int matrix_make_fp32(Matrix* M, hipStream_t s);
+int matrix_transpose(Matrix* M, hipStream_t s);
+
+template <typename TV>
+SellView<TV> view(const Matrix& M);
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (all asynchronous on `s`).
+// ---------------------------------------------------------------------------
+template <typename T>
+int spmv(const Matrix& M, const T* x, T* y, hipStream_t s);
+// r = b - A x; if rr_out != nullptr, rr_out = (r, r) (device scalar, deterministic).
+template <typename T>
+int residual(const Matrix& M, const T* x, const T* b, T* r, double* partials, unsigned* ticket,
+             double* rr_out, hipStream_t s);
+template <typename T>
+int diag_setup(Matrix& M, hipStream_t s);
+template <typename T>
+int diag_apply(const Matrix& M, const T* x, T* y, hipStream_t s);
+// Device dot, deterministic; result written to *out (device pointer).
+template <typename T>
+int dot(long long m, const T* x, const T* y, double* out, double* partials, unsigned* ticket,
+        hipStream_t s);
+template <typename T>
+int axpby(long long m, double alpha, const T* x, double beta, const T* y, T* z, hipStream_t s);
+
+// Number of reduction partials a launch over `rows` rows (or `slices`) may need.
+size_t partials_needed(long long n);
+
+// ---------------------------------------------------------------------------
+// Solver driver.
+// ---------------------------------------------------------------------------
+struct SolveParams {
+  int method = 1;        // 1 CG, 5 BiCGSTAB
+  int precond = 1;       // 0 NONE, 1 DIAG
+  int max_iter = 1000;
+  int init_x = 1;
+  double tol = 1e-8;
+  int chunk = 8;         // iterations enqueued between convergence polls
+  int keep_hist = 1;     // record per-iteration residuals
+};
+
+struct SolveResult {
+  int iter = 0;
+  double resid = 0.0;
+  int converged = 0;
+  int zero_rhs = 0;
+  int breakdown = 0;
+  double t_solve = 0.0;  // seconds, device-resident part
+  std::vector<double> hist;
+};
+
+// Work buffers of one solve, reusable across solves on the same matrix.
+struct Workspace {
+  DevArray<unsigned char> raw;
+  size_t bytes = 0;
+  int ensure(size_t need);
+};
+
+template <typename T>
+int krylov_solve(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, SolveResult* out,
+                 hipStream_t s);
+
+// FP32-inner / FP64-refined solve.  `x` and `b` are FP64 device vectors.
+struct RefineParams {
+  double tol = 1e-8;        // outer relative residual ||b - A x|| / ||b - A x0||
+  double inner_tol = 1e-4;  // inner FP32 relative residual per correction
+  int max_outer = 50;
+  int inner_max_iter = 1000;
+  int method = 1;
+  int precond = 1;
+  int init_x = 1;
+};
+struct RefineResult {
+  int outer = 0;
+  int inner_total = 0;
+  double resid = 0.0;
+  int converged = 0;
+  std::vector<double> outer_resid;
+  std::vector<int> inner_iters;
+};
+int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, Workspace& W,
+                 RefineResult* out, hipStream_t s);
+
+}  // namespace mh

@@ -1,0 +1,166 @@
+// Device-side building blocks: the sliced-ELL block layout, the SpMV row
+// loop, the block-Jacobi apply, and the generic fused SpMV kernel.
+//
+// Layout ("SELL-64", built once per matrix from monolis's BCSR arrays):
+//   rows are grouped in slices of 64 consecutive block rows, one wavefront per
+//   slice and one lane per block row.  Slice s holds w_s = max row length slots;
+//   slot k of lane l stores the k-th block of row 64*s+l (CSR order, so columns
+//   stay ascending).  For slot t = slice_ptr[s] + k:
+//     col[t*64 + l]                 0-based block column
+//     val[(t*B*B + e)*64 + l]       block entry e = r*B + c (row-major block)
+//   so every load of a slot is one contiguous 256 B (int) or 512 B (double)
+//   line-aligned request for the whole wavefront.  Slots past a row's length
+//   are padding (col = own row, val = 0) and are never loaded: the lane is
+//   masked off by the row length taken from monolis's `index` array.
+#pragma once
+
+#include "mh_common.hpp"
+
+namespace mh {
+
+template <typename TV>
+struct SellView {
+  int nrows;               // block rows that are multiplied (monolis N)
+  int nslices;             // ceil(nrows / 64)
+  const int* slice_ptr;    // [nslices + 1] slot offsets
+  const int* rowptr;       // [nrows + 1] monolis `index` (0-based offsets)
+  const int* col;          // [nslots * 64]
+  const TV* val;           // [nslots * 64 * B * B]
+};
+
+// acc[r] += sum_k sum_c val(k, r, c) * x(col_k, c) for the lane's row.  `xg`
+// gathers the B entries of the input vector at a block column.  Products are
+// accumulated in the order of monolis's matvec_33_R (matvec.f90:805-823):
+// block by block, and inside a block c = 1..B.
+template <typename TV, typename TA, int B, class XG>
+__device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane, int len, const XG& xg,
+                                         TA (&acc)[B]) {
+  const int k0 = M.slice_ptr[s];
+  const int w = M.slice_ptr[s + 1] - k0;
+  const int* colp = M.col + (size_t)k0 * kWave + lane;
+  const TV* valp = M.val + (size_t)k0 * kWave * (B * B) + lane;
+#pragma unroll 2
+  for (int k = 0; k < w; ++k) {
+    if (k < len) {
+      const int c = ld_stream(colp + (size_t)k * kWave);
+      TA xv[B];
+      xg(c, xv);
+      const TV* vp = valp + (size_t)k * kWave * (B * B);
+      TV a[B * B];
+#pragma unroll
+      for (int e = 0; e < B * B; ++e) a[e] = ld_stream(vp + e * kWave);
+#pragma unroll
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int cc = 0; cc < B; ++cc) acc[r] = fma((TA)a[r * B + cc], xv[cc], acc[r]);
+    }
+  }
+}
+
+// Block-Jacobi apply on one block row, in place.  D is the factored diagonal
+// block in structure-of-arrays form: entry e of row i at D[e * nrows + i].
+// The arithmetic order is that of diag_33.f90:123-135 for B = 3 and of
+// diag_nn.f90:207-228 otherwise (B = 1 reduces to x * D).
+template <typename T, typename TD, int B>
+__device__ __forceinline__ void diag_apply(const TD* __restrict__ D, int nrows, int i, T (&x)[B]) {
+  if constexpr (B == 1) {
+    x[0] = (T)D[i] * x[0];
+  } else if constexpr (B == 3) {
+    const size_t n = (size_t)nrows;
+    const T l3 = (T)D[3 * n + i], l6 = (T)D[6 * n + i], l7 = (T)D[7 * n + i];
+    const T l8 = (T)D[8 * n + i], l4 = (T)D[4 * n + i], l5 = (T)D[5 * n + i];
+    const T l0 = (T)D[i], l1 = (T)D[1 * n + i], l2 = (T)D[2 * n + i];
+    T x1 = x[0], x2 = x[1], x3 = x[2];
+    x2 = x2 - l3 * x1;
+    x3 = x3 - l6 * x1 - l7 * x2;
+    x3 = l8 * x3;
+    x2 = l4 * (x2 - l5 * x3);
+    x1 = l0 * (x1 - l2 * x3 - l1 * x2);
+    x[0] = x1;
+    x[1] = x2;
+    x[2] = x3;
+  } else {
+    const size_t n = (size_t)nrows;
+#pragma unroll
+    for (int j = 1; j < B; ++j)
+#pragma unroll
+      for (int k = 0; k < j; ++k) x[j] = x[j] - (T)D[(size_t)(B * j + k) * n + i] * x[k];
+#pragma unroll
+    for (int j = B - 1; j >= 0; --j) {
+#pragma unroll
+      for (int k = B - 1; k > j; --k) x[j] = x[j] - (T)D[(size_t)(B * j + k) * n + i] * x[k];
+      x[j] = (T)D[(size_t)((B + 1) * j) * n + i] * x[j];
+    }
+  }
+}
+
+// Generic fused SpMV kernel: one wavefront per 64-row slice, one lane per block
+// row.  `Op` supplies
+//   bool skip()                                  whole launch is a no-op (solver done)
+//   void gather(int col, TA (&xv)[B])            input vector at a block column
+//   void epilogue(int row, bool valid, const TA (&acc)[B], double (&part)[NV])
+//   void finalize(const double (&tot)[NV])       run once, by thread 0 of the last workgroup
+// and NV >= 0 partial sums that are reduced deterministically.
+template <typename TV, typename TA, int B, class Op>
+__global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op, double* partials,
+                                                 unsigned* ticket, int ntiles) {
+  const int tile = xcd_tile(ntiles);
+  if (tile < 0) return;
+  if (op.skip()) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int s = tile * kWpb + threadIdx.x / kWave;
+  const int row = s * kWave + lane;
+  const bool valid = (s < M.nslices) && (row < M.nrows);
+  int len = 0;
+  if (valid) len = M.rowptr[row + 1] - M.rowptr[row];
+  TA acc[B];
+#pragma unroll
+  for (int r = 0; r < B; ++r) acc[r] = (TA)0;
+  if (s < M.nslices) {
+    auto xg = [&](int c, TA(&xv)[B]) { op.gather(c, xv); };
+    sell_row<TV, TA, B>(M, s, lane, len, xg, acc);
+  }
+  constexpr int NV = Op::NV;
+  if constexpr (NV == 0) {
+    double dummy[1];
+    op.epilogue(row, valid, acc, dummy);
+  } else {
+    double part[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) part[k] = 0.0;
+    op.epilogue(row, valid, acc, part);
+    block_sum<NV>(part);
+    double tot[NV];
+    if (reduce_last<NV>(part, partials, tile, ntiles, ticket, tot)) {
+      if (threadIdx.x == 0) op.finalize(tot);
+    }
+  }
+}
+
+// Row-parallel (non-SpMV) fused vector kernel: one thread per block row.
+//   void row(int i, double (&part)[NV]);  void finalize(const double (&tot)[NV]);  bool skip();
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_rows(int nrows, Op op, double* partials, unsigned* ticket,
+                                                 int ntiles) {
+  const int tile = xcd_tile(ntiles);
+  if (tile < 0) return;
+  if (op.skip()) return;
+  const int i = tile * kBlock + threadIdx.x;
+  constexpr int NV = Op::NV;
+  if constexpr (NV == 0) {
+    double dummy[1];
+    if (i < nrows) op.row(i, dummy);
+  } else {
+    double part[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) part[k] = 0.0;
+    if (i < nrows) op.row(i, part);
+    block_sum<NV>(part);
+    double tot[NV];
+    if (reduce_last<NV>(part, partials, tile, ntiles, ticket, tot)) {
+      if (threadIdx.x == 0) op.finalize(tot);
+    }
+  }
+}
+
+}  // namespace mh

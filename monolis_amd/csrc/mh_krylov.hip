@@ -1,0 +1,832 @@
+// Device-resident CG and BiCGSTAB (monolis src/iterative/CG.f90:18-119,
+// src/iterative/BiCGSTAB.f90:18-192) and FP32-inner / FP64 iterative refinement.
+//
+// Every iteration is two (CG) or five (BiCGSTAB) fused kernels.  Scalars
+// (rho, alpha, beta, omega, residual) never leave the GPU: the last workgroup
+// of each reduction derives them and writes them to a KState in device
+// memory, and it also runs monolis's convergence test
+// (src/linalg/mat_converge.f90:90-141) and raises `done`.  The host enqueues
+// iterations in chunks and polls `done` through pinned memory one chunk behind,
+// so the GPU never waits for the CPU; kernels launched after `done` exit at
+// once.  The true-residual replacement every 200 iterations (CG.f90:91-95,
+// BiCGSTAB.f90:163-167) is kept.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+
+#include "mh_internal.hpp"
+#include "mh_kernels.hpp"
+
+namespace mh {
+
+namespace {
+
+constexpr int kIterRR = 200;
+
+enum : int { F_UPD = 1, F_CHECK = 2, F_FIRST = 4, F_XONLY = 8 };
+
+// ---- shared finalize: monolis_check_converge_R --------------------------------
+__device__ __forceinline__ bool finalize_check(KState* st, double* hist, int it, double rr) {
+  const double resid = sqrt(rr / st->B2);
+  st->rr = rr;
+  st->resid = resid;
+  st->iter = it;
+  if (hist) hist[it] = resid;
+  if (resid < st->tol) {
+    st->converged = 1;
+    st->done = 1;
+    return true;
+  }
+  if (it >= st->max_iter) {
+    st->done = 1;
+    return true;
+  }
+  return false;
+}
+
+// ---- residual used inside a solve (honours `done`, optional copy) -----------
+template <typename T, int B>
+struct OpResidSolve {
+  static constexpr int NV = 1;
+  const T* __restrict__ x;
+  const T* __restrict__ b;
+  T* __restrict__ r;
+  T* __restrict__ r2;   // optional second copy (BiCGSTAB r~)
+  double* out;          // (r, r) destination, may be null
+  const KState* st;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void gather(int c, T (&xv)[B]) const {
+#pragma unroll
+    for (int k = 0; k < B; ++k) xv[k] = x[(size_t)c * B + k];
+  }
+  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[1]) const {
+    if (!valid) return;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const T v = b[(size_t)row * B + k] - acc[k];
+      r[(size_t)row * B + k] = v;
+      if (r2) r2[(size_t)row * B + k] = v;
+      part[0] += (double)v * (double)v;
+    }
+  }
+  __device__ void finalize(const double (&tot)[1]) const {
+    if (out) *out = tot[0];
+  }
+};
+
+// ---- CG: q = A p with p = z + beta p_old formed on the fly; (p, q) -> alpha ---
+template <typename T, int B>
+struct OpCgSpmv {
+  static constexpr int NV = 1;
+  const T* __restrict__ z;
+  const T* __restrict__ pold;
+  T* __restrict__ pnew;
+  T* __restrict__ q;
+  KState* st;
+  T beta;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() { beta = (T)st->beta; }
+  __device__ void gather(int c, T (&xv)[B]) const {
+#pragma unroll
+    for (int k = 0; k < B; ++k) xv[k] = fma(beta, pold[(size_t)c * B + k], z[(size_t)c * B + k]);
+  }
+  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[1]) const {
+    if (!valid) return;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const size_t i = (size_t)row * B + k;
+      const T p = fma(beta, pold[i], z[i]);
+      pnew[i] = p;
+      q[i] = acc[k];
+      part[0] += (double)p * (double)acc[k];
+    }
+  }
+  __device__ void finalize(const double (&tot)[1]) const {
+    st->pq = tot[0];
+    st->alpha = st->rho / tot[0];
+  }
+};
+
+// ---- CG: x += alpha p; r -= alpha q; test; z = M^-1 r; rho, beta -------------
+template <typename T, int B, bool PREC>
+struct OpCgUpdate {
+  static constexpr int NV = 2;
+  T* __restrict__ x;
+  T* __restrict__ r;
+  T* __restrict__ z;
+  const T* __restrict__ p;
+  const T* __restrict__ q;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  double* hist;
+  int it;
+  int flags;
+  T alpha;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() { alpha = (T)st->alpha; }
+  __device__ void row(int i, double (&part)[2]) const {
+    T v[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) v[k] = r[(size_t)i * B + k];
+    if (flags & (F_UPD | F_XONLY)) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const size_t j = (size_t)i * B + k;
+        x[j] = fma(alpha, p[j], x[j]);  // X = alpha*P + X   (CG.f90:89)
+      }
+    }
+    if (flags & F_XONLY) return;
+    if (flags & F_UPD) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        const size_t j = (size_t)i * B + k;
+        v[k] = fma(-alpha, q[j], v[k]);  // R = -alpha*Q + R  (CG.f90:94)
+        r[j] = v[k];
+      }
+    }
+    T w[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      w[k] = v[k];
+      part[0] += (double)v[k] * (double)v[k];
+    }
+    if constexpr (PREC) diag_apply<T, T, B>(D, n, i, w);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      z[(size_t)i * B + k] = w[k];
+      part[1] += (double)v[k] * (double)w[k];
+    }
+  }
+  __device__ void finalize(const double (&tot)[2]) const {
+    if (flags & F_XONLY) return;
+    if (flags & F_CHECK) {
+      if (finalize_check(st, hist, it, tot[0])) return;
+    }
+    const double rho = tot[1];
+    st->beta = (flags & F_FIRST) ? 0.0 : rho / st->rho;
+    st->rho = rho;
+  }
+};
+
+// ---- BiCGSTAB ---------------------------------------------------------------
+// a: p = r + beta (p - omega v) (first: p = r); pt = M^-1 p
+template <typename T, int B, bool PREC>
+struct OpBiA {
+  static constexpr int NV = 0;
+  const T* __restrict__ r;
+  T* __restrict__ p;
+  const T* __restrict__ v;
+  T* __restrict__ pt;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  int first;
+  T beta, omega;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {
+    beta = (T)st->beta;
+    omega = (T)st->omega;
+  }
+  __device__ void row(int i, double (&)[1]) const {
+    T w[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const size_t j = (size_t)i * B + k;
+      const T pv = first ? r[j] : r[j] + beta * (p[j] - omega * v[j]);  // BiCGSTAB.f90:103
+      p[j] = pv;
+      w[k] = pv;
+    }
+    if constexpr (PREC) diag_apply<T, T, B>(D, n, i, w);
+#pragma unroll
+    for (int k = 0; k < B; ++k) pt[(size_t)i * B + k] = w[k];
+  }
+  __device__ void finalize(const double (&)[1]) const {}
+};
+
+// b: v = A pt; (rt, v) -> alpha = rho / c2
+template <typename T, int B>
+struct OpBiB {
+  static constexpr int NV = 1;
+  const T* __restrict__ pt;
+  T* __restrict__ v;
+  const T* __restrict__ rt;
+  KState* st;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void gather(int c, T (&xv)[B]) const {
+#pragma unroll
+    for (int k = 0; k < B; ++k) xv[k] = pt[(size_t)c * B + k];
+  }
+  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[1]) const {
+    if (!valid) return;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const size_t j = (size_t)row * B + k;
+      v[j] = acc[k];
+      part[0] += (double)rt[j] * (double)acc[k];
+    }
+  }
+  __device__ void finalize(const double (&tot)[1]) const {
+    const double c2 = tot[0];
+    if (c2 == 0.0) st->breakdown |= 2;
+    st->pq = c2;
+    st->alpha = st->rho / c2;
+  }
+};
+
+// c: s = r - alpha v; st = M^-1 s
+template <typename T, int B, bool PREC>
+struct OpBiC {
+  static constexpr int NV = 0;
+  const T* __restrict__ r;
+  const T* __restrict__ v;
+  T* __restrict__ s;
+  T* __restrict__ st_;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  T alpha;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() { alpha = (T)st->alpha; }
+  __device__ void row(int i, double (&)[1]) const {
+    T w[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const size_t j = (size_t)i * B + k;
+      const T sv = fma(-alpha, v[j], r[j]);  // S = -alpha*V + R  (BiCGSTAB.f90:118)
+      s[j] = sv;
+      w[k] = sv;
+    }
+    if constexpr (PREC) diag_apply<T, T, B>(D, n, i, w);
+#pragma unroll
+    for (int k = 0; k < B; ++k) st_[(size_t)i * B + k] = w[k];
+  }
+  __device__ void finalize(const double (&)[1]) const {}
+};
+
+// d: t = A st; (t, s), (t, t) -> omega
+template <typename T, int B>
+struct OpBiD {
+  static constexpr int NV = 2;
+  const T* __restrict__ stv;
+  T* __restrict__ t;
+  const T* __restrict__ s;
+  KState* st;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void gather(int c, T (&xv)[B]) const {
+#pragma unroll
+    for (int k = 0; k < B; ++k) xv[k] = stv[(size_t)c * B + k];
+  }
+  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[2]) const {
+    if (!valid) return;
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const size_t j = (size_t)row * B + k;
+      t[j] = acc[k];
+      part[0] += (double)acc[k] * (double)s[j];
+      part[1] += (double)acc[k] * (double)acc[k];
+    }
+  }
+  __device__ void finalize(const double (&tot)[2]) const {
+    st->ts = tot[0];
+    st->tt = tot[1];
+    if (tot[1] == 0.0) {
+      st->breakdown |= 4;
+      st->omega = 0.0;  // BiCGSTAB.f90:139-145
+    } else {
+      st->omega = tot[0] / tot[1];
+    }
+  }
+};
+
+// e: x += alpha pt + omega st; r = s - omega t; test; rho = (r, rt); beta
+template <typename T, int B>
+struct OpBiE {
+  static constexpr int NV = 2;
+  T* __restrict__ x;
+  T* __restrict__ r;
+  const T* __restrict__ pt;
+  const T* __restrict__ stv;
+  const T* __restrict__ s;
+  const T* __restrict__ t;
+  const T* __restrict__ rt;
+  KState* st;
+  double* hist;
+  int it;
+  int flags;
+  T alpha, omega;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {
+    alpha = (T)st->alpha;
+    omega = (T)st->omega;
+  }
+  __device__ void row(int i, double (&part)[2]) const {
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const size_t j = (size_t)i * B + k;
+      if (flags & (F_UPD | F_XONLY)) x[j] = x[j] + alpha * pt[j] + omega * stv[j];  // BiCGSTAB.f90:153
+      if (flags & F_XONLY) continue;
+      T rv;
+      if (flags & F_UPD) {
+        rv = fma(-omega, t[j], s[j]);  // R = -omega*T + S  (BiCGSTAB.f90:166)
+        r[j] = rv;
+      } else {
+        rv = r[j];
+      }
+      part[0] += (double)rv * (double)rv;
+      part[1] += (double)rv * (double)rt[j];
+    }
+  }
+  __device__ void finalize(const double (&tot)[2]) const {
+    if (flags & F_XONLY) return;
+    if (finalize_check(st, hist, it, tot[0])) return;
+    const double rho = tot[1];
+    if (rho == 0.0) st->breakdown |= 1;
+    st->beta = (rho / st->rho) * (st->alpha / st->omega);  // BiCGSTAB.f90:96
+    st->rho_prev = st->rho;
+    st->rho = rho;
+  }
+};
+
+// k_sell / k_rows take the op by value; these wrappers call begin() per thread.
+template <class Op>
+struct Begun : Op {};
+
+template <typename TV, typename TA, int B, class Op>
+__global__ void __launch_bounds__(kBlock) k_sell_b(SellView<TV> M, Op op0, double* partials, unsigned* ticket,
+                                                   int ntiles) {
+  const int tile = xcd_tile(ntiles);
+  if (tile < 0) return;
+  Op op = op0;
+  if (op.skip()) return;
+  op.begin();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int s = tile * kWpb + threadIdx.x / kWave;
+  const int row = s * kWave + lane;
+  const bool valid = (s < M.nslices) && (row < M.nrows);
+  int len = 0;
+  if (valid) len = M.rowptr[row + 1] - M.rowptr[row];
+  TA acc[B];
+#pragma unroll
+  for (int r = 0; r < B; ++r) acc[r] = (TA)0;
+  if (s < M.nslices) {
+    auto xg = [&](int c, TA(&xv)[B]) { op.gather(c, xv); };
+    sell_row<TV, TA, B>(M, s, lane, len, xg, acc);
+  }
+  constexpr int NV = Op::NV;
+  double part[NV > 0 ? NV : 1];
+#pragma unroll
+  for (int k = 0; k < (NV > 0 ? NV : 1); ++k) part[k] = 0.0;
+  op.epilogue(row, valid, acc, part);
+  if constexpr (NV > 0) {
+    block_sum<NV>(part);
+    double tot[NV];
+    if (reduce_last<NV>(part, partials, tile, ntiles, ticket, tot)) {
+      if (threadIdx.x == 0) op.finalize(tot);
+    }
+  }
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_rows_b(int nrows, Op op0, double* partials, unsigned* ticket,
+                                                   int ntiles) {
+  const int tile = xcd_tile(ntiles);
+  if (tile < 0) return;
+  Op op = op0;
+  if (op.skip()) return;
+  op.begin();
+  const int i = tile * kBlock + threadIdx.x;
+  constexpr int NV = Op::NV;
+  double part[NV > 0 ? NV : 1];
+#pragma unroll
+  for (int k = 0; k < (NV > 0 ? NV : 1); ++k) part[k] = 0.0;
+  if (i < nrows) op.row(i, part);
+  if constexpr (NV > 0) {
+    block_sum<NV>(part);
+    double tot[NV];
+    if (reduce_last<NV>(part, partials, tile, ntiles, ticket, tot)) {
+      if (threadIdx.x == 0) op.finalize(tot);
+    }
+  }
+}
+
+template <typename T, int B, class Op>
+int run_sell(const Matrix& M, const Op& op, double* partials, unsigned* ticket, hipStream_t s) {
+  const int ntiles = (M.nslices + kWpb - 1) / kWpb;
+  if (ntiles == 0) return 0;
+  k_sell_b<T, T, B, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(view<T>(M), op, partials, ticket, ntiles);
+  MH_HIP(hipGetLastError());
+  return 0;
+}
+
+template <class Op>
+int run_rows(int nrows, const Op& op, double* partials, unsigned* ticket, hipStream_t s) {
+  const int ntiles = (nrows + kBlock - 1) / kBlock;
+  if (ntiles == 0) return 0;
+  k_rows_b<Op><<<padded_grid(ntiles), kBlock, 0, s>>>(nrows, op, partials, ticket, ntiles);
+  MH_HIP(hipGetLastError());
+  return 0;
+}
+
+__global__ void k_state_init(KState* st, double tol, int max_iter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    KState z;
+    memset(&z, 0, sizeof(z));
+    z.tol = tol;
+    z.max_iter = max_iter;
+    *st = z;
+  }
+}
+
+// ---- workspace carving --------------------------------------------------------
+struct Carve {
+  unsigned char* base;
+  size_t off = 0;
+  template <typename U>
+  U* take(size_t count) {
+    off = (off + 255) / 256 * 256;
+    U* p = (U*)(base + off);
+    off += count * sizeof(U);
+    return p;
+  }
+};
+
+template <typename T>
+size_t cg_bytes(const Matrix& M, int nvec, int max_iter) {
+  const size_t vec = (size_t)M.np * M.b;
+  const size_t parts = partials_needed((long long)M.n) * 4 + partials_needed((long long)M.nslices * kWave) * 4;
+  return nvec * (vec * sizeof(T) + 256) + parts * sizeof(double) + sizeof(KState) + 256 +
+         ((size_t)max_iter + 2) * sizeof(double) + 4096;
+}
+
+// Polls `done` one chunk behind the enqueue front.
+struct Poller {
+  Context* c;
+  const KState* st;
+  std::deque<int> pending;
+  int next = 0;
+  int issue(hipStream_t s) {
+    const int slot = next;
+    next = (next + 1) % kFlagSlots;
+    c->h_flags[slot] = 0;
+    MH_HIP(hipMemcpyAsync(&c->h_flags[slot], &st->done, sizeof(int), hipMemcpyDeviceToHost, s));
+    MH_HIP(hipEventRecord(c->ev[slot], s));
+    pending.push_back(slot);
+    return 0;
+  }
+  // returns 1 when the oldest outstanding poll reports done
+  int check(size_t keep, int* done) {
+    *done = 0;
+    while (pending.size() > keep) {
+      const int slot = pending.front();
+      pending.pop_front();
+      MH_HIP(hipEventSynchronize(c->ev[slot]));
+      if (c->h_flags[slot]) {
+        *done = 1;
+        pending.clear();
+        return 0;
+      }
+    }
+    return 0;
+  }
+};
+
+template <typename T, int B, bool PREC>
+int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, SolveResult* out, hipStream_t s) {
+  Context* c = nullptr;
+  MH_TRY(context(&c));
+  const size_t vec = (size_t)M.np * B;
+  MH_TRY(W.ensure(cg_bytes<T>(M, 5, P.max_iter)));
+  Carve cv{W.raw.p};
+  T* r = cv.take<T>(vec);
+  T* z = cv.take<T>(vec);
+  T* q = cv.take<T>(vec);
+  T* p0 = cv.take<T>(vec);
+  T* p1 = cv.take<T>(vec);
+  const size_t nparts = partials_needed((long long)M.n) * 4 + partials_needed((long long)M.nslices * kWave) * 4;
+  double* parts = cv.take<double>(nparts);
+  KState* st = cv.take<KState>(1);
+  double* hist = cv.take<double>((size_t)P.max_iter + 2);
+  T* pbuf[2] = {p0, p1};
+  const T* D = nullptr;
+  if constexpr (PREC) D = (sizeof(T) == 8) ? (const T*)M.D64.p : (const T*)M.D32.p;
+  const int n = M.n;
+  const size_t m = (size_t)n * B;
+
+  k_state_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
+  MH_HIP(hipGetLastError());
+  if (P.init_x) MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+  MH_HIP(hipMemsetAsync(p0, 0, sizeof(T) * vec, s));
+  MH_HIP(hipMemsetAsync(p1, 0, sizeof(T) * vec, s));
+  if (vec > m) {  // halo part of z stays defined for gathers
+    MH_HIP(hipMemsetAsync(z + m, 0, sizeof(T) * (vec - m), s));
+  }
+  // R = B - A X; B2 = (R, R)   (CG.f90:61-63, mat_converge.f90:38)
+  {
+    OpResidSolve<T, B> op{x, b, r, nullptr, &st->B2, st};
+    MH_TRY((run_sell<T, B>(M, op, parts, &st->ticket[0], s)));
+  }
+  double B2 = 0.0;
+  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  out->hist.clear();
+  if (B2 == 0.0) {  // mat_converge.f90:40-48: X = 0, converged, no iteration
+    MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+    MH_HIP(hipStreamSynchronize(s));
+    out->zero_rhs = 1;
+    out->converged = 1;
+    out->iter = -1;
+    out->resid = 0.0;
+    return 0;
+  }
+  // first z = M^-1 r, rho_1
+  auto upd = [&](int it, int flags) -> int {
+    OpCgUpdate<T, B, PREC> op{x, r, z, pbuf[it & 1], q, D, n, st, P.keep_hist ? hist : nullptr, it, flags, (T)0};
+    return run_rows(n, op, parts, &st->ticket[1], s);
+  };
+  MH_TRY(upd(0, F_FIRST));
+  Poller poll{c, st};
+  int done = 0;
+  for (int it = 1; it <= P.max_iter && !done;) {
+    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    for (; it <= end; ++it) {
+      OpCgSpmv<T, B> sp{z, pbuf[(it + 1) & 1], pbuf[it & 1], q, st, (T)0};
+      MH_TRY((run_sell<T, B>(M, sp, parts, &st->ticket[2], s)));
+      if (it % kIterRR == 0) {
+        MH_TRY(upd(it, F_XONLY));
+        OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
+        MH_TRY((run_sell<T, B>(M, rs, parts, &st->ticket[3], s)));
+        MH_TRY(upd(it, F_CHECK));
+      } else {
+        MH_TRY(upd(it, F_UPD | F_CHECK));
+      }
+    }
+    MH_TRY(poll.issue(s));
+    MH_TRY(poll.check(1, &done));
+  }
+  MH_HIP(hipStreamSynchronize(s));
+  KState hs;
+  MH_HIP(hipMemcpy(&hs, st, sizeof(KState), hipMemcpyDeviceToHost));
+  out->iter = hs.iter;
+  out->resid = hs.resid;
+  out->converged = hs.converged;
+  out->breakdown = hs.breakdown;
+  if (P.keep_hist && hs.iter > 0) {
+    out->hist.resize((size_t)hs.iter + 1);
+    MH_HIP(hipMemcpy(out->hist.data(), hist, sizeof(double) * ((size_t)hs.iter + 1), hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+template <typename T, int B, bool PREC>
+int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, SolveResult* out,
+                  hipStream_t s) {
+  Context* c = nullptr;
+  MH_TRY(context(&c));
+  const size_t vec = (size_t)M.np * B;
+  MH_TRY(W.ensure(cg_bytes<T>(M, 8, P.max_iter)));
+  Carve cv{W.raw.p};
+  T* r = cv.take<T>(vec);
+  T* rt = cv.take<T>(vec);
+  T* p = cv.take<T>(vec);
+  T* pt = cv.take<T>(vec);
+  T* sv = cv.take<T>(vec);
+  T* stv = cv.take<T>(vec);
+  T* t = cv.take<T>(vec);
+  T* v = cv.take<T>(vec);
+  const size_t nparts = partials_needed((long long)M.n) * 4 + partials_needed((long long)M.nslices * kWave) * 4;
+  double* parts = cv.take<double>(nparts);
+  KState* st = cv.take<KState>(1);
+  double* hist = cv.take<double>((size_t)P.max_iter + 2);
+  const T* D = nullptr;
+  if constexpr (PREC) D = (sizeof(T) == 8) ? (const T*)M.D64.p : (const T*)M.D32.p;
+  const int n = M.n;
+  const size_t m = (size_t)n * B;
+
+  k_state_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
+  MH_HIP(hipGetLastError());
+  if (P.init_x) MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+  if (vec > m) {
+    MH_HIP(hipMemsetAsync(pt + m, 0, sizeof(T) * (vec - m), s));
+    MH_HIP(hipMemsetAsync(stv + m, 0, sizeof(T) * (vec - m), s));
+  }
+  MH_HIP(hipMemsetAsync(v, 0, sizeof(T) * vec, s));
+  // R = B - A X, RT = R, B2 = (R, R); rho_1 = (R, RT) = B2
+  {
+    OpResidSolve<T, B> op{x, b, r, rt, &st->B2, st};
+    MH_TRY((run_sell<T, B>(M, op, parts, &st->ticket[0], s)));
+  }
+  double B2 = 0.0;
+  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  out->hist.clear();
+  if (B2 == 0.0) {
+    MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+    MH_HIP(hipStreamSynchronize(s));
+    out->zero_rhs = 1;
+    out->converged = 1;
+    out->iter = -1;
+    out->resid = 0.0;
+    return 0;
+  }
+  // rho of iteration 1 is (R, RT) with RT == R: the same sum as B2.
+  MH_HIP(hipMemcpyAsync(&st->rho, &st->B2, sizeof(double), hipMemcpyDeviceToDevice, s));
+  Poller poll{c, st};
+  int done = 0;
+  double* hp = P.keep_hist ? hist : nullptr;
+  for (int it = 1; it <= P.max_iter && !done;) {
+    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    for (; it <= end; ++it) {
+      OpBiA<T, B, PREC> a{r, p, v, pt, D, n, st, it == 1, (T)0, (T)0};
+      MH_TRY(run_rows(n, a, parts, &st->ticket[1], s));
+      OpBiB<T, B> bb{pt, v, rt, st};
+      MH_TRY((run_sell<T, B>(M, bb, parts, &st->ticket[2], s)));
+      OpBiC<T, B, PREC> cc{r, v, sv, stv, D, n, st, (T)0};
+      MH_TRY(run_rows(n, cc, parts, &st->ticket[1], s));
+      OpBiD<T, B> dd{stv, t, sv, st};
+      MH_TRY((run_sell<T, B>(M, dd, parts, &st->ticket[2], s)));
+      if (it % kIterRR == 0) {
+        OpBiE<T, B> e1{x, r, pt, stv, sv, t, rt, st, hp, it, F_XONLY, (T)0, (T)0};
+        MH_TRY(run_rows(n, e1, parts, &st->ticket[1], s));
+        OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
+        MH_TRY((run_sell<T, B>(M, rs, parts, &st->ticket[3], s)));
+        OpBiE<T, B> e2{x, r, pt, stv, sv, t, rt, st, hp, it, F_CHECK, (T)0, (T)0};
+        MH_TRY(run_rows(n, e2, parts, &st->ticket[1], s));
+      } else {
+        OpBiE<T, B> e{x, r, pt, stv, sv, t, rt, st, hp, it, F_UPD | F_CHECK, (T)0, (T)0};
+        MH_TRY(run_rows(n, e, parts, &st->ticket[1], s));
+      }
+    }
+    MH_TRY(poll.issue(s));
+    MH_TRY(poll.check(1, &done));
+  }
+  MH_HIP(hipStreamSynchronize(s));
+  KState hs;
+  MH_HIP(hipMemcpy(&hs, st, sizeof(KState), hipMemcpyDeviceToHost));
+  out->iter = hs.iter;
+  out->resid = hs.resid;
+  out->converged = hs.converged;
+  out->breakdown = hs.breakdown;
+  if (P.keep_hist && hs.iter > 0) {
+    out->hist.resize((size_t)hs.iter + 1);
+    MH_HIP(hipMemcpy(out->hist.data(), hist, sizeof(double) * ((size_t)hs.iter + 1), hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+#define MH_DISPATCH_B(b, ...)                                            \
+  switch (b) {                                                           \
+    case 1: { constexpr int B = 1; __VA_ARGS__; } break;                  \
+    case 2: { constexpr int B = 2; __VA_ARGS__; } break;                  \
+    case 3: { constexpr int B = 3; __VA_ARGS__; } break;                  \
+    case 4: { constexpr int B = 4; __VA_ARGS__; } break;                  \
+    case 5: { constexpr int B = 5; __VA_ARGS__; } break;                  \
+    case 6: { constexpr int B = 6; __VA_ARGS__; } break;                  \
+    case 7: { constexpr int B = 7; __VA_ARGS__; } break;                  \
+    case 8: { constexpr int B = 8; __VA_ARGS__; } break;                  \
+    default: set_error("unsupported block size %d", b); return -2;       \
+  }
+
+}  // namespace
+
+template <typename T>
+int krylov_solve(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, SolveResult* out,
+                 hipStream_t s) {
+  if (P.precond != 0 && P.precond != 1) {
+    set_error("preconditioner %d is not implemented on the GPU path (NONE=0, DIAG=1)", P.precond);
+    return -3;
+  }
+  if (P.method != 1 && P.method != 5) {
+    set_error("method %d is not implemented on the GPU path (CG=1, BiCGSTAB=5)", P.method);
+    return -4;
+  }
+  if (P.max_iter < 0) {
+    set_error("max_iter must be >= 0");
+    return -2;
+  }
+  if (sizeof(T) == 4) MH_TRY(matrix_make_fp32(&M, s));
+  if (P.precond == 1) MH_TRY(diag_setup<T>(M, s));
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = 0;
+  if (P.method == 1) {
+    if (P.precond == 1) {
+      MH_DISPATCH_B(M.b, rc = (cg_impl<T, B, true>(M, P, b, x, W, out, s)));
+    } else {
+      MH_DISPATCH_B(M.b, rc = (cg_impl<T, B, false>(M, P, b, x, W, out, s)));
+    }
+  } else {
+    if (P.precond == 1) {
+      MH_DISPATCH_B(M.b, rc = (bicgstab_impl<T, B, true>(M, P, b, x, W, out, s)));
+    } else {
+      MH_DISPATCH_B(M.b, rc = (bicgstab_impl<T, B, false>(M, P, b, x, W, out, s)));
+    }
+  }
+  out->t_solve = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+template int krylov_solve<double>(Matrix&, const SolveParams&, const double*, double*, Workspace&, SolveResult*,
+                                  hipStream_t);
+template int krylov_solve<float>(Matrix&, const SolveParams&, const float*, float*, Workspace&, SolveResult*,
+                                 hipStream_t);
+
+// ---------------------------------------------------------------------------
+// FP32-inner / FP64-outer iterative refinement.
+// ---------------------------------------------------------------------------
+__global__ void k_scale_to_fp32(size_t m, const double* __restrict__ r, double s, float* __restrict__ r32) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
+    r32[i] = (float)(r[i] * s);
+}
+__global__ void k_add_scaled(size_t m, double* __restrict__ x, const float* __restrict__ d, double s) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = fma(s, (double)d[i], x[i]);
+}
+
+static int grid_1d(size_t m) {
+  size_t g = (m + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  return (int)(g > 0 ? g : 1);
+}
+
+int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, Workspace& W, RefineResult* out,
+                 hipStream_t s) {
+  const size_t vec = (size_t)M.np * M.b;
+  const size_t m = (size_t)M.n * M.b;
+  // outer buffers live in their own allocation; W serves the inner solve
+  DevArray<double> r64, scal;
+  DevArray<float> r32, d32;
+  MH_TRY(r64.alloc(vec));
+  MH_TRY(r32.alloc(vec));
+  MH_TRY(d32.alloc(vec));
+  const size_t nparts = partials_needed((long long)M.nslices * kWave) * 2;
+  MH_TRY(scal.alloc(nparts + 64));
+  double* parts = scal.p;
+  double* rr_dev = scal.p + nparts;
+  unsigned* ticket = (unsigned*)(scal.p + nparts + 8);
+  MH_HIP(hipMemsetAsync(ticket, 0, 64, s));
+  MH_TRY(matrix_make_fp32(&M, s));
+  if (P.precond == 1) {
+    MH_TRY(diag_setup<double>(M, s));
+    MH_TRY(diag_setup<float>(M, s));
+  }
+  if (P.init_x) MH_HIP(hipMemsetAsync(x, 0, sizeof(double) * m, s));
+  if (vec > m) {
+    MH_HIP(hipMemsetAsync(r32.p + m, 0, sizeof(float) * (vec - m), s));
+    MH_HIP(hipMemsetAsync(d32.p + m, 0, sizeof(float) * (vec - m), s));
+  }
+  MH_TRY(residual<double>(M, x, b, r64.p, parts, ticket, rr_dev, s));
+  double rr = 0.0;
+  MH_HIP(hipMemcpyAsync(&rr, rr_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  out->outer_resid.clear();
+  out->inner_iters.clear();
+  out->inner_total = 0;
+  out->outer = 0;
+  const double B2 = rr;
+  if (B2 == 0.0) {
+    MH_HIP(hipMemsetAsync(x, 0, sizeof(double) * m, s));
+    MH_HIP(hipStreamSynchronize(s));
+    out->converged = 1;
+    out->resid = 0.0;
+    return 0;
+  }
+  double resid = 1.0;
+  out->outer_resid.push_back(resid);
+  SolveParams ip;
+  ip.method = P.method;
+  ip.precond = P.precond;
+  ip.max_iter = P.inner_max_iter;
+  ip.init_x = 1;
+  ip.tol = P.inner_tol;
+  ip.keep_hist = 0;
+  for (int k = 1; k <= P.max_outer; ++k) {
+    const double nr = std::sqrt(rr);
+    k_scale_to_fp32<<<grid_1d(m), kBlock, 0, s>>>(m, r64.p, 1.0 / nr, r32.p);
+    MH_HIP(hipGetLastError());
+    SolveResult ir;
+    MH_TRY(krylov_solve<float>(M, ip, r32.p, d32.p, W, &ir, s));
+    out->inner_iters.push_back(ir.iter);
+    out->inner_total += ir.iter > 0 ? ir.iter : 0;
+    k_add_scaled<<<grid_1d(m), kBlock, 0, s>>>(m, x, d32.p, nr);
+    MH_HIP(hipGetLastError());
+    MH_TRY(residual<double>(M, x, b, r64.p, parts, ticket, rr_dev, s));
+    MH_HIP(hipMemcpyAsync(&rr, rr_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+    MH_HIP(hipStreamSynchronize(s));
+    resid = std::sqrt(rr / B2);
+    out->outer_resid.push_back(resid);
+    out->outer = k;
+    if (resid < P.tol) {
+      out->converged = 1;
+      break;
+    }
+    if (!(resid == resid)) break;  // NaN: stop
+  }
+  out->resid = resid;
+  return 0;
+}
+
+}  // namespace mh

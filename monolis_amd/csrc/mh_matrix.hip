@@ -1,0 +1,569 @@
+// Matrix ingress (monolis BCSR -> SELL-64), SpMV, BLAS-1 and block-Jacobi.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "mh_internal.hpp"
+#include "mh_kernels.hpp"
+
+namespace mh {
+
+// ---------------------------------------------------------------------------
+// Error state and context
+// ---------------------------------------------------------------------------
+static thread_local char g_err[1024] = {0};
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+const char* last_error() { return g_err; }
+void clear_error() { g_err[0] = 0; }
+
+static Context g_ctx;
+static std::mutex g_ctx_mu;
+
+int context(Context** out) {
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  if (!g_ctx.ok) {
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1) {
+      set_error("no HIP device available (%s)", e == hipSuccess ? "count 0" : hipGetErrorString(e));
+      return -1;
+    }
+    MH_HIP(hipGetDevice(&g_ctx.device));
+    MH_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    MH_HIP(hipHostMalloc((void**)&g_ctx.h_flags, sizeof(int) * kFlagSlots, hipHostMallocDefault));
+    for (int i = 0; i < kFlagSlots; ++i) MH_HIP(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
+    g_ctx.ok = true;
+  }
+  *out = &g_ctx;
+  return 0;
+}
+
+template <typename T>
+int DevArray<T>::alloc(size_t count) {
+  release();
+  if (count == 0) count = 1;
+  MH_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+  n = count;
+  return 0;
+}
+template <typename T>
+void DevArray<T>::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+template struct DevArray<int>;
+template struct DevArray<unsigned>;
+template struct DevArray<double>;
+template struct DevArray<float>;
+template struct DevArray<unsigned char>;
+
+int is_device_pointer(const void* p) {
+  if (!p) return 0;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // plain host memory reports an error: clear it
+    return 0;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+int upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return 0;
+  MH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, s));
+  return 0;
+}
+
+int Workspace::ensure(size_t need) {
+  if (need <= bytes) return 0;
+  MH_TRY(raw.alloc(need));
+  bytes = need;
+  return 0;
+}
+
+size_t partials_needed(long long n) {
+  // one partial slot-group per 256-row workgroup (or per 4 slices) + margin
+  return (size_t)((n + kBlock - 1) / kBlock + 8);
+}
+
+// ---------------------------------------------------------------------------
+// SELL-64 construction
+// ---------------------------------------------------------------------------
+__global__ void k_slice_width(int n, int nslices, const int* __restrict__ rowptr, int* __restrict__ width) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = t / kWave;
+  int len = 0;
+  if (t < n) len = rowptr[t + 1] - rowptr[t];
+  len = wave_max(len);
+  if ((threadIdx.x & (kWave - 1)) == 0 && s < nslices) width[s] = len;
+}
+
+// Structural check of caller arrays before any kernel indexes with them:
+// index[0] == 0, index non-decreasing within [0, nz], 1 <= item <= np.
+__global__ void k_validate(int np, int nz, const int* __restrict__ rowptr, const int* __restrict__ item,
+                           int* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const int a = rowptr[i], b = rowptr[i + 1];
+  if (i == 0 && a != 0) atomicOr(bad, 1);
+  if (i == np - 1 && b != nz) atomicOr(bad, 1);
+  if (a < 0 || b < a || b > nz) {
+    atomicOr(bad, 2);
+    return;
+  }
+  for (int j = a; j < b; ++j) {
+    const int c = item[j];
+    if (c < 1 || c > np) {
+      atomicOr(bad, 4);
+      return;
+    }
+  }
+}
+
+// One thread per (slice, lane); writes every slot of that lane, padding included.
+template <typename TV>
+__global__ void k_sell_fill(int n, int nslices, int b, const int* __restrict__ rowptr, const int* __restrict__ item,
+                            const double* __restrict__ A, const int* __restrict__ slice_ptr,
+                            int* __restrict__ col, TV* __restrict__ val, int fill_col) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nslices * kWave) return;
+  const int s = (int)(t / kWave);
+  const int lane = (int)(t % kWave);
+  const int row = (int)t;
+  const int k0 = slice_ptr[s];
+  const int w = slice_ptr[s + 1] - k0;
+  const int bb = b * b;
+  int j0 = 0, len = 0;
+  if (row < n) {
+    j0 = rowptr[row];
+    len = rowptr[row + 1] - j0;
+  }
+  for (int k = 0; k < w; ++k) {
+    const size_t slot = (size_t)(k0 + k);
+    if (k < len) {
+      if (fill_col) col[slot * kWave + lane] = item[j0 + k] - 1;
+      for (int e = 0; e < bb; ++e)
+        val[(slot * bb + e) * kWave + lane] = (TV)A[(size_t)(j0 + k) * bb + e];
+    } else {
+      if (fill_col) col[slot * kWave + lane] = row < n ? row : 0;
+      for (int e = 0; e < bb; ++e) val[(slot * bb + e) * kWave + lane] = (TV)0;
+    }
+  }
+}
+
+__global__ void k_to_fp32(size_t m, const double* __restrict__ a, float* __restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+static int grid_1d(size_t m) {
+  size_t g = (m + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  return (int)(g > 0 ? g : 1);
+}
+
+Matrix::~Matrix() { delete transpose; }
+
+template <typename TV>
+SellView<TV> view(const Matrix& M) {
+  SellView<TV> v;
+  v.nrows = M.n;
+  v.nslices = M.nslices;
+  v.slice_ptr = M.slice_ptr.p;
+  v.rowptr = M.rowptr.p;
+  v.col = M.col.p;
+  if constexpr (sizeof(TV) == 8)
+    v.val = (const TV*)M.val64.p;
+  else
+    v.val = (const TV*)M.val32.p;
+  return v;
+}
+template SellView<double> view<double>(const Matrix&);
+template SellView<float> view<float>(const Matrix&);
+
+static int fill_values(Matrix* M, const double* A, hipStream_t s, int fill_col) {
+  const size_t nvals = (size_t)M->nz * M->b * M->b;
+  const double* dA = A;
+  DevArray<double> tmp;
+  if (!is_device_pointer(A)) {
+    MH_TRY(tmp.alloc(nvals));
+    MH_TRY(upload(tmp.p, A, nvals * sizeof(double), s));
+    dA = tmp.p;
+  }
+  const long long threads = (long long)M->nslices * kWave;
+  k_sell_fill<double><<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+      M->n, M->nslices, M->b, M->rowptr.p, M->csr_item.p, dA, M->slice_ptr.p, M->col.p, M->val64.p, fill_col);
+  MH_HIP(hipGetLastError());
+  if (tmp.p) MH_HIP(hipStreamSynchronize(s));  // tmp is released on return
+  M->diag_ready = M->diag32_ready = false;
+  if (M->val32_ready) {
+    M->val32_ready = false;
+    MH_TRY(matrix_make_fp32(M, s));
+  }
+  return 0;
+}
+
+int matrix_create(Matrix** out, int n, int np, int nz, int b, const int* index, const int* item,
+                  const double* A, hipStream_t s) {
+  *out = nullptr;
+  if (n < 0 || np < n || nz < 0 || b < 1) {
+    set_error("invalid matrix sizes n=%d np=%d nz=%d ndof=%d", n, np, nz, b);
+    return -2;
+  }
+  if (b > kMaxB) {
+    set_error("block size %d is larger than the supported maximum %d", b, kMaxB);
+    return -2;
+  }
+  Matrix* M = new Matrix();
+  M->n = n;
+  M->np = np;
+  M->nz = nz;
+  M->b = b;
+  M->nslices = (n + kWave - 1) / kWave;
+  int rc = 0;
+  do {
+    if ((rc = M->rowptr.alloc((size_t)np + 1))) break;
+    if ((rc = upload(M->rowptr.p, index, sizeof(int) * ((size_t)np + 1), s))) break;
+    if ((rc = M->csr_item.alloc((size_t)nz))) break;
+    if ((rc = upload(M->csr_item.p, item, sizeof(int) * (size_t)nz, s))) break;
+    // slice widths -> slice_ptr (exclusive scan on the host: nslices ints)
+    DevArray<int> width;
+    if ((rc = width.alloc((size_t)M->nslices + 1))) break;
+    if (np > 0) {
+      int bad = 0;
+      if (hipMemsetAsync(width.p + M->nslices, 0, sizeof(int), s) != hipSuccess) {
+        set_error("memset failed");
+        rc = -1;
+        break;
+      }
+      k_validate<<<(np + kBlock - 1) / kBlock, kBlock, 0, s>>>(np, nz, M->rowptr.p, M->csr_item.p,
+                                                               width.p + M->nslices);
+      if (hipGetLastError() != hipSuccess ||
+          hipMemcpyAsync(&bad, width.p + M->nslices, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        set_error("CSR validation kernel failed");
+        rc = -1;
+        break;
+      }
+      if (bad) {
+        set_error("invalid BCSR arrays (%s%s%s)", (bad & 1) ? "index[0] != 0 or index[NP] != nz; " : "",
+                  (bad & 2) ? "index not non-decreasing within [0, nz]; " : "",
+                  (bad & 4) ? "item outside 1..NP" : "");
+        rc = -2;
+        break;
+      }
+    }
+    if (M->nslices > 0) {
+      const long long threads = (long long)M->nslices * kWave;
+      k_slice_width<<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(n, M->nslices, M->rowptr.p,
+                                                                                 width.p);
+      if (hipGetLastError() != hipSuccess) {
+        set_error("slice width kernel launch failed");
+        rc = -1;
+        break;
+      }
+    }
+    std::vector<int> hw((size_t)M->nslices + 1, 0), hp((size_t)M->nslices + 1, 0);
+    if (M->nslices > 0) {
+      if (hipMemcpyAsync(hw.data(), width.p, sizeof(int) * M->nslices, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        set_error("slice width copy failed");
+        rc = -1;
+        break;
+      }
+    }
+    size_t acc = 0;
+    for (int i = 0; i < M->nslices; ++i) {
+      hp[i] = (int)acc;
+      acc += (size_t)hw[i];
+    }
+    hp[M->nslices] = (int)acc;
+    if (acc * kWave * (size_t)b * b >= ((size_t)1 << 40)) {
+      set_error("matrix too large");
+      rc = -2;
+      break;
+    }
+    M->nslots = acc;
+    if ((rc = M->slice_ptr.alloc((size_t)M->nslices + 1))) break;
+    if ((rc = upload(M->slice_ptr.p, hp.data(), sizeof(int) * ((size_t)M->nslices + 1), s))) break;
+    if ((rc = M->col.alloc(acc * kWave))) break;
+    if ((rc = M->val64.alloc(acc * kWave * (size_t)b * b))) break;
+    if ((rc = fill_values(M, A, s, 1))) break;
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      set_error("matrix build failed");
+      rc = -1;
+      break;
+    }
+  } while (0);
+  if (rc) {
+    delete M;
+    return rc;
+  }
+  *out = M;
+  return 0;
+}
+
+int matrix_set_values(Matrix* M, const double* A, hipStream_t s) { return fill_values(M, A, s, 0); }
+
+int matrix_make_fp32(Matrix* M, hipStream_t s) {
+  if (M->val32_ready) return 0;
+  if (M->val32.n < M->val64.n) MH_TRY(M->val32.alloc(M->val64.n));
+  k_to_fp32<<<grid_1d(M->val64.n), kBlock, 0, s>>>(M->val64.n, M->val64.p, M->val32.p);
+  MH_HIP(hipGetLastError());
+  M->val32_ready = true;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// SpMV ops
+// ---------------------------------------------------------------------------
+template <typename T, int B>
+struct OpSpmv {
+  static constexpr int NV = 0;
+  const T* __restrict__ x;
+  T* __restrict__ y;
+  __device__ bool skip() const { return false; }
+  __device__ void gather(int c, T (&xv)[B]) const {
+#pragma unroll
+    for (int k = 0; k < B; ++k) xv[k] = x[(size_t)c * B + k];
+  }
+  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&)[1]) const {
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) y[(size_t)row * B + k] = acc[k];
+    }
+  }
+  __device__ void finalize(const double (&)[1]) const {}
+};
+
+template <typename T, int B>
+struct OpResid {
+  static constexpr int NV = 1;
+  const T* __restrict__ x;
+  const T* __restrict__ b;
+  T* __restrict__ r;
+  double* out;
+  __device__ bool skip() const { return false; }
+  __device__ void gather(int c, T (&xv)[B]) const {
+#pragma unroll
+    for (int k = 0; k < B; ++k) xv[k] = x[(size_t)c * B + k];
+  }
+  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[1]) const {
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        // monolis_residual_main_R: R = 1*B + (-1)*(A X)   (matvec.f90:901)
+        const T v = b[(size_t)row * B + k] - acc[k];
+        r[(size_t)row * B + k] = v;
+        part[0] += (double)v * (double)v;
+      }
+    }
+  }
+  __device__ void finalize(const double (&tot)[1]) const {
+    if (out) *out = tot[0];
+  }
+};
+
+template <typename TV, typename T, int B, class Op>
+static int launch_sell(const Matrix& M, const Op& op, double* partials, unsigned* ticket, hipStream_t s) {
+  const int ntiles = (M.nslices + kWpb - 1) / kWpb;
+  if (ntiles == 0) return 0;
+  k_sell<TV, T, B, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(view<TV>(M), op, partials, ticket, ntiles);
+  MH_HIP(hipGetLastError());
+  return 0;
+}
+
+#define MH_DISPATCH_B(b, ...)                                            \
+  switch (b) {                                                           \
+    case 1: { constexpr int B = 1; __VA_ARGS__; } break;                  \
+    case 2: { constexpr int B = 2; __VA_ARGS__; } break;                  \
+    case 3: { constexpr int B = 3; __VA_ARGS__; } break;                  \
+    case 4: { constexpr int B = 4; __VA_ARGS__; } break;                  \
+    case 5: { constexpr int B = 5; __VA_ARGS__; } break;                  \
+    case 6: { constexpr int B = 6; __VA_ARGS__; } break;                  \
+    case 7: { constexpr int B = 7; __VA_ARGS__; } break;                  \
+    case 8: { constexpr int B = 8; __VA_ARGS__; } break;                  \
+    default: set_error("unsupported block size %d", b); return -2;       \
+  }
+
+template <typename T>
+int spmv(const Matrix& M, const T* x, T* y, hipStream_t s) {
+  if (sizeof(T) == 4 && !M.val32_ready) {
+    set_error("fp32 values not built");
+    return -2;
+  }
+  MH_DISPATCH_B(M.b, {
+    OpSpmv<T, B> op{x, y};
+    MH_TRY((launch_sell<T, T, B>(M, op, nullptr, nullptr, s)));
+  });
+  return 0;
+}
+template int spmv<double>(const Matrix&, const double*, double*, hipStream_t);
+template int spmv<float>(const Matrix&, const float*, float*, hipStream_t);
+
+template <typename T>
+int residual(const Matrix& M, const T* x, const T* b, T* r, double* partials, unsigned* ticket,
+             double* rr_out, hipStream_t s) {
+  MH_DISPATCH_B(M.b, {
+    OpResid<T, B> op{x, b, r, rr_out};
+    MH_TRY((launch_sell<T, T, B>(M, op, partials, ticket, s)));
+  });
+  return 0;
+}
+template int residual<double>(const Matrix&, const double*, const double*, double*, double*, unsigned*,
+                              double*, hipStream_t);
+template int residual<float>(const Matrix&, const float*, const float*, float*, double*, unsigned*, double*,
+                             hipStream_t);
+
+// ---------------------------------------------------------------------------
+// BLAS-1: deterministic dot, AXPBY
+// ---------------------------------------------------------------------------
+template <typename T>
+struct OpDot {
+  static constexpr int NV = 1;
+  const T* __restrict__ x;
+  const T* __restrict__ y;
+  double* out;
+  __device__ bool skip() const { return false; }
+  __device__ void row(int i, double (&part)[1]) const { part[0] += (double)x[i] * (double)y[i]; }
+  __device__ void finalize(const double (&tot)[1]) const { *out = tot[0]; }
+};
+
+template <typename T>
+int dot(long long m, const T* x, const T* y, double* out, double* partials, unsigned* ticket, hipStream_t s) {
+  if (m > 0x7fffffffLL) {
+    set_error("vector too long");
+    return -2;
+  }
+  const int ntiles = (int)((m + kBlock - 1) / kBlock);
+  if (ntiles == 0) {
+    MH_HIP(hipMemsetAsync(out, 0, sizeof(double), s));
+    return 0;
+  }
+  OpDot<T> op{x, y, out};
+  k_rows<OpDot<T>><<<padded_grid(ntiles), kBlock, 0, s>>>((int)m, op, partials, ticket, ntiles);
+  MH_HIP(hipGetLastError());
+  return 0;
+}
+template int dot<double>(long long, const double*, const double*, double*, double*, unsigned*, hipStream_t);
+template int dot<float>(long long, const float*, const float*, double*, double*, unsigned*, hipStream_t);
+
+template <typename T>
+__global__ void k_axpby(long long m, T alpha, const T* __restrict__ x, T beta, const T* __restrict__ y,
+                        T* __restrict__ z) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+    z[i] = alpha * x[i] + beta * y[i];
+}
+
+template <typename T>
+int axpby(long long m, double alpha, const T* x, double beta, const T* y, T* z, hipStream_t s) {
+  if (m <= 0) return 0;
+  k_axpby<T><<<grid_1d((size_t)m), kBlock, 0, s>>>(m, (T)alpha, x, (T)beta, y, z);
+  MH_HIP(hipGetLastError());
+  return 0;
+}
+template int axpby<double>(long long, double, const double*, double, const double*, double*, hipStream_t);
+template int axpby<float>(long long, double, const float*, double, const float*, float*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Block-Jacobi setup: diagonal block extraction + no-pivot LU with reciprocal
+// pivots, in place in structure-of-arrays D (entry e of row i at D[e*n + i]).
+// b == 3 follows diag_33.f90:40-96 (every row is factored, a missing diagonal
+// block becomes the identity); other b follow diag_nn.f90:45-80 (only rows
+// that hold a diagonal block are factored, the others stay zero).
+// ---------------------------------------------------------------------------
+__global__ void k_diag_setup(int n, int b, const int* __restrict__ slice_ptr, const int* __restrict__ rowptr,
+                             const int* __restrict__ col, const double* __restrict__ val, double* __restrict__ D) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = i / kWave, lane = i % kWave;
+  const int bb = b * b;
+  const size_t N = (size_t)n;
+  const int k0 = slice_ptr[s];
+  const int len = rowptr[i + 1] - rowptr[i];
+  int found = 0;
+  for (int e = 0; e < bb; ++e) D[(size_t)e * N + i] = 0.0;
+  for (int k = 0; k < len; ++k) {
+    const size_t slot = (size_t)(k0 + k);
+    if (col[slot * kWave + lane] == i) {
+      found = 1;
+      for (int e = 0; e < bb; ++e) D[(size_t)e * N + i] = val[(slot * bb + e) * kWave + lane];
+    }
+  }
+  if (!found && b != 3) return;
+#define LU(r, c) D[(size_t)((r) * b + (c)) * N + i]
+  for (int k = 0; k < b; ++k) {
+    if (LU(k, k) == 0.0) LU(k, k) = 1.0;
+    LU(k, k) = 1.0 / LU(k, k);
+    for (int l = k + 1; l < b; ++l) {
+      LU(l, k) = LU(l, k) * LU(k, k);
+      for (int j = k + 1; j < b; ++j) LU(l, j) = LU(l, j) - LU(l, k) * LU(k, j);
+    }
+  }
+#undef LU
+}
+
+template <typename T>
+int diag_setup(Matrix& M, hipStream_t s) {
+  if (!M.diag_ready) {
+    const size_t need = (size_t)M.n * M.b * M.b;
+    if (M.D64.n < need) MH_TRY(M.D64.alloc(need));
+    if (M.n > 0) {
+      k_diag_setup<<<(M.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(M.n, M.b, M.slice_ptr.p, M.rowptr.p, M.col.p,
+                                                                   M.val64.p, M.D64.p);
+      MH_HIP(hipGetLastError());
+    }
+    M.diag_ready = true;
+    M.diag32_ready = false;
+  }
+  if (sizeof(T) == 4 && !M.diag32_ready) {
+    const size_t need = (size_t)M.n * M.b * M.b;
+    if (M.D32.n < need) MH_TRY(M.D32.alloc(need));
+    if (need > 0) {
+      k_to_fp32<<<grid_1d(need), kBlock, 0, s>>>(need, M.D64.p, M.D32.p);
+      MH_HIP(hipGetLastError());
+    }
+    M.diag32_ready = true;
+  }
+  return 0;
+}
+template int diag_setup<double>(Matrix&, hipStream_t);
+template int diag_setup<float>(Matrix&, hipStream_t);
+
+template <typename T, typename TD, int B>
+__global__ void k_diag_apply(int n, const TD* __restrict__ D, const T* __restrict__ x, T* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T v[B];
+#pragma unroll
+  for (int k = 0; k < B; ++k) v[k] = x[(size_t)i * B + k];
+  diag_apply<T, TD, B>(D, n, i, v);
+#pragma unroll
+  for (int k = 0; k < B; ++k) y[(size_t)i * B + k] = v[k];
+}
+
+template <typename T>
+int diag_apply(const Matrix& M, const T* x, T* y, hipStream_t s) {
+  if (M.n == 0) return 0;
+  const T* D = nullptr;
+  if constexpr (sizeof(T) == 8) D = (const T*)M.D64.p;
+  else D = (const T*)M.D32.p;
+  MH_DISPATCH_B(M.b, {
+    k_diag_apply<T, T, B><<<(M.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(M.n, D, x, y);
+    MH_HIP(hipGetLastError());
+  });
+  return 0;
+}
+template int diag_apply<double>(const Matrix&, const double*, double*, hipStream_t);
+template int diag_apply<float>(const Matrix&, const float*, float*, hipStream_t);
+
+}  // namespace mh

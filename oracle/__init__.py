@@ -1,0 +1,178 @@
+"""CPU oracle for the monolis Krylov path -- TEST INFRASTRUCTURE ONLY.
+
+This package restates monolis's FP64 CPU algorithm (SpMV, dot, AXPBY,
+convergence test, diagonal / block-Jacobi preconditioner, CG, BiCGSTAB) in
+plain C (``monolis_oracle.c``) and exposes it through ctypes.  Only the test
+suite, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
+may import it, and only to check results or to time the CPU baseline.  The
+HIP product path never imports or calls anything here.
+
+Parity status of each function and what pins it is listed in the C file
+header.  Nothing here is "parity unpinned": every function is checked by a
+reference golden vector / closed form or by a textbook identity in
+``tests/test_oracle_*.py``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+# Iarray / Rarray slots (C index k == Fortran index k), src/define/def_solver_prm.f90:90-163
+I_METHOD, I_PRECOND, I_MAX_ITER, I_CUR_ITER, I_IERR = 1, 2, 3, 4, 5
+I_IS_INIT_X, I_IS_ERROR_ABORT = 10, 16
+R_TOL, R_CUR_RESID = 1, 2
+ITER_CG, ITER_BICGSTAB = 1, 5
+PREC_NONE, PREC_DIAG = 0, 1
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c_int, c_double = ctypes.c_int, ctypes.c_double
+        L.orc_set_threads.argtypes = [c_int]
+        L.orc_get_threads.restype = c_int
+        L.orc_matvec.argtypes = [c_int, c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.orc_inner_product.argtypes = [c_int, _f64p, _f64p]
+        L.orc_inner_product.restype = c_double
+        L.orc_sorted_inner_product.argtypes = [c_int, _f64p, _f64p]
+        L.orc_sorted_inner_product.restype = c_double
+        L.orc_vec_axpby.argtypes = [c_int, c_double, _f64p, c_double, _f64p, _f64p]
+        L.orc_vec_copy.argtypes = [c_int, _f64p, _f64p]
+        L.orc_check_converge.argtypes = [c_int, _f64p, c_double, c_int, _i32p, _f64p]
+        L.orc_check_converge.restype = c_int
+        L.orc_diag_nn_setup.argtypes = [c_int, c_int, _i32p, _i32p, _f64p, _f64p]
+        L.orc_diag_33_setup.argtypes = [c_int, _i32p, _i32p, _f64p, _f64p]
+        L.orc_diag_nn_apply.argtypes = [c_int, c_int, _f64p, _f64p, _f64p]
+        L.orc_diag_33_apply.argtypes = [c_int, _f64p, _f64p, _f64p]
+        L.orc_diag_V_setup.argtypes = [c_int, _i32p, _i32p, _i32p, _i32p, _f64p, _i64p, _f64p]
+        L.orc_diag_V_apply.argtypes = [c_int, _i32p, _i32p, _i64p, _f64p, _f64p, _f64p]
+        L.orc_solve.argtypes = [c_int, c_int, c_int, c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p, _f64p, _f64p, _f64p, _i32p, _i32p, _i32p, _f64p]
+        L.orc_solve.restype = c_int
+        L.orc_time_matvec.argtypes = [c_int, c_int, c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.orc_time_matvec.restype = c_double
+        _lib = L
+    return _lib
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def matvec(mat, x):
+    """y = A x over the N internal block rows (src/linalg/matvec.f90:123-188)."""
+    ndof = mat.ndof
+    m = mat.n_dof_index[mat.n] if ndof == -1 else mat.n * ndof
+    y = np.zeros(max(m, 1), dtype=np.float64)
+    lib().orc_matvec(mat.n, ndof, _ptr(mat.n_dof_list), _ptr(mat.n_dof_index), _ptr(mat.n_dof_index2),
+                     mat.index, mat.item, mat.A, np.ascontiguousarray(x, dtype=np.float64), y)
+    return y[:m]
+
+
+def inner_product(x, y):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    return float(lib().orc_inner_product(len(x), x, y))
+
+
+def sorted_inner_product(x, y):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    return float(lib().orc_sorted_inner_product(len(x), x, y))
+
+
+def vec_axpby(alpha, x, beta, y):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    z = np.empty_like(x)
+    lib().orc_vec_axpby(len(x), float(alpha), x, float(beta), y, z)
+    return z
+
+
+def check_converge(r, B2, it, tol):
+    Iarray = np.zeros(100, np.int32)
+    Rarray = np.zeros(100, np.float64)
+    Rarray[R_TOL] = tol
+    r = np.ascontiguousarray(r, np.float64)
+    conv = lib().orc_check_converge(len(r), r, float(B2), int(it), Iarray, Rarray)
+    return bool(conv), int(Iarray[I_CUR_ITER]), float(Rarray[R_CUR_RESID])
+
+
+def diag_setup(mat):
+    """Factored diagonal blocks, the layout of PREC%R%D (diag_33.f90 / diag_nn.f90)."""
+    n, nd = mat.n, mat.ndof
+    D = np.zeros(max(nd * nd * n, 1), np.float64)
+    if nd == 3:
+        lib().orc_diag_33_setup(n, mat.index, mat.item, mat.A, D)
+    else:
+        lib().orc_diag_nn_setup(n, nd, mat.index, mat.item, mat.A, D)
+    return D
+
+
+def diag_apply(mat, D, x):
+    n, nd = mat.n, mat.ndof
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros_like(x)
+    if nd == 3:
+        lib().orc_diag_33_apply(n, D, x, y)
+    else:
+        lib().orc_diag_nn_apply(n, nd, D, x, y)
+    return y
+
+
+def default_arrays(method=ITER_CG, precond=PREC_DIAG, tol=1e-8, max_iter=1000):
+    """Iarray / Rarray as monolis_prm_initialize sets them (def_solver_prm.f90:183-222)."""
+    Iarray = np.zeros(100, np.int32)
+    Rarray = np.zeros(100, np.float64)
+    Iarray[1], Iarray[2], Iarray[3], Iarray[4], Iarray[5] = method, precond, max_iter, 0, -1
+    Iarray[10], Iarray[16] = 1, 1
+    Rarray[1] = tol
+    return Iarray, Rarray
+
+
+def solve(mat, b, x0=None, Iarray=None, Rarray=None, **kw):
+    """monolis_solve_main_R on one rank.  Returns (x, status, Iarray, Rarray)."""
+    if Iarray is None:
+        Iarray, Rarray = default_arrays(**kw)
+    ndof = mat.ndof
+    m = mat.n_dof_index[mat.n] if ndof == -1 else mat.n * ndof
+    x = np.zeros(max(m, 1), np.float64) if x0 is None else np.array(x0, np.float64, copy=True)
+    b = np.ascontiguousarray(b, np.float64)
+    st = lib().orc_solve(mat.n, mat.n, len(mat.item), ndof, _ptr(mat.n_dof_list), _ptr(mat.n_dof_index),
+                         _ptr(mat.n_dof_index2), mat.A, x, b, mat.index, mat.item, Iarray, Rarray)
+    return x[:m], int(st), Iarray, Rarray
+
+
+def time_matvec(mat, x, reps):
+    y = np.zeros(mat.n * mat.ndof, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    return float(lib().orc_time_matvec(int(reps), mat.n, mat.ndof, mat.index, mat.item, mat.A, x, y))

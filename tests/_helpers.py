@@ -1,0 +1,89 @@
+"""Shared test helpers: build reference test matrices from the golden fixtures."""
+import json
+import os
+
+import numpy as np
+
+from monolis_amd import matrix as mm
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "monolis_kat.json")
+
+
+def kat():
+    with open(GOLDEN) as f:
+        return json.load(f)
+
+
+def from_mesh_case(case):
+    m = mm.nonzero_pattern_by_simple_mesh(case["n_node"], case["ndof"], case["elems"])
+    for i, j, si, sj, v in case.get("add", []):
+        mm.add_scalar(m, i, j, si, sj, v)
+    for i, j, si, sj, v in case.get("set", []):
+        mm.set_scalar(m, i, j, si, sj, v)
+    return m
+
+
+def mtx_matrix(case):
+    """Symmetric matrix of mtx.dat as main.f90:176-190 assembles it."""
+    n = case["n_node"]
+    elems = [[i, j] for i, j, _ in case["entries"] if i != j]
+    m = mm.nonzero_pattern_by_simple_mesh(n, 1, elems)
+    for i, j, v in case["entries"]:
+        mm.add_scalar(m, i, j, 0, 0, v)
+        if i != j:
+            mm.add_scalar(m, j, i, 0, 0, v)
+    return m
+
+
+def chain_matrix(n_node, ndof, seed):
+    """Shape of src_test/iterative/CG_test.f90:40-70: 10-node chain, diagonal
+    U[0,1)+2 on the block diagonals, symmetric U[0,1) off-diagonal blocks."""
+    rng = np.random.default_rng(seed)
+    elems = [[i, i + 1] for i in range(n_node - 1)]
+    m = mm.nonzero_pattern_by_simple_mesh(n_node, ndof, elems)
+    for i in range(n_node):
+        for d in range(ndof):
+            mm.add_scalar(m, i, i, d, d, rng.random() + 2.0)
+    for a, b in elems:
+        for i2 in range(ndof):
+            for j2 in range(ndof):
+                v = rng.random()
+                mm.add_scalar(m, a, b, i2, j2, v)
+                mm.add_scalar(m, b, a, j2, i2, v)
+    return m
+
+
+def random_bcsr(n, ndof, density, seed, spd=False, diag_missing=()):
+    """Random block pattern (diagonal included unless listed) with random values."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    for i in range(n):
+        cols = set(np.nonzero(rng.random(n) < density)[0].tolist())
+        if i not in diag_missing:
+            cols.add(i)
+        else:
+            cols.discard(i)
+        if spd:
+            pass
+        rows.append(sorted(cols))
+    if spd:  # symmetrise the pattern
+        sets = [set(r) for r in rows]
+        for i, r in enumerate(rows):
+            for c in r:
+                sets[c].add(i)
+        rows = [sorted(s) for s in sets]
+    index = np.zeros(n + 1, np.int32)
+    index[1:] = np.cumsum([len(r) for r in rows])
+    item = np.array([c + 1 for r in rows for c in r], np.int32)
+    b2 = ndof * ndof
+    A = rng.uniform(-1, 1, size=len(item) * b2)
+    m = mm.Matrix(n, n, ndof, index, item, A).ensure_dof_index()
+    if spd:
+        D = m.dense()
+        D = 0.5 * (D + D.T)
+        D += np.eye(len(D)) * (np.abs(D).sum(1).max() + 1.0)
+        for i in range(n):
+            for k in range(index[i], index[i + 1]):
+                c = item[k] - 1
+                m.A[k * b2:(k + 1) * b2] = D[i * ndof:(i + 1) * ndof, c * ndof:(c + 1) * ndof].reshape(-1)
+    return m

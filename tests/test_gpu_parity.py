@@ -1,0 +1,288 @@
+"""GPU parity of the HIP path against the CPU oracle and monolis's golden vectors.
+
+Every test calls libmonolis_hip.so through its C-ABI (ctypes) and compares with
+the oracle on the same seeded inputs.  Bars (stated per test):
+  SpMV: max |y_gpu - y_orc| <= 1e-12 * max |y_orc| (and exact for the integer KATs);
+  dot:  relative difference <= 1e-12;
+  solves: same convergence status, iteration counts within +-2 % (+-1 below 50),
+          relative residual below the tolerance, ||x_gpu - x_orc|| / ||x_orc|| <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from monolis_amd import api, workloads as W
+
+from _helpers import chain_matrix, from_mesh_case, kat, mtx_matrix, random_bcsr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init(hip):
+    oracle.set_threads(4)
+    yield
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _iters_close(a, b):
+    return abs(a - b) <= max(1, int(0.02 * b))
+
+
+# --- SpMV -------------------------------------------------------------------------
+@pytest.mark.parametrize("case", kat()["matvec"], ids=lambda c: c["name"])
+def test_matvec_reference_kat(case):
+    m = from_mesh_case(case)
+    y = api.monolis_matvec_product_R(api.Monolis(m), np.array(case["x"]))
+    assert np.array_equal(y[:m.nndof], np.array(case["y"]))
+
+
+@pytest.mark.parametrize("ndof", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_matvec_random_pattern(ndof):
+    m = random_bcsr(203, ndof, 0.05, seed=100 + ndof, diag_missing=(5, 77))
+    x = np.random.default_rng(ndof).uniform(-1, 1, m.npndof)
+    y = api.monolis_matvec_product_R(api.Monolis(m), x)[:m.nndof]
+    assert _rel(y, oracle.matvec(m, x)) <= 1e-12
+
+
+def test_matvec_empty_rows_and_ragged_slices():
+    # rows of length 0, 1 and long rows in one slice; n not a multiple of 64
+    m = random_bcsr(130, 3, 0.0, seed=3, diag_missing=tuple(range(0, 130, 7)))
+    rng = np.random.default_rng(0)
+    rows = [sorted(set(rng.integers(0, 130, size=int(rng.integers(0, 40))).tolist())) for _ in range(130)]
+    m.index = np.zeros(131, np.int32)
+    m.index[1:] = np.cumsum([len(r) for r in rows])
+    m.item = np.array([c + 1 for r in rows for c in r], np.int32)
+    m.A = rng.uniform(-1, 1, len(m.item) * 9)
+    m.n_dof_list = None
+    m.ensure_dof_index()
+    x = rng.uniform(-1, 1, 390)
+    y = api.monolis_matvec_product_R(api.Monolis(m), x)[:390]
+    assert _rel(y, oracle.matvec(m, x)) <= 1e-12
+
+
+def test_matvec_stencil27_device_resident():
+    m = W.stencil27_bcsr3(24, seed=5)
+    x = np.random.default_rng(1).uniform(-1, 1, m.nndof)
+    dm = api.DeviceMatrix(m)
+    dx = api.DeviceBuffer.from_numpy(x)
+    dy = api.DeviceBuffer(m.nndof)
+    dm.matvec(dx, dy)
+    assert _rel(dy.numpy(), oracle.matvec(m, x)) <= 1e-12
+    # FP32 layout: values and vectors rounded to float, same product within FP32 error
+    dx32 = api.DeviceBuffer.from_numpy(x.astype(np.float32))
+    dy32 = api.DeviceBuffer(m.nndof, np.float32)
+    dm.matvec(dx32, dy32)
+    ref = oracle.matvec(m, x)
+    assert _rel(dy32.numpy().astype(np.float64), ref) <= 1e-5
+
+
+def test_matvec_values_update():
+    m = W.stencil27_bcsr3(10, seed=1)
+    dm = api.DeviceMatrix(m)
+    x = np.random.default_rng(2).uniform(-1, 1, m.nndof)
+    dx = api.DeviceBuffer.from_numpy(x)
+    dy = api.DeviceBuffer(m.nndof)
+    m2 = W.stencil27_bcsr3(10, seed=9)
+    dm.set_values(m2.A)
+    dm.matvec(dx, dy)
+    assert _rel(dy.numpy(), oracle.matvec(m2, x)) <= 1e-12
+
+
+# --- BLAS-1 ---------------------------------------------------------------------------
+def test_inner_product_kat_and_random():
+    k = kat()["blas"]["inner_product"]
+    mon = api.Monolis(chain_matrix(2, 2, 0))
+    assert api.monolis_inner_product_R(mon, 2, np.array(k["x"]), np.array(k["y"])) == k["sum"]
+    rng = np.random.default_rng(3)
+    x, y = rng.uniform(-1, 1, 1_000_003), rng.uniform(-1, 1, 1_000_003)
+    s = api.inner_product_device(api.DeviceBuffer.from_numpy(x), api.DeviceBuffer.from_numpy(y))
+    ref = oracle.inner_product(x, y)
+    assert abs(s - ref) <= 1e-12 * abs(ref) + 1e-12 * np.sqrt(len(x))
+    # deterministic: same bits on a second call
+    s2 = api.inner_product_device(api.DeviceBuffer.from_numpy(x), api.DeviceBuffer.from_numpy(y))
+    assert s == s2
+
+
+# --- solves through the flat ABI -------------------------------------------------------
+def _solve_flat(m, b, method=1, precond=1, tol=1e-8, max_iter=1000, x0=None, init_x=True, abort=False):
+    mon = api.Monolis(m, api.Prm().quiet())
+    api.monolis_set_method(mon, method)
+    api.monolis_set_precond(mon, precond)
+    api.monolis_set_tolerance(mon, tol)
+    api.monolis_set_maxiter(mon, max_iter)
+    api.monolis_set_init_x(mon, init_x)
+    api.monolis_set_error_abort(mon, abort)
+    x = np.zeros(m.npndof) if x0 is None else np.array(x0, np.float64)
+    api.monolis_solve_R(mon, b, x)
+    return x, mon
+
+
+@pytest.mark.parametrize("method", [1, 5])
+@pytest.mark.parametrize("precond", [0, 1])
+def test_solve_mtx_dat(method, precond):
+    case = kat()["mtx"]
+    m = mtx_matrix(case)
+    b = oracle.matvec(m, np.ones(10))
+    x, mon = _solve_flat(m, b, method, precond, tol=case["tol"])
+    assert api.monolis_get_error_tag(mon) == 0
+    assert api.monolis_get_converge_iter(mon) >= 1
+    assert api.monolis_get_converge_residual(mon) <= 1e-10
+    assert np.allclose(x, 1.0, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("ndof", [1, 2, 3])
+@pytest.mark.parametrize("precond", [0, 1])
+@pytest.mark.parametrize("method", [1, 5])
+def test_solve_chain_matches_oracle(ndof, precond, method):
+    m = chain_matrix(10, ndof, seed=ndof)
+    b = oracle.matvec(m, np.ones(10 * ndof))
+    x, mon = _solve_flat(m, b, method, precond, tol=1e-10)
+    xo, st, I, R = oracle.solve(m, b, method=method, precond=precond, tol=1e-10)
+    assert api.monolis_get_error_tag(mon) == 0 and st == 0
+    assert _iters_close(api.monolis_get_converge_iter(mon), int(I[4]))
+    assert np.allclose(x, 1.0, rtol=0, atol=1e-8)
+
+
+def test_solve_c_reference_case():
+    case = kat()["solve_c"]
+    m = from_mesh_case(case)
+    b = oracle.matvec(m, np.ones(20))
+    x, mon = _solve_flat(m, b, tol=case["tol"])
+    assert np.allclose(x, case["x_expected"], rtol=0, atol=1e-8)
+
+
+def test_solve_poisson32_cg_diag_matches_oracle():
+    """The CPU-runnable workload: 7-point Poisson 32^3, CG + DIAG, tol 1e-8."""
+    m = W.poisson7(32)
+    b = W.uniform_vector(m.nndof, seed=0)
+    x, mon = _solve_flat(m, b)
+    xo, st, I, R = oracle.solve(m, b)
+    it_gpu, it_orc = api.monolis_get_converge_iter(mon), int(I[4])
+    assert st == 0 and api.monolis_get_error_tag(mon) == 0
+    assert _iters_close(it_gpu, it_orc), (it_gpu, it_orc)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+    true = np.linalg.norm(b - m.to_scipy() @ x) / np.linalg.norm(b)
+    assert true < 1e-8 * 1.01
+
+
+def test_solve_residual_replacement_beyond_200():
+    m = W.poisson7(64)
+    b = W.uniform_vector(m.nndof, seed=0)
+    x, mon = _solve_flat(m, b, precond=0, tol=1e-10, max_iter=2000)
+    xo, st, I, R = oracle.solve(m, b, precond=0, tol=1e-10, max_iter=2000)
+    it_gpu = api.monolis_get_converge_iter(mon)
+    assert it_gpu > 200 and _iters_close(it_gpu, int(I[4])), (it_gpu, int(I[4]))
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_bicgstab_convdiff_matches_oracle(precond):
+    m = W.convdiff7(16, 0.5)
+    xt = W.uniform_vector(m.nndof, seed=1)
+    b = oracle.matvec(m, xt)
+    x, mon = _solve_flat(m, b, method=5, precond=precond)
+    xo, st, I, R = oracle.solve(m, b, method=5, precond=precond)
+    assert st == 0 and api.monolis_get_error_tag(mon) == 0
+    assert _iters_close(api.monolis_get_converge_iter(mon), int(I[4]))
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+
+
+def test_block_jacobi_bcsr3_matches_oracle():
+    m = W.stencil27_bcsr3(16, seed=3)
+    xt = W.uniform_vector(m.nndof, seed=4)
+    b = oracle.matvec(m, xt)
+    x, mon = _solve_flat(m, b)
+    xo, st, I, R = oracle.solve(m, b)
+    assert _iters_close(api.monolis_get_converge_iter(mon), int(I[4]))
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+    assert np.linalg.norm(x - xt) / np.linalg.norm(xt) <= 1e-6
+
+
+def test_edge_cases_flat_abi():
+    m = chain_matrix(10, 2, seed=9)
+    # b = 0 -> x = 0, cur_iter untouched
+    mon = api.Monolis(m, api.Prm().quiet())
+    mon.prm.Iarray[4] = 77
+    x = np.full(20, 3.0)
+    api.monolis_solve_R(mon, np.zeros(20), x)
+    assert np.all(x == 0.0) and mon.prm.Iarray[4] == 77 and api.monolis_get_error_tag(mon) == 0
+    # warm start at the exact solution: ||r0|| = 0 also resets x (monolis rule)
+    b = oracle.matvec(m, np.ones(20))
+    x, mon = _solve_flat(m, b, x0=np.ones(20), init_x=False)
+    assert np.all(x == 0.0)
+    # max_iter = 1 without abort -> status 1, one iteration
+    x, mon = _solve_flat(m, b, max_iter=1)
+    assert api.monolis_get_error_tag(mon) == 1 and api.monolis_get_converge_iter(mon) == 1
+    # unsupported preconditioner (SOR) without abort -> error status, x untouched
+    x0 = np.full(20, 5.0)
+    x, mon = _solve_flat(m, b, precond=2, x0=x0)
+    assert api.monolis_get_error_tag(mon) < 0 and np.all(x == 5.0)
+    # zero diagonal block entry: factorisation replaces the pivot by 1, stays finite
+    mz = chain_matrix(10, 3, seed=2)
+    mz.A[0] = 0.0
+    bz = oracle.matvec(mz, np.ones(30))
+    x, mon = _solve_flat(mz, bz, max_iter=50)
+    xo, st, I, R = oracle.solve(mz, bz, max_iter=50, tol=1e-8)
+    assert np.all(np.isfinite(x))
+    assert _iters_close(api.monolis_get_converge_iter(mon), int(I[4]))
+
+
+# --- device-resident API ---------------------------------------------------------------
+def test_device_solve_matches_flat_and_history():
+    m = W.stencil27_bcsr3(12, seed=8)
+    b = oracle.matvec(m, W.uniform_vector(m.nndof, 3))
+    dm = api.DeviceMatrix(m)
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(m.npndof)
+    prm = api.Prm().quiet()
+    hist = dm.solve(db, dx, prm, hist_len=200)
+    it = int(prm.Iarray[4])
+    xo, st, I, R = oracle.solve(m, b)
+    assert it == int(I[4]) or _iters_close(it, int(I[4]))
+    assert abs(hist[it] - prm.Rarray[2]) == 0.0 and hist[it] < 1e-8 <= hist[it - 1]
+    assert np.linalg.norm(dx.numpy() - xo) / np.linalg.norm(xo) <= 1e-6
+    # repeated solve on the same handle: identical bits (deterministic reductions)
+    dx2 = api.DeviceBuffer(m.npndof)
+    dm.solve(db, dx2, api.Prm().quiet())
+    assert np.array_equal(dx.numpy(), dx2.numpy())
+
+
+@pytest.mark.parametrize("scaled", [False, True])
+def test_refined_solve_reaches_fp64_accuracy(scaled):
+    m = W.stencil27_bcsr3(16, seed=11)
+    if scaled:
+        m = W.scale_symmetric(m, 1e3, seed=2)
+    xt = W.uniform_vector(m.nndof, seed=12)
+    b = oracle.matvec(m, xt)
+    dm = api.DeviceMatrix(m)
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(m.npndof)
+    prm = api.Prm().quiet()
+    api_m = api.Monolis(m, prm)
+    api.monolis_set_tolerance(api_m, 1e-10)
+    st = dm.solve_refined(db, dx, prm, inner_tol=1e-4, max_outer=30)
+    assert st["converged"] and st["resid"] < 1e-10
+    x = dx.numpy()
+    true = np.linalg.norm(b - oracle.matvec(m, x)) / np.linalg.norm(b)
+    assert true < 1e-10
+    xo, s0, I, R = oracle.solve(m, b, tol=1e-10, max_iter=5000)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+
+
+def test_invalid_csr_is_rejected_without_launching():
+    m = chain_matrix(10, 2, seed=1)
+    bad = m.item.copy()
+    bad[3] = 11                      # column outside 1..NP
+    from monolis_amd import matrix as mm
+    mb = mm.Matrix(m.n, m.np_, 2, m.index.copy(), bad, m.A.copy()).ensure_dof_index()
+    with pytest.raises(api._lib.MonolisHipError, match="item outside"):
+        api.DeviceMatrix(mb)
+    idx = m.index.copy()
+    idx[4] = idx[5] + 1              # decreasing offsets
+    mb = mm.Matrix(m.n, m.np_, 2, idx, m.item.copy(), m.A.copy()).ensure_dof_index()
+    with pytest.raises(api._lib.MonolisHipError, match="non-decreasing"):
+        api.DeviceMatrix(mb)
