@@ -42,6 +42,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU SpMV sample")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only run the streaming-copy probe (1 GiB read + 1 GiB write per launch)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -57,6 +59,23 @@ def main() -> int:
     L = _lib.load()
     _lib.check(L.monolis_hip_init(), "monolis_hip_init")
 
+    def copy_probe(reps):
+        import ctypes
+        n = 1 << 27                                  # 1 GiB of doubles each way
+        src, dst = api.DeviceBuffer(n), api.DeviceBuffer(n)
+        src.zero()
+        ms_c = ctypes.c_double()
+        _lib.check(L.monolis_hip_stream_copy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), n, reps,
+                                             ctypes.byref(ms_c)), "stream_copy")
+        return {"bytes_per_launch": 16 * n, "ms": round(ms_c.value, 5),
+                "gbs": round(16 * n / (ms_c.value * 1e-3) / 1e9, 1)}
+
+    if args.copy_only:
+        cp = copy_probe(args.steps)
+        log(f"stream copy: {cp['ms']:.3f} ms, {cp['gbs']:.0f} GB/s")
+        print(json.dumps({"stream_copy": cp}), flush=True)
+        return 0
+
     t0 = time.time()
     m = W.stencil27_bcsr3(args.grid, seed=args.seed)
     xt = W.uniform_vector(m.nndof, seed=args.seed + 1)
@@ -68,6 +87,11 @@ def main() -> int:
     dx = api.DeviceBuffer.from_numpy(W.uniform_vector(m.nndof, seed=args.seed + 2))
     dy = api.DeviceBuffer(m.nndof)
     log(f"uploaded; GPU layout {info['bytes_fp64'] / 1e9:.3f} GB ({info['slots']} slots)")
+
+    # --- autotuner (outside the timed region): per-matrix SELL kernel variant ---
+    tune = dm.autotune(fp32=True, reps=10)
+    log(f"autotune: FP64 variant {tune['best64']} ({min(tune['ms64']) * 1e3:.1f} us), "
+        f"FP32 variant {tune['best32']} ({min(tune['ms32']) * 1e3:.1f} us)")
 
     # --- headline: SpMV, HIP events around exactly K launches -------------------
     _lib.check(L.monolis_hip_synchronize(), "sync")
@@ -83,6 +107,9 @@ def main() -> int:
     B32 = metrics.spmv_bytes(m.n, m.nz, 3, 4)
     gbs32 = B32 / (ms32 * 1e-3) / 1e9
     log(f"SpMV FP32: {ms32 * 1e3:.1f} us/launch, {gbs32:.0f} GB/s")
+
+    cp = copy_probe(20)
+    log(f"stream copy probe: {cp['gbs']:.0f} GB/s")
 
     out = {
         "metric": METRIC,
@@ -104,6 +131,10 @@ def main() -> int:
                      "frac": round(gbs / metrics.HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_sell<double,double,3,OpSpmv>", "algorithmic_bytes": B64},
         "spmv_fp32": {"ms": round(ms32, 5), "gbs": round(gbs32, 1), "algorithmic_bytes": B32},
+        "stream_copy": cp,
+        "autotune": {"variant_fp64": tune["best64"], "variant_fp32": tune["best32"],
+                     "us_fp64": [round(v * 1e3, 1) for v in tune["ms64"]],
+                     "us_fp32": [round(v * 1e3, 1) for v in tune["ms32"]]},
     }
 
     # --- CG FP64 vs FP32-refined on the same system -------------------------------

@@ -116,6 +116,24 @@ int monolis_hip_solve_refined_R(monolis_hip_matrix* M, const double* b, double* 
 int monolis_hip_time_matvec(monolis_hip_matrix* M, int fp32, const void* x, void* y, int warmup, int reps,
                             double* ms);
 
+/* Autotuner (no monolis counterpart): times each of the SELL SpMV kernel
+ * variants (slots per step x temporal / non-temporal matrix streams) on this
+ * matrix with HIP events, `reps` launches each, and keeps the fastest one per
+ * precision for every later SpMV and solve on the handle.  best[0] / best[1]
+ * receive the FP64 / FP32 choice; ms (if not NULL) receives 2 x
+ * monolis_hip_num_variants() average times, FP64 first.  fp32 = 0 tunes FP64
+ * only. */
+int monolis_hip_num_variants(void);
+int monolis_hip_matrix_autotune(monolis_hip_matrix* M, int fp32, int reps, int* best, double* ms);
+/* Forces a variant (0 = default) for one precision (fp32 = 0 or 1). */
+int monolis_hip_matrix_set_variant(monolis_hip_matrix* M, int fp32, int variant);
+
+/* Bandwidth probe (no monolis counterpart): a plain streaming copy kernel
+ * dst[i] = src[i] over n doubles, `reps` timed launches on the library stream;
+ * *ms = average per launch.  Used to calibrate HBM byte counters and to state
+ * the copy bandwidth the SpMV is compared with. */
+int monolis_hip_stream_copy(double* dst, const double* src, long long n, int reps, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

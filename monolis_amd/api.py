@@ -219,8 +219,13 @@ class DeviceMatrix:
         return dict(zip(keys, list(a)))
 
     def set_values(self, A) -> None:
-        ptr = A.ptr if isinstance(A, DeviceBuffer) else _p(np.ascontiguousarray(A, np.float64))
-        _lib.check(_lib.load().monolis_hip_matrix_set_values(ctypes.c_void_p(self.h), ctypes.c_void_p(ptr) if isinstance(A, DeviceBuffer) else ptr), "set_values")
+        """New block values, same pattern (host array or DeviceBuffer)."""
+        if isinstance(A, DeviceBuffer):
+            ptr = ctypes.c_void_p(A.ptr)
+        else:
+            A = np.ascontiguousarray(A, np.float64)
+            ptr = _p(A)
+        _lib.check(_lib.load().monolis_hip_matrix_set_values(ctypes.c_void_p(self.h), ptr), "set_values")
 
     def matvec(self, x: DeviceBuffer, y: DeviceBuffer) -> None:
         L = _lib.load()
@@ -244,6 +249,19 @@ class DeviceMatrix:
                                                      float(inner_tol), int(max_outer), _p(st))
         _lib.check(rc, "monolis_hip_solve_refined_R")
         return {"outer": int(st[0]), "inner_total": int(st[1]), "resid": float(st[2]), "converged": bool(st[3])}
+
+    def autotune(self, fp32: bool = True, reps: int = 20) -> dict:
+        L = _lib.load()
+        nv = L.monolis_hip_num_variants()
+        best = (ctypes.c_int * 2)()
+        ms = (ctypes.c_double * (2 * nv))()
+        _lib.check(L.monolis_hip_matrix_autotune(ctypes.c_void_p(self.h), int(fp32), int(reps), best, ms), "autotune")
+        return {"best64": best[0], "best32": best[1] if fp32 else None,
+                "ms64": list(ms)[:nv], "ms32": list(ms)[nv:] if fp32 else None}
+
+    def set_variant(self, variant: int, fp32: bool = False) -> None:
+        _lib.check(_lib.load().monolis_hip_matrix_set_variant(ctypes.c_void_p(self.h), int(fp32), int(variant)),
+                   "set_variant")
 
     def time_matvec(self, x: DeviceBuffer, y: DeviceBuffer, warmup: int = 5, reps: int = 50) -> float:
         ms = ctypes.c_double()

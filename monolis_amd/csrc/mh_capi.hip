@@ -9,6 +9,7 @@
 
 #include "../../include/monolis_hip.h"
 #include "mh_internal.hpp"
+#include "mh_kernels.hpp"
 
 using mh::Context;
 using mh::Matrix;
@@ -343,6 +344,109 @@ int monolis_hip_time_matvec(monolis_hip_matrix* H, int fp32, const void* x, void
   MH_HIP(hipEventCreate(&e1));
   MH_HIP(hipEventRecord(e0, c->stream));
   for (int i = 0; i < reps; ++i) MH_TRY(one());
+  MH_HIP(hipEventRecord(e1, c->stream));
+  MH_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  MH_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = reps > 0 ? (double)t / reps : 0.0;
+  return 0;
+}
+
+int monolis_hip_num_variants(void) { return mh::kNumVariants; }
+
+int monolis_hip_matrix_set_variant(monolis_hip_matrix* H, int fp32, int variant) {
+  if (variant < 0 || variant >= mh::kNumVariants) {
+    mh::set_error("variant %d out of range [0, %d)", variant, mh::kNumVariants);
+    return ST_ARGS;
+  }
+  if (fp32) H->M->variant32 = variant;
+  else H->M->variant64 = variant;
+  return 0;
+}
+
+int monolis_hip_matrix_autotune(monolis_hip_matrix* H, int fp32, int reps, int* best, double* ms) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  Matrix& M = *H->M;
+  if (reps < 1) reps = 1;
+  const size_t vec = (size_t)M.np * M.b;
+  mh::DevArray<double> x, y;
+  MH_TRY(x.alloc(vec));
+  MH_TRY(y.alloc(vec));
+  MH_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * vec, c->stream));
+  if (fp32) MH_TRY(mh::matrix_make_fp32(&M, c->stream));
+  hipEvent_t e0, e1;
+  MH_HIP(hipEventCreate(&e0));
+  MH_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  for (int prec = 0; prec < (fp32 ? 2 : 1) && !rc; ++prec) {
+    double best_ms = 1e300;
+    int best_v = 0;
+    for (int v = 0; v < mh::kNumVariants && !rc; ++v) {
+      auto one = [&]() -> int {
+        if (prec == 0) return mh::spmv_variant<double>(M, x.p, y.p, v, c->stream);
+        return mh::spmv_variant<float>(M, (const float*)x.p, (float*)y.p, v, c->stream);
+      };
+      for (int i = 0; i < 3 && !rc; ++i) rc = one();
+      if (rc) break;
+      if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = -1; break; }
+      for (int i = 0; i < reps && !rc; ++i) rc = one();
+      if (rc) break;
+      float t = 0.f;
+      if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+        mh::set_error("autotune timing failed");
+        rc = -1;
+        break;
+      }
+      const double avg = (double)t / reps;
+      if (ms) ms[prec * mh::kNumVariants + v] = avg;
+      if (avg < best_ms) {
+        best_ms = avg;
+        best_v = v;
+      }
+    }
+    if (!rc) {
+      if (prec == 0) M.variant64 = best_v;
+      else M.variant32 = best_v;
+      if (best) best[prec] = best_v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+__global__ void __launch_bounds__(256) k_stream_copy(long long n, const double* __restrict__ src,
+                                                     double* __restrict__ dst) {
+  const long long stride = (long long)gridDim.x * blockDim.x * 2;
+  for (long long i = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < n; i += stride) {
+    if (i + 1 < n) {
+      const double2 v = *reinterpret_cast<const double2*>(src + i);
+      *reinterpret_cast<double2*>(dst + i) = v;
+    } else {
+      dst[i] = src[i];
+    }
+  }
+}
+
+int monolis_hip_stream_copy(double* dst, const double* src, long long n, int reps, double* ms) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (n <= 0) {
+    *ms = 0.0;
+    return 0;
+  }
+  const int grid = 256 * 8 * 4;  // 8 waves per CU worth of workgroups on 256 CUs, grid-stride
+  k_stream_copy<<<grid, 256, 0, c->stream>>>(n, src, dst);
+  MH_HIP(hipGetLastError());
+  hipEvent_t e0, e1;
+  MH_HIP(hipEventCreate(&e0));
+  MH_HIP(hipEventCreate(&e1));
+  MH_HIP(hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; ++r) k_stream_copy<<<grid, 256, 0, c->stream>>>(n, src, dst);
   MH_HIP(hipEventRecord(e1, c->stream));
   MH_HIP(hipEventSynchronize(e1));
   float t = 0.f;

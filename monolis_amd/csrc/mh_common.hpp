@@ -149,5 +149,10 @@ template <typename T>
 __device__ __forceinline__ T ld_stream(const T* p) {
   return __builtin_nontemporal_load(p);
 }
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_val(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
 
 }  // namespace mh

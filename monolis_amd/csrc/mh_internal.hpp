@@ -89,6 +89,8 @@ struct Matrix {
   DevArray<double> D64;
   DevArray<float> D32;
   bool diag_ready = false, diag32_ready = false, val32_ready = false;
+  // SELL kernel variant per precision (0 = default); set by the autotuner
+  int variant64 = 0, variant32 = 0;
   // transpose (for adjoint solves of nonsymmetric systems), built lazily
   Matrix* transpose = nullptr;
   ~Matrix();
@@ -108,6 +110,9 @@ SellView<TV> view(const Matrix& M);
 // ---------------------------------------------------------------------------
 template <typename T>
 int spmv(const Matrix& M, const T* x, T* y, hipStream_t s);
+// SpMV with an explicit kernel variant (autotuner / sweeps).
+template <typename T>
+int spmv_variant(const Matrix& M, const T* x, T* y, int variant, hipStream_t s);
 // r = b - A x; if rr_out != nullptr, rr_out = (r, r) (device scalar, deterministic).
 template <typename T>
 int residual(const Matrix& M, const T* x, const T* b, T* r, double* partials, unsigned* ticket,

@@ -32,30 +32,87 @@ struct SellView {
 // gathers the B entries of the input vector at a block column.  Products are
 // accumulated in the order of monolis's matvec_33_R (matvec.f90:805-823):
 // block by block, and inside a block c = 1..B.
-template <typename TV, typename TA, int B, class XG>
+//
+// The slot loop is wave-uniform (one slice per wavefront) and processes U slots
+// per step with every load issued up front: U column ids, U*B*B value planes and
+// then U gathers.  Padding slots are always safe to load (col = own row,
+// val = 0), so no lane branches; a select keeps a padded slot from touching the
+// accumulator, which preserves IEEE semantics (0 * inf never enters a sum).
+template <typename TV, typename TA, int B, int U, bool NT, class XG>
 __device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane, int len, const XG& xg,
                                          TA (&acc)[B]) {
   const int k0 = M.slice_ptr[s];
   const int w = M.slice_ptr[s + 1] - k0;
   const int* colp = M.col + (size_t)k0 * kWave + lane;
   const TV* valp = M.val + (size_t)k0 * kWave * (B * B) + lane;
-#pragma unroll 2
-  for (int k = 0; k < w; ++k) {
-    if (k < len) {
-      const int c = ld_stream(colp + (size_t)k * kWave);
-      TA xv[B];
-      xg(c, xv);
-      const TV* vp = valp + (size_t)k * kWave * (B * B);
-      TV a[B * B];
+  int k = 0;
+  for (; k + U <= w; k += U) {
+    int c[U];
+    TV a[U][B * B];
+    TA xv[U][B];
 #pragma unroll
-      for (int e = 0; e < B * B; ++e) a[e] = ld_stream(vp + e * kWave);
+    for (int u = 0; u < U; ++u) c[u] = ld_val<NT>(colp + (size_t)(k + u) * kWave);
 #pragma unroll
-      for (int r = 0; r < B; ++r)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int cc = 0; cc < B; ++cc) acc[r] = fma((TA)a[r * B + cc], xv[cc], acc[r]);
+      for (int e = 0; e < B * B; ++e) a[u][e] = ld_val<NT>(valp + ((size_t)(k + u) * (B * B) + e) * kWave);
+#pragma unroll
+    for (int u = 0; u < U; ++u) xg(c[u], xv[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool on = (k + u) < len;
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+        TA t = acc[r];
+#pragma unroll
+        for (int cc = 0; cc < B; ++cc) t = fma((TA)a[u][r * B + cc], xv[u][cc], t);
+        acc[r] = on ? t : acc[r];
+      }
+    }
+  }
+  for (; k < w; ++k) {
+    const int c = ld_val<NT>(colp + (size_t)k * kWave);
+    TV a[B * B];
+#pragma unroll
+    for (int e = 0; e < B * B; ++e) a[e] = ld_val<NT>(valp + ((size_t)k * (B * B) + e) * kWave);
+    TA xv[B];
+    xg(c, xv);
+    const bool on = k < len;
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      TA t = acc[r];
+#pragma unroll
+      for (int cc = 0; cc < B; ++cc) t = fma((TA)a[r * B + cc], xv[cc], t);
+      acc[r] = on ? t : acc[r];
     }
   }
 }
+
+// Kernel variants of the SELL row loop: U slots per step (loads in flight per
+// wave vs. VGPR budget) and temporal / non-temporal matrix streams.  Variant 0
+// is the default; monolis_hip_matrix_autotune times the others per matrix.
+struct SellVariant {
+  int unroll;
+  bool nt;
+};
+constexpr int kNumVariants = 8;
+__host__ __device__ constexpr SellVariant sell_variant(int v) {
+  return v == 1 ? SellVariant{1, true}
+       : v == 2 ? SellVariant{2, true}
+       : v == 3 ? SellVariant{4, true}
+       : v == 4 ? SellVariant{3, false}
+       : v == 5 ? SellVariant{2, false}
+       : v == 6 ? SellVariant{4, false}
+       : v == 7 ? SellVariant{1, false}
+                : SellVariant{3, true};
+}
+// B > 3 blocks carry B*B values per slot: cap the unroll to bound VGPRs.
+template <int B, int V>
+struct SellCfg {
+  static constexpr int U0 = sell_variant(V).unroll;
+  static constexpr int U = B == 1 ? U0 * 2 : B <= 3 ? U0 : B <= 5 ? (U0 < 2 ? U0 : 2) : 1;
+  static constexpr bool NT = sell_variant(V).nt;
+};
 
 // Block-Jacobi apply on one block row, in place.  D is the factored diagonal
 // block in structure-of-arrays form: entry e of row i at D[e * nrows + i].
@@ -101,14 +158,14 @@ __device__ __forceinline__ void diag_apply(const TD* __restrict__ D, int nrows, 
 //   void epilogue(int row, bool valid, const TA (&acc)[B], double (&part)[NV])
 //   void finalize(const double (&tot)[NV])       run once, by thread 0 of the last workgroup
 // and NV >= 0 partial sums that are reduced deterministically.
-template <typename TV, typename TA, int B, class Op>
+template <typename TV, typename TA, int B, int V, class Op>
 __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op, double* partials,
                                                  unsigned* ticket, int ntiles) {
   const int tile = xcd_tile(ntiles);
   if (tile < 0) return;
   if (op.skip()) return;
   const int lane = threadIdx.x & (kWave - 1);
-  const int s = tile * kWpb + threadIdx.x / kWave;
+  const int s = __builtin_amdgcn_readfirstlane(tile * kWpb + (int)(threadIdx.x / kWave));
   const int row = s * kWave + lane;
   const bool valid = (s < M.nslices) && (row < M.nrows);
   int len = 0;
@@ -118,7 +175,7 @@ __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op, double* 
   for (int r = 0; r < B; ++r) acc[r] = (TA)0;
   if (s < M.nslices) {
     auto xg = [&](int c, TA(&xv)[B]) { op.gather(c, xv); };
-    sell_row<TV, TA, B>(M, s, lane, len, xg, acc);
+    sell_row<TV, TA, B, SellCfg<B, V>::U, SellCfg<B, V>::NT>(M, s, lane, len, xg, acc);
   }
   constexpr int NV = Op::NV;
   if constexpr (NV == 0) {
@@ -136,6 +193,19 @@ __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op, double* 
     }
   }
 }
+
+// Dispatches a runtime variant index to the k_sell instantiation.
+#define MH_SELL_VARIANTS(v, ...)                                    \
+  switch (v) {                                                      \
+    case 1: { constexpr int V = 1; __VA_ARGS__; } break;            \
+    case 2: { constexpr int V = 2; __VA_ARGS__; } break;            \
+    case 3: { constexpr int V = 3; __VA_ARGS__; } break;            \
+    case 4: { constexpr int V = 4; __VA_ARGS__; } break;            \
+    case 5: { constexpr int V = 5; __VA_ARGS__; } break;            \
+    case 6: { constexpr int V = 6; __VA_ARGS__; } break;            \
+    case 7: { constexpr int V = 7; __VA_ARGS__; } break;            \
+    default: { constexpr int V = 0; __VA_ARGS__; } break;           \
+  }
 
 // Row-parallel (non-SpMV) fused vector kernel: one thread per block row.
 //   void row(int i, double (&part)[NV]);  void finalize(const double (&tot)[NV]);  bool skip();

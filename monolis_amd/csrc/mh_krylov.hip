@@ -357,7 +357,7 @@ struct OpBiE {
 template <class Op>
 struct Begun : Op {};
 
-template <typename TV, typename TA, int B, class Op>
+template <typename TV, typename TA, int B, int V, class Op>
 __global__ void __launch_bounds__(kBlock) k_sell_b(SellView<TV> M, Op op0, double* partials, unsigned* ticket,
                                                    int ntiles) {
   const int tile = xcd_tile(ntiles);
@@ -366,7 +366,7 @@ __global__ void __launch_bounds__(kBlock) k_sell_b(SellView<TV> M, Op op0, doubl
   if (op.skip()) return;
   op.begin();
   const int lane = threadIdx.x & (kWave - 1);
-  const int s = tile * kWpb + threadIdx.x / kWave;
+  const int s = __builtin_amdgcn_readfirstlane(tile * kWpb + (int)(threadIdx.x / kWave));
   const int row = s * kWave + lane;
   const bool valid = (s < M.nslices) && (row < M.nrows);
   int len = 0;
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(kBlock) k_sell_b(SellView<TV> M, Op op0, doubl
   for (int r = 0; r < B; ++r) acc[r] = (TA)0;
   if (s < M.nslices) {
     auto xg = [&](int c, TA(&xv)[B]) { op.gather(c, xv); };
-    sell_row<TV, TA, B>(M, s, lane, len, xg, acc);
+    sell_row<TV, TA, B, SellCfg<B, V>::U, SellCfg<B, V>::NT>(M, s, lane, len, xg, acc);
   }
   constexpr int NV = Op::NV;
   double part[NV > 0 ? NV : 1];
@@ -415,11 +415,20 @@ __global__ void __launch_bounds__(kBlock) k_rows_b(int nrows, Op op0, double* pa
   }
 }
 
+// Solver SpMVs use the matrix's tuned variant for their precision when `tuned`
+// is set (the per-iteration q = A p / v = A pt / t = A st), else the default.
 template <typename T, int B, class Op>
-int run_sell(const Matrix& M, const Op& op, double* partials, unsigned* ticket, hipStream_t s) {
+int run_sell(const Matrix& M, const Op& op, double* partials, unsigned* ticket, hipStream_t s,
+             bool tuned = false) {
   const int ntiles = (M.nslices + kWpb - 1) / kWpb;
   if (ntiles == 0) return 0;
-  k_sell_b<T, T, B, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(view<T>(M), op, partials, ticket, ntiles);
+  const int var = tuned ? (sizeof(T) == 8 ? M.variant64 : M.variant32) : 0;
+  if constexpr (B == 1 || B == 3) {
+    MH_SELL_VARIANTS(var, k_sell_b<T, T, B, V, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(
+                              view<T>(M), op, partials, ticket, ntiles));
+  } else {
+    k_sell_b<T, T, B, 0, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(view<T>(M), op, partials, ticket, ntiles);
+  }
   MH_HIP(hipGetLastError());
   return 0;
 }
@@ -556,7 +565,7 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
     const int end = std::min(it + P.chunk - 1, P.max_iter);
     for (; it <= end; ++it) {
       OpCgSpmv<T, B> sp{z, pbuf[(it + 1) & 1], pbuf[it & 1], q, st, (T)0};
-      MH_TRY((run_sell<T, B>(M, sp, parts, &st->ticket[2], s)));
+      MH_TRY((run_sell<T, B>(M, sp, parts, &st->ticket[2], s, true)));
       if (it % kIterRR == 0) {
         MH_TRY(upd(it, F_XONLY));
         OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
@@ -645,11 +654,11 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
       OpBiA<T, B, PREC> a{r, p, v, pt, D, n, st, it == 1, (T)0, (T)0};
       MH_TRY(run_rows(n, a, parts, &st->ticket[1], s));
       OpBiB<T, B> bb{pt, v, rt, st};
-      MH_TRY((run_sell<T, B>(M, bb, parts, &st->ticket[2], s)));
+      MH_TRY((run_sell<T, B>(M, bb, parts, &st->ticket[2], s, true)));
       OpBiC<T, B, PREC> cc{r, v, sv, stv, D, n, st, (T)0};
       MH_TRY(run_rows(n, cc, parts, &st->ticket[1], s));
       OpBiD<T, B> dd{stv, t, sv, st};
-      MH_TRY((run_sell<T, B>(M, dd, parts, &st->ticket[2], s)));
+      MH_TRY((run_sell<T, B>(M, dd, parts, &st->ticket[2], s, true)));
       if (it % kIterRR == 0) {
         OpBiE<T, B> e1{x, r, pt, stv, sv, t, rt, st, hp, it, F_XONLY, (T)0, (T)0};
         MH_TRY(run_rows(n, e1, parts, &st->ticket[1], s));

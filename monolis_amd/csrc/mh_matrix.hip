@@ -374,10 +374,16 @@ struct OpResid {
 };
 
 template <typename TV, typename T, int B, class Op>
-static int launch_sell(const Matrix& M, const Op& op, double* partials, unsigned* ticket, hipStream_t s) {
+static int launch_sell(const Matrix& M, const Op& op, double* partials, unsigned* ticket, hipStream_t s,
+                       int variant = 0) {
   const int ntiles = (M.nslices + kWpb - 1) / kWpb;
   if (ntiles == 0) return 0;
-  k_sell<TV, T, B, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(view<TV>(M), op, partials, ticket, ntiles);
+  if constexpr (B == 1 || B == 3) {
+    MH_SELL_VARIANTS(variant, k_sell<TV, T, B, V, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(
+                                  view<TV>(M), op, partials, ticket, ntiles));
+  } else {
+    k_sell<TV, T, B, 0, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(view<TV>(M), op, partials, ticket, ntiles);
+  }
   MH_HIP(hipGetLastError());
   return 0;
 }
@@ -396,16 +402,27 @@ static int launch_sell(const Matrix& M, const Op& op, double* partials, unsigned
   }
 
 template <typename T>
-int spmv(const Matrix& M, const T* x, T* y, hipStream_t s) {
+int spmv_variant(const Matrix& M, const T* x, T* y, int variant, hipStream_t s) {
   if (sizeof(T) == 4 && !M.val32_ready) {
     set_error("fp32 values not built");
     return -2;
   }
+  if (variant < 0 || variant >= kNumVariants) {
+    set_error("SpMV variant %d out of range", variant);
+    return -2;
+  }
   MH_DISPATCH_B(M.b, {
     OpSpmv<T, B> op{x, y};
-    MH_TRY((launch_sell<T, T, B>(M, op, nullptr, nullptr, s)));
+    MH_TRY((launch_sell<T, T, B>(M, op, nullptr, nullptr, s, variant)));
   });
   return 0;
+}
+template int spmv_variant<double>(const Matrix&, const double*, double*, int, hipStream_t);
+template int spmv_variant<float>(const Matrix&, const float*, float*, int, hipStream_t);
+
+template <typename T>
+int spmv(const Matrix& M, const T* x, T* y, hipStream_t s) {
+  return spmv_variant<T>(M, x, y, sizeof(T) == 8 ? M.variant64 : M.variant32, s);
 }
 template int spmv<double>(const Matrix&, const double*, double*, hipStream_t);
 template int spmv<float>(const Matrix&, const float*, float*, hipStream_t);

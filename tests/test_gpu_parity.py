@@ -20,8 +20,20 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module", autouse=True)
 def _init(hip):
-    oracle.set_threads(4)
+    oracle.set_threads(1)      # sequential sums: the reference's single-thread order
     yield
+
+
+def _oracle_iter_spread(m, b, **kw):
+    """Iteration counts of the oracle under several summation orders (OpenMP
+    thread counts).  Near finite termination the count itself depends on the
+    reduction order; the GPU (another order) must fall inside this spread."""
+    its = []
+    for th in (1, 2, 3, 4, 5, 8):
+        oracle.set_threads(th)
+        its.append(int(oracle.solve(m, b, **kw)[2][4]))
+    oracle.set_threads(1)
+    return min(its), max(its)
 
 
 def _rel(a, b):
@@ -78,6 +90,31 @@ def test_matvec_stencil27_device_resident():
     dm.matvec(dx32, dy32)
     ref = oracle.matvec(m, x)
     assert _rel(dy32.numpy().astype(np.float64), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("gen", ["stencil27", "poisson7"])
+def test_matvec_every_kernel_variant_is_bit_identical(gen):
+    """The autotuner may pick any variant: all of them must give the same bits
+    (same per-row accumulation order, only load scheduling differs)."""
+    from monolis_amd import _lib
+    m = W.stencil27_bcsr3(13, seed=4) if gen == "stencil27" else W.poisson7(23)
+    x = np.random.default_rng(5).uniform(-1, 1, m.nndof)
+    dm = api.DeviceMatrix(m)
+    dx = api.DeviceBuffer.from_numpy(x)
+    dy = api.DeviceBuffer(m.nndof)
+    ref = oracle.matvec(m, x)
+    outs = []
+    for v in range(_lib.load().monolis_hip_num_variants()):
+        dm.set_variant(v)
+        dm.matvec(dx, dy)
+        outs.append(dy.numpy())
+    assert _rel(outs[0], ref) <= 1e-12
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    tune = dm.autotune(fp32=True, reps=2)
+    assert 0 <= tune["best64"] < len(outs) and 0 <= tune["best32"] < len(outs)
+    dm.matvec(dx, dy)
+    assert np.array_equal(dy.numpy(), outs[0])
 
 
 def test_matvec_values_update():
@@ -143,7 +180,9 @@ def test_solve_chain_matches_oracle(ndof, precond, method):
     x, mon = _solve_flat(m, b, method, precond, tol=1e-10)
     xo, st, I, R = oracle.solve(m, b, method=method, precond=precond, tol=1e-10)
     assert api.monolis_get_error_tag(mon) == 0 and st == 0
-    assert _iters_close(api.monolis_get_converge_iter(mon), int(I[4]))
+    lo, hi = _oracle_iter_spread(m, b, method=method, precond=precond, tol=1e-10)
+    it = api.monolis_get_converge_iter(mon)
+    assert _iters_close(it, int(I[4])) or lo - 1 <= it <= hi + 1, (it, int(I[4]), lo, hi)
     assert np.allclose(x, 1.0, rtol=0, atol=1e-8)
 
 
