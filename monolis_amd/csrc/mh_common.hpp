@@ -98,49 +98,39 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
   __syncthreads();
 }
 
-__device__ __forceinline__ void store_agent(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_agent(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Publishes this workgroup's NV partial sums (thread 0 holds them) and returns
-// true in every thread of the workgroup that arrives last.  That workgroup then
-// owns `total[k]` = fixed-order sum over all partials.  The ticket is reset by
-// the last workgroup, so it is ready for the next launch.
+// Cross-workgroup reductions are two-phase: every workgroup stores its NV
+// partial sums with plain stores, and a one-workgroup finalize kernel launched
+// next on the same stream sums them in a fixed order.  The kernel boundary
+// provides the visibility across XCDs; no device-scope fence is executed inside
+// a kernel (on a multi-XCD GPU that fence writes back and invalidates the XCD's
+// L2, which would also evict the gathered vector).
 template <int NV>
-__device__ __forceinline__ bool reduce_last(const double (&v)[NV], double* partials, int tile,
-                                            int ntiles, unsigned* ticket, double (&total)[NV]) {
-  __shared__ int is_last;
+__device__ __forceinline__ void store_partials(const double (&v)[NV], double* partials, int tile) {
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) store_agent(&partials[(size_t)tile * NV + k], v[k]);
-    __threadfence();
-    const unsigned t = atomicAdd(ticket, 1u);
-    is_last = (t == (unsigned)ntiles - 1u);
+    for (int k = 0; k < NV; ++k) partials[(size_t)tile * NV + k] = v[k];
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+}
+
+// Fixed-order sum of ntiles x NV partials by one workgroup; valid in every thread.
+template <int NV>
+__device__ __forceinline__ void sum_partials(const double* partials, int ntiles, double (&total)[NV]) {
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
   for (int i = threadIdx.x; i < ntiles; i += kBlock) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] += load_agent(&partials[(size_t)i * NV + k]);
+    for (int k = 0; k < NV; ++k) acc[k] += partials[(size_t)i * NV + k];
   }
   block_sum<NV>(acc);
   __shared__ double tot[NV];
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) tot[k] = acc[k];
-    *ticket = 0u;
   }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < NV; ++k) total[k] = tot[k];
-  return true;
 }
 
 // Streaming loads: matrix values and column ids are read once per SpMV, so

@@ -156,8 +156,8 @@ __device__ __forceinline__ void diag_apply(const TD* __restrict__ D, int nrows, 
 //   bool skip()                                  whole launch is a no-op (solver done)
 //   void gather(int col, TA (&xv)[B])            input vector at a block column
 //   void epilogue(int row, bool valid, const TA (&acc)[B], double (&part)[NV])
-//   void finalize(const double (&tot)[NV])       run once, by thread 0 of the last workgroup
-// and NV >= 0 partial sums that are reduced deterministically.
+//   void finalize(const double (&tot)[NV])       run once by k_finalize (thread 0)
+// and NV >= 0 partial sums per workgroup, stored for the finalize kernel.
 template <typename TV, typename TA, int B, int V, class Op>
 __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op, double* partials,
                                                  unsigned* ticket, int ntiles) {
@@ -187,10 +187,7 @@ __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op, double* 
     for (int k = 0; k < NV; ++k) part[k] = 0.0;
     op.epilogue(row, valid, acc, part);
     block_sum<NV>(part);
-    double tot[NV];
-    if (reduce_last<NV>(part, partials, tile, ntiles, ticket, tot)) {
-      if (threadIdx.x == 0) op.finalize(tot);
-    }
+    store_partials<NV>(part, partials, tile);
   }
 }
 
@@ -226,11 +223,18 @@ __global__ void __launch_bounds__(kBlock) k_rows(int nrows, Op op, double* parti
     for (int k = 0; k < NV; ++k) part[k] = 0.0;
     if (i < nrows) op.row(i, part);
     block_sum<NV>(part);
-    double tot[NV];
-    if (reduce_last<NV>(part, partials, tile, ntiles, ticket, tot)) {
-      if (threadIdx.x == 0) op.finalize(tot);
-    }
+    store_partials<NV>(part, partials, tile);
   }
+}
+
+// Second phase of a reduction: one workgroup, fixed-order sum, op.finalize().
+template <class Op>
+__global__ void __launch_bounds__(kBlock) k_finalize(Op op, const double* partials, int ntiles) {
+  if (op.skip()) return;
+  constexpr int NV = Op::NV;
+  double tot[NV];
+  sum_partials<NV>(partials, ntiles, tot);
+  if (threadIdx.x == 0) op.finalize(tot);
 }
 
 }  // namespace mh

@@ -385,10 +385,7 @@ __global__ void __launch_bounds__(kBlock) k_sell_b(SellView<TV> M, Op op0, doubl
   op.epilogue(row, valid, acc, part);
   if constexpr (NV > 0) {
     block_sum<NV>(part);
-    double tot[NV];
-    if (reduce_last<NV>(part, partials, tile, ntiles, ticket, tot)) {
-      if (threadIdx.x == 0) op.finalize(tot);
-    }
+    store_partials<NV>(part, partials, tile);
   }
 }
 
@@ -408,10 +405,7 @@ __global__ void __launch_bounds__(kBlock) k_rows_b(int nrows, Op op0, double* pa
   if (i < nrows) op.row(i, part);
   if constexpr (NV > 0) {
     block_sum<NV>(part);
-    double tot[NV];
-    if (reduce_last<NV>(part, partials, tile, ntiles, ticket, tot)) {
-      if (threadIdx.x == 0) op.finalize(tot);
-    }
+    store_partials<NV>(part, partials, tile);
   }
 }
 
@@ -430,6 +424,10 @@ int run_sell(const Matrix& M, const Op& op, double* partials, unsigned* ticket, 
     k_sell_b<T, T, B, 0, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(view<T>(M), op, partials, ticket, ntiles);
   }
   MH_HIP(hipGetLastError());
+  if constexpr (Op::NV > 0) {
+    k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, ntiles);
+    MH_HIP(hipGetLastError());
+  }
   return 0;
 }
 
@@ -439,6 +437,10 @@ int run_rows(int nrows, const Op& op, double* partials, unsigned* ticket, hipStr
   if (ntiles == 0) return 0;
   k_rows_b<Op><<<padded_grid(ntiles), kBlock, 0, s>>>(nrows, op, partials, ticket, ntiles);
   MH_HIP(hipGetLastError());
+  if constexpr (Op::NV > 0) {
+    k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, ntiles);
+    MH_HIP(hipGetLastError());
+  }
   return 0;
 }
 

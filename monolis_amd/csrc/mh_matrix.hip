@@ -385,6 +385,10 @@ static int launch_sell(const Matrix& M, const Op& op, double* partials, unsigned
     k_sell<TV, T, B, 0, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(view<TV>(M), op, partials, ticket, ntiles);
   }
   MH_HIP(hipGetLastError());
+  if constexpr (Op::NV > 0) {
+    k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, ntiles);
+    MH_HIP(hipGetLastError());
+  }
   return 0;
 }
 
@@ -468,6 +472,8 @@ int dot(long long m, const T* x, const T* y, double* out, double* partials, unsi
   }
   OpDot<T> op{x, y, out};
   k_rows<OpDot<T>><<<padded_grid(ntiles), kBlock, 0, s>>>((int)m, op, partials, ticket, ntiles);
+  MH_HIP(hipGetLastError());
+  k_finalize<OpDot<T>><<<1, kBlock, 0, s>>>(op, partials, ntiles);
   MH_HIP(hipGetLastError());
   return 0;
 }
