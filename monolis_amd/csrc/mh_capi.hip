@@ -258,8 +258,6 @@ int monolis_hip_inner_product_R(int m, const double* x, const double* y, double*
   mh::DevArray<double> scratch;
   const size_t np = mh::partials_needed(m);
   MH_TRY(scratch.alloc(np + 16));
-  unsigned* ticket = (unsigned*)(scratch.p + np + 8);
-  MH_HIP(hipMemsetAsync(ticket, 0, 8 * sizeof(double), c->stream));
   const double* dx = x;
   const double* dy = y;
   mh::DevArray<double> hx, hy;
@@ -273,7 +271,7 @@ int monolis_hip_inner_product_R(int m, const double* x, const double* y, double*
     MH_TRY(mh::upload(hy.p, y, sizeof(double) * (size_t)m, c->stream));
     dy = hy.p;
   }
-  MH_TRY(mh::dot<double>(m, dx, dy, scratch.p + np, scratch.p, ticket, c->stream));
+  MH_TRY(mh::dot<double>(m, dx, dy, scratch.p + np, scratch.p, c->stream));
   MH_HIP(hipMemcpyAsync(sum, scratch.p + np, sizeof(double), hipMemcpyDefault, c->stream));
   MH_HIP(hipStreamSynchronize(c->stream));
   return 0;

@@ -69,7 +69,6 @@ struct KState {
   int done;
   int breakdown;    // bit mask of exact-zero dot products seen (BiCGSTAB)
   int pad[3];
-  unsigned ticket[4];
 };
 
 // ---------------------------------------------------------------------------
@@ -115,16 +114,14 @@ template <typename T>
 int spmv_variant(const Matrix& M, const T* x, T* y, int variant, hipStream_t s);
 // r = b - A x; if rr_out != nullptr, rr_out = (r, r) (device scalar, deterministic).
 template <typename T>
-int residual(const Matrix& M, const T* x, const T* b, T* r, double* partials, unsigned* ticket,
-             double* rr_out, hipStream_t s);
+int residual(const Matrix& M, const T* x, const T* b, T* r, double* partials, double* rr_out, hipStream_t s);
 template <typename T>
 int diag_setup(Matrix& M, hipStream_t s);
 template <typename T>
 int diag_apply(const Matrix& M, const T* x, T* y, hipStream_t s);
 // Device dot, deterministic; result written to *out (device pointer).
 template <typename T>
-int dot(long long m, const T* x, const T* y, double* out, double* partials, unsigned* ticket,
-        hipStream_t s);
+int dot(long long m, const T* x, const T* y, double* out, double* partials, hipStream_t s);
 template <typename T>
 int axpby(long long m, double alpha, const T* x, double beta, const T* y, T* z, hipStream_t s);
 

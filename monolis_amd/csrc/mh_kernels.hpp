@@ -88,30 +88,27 @@ __device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane,
   }
 }
 
-// Kernel variants of the SELL row loop: U slots per step (loads in flight per
-// wave vs. VGPR budget) and temporal / non-temporal matrix streams.  Variant 0
-// is the default; monolis_hip_matrix_autotune times the others per matrix.
+// Kernel variants of the SELL SpMV.  A variant fixes
+//   unroll: slots per step in sell_row (loads in flight per wave vs. VGPRs), and
+//   wpc:    0 = one wavefront per slice over a grid covering every slice, or
+//           w > 0 = a persistent grid of w wavefronts per CU, each wavefront
+//           walking a contiguous range of slices (fewer, longer HBM streams).
+// Matrix streams are always non-temporal.  Variant 0 is the default before
+// tuning; monolis_hip_matrix_autotune times all of them per matrix.
 struct SellVariant {
   int unroll;
-  bool nt;
+  int wpc;
 };
-constexpr int kNumVariants = 8;
+constexpr int kNumVariants = 16;
 __host__ __device__ constexpr SellVariant sell_variant(int v) {
-  return v == 1 ? SellVariant{1, true}
-       : v == 2 ? SellVariant{2, true}
-       : v == 3 ? SellVariant{4, true}
-       : v == 4 ? SellVariant{3, false}
-       : v == 5 ? SellVariant{2, false}
-       : v == 6 ? SellVariant{4, false}
-       : v == 7 ? SellVariant{1, false}
-                : SellVariant{3, true};
+  constexpr int u[kNumVariants] = {3, 2, 4, 1, 3, 2, 4, 1, 3, 2, 4, 3, 2, 4, 3, 4};
+  constexpr int w[kNumVariants] = {8, 0, 0, 0, 0, 8, 8, 8, 4, 4, 4, 12, 12, 12, 16, 16};
+  return SellVariant{u[v], w[v]};
 }
 // B > 3 blocks carry B*B values per slot: cap the unroll to bound VGPRs.
-template <int B, int V>
+template <int B, int U0>
 struct SellCfg {
-  static constexpr int U0 = sell_variant(V).unroll;
   static constexpr int U = B == 1 ? U0 * 2 : B <= 3 ? U0 : B <= 5 ? (U0 < 2 ? U0 : 2) : 1;
-  static constexpr bool NT = sell_variant(V).nt;
 };
 
 // Block-Jacobi apply on one block row, in place.  D is the factored diagonal
@@ -151,77 +148,80 @@ __device__ __forceinline__ void diag_apply(const TD* __restrict__ D, int nrows, 
   }
 }
 
-// Generic fused SpMV kernel: one wavefront per 64-row slice, one lane per block
-// row.  `Op` supplies
+// Generic fused SpMV kernel: one lane per block row, one wavefront per 64-row
+// slice at a time.  With spw == 0 wavefront w of the grid handles slice w; with
+// spw == 1 the nwg*4 wavefronts split the slices into contiguous ranges.
+// Logical workgroups are laid out XCD by XCD (xcd_tile), so each XCD streams one
+// contiguous part of the matrix.  `Op` supplies
 //   bool skip()                                  whole launch is a no-op (solver done)
+//   void begin()                                 per-thread setup (reads device scalars)
 //   void gather(int col, TA (&xv)[B])            input vector at a block column
 //   void epilogue(int row, bool valid, const TA (&acc)[B], double (&part)[NV])
 //   void finalize(const double (&tot)[NV])       run once by k_finalize (thread 0)
 // and NV >= 0 partial sums per workgroup, stored for the finalize kernel.
-template <typename TV, typename TA, int B, int V, class Op>
-__global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op, double* partials,
-                                                 unsigned* ticket, int ntiles) {
-  const int tile = xcd_tile(ntiles);
-  if (tile < 0) return;
+template <typename TV, typename TA, int B, int U, class Op>
+__global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op0, double* partials, int nwg, int spw) {
+  const int wg = xcd_tile(nwg);
+  if (wg < 0) return;
+  Op op = op0;
   if (op.skip()) return;
+  op.begin();
   const int lane = threadIdx.x & (kWave - 1);
-  const int s = __builtin_amdgcn_readfirstlane(tile * kWpb + (int)(threadIdx.x / kWave));
-  const int row = s * kWave + lane;
-  const bool valid = (s < M.nslices) && (row < M.nrows);
-  int len = 0;
-  if (valid) len = M.rowptr[row + 1] - M.rowptr[row];
-  TA acc[B];
-#pragma unroll
-  for (int r = 0; r < B; ++r) acc[r] = (TA)0;
-  if (s < M.nslices) {
-    auto xg = [&](int c, TA(&xv)[B]) { op.gather(c, xv); };
-    sell_row<TV, TA, B, SellCfg<B, V>::U, SellCfg<B, V>::NT>(M, s, lane, len, xg, acc);
+  const int lw = __builtin_amdgcn_readfirstlane(wg * kWpb + (int)(threadIdx.x / kWave));
+  int s0 = lw, s1 = lw + 1;
+  if (spw) {
+    const long long W = (long long)nwg * kWpb;
+    s0 = (int)((long long)lw * M.nslices / W);
+    s1 = (int)((long long)(lw + 1) * M.nslices / W);
   }
+  if (s1 > M.nslices) s1 = M.nslices;
   constexpr int NV = Op::NV;
-  if constexpr (NV == 0) {
-    double dummy[1];
-    op.epilogue(row, valid, acc, dummy);
-  } else {
-    double part[NV];
+  double part[NV > 0 ? NV : 1];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) part[k] = 0.0;
+  for (int k = 0; k < (NV > 0 ? NV : 1); ++k) part[k] = 0.0;
+  for (int s = s0; s < s1; ++s) {
+    const int row = s * kWave + lane;
+    const bool valid = row < M.nrows;
+    int len = 0;
+    if (valid) len = M.rowptr[row + 1] - M.rowptr[row];
+    TA acc[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) acc[r] = (TA)0;
+    auto xg = [&](int c, TA(&xv)[B]) { op.gather(c, xv); };
+    sell_row<TV, TA, B, SellCfg<B, U>::U, true>(M, s, lane, len, xg, acc);
     op.epilogue(row, valid, acc, part);
+  }
+  if constexpr (NV > 0) {
     block_sum<NV>(part);
-    store_partials<NV>(part, partials, tile);
+    store_partials<NV>(part, partials, wg);
   }
 }
 
-// Dispatches a runtime variant index to the k_sell instantiation.
-#define MH_SELL_VARIANTS(v, ...)                                    \
-  switch (v) {                                                      \
-    case 1: { constexpr int V = 1; __VA_ARGS__; } break;            \
-    case 2: { constexpr int V = 2; __VA_ARGS__; } break;            \
-    case 3: { constexpr int V = 3; __VA_ARGS__; } break;            \
-    case 4: { constexpr int V = 4; __VA_ARGS__; } break;            \
-    case 5: { constexpr int V = 5; __VA_ARGS__; } break;            \
-    case 6: { constexpr int V = 6; __VA_ARGS__; } break;            \
-    case 7: { constexpr int V = 7; __VA_ARGS__; } break;            \
-    default: { constexpr int V = 0; __VA_ARGS__; } break;           \
+// Dispatches a runtime unroll to the k_sell instantiation.
+#define MH_SELL_UNROLL(u, ...)                                      \
+  switch (u) {                                                      \
+    case 1: { constexpr int U = 1; __VA_ARGS__; } break;            \
+    case 2: { constexpr int U = 2; __VA_ARGS__; } break;            \
+    case 4: { constexpr int U = 4; __VA_ARGS__; } break;            \
+    default: { constexpr int U = 3; __VA_ARGS__; } break;           \
   }
 
 // Row-parallel (non-SpMV) fused vector kernel: one thread per block row.
-//   void row(int i, double (&part)[NV]);  void finalize(const double (&tot)[NV]);  bool skip();
+//   bool skip(); void begin(); void row(int i, double (&part)[NV]); void finalize(...)
 template <class Op>
-__global__ void __launch_bounds__(kBlock) k_rows(int nrows, Op op, double* partials, unsigned* ticket,
-                                                 int ntiles) {
+__global__ void __launch_bounds__(kBlock) k_rows(int nrows, Op op0, double* partials, int ntiles) {
   const int tile = xcd_tile(ntiles);
   if (tile < 0) return;
+  Op op = op0;
   if (op.skip()) return;
+  op.begin();
   const int i = tile * kBlock + threadIdx.x;
   constexpr int NV = Op::NV;
-  if constexpr (NV == 0) {
-    double dummy[1];
-    if (i < nrows) op.row(i, dummy);
-  } else {
-    double part[NV];
+  double part[NV > 0 ? NV : 1];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) part[k] = 0.0;
-    if (i < nrows) op.row(i, part);
+  for (int k = 0; k < (NV > 0 ? NV : 1); ++k) part[k] = 0.0;
+  if (i < nrows) op.row(i, part);
+  if constexpr (NV > 0) {
     block_sum<NV>(part);
     store_partials<NV>(part, partials, tile);
   }
@@ -230,7 +230,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(int nrows, Op op, double* parti
 // Second phase of a reduction: one workgroup, fixed-order sum, op.finalize().
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_finalize(Op op, const double* partials, int ntiles) {
-  if (op.skip()) return;
+  if (op.skip()) return;  // every thread reads the same flag: uniform
   constexpr int NV = Op::NV;
   double tot[NV];
   sum_partials<NV>(partials, ntiles, tot);

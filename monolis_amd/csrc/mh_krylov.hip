@@ -17,7 +17,7 @@
 #include <deque>
 
 #include "mh_internal.hpp"
-#include "mh_kernels.hpp"
+#include "mh_launch.hpp"
 
 namespace mh {
 
@@ -77,31 +77,46 @@ struct OpResidSolve {
   }
 };
 
-// ---- CG: q = A p with p = z + beta p_old formed on the fly; (p, q) -> alpha ---
+// ---- CG: p = z + beta p (CG.f90:80-84; first iteration beta = 0, p = z) -------
 template <typename T, int B>
-struct OpCgSpmv {
-  static constexpr int NV = 1;
+struct OpCgP {
+  static constexpr int NV = 0;
   const T* __restrict__ z;
-  const T* __restrict__ pold;
-  T* __restrict__ pnew;
-  T* __restrict__ q;
+  T* __restrict__ p;
   KState* st;
   T beta;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() { beta = (T)st->beta; }
+  __device__ void row(int i, double (&)[1]) const {
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const size_t j = (size_t)i * B + k;
+      p[j] = fma(beta, p[j], z[j]);  // P = beta*P + 1*Z
+    }
+  }
+  __device__ void finalize(const double (&)[1]) const {}
+};
+
+// ---- CG: q = A p; (p, q) -> alpha = rho / (p, q) (CG.f90:86-88) -----------------
+template <typename T, int B>
+struct OpCgSpmv {
+  static constexpr int NV = 1;
+  const T* __restrict__ p;
+  T* __restrict__ q;
+  KState* st;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
   __device__ void gather(int c, T (&xv)[B]) const {
 #pragma unroll
-    for (int k = 0; k < B; ++k) xv[k] = fma(beta, pold[(size_t)c * B + k], z[(size_t)c * B + k]);
+    for (int k = 0; k < B; ++k) xv[k] = p[(size_t)c * B + k];
   }
   __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[1]) const {
     if (!valid) return;
 #pragma unroll
     for (int k = 0; k < B; ++k) {
       const size_t i = (size_t)row * B + k;
-      const T p = fma(beta, pold[i], z[i]);
-      pnew[i] = p;
       q[i] = acc[k];
-      part[0] += (double)p * (double)acc[k];
+      part[0] += (double)p[i] * (double)acc[k];
     }
   }
   __device__ void finalize(const double (&tot)[1]) const {
@@ -353,97 +368,6 @@ struct OpBiE {
   }
 };
 
-// k_sell / k_rows take the op by value; these wrappers call begin() per thread.
-template <class Op>
-struct Begun : Op {};
-
-template <typename TV, typename TA, int B, int V, class Op>
-__global__ void __launch_bounds__(kBlock) k_sell_b(SellView<TV> M, Op op0, double* partials, unsigned* ticket,
-                                                   int ntiles) {
-  const int tile = xcd_tile(ntiles);
-  if (tile < 0) return;
-  Op op = op0;
-  if (op.skip()) return;
-  op.begin();
-  const int lane = threadIdx.x & (kWave - 1);
-  const int s = __builtin_amdgcn_readfirstlane(tile * kWpb + (int)(threadIdx.x / kWave));
-  const int row = s * kWave + lane;
-  const bool valid = (s < M.nslices) && (row < M.nrows);
-  int len = 0;
-  if (valid) len = M.rowptr[row + 1] - M.rowptr[row];
-  TA acc[B];
-#pragma unroll
-  for (int r = 0; r < B; ++r) acc[r] = (TA)0;
-  if (s < M.nslices) {
-    auto xg = [&](int c, TA(&xv)[B]) { op.gather(c, xv); };
-    sell_row<TV, TA, B, SellCfg<B, V>::U, SellCfg<B, V>::NT>(M, s, lane, len, xg, acc);
-  }
-  constexpr int NV = Op::NV;
-  double part[NV > 0 ? NV : 1];
-#pragma unroll
-  for (int k = 0; k < (NV > 0 ? NV : 1); ++k) part[k] = 0.0;
-  op.epilogue(row, valid, acc, part);
-  if constexpr (NV > 0) {
-    block_sum<NV>(part);
-    store_partials<NV>(part, partials, tile);
-  }
-}
-
-template <class Op>
-__global__ void __launch_bounds__(kBlock) k_rows_b(int nrows, Op op0, double* partials, unsigned* ticket,
-                                                   int ntiles) {
-  const int tile = xcd_tile(ntiles);
-  if (tile < 0) return;
-  Op op = op0;
-  if (op.skip()) return;
-  op.begin();
-  const int i = tile * kBlock + threadIdx.x;
-  constexpr int NV = Op::NV;
-  double part[NV > 0 ? NV : 1];
-#pragma unroll
-  for (int k = 0; k < (NV > 0 ? NV : 1); ++k) part[k] = 0.0;
-  if (i < nrows) op.row(i, part);
-  if constexpr (NV > 0) {
-    block_sum<NV>(part);
-    store_partials<NV>(part, partials, tile);
-  }
-}
-
-// Solver SpMVs use the matrix's tuned variant for their precision when `tuned`
-// is set (the per-iteration q = A p / v = A pt / t = A st), else the default.
-template <typename T, int B, class Op>
-int run_sell(const Matrix& M, const Op& op, double* partials, unsigned* ticket, hipStream_t s,
-             bool tuned = false) {
-  const int ntiles = (M.nslices + kWpb - 1) / kWpb;
-  if (ntiles == 0) return 0;
-  const int var = tuned ? (sizeof(T) == 8 ? M.variant64 : M.variant32) : 0;
-  if constexpr (B == 1 || B == 3) {
-    MH_SELL_VARIANTS(var, k_sell_b<T, T, B, V, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(
-                              view<T>(M), op, partials, ticket, ntiles));
-  } else {
-    k_sell_b<T, T, B, 0, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(view<T>(M), op, partials, ticket, ntiles);
-  }
-  MH_HIP(hipGetLastError());
-  if constexpr (Op::NV > 0) {
-    k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, ntiles);
-    MH_HIP(hipGetLastError());
-  }
-  return 0;
-}
-
-template <class Op>
-int run_rows(int nrows, const Op& op, double* partials, unsigned* ticket, hipStream_t s) {
-  const int ntiles = (nrows + kBlock - 1) / kBlock;
-  if (ntiles == 0) return 0;
-  k_rows_b<Op><<<padded_grid(ntiles), kBlock, 0, s>>>(nrows, op, partials, ticket, ntiles);
-  MH_HIP(hipGetLastError());
-  if constexpr (Op::NV > 0) {
-    k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, ntiles);
-    MH_HIP(hipGetLastError());
-  }
-  return 0;
-}
-
 __global__ void k_state_init(KState* st, double tol, int max_iter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     KState z;
@@ -516,31 +440,29 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
   Carve cv{W.raw.p};
   T* r = cv.take<T>(vec);
   T* z = cv.take<T>(vec);
-  T* q = cv.take<T>(vec);
-  T* p0 = cv.take<T>(vec);
-  T* p1 = cv.take<T>(vec);
+  T* q = cv.take<T>(vec);  // p follows
+  T* p = cv.take<T>(vec);
   const size_t nparts = partials_needed((long long)M.n) * 4 + partials_needed((long long)M.nslices * kWave) * 4;
   double* parts = cv.take<double>(nparts);
   KState* st = cv.take<KState>(1);
   double* hist = cv.take<double>((size_t)P.max_iter + 2);
-  T* pbuf[2] = {p0, p1};
   const T* D = nullptr;
   if constexpr (PREC) D = (sizeof(T) == 8) ? (const T*)M.D64.p : (const T*)M.D32.p;
   const int n = M.n;
   const size_t m = (size_t)n * B;
+  const int var = sizeof(T) == 8 ? M.variant64 : M.variant32;
 
   k_state_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
   MH_HIP(hipGetLastError());
   if (P.init_x) MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
-  MH_HIP(hipMemsetAsync(p0, 0, sizeof(T) * vec, s));
-  MH_HIP(hipMemsetAsync(p1, 0, sizeof(T) * vec, s));
+  MH_HIP(hipMemsetAsync(p, 0, sizeof(T) * vec, s));
   if (vec > m) {  // halo part of z stays defined for gathers
     MH_HIP(hipMemsetAsync(z + m, 0, sizeof(T) * (vec - m), s));
   }
   // R = B - A X; B2 = (R, R)   (CG.f90:61-63, mat_converge.f90:38)
   {
     OpResidSolve<T, B> op{x, b, r, nullptr, &st->B2, st};
-    MH_TRY((run_sell<T, B>(M, op, parts, &st->ticket[0], s)));
+    MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
   }
   double B2 = 0.0;
   MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -557,8 +479,8 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
   }
   // first z = M^-1 r, rho_1
   auto upd = [&](int it, int flags) -> int {
-    OpCgUpdate<T, B, PREC> op{x, r, z, pbuf[it & 1], q, D, n, st, P.keep_hist ? hist : nullptr, it, flags, (T)0};
-    return run_rows(n, op, parts, &st->ticket[1], s);
+    OpCgUpdate<T, B, PREC> op{x, r, z, p, q, D, n, st, P.keep_hist ? hist : nullptr, it, flags, (T)0};
+    return launch_rows(n, op, parts, s);
   };
   MH_TRY(upd(0, F_FIRST));
   Poller poll{c, st};
@@ -566,12 +488,14 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
   for (int it = 1; it <= P.max_iter && !done;) {
     const int end = std::min(it + P.chunk - 1, P.max_iter);
     for (; it <= end; ++it) {
-      OpCgSpmv<T, B> sp{z, pbuf[(it + 1) & 1], pbuf[it & 1], q, st, (T)0};
-      MH_TRY((run_sell<T, B>(M, sp, parts, &st->ticket[2], s, true)));
+      OpCgP<T, B> pu{z, p, st, (T)0};
+      MH_TRY(launch_rows(n, pu, parts, s));
+      OpCgSpmv<T, B> sp{p, q, st};
+      MH_TRY((launch_sell<T, B>(M, sp, parts, var, s)));
       if (it % kIterRR == 0) {
         MH_TRY(upd(it, F_XONLY));
         OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
-        MH_TRY((run_sell<T, B>(M, rs, parts, &st->ticket[3], s)));
+        MH_TRY((launch_sell<T, B>(M, rs, parts, var, s)));
         MH_TRY(upd(it, F_CHECK));
       } else {
         MH_TRY(upd(it, F_UPD | F_CHECK));
@@ -618,6 +542,7 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   if constexpr (PREC) D = (sizeof(T) == 8) ? (const T*)M.D64.p : (const T*)M.D32.p;
   const int n = M.n;
   const size_t m = (size_t)n * B;
+  const int var = sizeof(T) == 8 ? M.variant64 : M.variant32;
 
   k_state_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
   MH_HIP(hipGetLastError());
@@ -630,7 +555,7 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   // R = B - A X, RT = R, B2 = (R, R); rho_1 = (R, RT) = B2
   {
     OpResidSolve<T, B> op{x, b, r, rt, &st->B2, st};
-    MH_TRY((run_sell<T, B>(M, op, parts, &st->ticket[0], s)));
+    MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
   }
   double B2 = 0.0;
   MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -654,23 +579,23 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
     const int end = std::min(it + P.chunk - 1, P.max_iter);
     for (; it <= end; ++it) {
       OpBiA<T, B, PREC> a{r, p, v, pt, D, n, st, it == 1, (T)0, (T)0};
-      MH_TRY(run_rows(n, a, parts, &st->ticket[1], s));
+      MH_TRY(launch_rows(n, a, parts, s));
       OpBiB<T, B> bb{pt, v, rt, st};
-      MH_TRY((run_sell<T, B>(M, bb, parts, &st->ticket[2], s, true)));
+      MH_TRY((launch_sell<T, B>(M, bb, parts, var, s)));
       OpBiC<T, B, PREC> cc{r, v, sv, stv, D, n, st, (T)0};
-      MH_TRY(run_rows(n, cc, parts, &st->ticket[1], s));
+      MH_TRY(launch_rows(n, cc, parts, s));
       OpBiD<T, B> dd{stv, t, sv, st};
-      MH_TRY((run_sell<T, B>(M, dd, parts, &st->ticket[2], s, true)));
+      MH_TRY((launch_sell<T, B>(M, dd, parts, var, s)));
       if (it % kIterRR == 0) {
         OpBiE<T, B> e1{x, r, pt, stv, sv, t, rt, st, hp, it, F_XONLY, (T)0, (T)0};
-        MH_TRY(run_rows(n, e1, parts, &st->ticket[1], s));
+        MH_TRY(launch_rows(n, e1, parts, s));
         OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
-        MH_TRY((run_sell<T, B>(M, rs, parts, &st->ticket[3], s)));
+        MH_TRY((launch_sell<T, B>(M, rs, parts, var, s)));
         OpBiE<T, B> e2{x, r, pt, stv, sv, t, rt, st, hp, it, F_CHECK, (T)0, (T)0};
-        MH_TRY(run_rows(n, e2, parts, &st->ticket[1], s));
+        MH_TRY(launch_rows(n, e2, parts, s));
       } else {
         OpBiE<T, B> e{x, r, pt, stv, sv, t, rt, st, hp, it, F_UPD | F_CHECK, (T)0, (T)0};
-        MH_TRY(run_rows(n, e, parts, &st->ticket[1], s));
+        MH_TRY(launch_rows(n, e, parts, s));
       }
     }
     MH_TRY(poll.issue(s));
@@ -689,19 +614,6 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   }
   return 0;
 }
-
-#define MH_DISPATCH_B(b, ...)                                            \
-  switch (b) {                                                           \
-    case 1: { constexpr int B = 1; __VA_ARGS__; } break;                  \
-    case 2: { constexpr int B = 2; __VA_ARGS__; } break;                  \
-    case 3: { constexpr int B = 3; __VA_ARGS__; } break;                  \
-    case 4: { constexpr int B = 4; __VA_ARGS__; } break;                  \
-    case 5: { constexpr int B = 5; __VA_ARGS__; } break;                  \
-    case 6: { constexpr int B = 6; __VA_ARGS__; } break;                  \
-    case 7: { constexpr int B = 7; __VA_ARGS__; } break;                  \
-    case 8: { constexpr int B = 8; __VA_ARGS__; } break;                  \
-    default: set_error("unsupported block size %d", b); return -2;       \
-  }
 
 }  // namespace
 
@@ -777,8 +689,6 @@ int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, W
   MH_TRY(scal.alloc(nparts + 64));
   double* parts = scal.p;
   double* rr_dev = scal.p + nparts;
-  unsigned* ticket = (unsigned*)(scal.p + nparts + 8);
-  MH_HIP(hipMemsetAsync(ticket, 0, 64, s));
   MH_TRY(matrix_make_fp32(&M, s));
   if (P.precond == 1) {
     MH_TRY(diag_setup<double>(M, s));
@@ -789,7 +699,7 @@ int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, W
     MH_HIP(hipMemsetAsync(r32.p + m, 0, sizeof(float) * (vec - m), s));
     MH_HIP(hipMemsetAsync(d32.p + m, 0, sizeof(float) * (vec - m), s));
   }
-  MH_TRY(residual<double>(M, x, b, r64.p, parts, ticket, rr_dev, s));
+  MH_TRY(residual<double>(M, x, b, r64.p, parts, rr_dev, s));
   double rr = 0.0;
   MH_HIP(hipMemcpyAsync(&rr, rr_dev, sizeof(double), hipMemcpyDeviceToHost, s));
   MH_HIP(hipStreamSynchronize(s));
@@ -824,7 +734,7 @@ int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, W
     out->inner_total += ir.iter > 0 ? ir.iter : 0;
     k_add_scaled<<<grid_1d(m), kBlock, 0, s>>>(m, x, d32.p, nr);
     MH_HIP(hipGetLastError());
-    MH_TRY(residual<double>(M, x, b, r64.p, parts, ticket, rr_dev, s));
+    MH_TRY(residual<double>(M, x, b, r64.p, parts, rr_dev, s));
     MH_HIP(hipMemcpyAsync(&rr, rr_dev, sizeof(double), hipMemcpyDeviceToHost, s));
     MH_HIP(hipStreamSynchronize(s));
     resid = std::sqrt(rr / B2);

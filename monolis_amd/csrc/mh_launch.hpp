@@ -1,0 +1,82 @@
+// Launch helpers shared by the translation units: SELL SpMV variants and
+// row-parallel kernels, each followed by its reduction finalize kernel.
+#pragma once
+
+#include "mh_internal.hpp"
+#include "mh_kernels.hpp"
+
+namespace mh {
+
+// Compute units of the current device (context init reads it once).
+int cu_count();
+
+// Launch geometry of a SELL variant: logical workgroups and the
+// slices-per-wave mode (0: one slice per wavefront, 1: contiguous ranges).
+inline void sell_geometry(const Matrix& M, int variant, int* nwg, int* spw) {
+  const SellVariant v = sell_variant(variant);
+  const int full = (M.nslices + kWpb - 1) / kWpb;
+  if (v.wpc == 0) {
+    *nwg = full;
+    *spw = 0;
+  } else {
+    int g = cu_count() * v.wpc / kWpb;
+    if (g > full) g = full;
+    if (g < 1) g = 1;
+    *nwg = g;
+    *spw = 1;
+  }
+}
+
+// Partials a SELL launch of this matrix may write (per value of NV).
+inline size_t sell_partials(const Matrix& M) { return (size_t)(M.nslices + kWpb - 1) / kWpb + 8; }
+
+template <typename T, int B, class Op>
+int launch_sell(const Matrix& M, const Op& op, double* partials, int variant, hipStream_t s) {
+  int nwg = 0, spw = 0;
+  if (M.nslices > 0) {
+    sell_geometry(M, variant, &nwg, &spw);
+    const int u = sell_variant(variant).unroll;
+    if constexpr (B == 1 || B == 3) {
+      MH_SELL_UNROLL(u, k_sell<T, T, B, U, Op><<<padded_grid(nwg), kBlock, 0, s>>>(view<T>(M), op, partials,
+                                                                                   nwg, spw));
+    } else {
+      (void)u;
+      k_sell<T, T, B, 3, Op><<<padded_grid(nwg), kBlock, 0, s>>>(view<T>(M), op, partials, nwg, spw);
+    }
+    MH_HIP(hipGetLastError());
+  }
+  if constexpr (Op::NV > 0) {
+    k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, nwg);
+    MH_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+template <class Op>
+int launch_rows(int nrows, const Op& op, double* partials, hipStream_t s) {
+  const int ntiles = (nrows + kBlock - 1) / kBlock;
+  if (ntiles > 0) {
+    k_rows<Op><<<padded_grid(ntiles), kBlock, 0, s>>>(nrows, op, partials, ntiles);
+    MH_HIP(hipGetLastError());
+  }
+  if constexpr (Op::NV > 0) {
+    k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, ntiles);
+    MH_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+#define MH_DISPATCH_B(b, ...)                                            \
+  switch (b) {                                                           \
+    case 1: { constexpr int B = 1; __VA_ARGS__; } break;                  \
+    case 2: { constexpr int B = 2; __VA_ARGS__; } break;                  \
+    case 3: { constexpr int B = 3; __VA_ARGS__; } break;                  \
+    case 4: { constexpr int B = 4; __VA_ARGS__; } break;                  \
+    case 5: { constexpr int B = 5; __VA_ARGS__; } break;                  \
+    case 6: { constexpr int B = 6; __VA_ARGS__; } break;                  \
+    case 7: { constexpr int B = 7; __VA_ARGS__; } break;                  \
+    case 8: { constexpr int B = 8; __VA_ARGS__; } break;                  \
+    default: set_error("unsupported block size %d", b); return -2;       \
+  }
+
+}  // namespace mh

@@ -6,7 +6,7 @@
 #include <vector>
 
 #include "mh_internal.hpp"
-#include "mh_kernels.hpp"
+#include "mh_launch.hpp"
 
 namespace mh {
 
@@ -26,6 +26,9 @@ void clear_error() { g_err[0] = 0; }
 
 static Context g_ctx;
 static std::mutex g_ctx_mu;
+static int g_cus = 256;
+
+int cu_count() { return g_cus; }
 
 int context(Context** out) {
   std::lock_guard<std::mutex> lk(g_ctx_mu);
@@ -37,6 +40,7 @@ int context(Context** out) {
       return -1;
     }
     MH_HIP(hipGetDevice(&g_ctx.device));
+    MH_HIP(hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, g_ctx.device));
     MH_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     MH_HIP(hipHostMalloc((void**)&g_ctx.h_flags, sizeof(int) * kFlagSlots, hipHostMallocDefault));
     for (int i = 0; i < kFlagSlots; ++i) MH_HIP(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
@@ -332,6 +336,7 @@ struct OpSpmv {
   const T* __restrict__ x;
   T* __restrict__ y;
   __device__ bool skip() const { return false; }
+  __device__ void begin() {}
   __device__ void gather(int c, T (&xv)[B]) const {
 #pragma unroll
     for (int k = 0; k < B; ++k) xv[k] = x[(size_t)c * B + k];
@@ -353,6 +358,7 @@ struct OpResid {
   T* __restrict__ r;
   double* out;
   __device__ bool skip() const { return false; }
+  __device__ void begin() {}
   __device__ void gather(int c, T (&xv)[B]) const {
 #pragma unroll
     for (int k = 0; k < B; ++k) xv[k] = x[(size_t)c * B + k];
@@ -373,38 +379,6 @@ struct OpResid {
   }
 };
 
-template <typename TV, typename T, int B, class Op>
-static int launch_sell(const Matrix& M, const Op& op, double* partials, unsigned* ticket, hipStream_t s,
-                       int variant = 0) {
-  const int ntiles = (M.nslices + kWpb - 1) / kWpb;
-  if (ntiles == 0) return 0;
-  if constexpr (B == 1 || B == 3) {
-    MH_SELL_VARIANTS(variant, k_sell<TV, T, B, V, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(
-                                  view<TV>(M), op, partials, ticket, ntiles));
-  } else {
-    k_sell<TV, T, B, 0, Op><<<padded_grid(ntiles), kBlock, 0, s>>>(view<TV>(M), op, partials, ticket, ntiles);
-  }
-  MH_HIP(hipGetLastError());
-  if constexpr (Op::NV > 0) {
-    k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, ntiles);
-    MH_HIP(hipGetLastError());
-  }
-  return 0;
-}
-
-#define MH_DISPATCH_B(b, ...)                                            \
-  switch (b) {                                                           \
-    case 1: { constexpr int B = 1; __VA_ARGS__; } break;                  \
-    case 2: { constexpr int B = 2; __VA_ARGS__; } break;                  \
-    case 3: { constexpr int B = 3; __VA_ARGS__; } break;                  \
-    case 4: { constexpr int B = 4; __VA_ARGS__; } break;                  \
-    case 5: { constexpr int B = 5; __VA_ARGS__; } break;                  \
-    case 6: { constexpr int B = 6; __VA_ARGS__; } break;                  \
-    case 7: { constexpr int B = 7; __VA_ARGS__; } break;                  \
-    case 8: { constexpr int B = 8; __VA_ARGS__; } break;                  \
-    default: set_error("unsupported block size %d", b); return -2;       \
-  }
-
 template <typename T>
 int spmv_variant(const Matrix& M, const T* x, T* y, int variant, hipStream_t s) {
   if (sizeof(T) == 4 && !M.val32_ready) {
@@ -417,7 +391,7 @@ int spmv_variant(const Matrix& M, const T* x, T* y, int variant, hipStream_t s) 
   }
   MH_DISPATCH_B(M.b, {
     OpSpmv<T, B> op{x, y};
-    MH_TRY((launch_sell<T, T, B>(M, op, nullptr, nullptr, s, variant)));
+    MH_TRY((launch_sell<T, B>(M, op, nullptr, variant, s)));
   });
   return 0;
 }
@@ -432,18 +406,16 @@ template int spmv<double>(const Matrix&, const double*, double*, hipStream_t);
 template int spmv<float>(const Matrix&, const float*, float*, hipStream_t);
 
 template <typename T>
-int residual(const Matrix& M, const T* x, const T* b, T* r, double* partials, unsigned* ticket,
-             double* rr_out, hipStream_t s) {
+int residual(const Matrix& M, const T* x, const T* b, T* r, double* partials, double* rr_out, hipStream_t s) {
   MH_DISPATCH_B(M.b, {
     OpResid<T, B> op{x, b, r, rr_out};
-    MH_TRY((launch_sell<T, T, B>(M, op, partials, ticket, s)));
+    MH_TRY((launch_sell<T, B>(M, op, partials, sizeof(T) == 8 ? M.variant64 : M.variant32, s)));
   });
   return 0;
 }
-template int residual<double>(const Matrix&, const double*, const double*, double*, double*, unsigned*,
-                              double*, hipStream_t);
-template int residual<float>(const Matrix&, const float*, const float*, float*, double*, unsigned*, double*,
-                             hipStream_t);
+template int residual<double>(const Matrix&, const double*, const double*, double*, double*, double*,
+                              hipStream_t);
+template int residual<float>(const Matrix&, const float*, const float*, float*, double*, double*, hipStream_t);
 
 // ---------------------------------------------------------------------------
 // BLAS-1: deterministic dot, AXPBY
@@ -455,30 +427,22 @@ struct OpDot {
   const T* __restrict__ y;
   double* out;
   __device__ bool skip() const { return false; }
+  __device__ void begin() {}
   __device__ void row(int i, double (&part)[1]) const { part[0] += (double)x[i] * (double)y[i]; }
   __device__ void finalize(const double (&tot)[1]) const { *out = tot[0]; }
 };
 
 template <typename T>
-int dot(long long m, const T* x, const T* y, double* out, double* partials, unsigned* ticket, hipStream_t s) {
+int dot(long long m, const T* x, const T* y, double* out, double* partials, hipStream_t s) {
   if (m > 0x7fffffffLL) {
     set_error("vector too long");
     return -2;
   }
-  const int ntiles = (int)((m + kBlock - 1) / kBlock);
-  if (ntiles == 0) {
-    MH_HIP(hipMemsetAsync(out, 0, sizeof(double), s));
-    return 0;
-  }
   OpDot<T> op{x, y, out};
-  k_rows<OpDot<T>><<<padded_grid(ntiles), kBlock, 0, s>>>((int)m, op, partials, ticket, ntiles);
-  MH_HIP(hipGetLastError());
-  k_finalize<OpDot<T>><<<1, kBlock, 0, s>>>(op, partials, ntiles);
-  MH_HIP(hipGetLastError());
-  return 0;
+  return launch_rows((int)m, op, partials, s);
 }
-template int dot<double>(long long, const double*, const double*, double*, double*, unsigned*, hipStream_t);
-template int dot<float>(long long, const float*, const float*, double*, double*, unsigned*, hipStream_t);
+template int dot<double>(long long, const double*, const double*, double*, double*, hipStream_t);
+template int dot<float>(long long, const float*, const float*, double*, double*, hipStream_t);
 
 template <typename T>
 __global__ void k_axpby(long long m, T alpha, const T* __restrict__ x, T beta, const T* __restrict__ y,
