@@ -42,6 +42,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU SpMV sample")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="force this SpMV kernel variant instead of running the autotuner")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the streaming-copy probe (1 GiB read + 1 GiB write per launch)")
     args = ap.parse_args()
@@ -89,9 +91,14 @@ def main() -> int:
     log(f"uploaded; GPU layout {info['bytes_fp64'] / 1e9:.3f} GB ({info['slots']} slots)")
 
     # --- autotuner (outside the timed region): per-matrix SELL kernel variant ---
-    tune = dm.autotune(fp32=True, reps=10)
-    log(f"autotune: FP64 variant {tune['best64']} ({min(tune['ms64']) * 1e3:.1f} us), "
-        f"FP32 variant {tune['best32']} ({min(tune['ms32']) * 1e3:.1f} us)")
+    if args.variant >= 0:
+        dm.set_variant(args.variant, fp32=False)
+        dm.set_variant(args.variant, fp32=True)
+        tune = {"best64": args.variant, "best32": args.variant, "ms64": [], "ms32": []}
+    else:
+        tune = dm.autotune(fp32=True, reps=10)
+        log(f"autotune: FP64 variant {tune['best64']} ({min(tune['ms64']) * 1e3:.1f} us), "
+            f"FP32 variant {tune['best32']} ({min(tune['ms32']) * 1e3:.1f} us)")
 
     # --- headline: SpMV, HIP events around exactly K launches -------------------
     _lib.check(L.monolis_hip_synchronize(), "sync")

@@ -111,6 +111,66 @@ struct SellCfg {
   static constexpr int U = B == 1 ? U0 * 2 : B <= 3 ? U0 : B <= 5 ? (U0 < 2 ? U0 : 2) : 1;
 };
 
+// Wave-local ordering of LDS accesses between lanes of one wavefront (the LDS
+// region is private to the wavefront, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-row vector I/O of one wavefront covering 64 consecutive block rows
+// (row0 .. row0+63).  A lane owns one block row, i.e. B consecutive entries, so
+// direct per-lane access would be a 8*B-byte strided pattern that writes
+// partial cache lines.  For B > 1 the values go through a 64*B LDS tile and
+// are read / written by the wavefront as B contiguous 64-entry runs.  Every
+// lane of the wavefront must call load/store (rows >= nrows are skipped).
+template <typename T, int B>
+struct WaveRows {
+  T* lds;     // 64*B entries private to this wavefront (unused for B == 1)
+  int row0;
+  int nrows;
+  int lane;
+  __device__ __forceinline__ int count() const {
+    const int c = nrows - row0;
+    return (c < kWave ? c : kWave) * B;
+  }
+  __device__ __forceinline__ void load(const T* __restrict__ src, T (&v)[B]) const {
+    if constexpr (B == 1) {
+      v[0] = (row0 + lane < nrows) ? src[(size_t)row0 + lane] : (T)0;
+    } else {
+      const int cnt = count();
+      const T* base = src + (size_t)row0 * B;
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        const int e = j * kWave + lane;
+        if (e < cnt) lds[e] = base[e];
+      }
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < B; ++k) v[k] = lds[lane * B + k];
+      wave_sync();
+    }
+  }
+  __device__ __forceinline__ void store(T* __restrict__ dst, const T (&v)[B]) const {
+    if constexpr (B == 1) {
+      if (row0 + lane < nrows) dst[(size_t)row0 + lane] = v[0];
+    } else {
+      const int cnt = count();
+      T* base = dst + (size_t)row0 * B;
+#pragma unroll
+      for (int k = 0; k < B; ++k) lds[lane * B + k] = v[k];
+      wave_sync();
+#pragma unroll
+      for (int j = 0; j < B; ++j) {
+        const int e = j * kWave + lane;
+        if (e < cnt) base[e] = lds[e];
+      }
+      wave_sync();
+    }
+  }
+};
+
 // Block-Jacobi apply on one block row, in place.  D is the factored diagonal
 // block in structure-of-arrays form: entry e of row i at D[e * nrows + i].
 // The arithmetic order is that of diag_33.f90:123-135 for B = 3 and of
@@ -156,7 +216,8 @@ __device__ __forceinline__ void diag_apply(const TD* __restrict__ D, int nrows, 
 //   bool skip()                                  whole launch is a no-op (solver done)
 //   void begin()                                 per-thread setup (reads device scalars)
 //   void gather(int col, TA (&xv)[B])            input vector at a block column
-//   void epilogue(int row, bool valid, const TA (&acc)[B], double (&part)[NV])
+//   void epilogue(int row, bool valid, const TA (&acc)[B], double (&part)[NV],
+//                 const WaveRows<TA, B>& io)     (every lane calls it; vectors via io)
 //   void finalize(const double (&tot)[NV])       run once by k_finalize (thread 0)
 // and NV >= 0 partial sums per workgroup, stored for the finalize kernel.
 template <typename TV, typename TA, int B, int U, class Op>
@@ -179,6 +240,7 @@ __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op0, double*
   double part[NV > 0 ? NV : 1];
 #pragma unroll
   for (int k = 0; k < (NV > 0 ? NV : 1); ++k) part[k] = 0.0;
+  __shared__ TA lds[kWpb * kWave * B];
   for (int s = s0; s < s1; ++s) {
     const int row = s * kWave + lane;
     const bool valid = row < M.nrows;
@@ -189,7 +251,8 @@ __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op0, double*
     for (int r = 0; r < B; ++r) acc[r] = (TA)0;
     auto xg = [&](int c, TA(&xv)[B]) { op.gather(c, xv); };
     sell_row<TV, TA, B, SellCfg<B, U>::U, true>(M, s, lane, len, xg, acc);
-    op.epilogue(row, valid, acc, part);
+    const WaveRows<TA, B> io{lds + (threadIdx.x / kWave) * kWave * B, s * kWave, M.nrows, lane};
+    op.epilogue(row, valid, acc, part, io);
   }
   if constexpr (NV > 0) {
     block_sum<NV>(part);
@@ -207,7 +270,11 @@ __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op0, double*
   }
 
 // Row-parallel (non-SpMV) fused vector kernel: one thread per block row.
-//   bool skip(); void begin(); void row(int i, double (&part)[NV]); void finalize(...)
+//   using VT (vector type) and BS (block size);
+//   bool skip(); void begin();
+//   void row(int i, bool valid, double (&part)[NV], const WaveRows<VT, BS>& io)
+//       (every lane calls it; per-row vectors via io)
+//   void finalize(...)
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_rows(int nrows, Op op0, double* partials, int ntiles) {
   const int tile = xcd_tile(ntiles);
@@ -215,12 +282,19 @@ __global__ void __launch_bounds__(kBlock) k_rows(int nrows, Op op0, double* part
   Op op = op0;
   if (op.skip()) return;
   op.begin();
+  using VT = typename Op::VT;
+  constexpr int BS = Op::BS;
+  __shared__ VT lds[kWpb * kWave * BS];
   const int i = tile * kBlock + threadIdx.x;
   constexpr int NV = Op::NV;
   double part[NV > 0 ? NV : 1];
 #pragma unroll
   for (int k = 0; k < (NV > 0 ? NV : 1); ++k) part[k] = 0.0;
-  if (i < nrows) op.row(i, part);
+  const int row0 = tile * kBlock + (int)(threadIdx.x / kWave) * kWave;
+  if (row0 < nrows) {
+    const WaveRows<VT, BS> io{lds + (threadIdx.x / kWave) * kWave * BS, row0, nrows, (int)(threadIdx.x & (kWave - 1))};
+    op.row(i, i < nrows, part, io);
+  }
   if constexpr (NV > 0) {
     block_sum<NV>(part);
     store_partials<NV>(part, partials, tile);

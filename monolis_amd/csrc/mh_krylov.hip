@@ -62,15 +62,16 @@ struct OpResidSolve {
 #pragma unroll
     for (int k = 0; k < B; ++k) xv[k] = x[(size_t)c * B + k];
   }
-  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[1]) const {
-    if (!valid) return;
+  __device__ void epilogue(int, bool valid, const T (&acc)[B], double (&part)[1], const WaveRows<T, B>& io) const {
+    T v[B];
+    io.load(b, v);
 #pragma unroll
     for (int k = 0; k < B; ++k) {
-      const T v = b[(size_t)row * B + k] - acc[k];
-      r[(size_t)row * B + k] = v;
-      if (r2) r2[(size_t)row * B + k] = v;
-      part[0] += (double)v * (double)v;
+      v[k] = v[k] - acc[k];  // R = 1*B + (-1)*(A X)  (matvec.f90:901)
+      if (valid) part[0] += (double)v[k] * (double)v[k];
     }
+    io.store(r, v);
+    if (r2) io.store(r2, v);
   }
   __device__ void finalize(const double (&tot)[1]) const {
     if (out) *out = tot[0];
@@ -78,8 +79,11 @@ struct OpResidSolve {
 };
 
 // ---- CG: p = z + beta p (CG.f90:80-84; first iteration beta = 0, p = z) -------
-template <typename T, int B>
+// Elementwise: launched over n*B entries.
+template <typename T>
 struct OpCgP {
+  using VT = T;
+  static constexpr int BS = 1;
   static constexpr int NV = 0;
   const T* __restrict__ z;
   T* __restrict__ p;
@@ -87,12 +91,8 @@ struct OpCgP {
   T beta;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() { beta = (T)st->beta; }
-  __device__ void row(int i, double (&)[1]) const {
-#pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const size_t j = (size_t)i * B + k;
-      p[j] = fma(beta, p[j], z[j]);  // P = beta*P + 1*Z
-    }
+  __device__ void row(int i, bool valid, double (&)[1], const WaveRows<T, 1>&) const {
+    if (valid) p[i] = fma(beta, p[i], z[i]);  // P = beta*P + 1*Z
   }
   __device__ void finalize(const double (&)[1]) const {}
 };
@@ -110,13 +110,13 @@ struct OpCgSpmv {
 #pragma unroll
     for (int k = 0; k < B; ++k) xv[k] = p[(size_t)c * B + k];
   }
-  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[1]) const {
-    if (!valid) return;
+  __device__ void epilogue(int, bool valid, const T (&acc)[B], double (&part)[1], const WaveRows<T, B>& io) const {
+    io.store(q, acc);
+    T pv[B];
+    io.load(p, pv);
+    if (valid) {
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const size_t i = (size_t)row * B + k;
-      q[i] = acc[k];
-      part[0] += (double)p[i] * (double)acc[k];
+      for (int k = 0; k < B; ++k) part[0] += (double)pv[k] * (double)acc[k];
     }
   }
   __device__ void finalize(const double (&tot)[1]) const {
@@ -128,6 +128,8 @@ struct OpCgSpmv {
 // ---- CG: x += alpha p; r -= alpha q; test; z = M^-1 r; rho, beta -------------
 template <typename T, int B, bool PREC>
 struct OpCgUpdate {
+  using VT = T;
+  static constexpr int BS = B;
   static constexpr int NV = 2;
   T* __restrict__ x;
   T* __restrict__ r;
@@ -143,38 +145,37 @@ struct OpCgUpdate {
   T alpha;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() { alpha = (T)st->alpha; }
-  __device__ void row(int i, double (&part)[2]) const {
-    T v[B];
-#pragma unroll
-    for (int k = 0; k < B; ++k) v[k] = r[(size_t)i * B + k];
+  __device__ void row(int i, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
     if (flags & (F_UPD | F_XONLY)) {
+      T xv[B], pv[B];
+      io.load(x, xv);
+      io.load(p, pv);
 #pragma unroll
-      for (int k = 0; k < B; ++k) {
-        const size_t j = (size_t)i * B + k;
-        x[j] = fma(alpha, p[j], x[j]);  // X = alpha*P + X   (CG.f90:89)
-      }
+      for (int k = 0; k < B; ++k) xv[k] = fma(alpha, pv[k], xv[k]);  // X = alpha*P + X  (CG.f90:89)
+      io.store(x, xv);
     }
     if (flags & F_XONLY) return;
+    T v[B];
+    io.load(r, v);
     if (flags & F_UPD) {
+      T qv[B];
+      io.load(q, qv);
 #pragma unroll
-      for (int k = 0; k < B; ++k) {
-        const size_t j = (size_t)i * B + k;
-        v[k] = fma(-alpha, q[j], v[k]);  // R = -alpha*Q + R  (CG.f90:94)
-        r[j] = v[k];
-      }
+      for (int k = 0; k < B; ++k) v[k] = fma(-alpha, qv[k], v[k]);  // R = -alpha*Q + R  (CG.f90:94)
+      io.store(r, v);
     }
     T w[B];
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      w[k] = v[k];
-      part[0] += (double)v[k] * (double)v[k];
-    }
-    if constexpr (PREC) diag_apply<T, T, B>(D, n, i, w);
+    for (int k = 0; k < B; ++k) w[k] = v[k];
+    if (valid) {
+      if constexpr (PREC) diag_apply<T, T, B>(D, n, i, w);
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      z[(size_t)i * B + k] = w[k];
-      part[1] += (double)v[k] * (double)w[k];
+      for (int k = 0; k < B; ++k) {
+        part[0] += (double)v[k] * (double)v[k];
+        part[1] += (double)v[k] * (double)w[k];
+      }
     }
+    io.store(z, w);
   }
   __device__ void finalize(const double (&tot)[2]) const {
     if (flags & F_XONLY) return;
@@ -191,6 +192,8 @@ struct OpCgUpdate {
 // a: p = r + beta (p - omega v) (first: p = r); pt = M^-1 p
 template <typename T, int B, bool PREC>
 struct OpBiA {
+  using VT = T;
+  static constexpr int BS = B;
   static constexpr int NV = 0;
   const T* __restrict__ r;
   T* __restrict__ p;
@@ -206,18 +209,21 @@ struct OpBiA {
     beta = (T)st->beta;
     omega = (T)st->omega;
   }
-  __device__ void row(int i, double (&)[1]) const {
-    T w[B];
+  __device__ void row(int i, bool valid, double (&)[1], const WaveRows<T, B>& io) const {
+    T pv[B];
+    io.load(r, pv);
+    if (!first) {
+      T po[B], vv[B];
+      io.load(p, po);
+      io.load(v, vv);
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const size_t j = (size_t)i * B + k;
-      const T pv = first ? r[j] : r[j] + beta * (p[j] - omega * v[j]);  // BiCGSTAB.f90:103
-      p[j] = pv;
-      w[k] = pv;
+      for (int k = 0; k < B; ++k) pv[k] = pv[k] + beta * (po[k] - omega * vv[k]);  // BiCGSTAB.f90:103
     }
-    if constexpr (PREC) diag_apply<T, T, B>(D, n, i, w);
-#pragma unroll
-    for (int k = 0; k < B; ++k) pt[(size_t)i * B + k] = w[k];
+    io.store(p, pv);
+    if constexpr (PREC) {
+      if (valid) diag_apply<T, T, B>(D, n, i, pv);
+    }
+    io.store(pt, pv);
   }
   __device__ void finalize(const double (&)[1]) const {}
 };
@@ -236,13 +242,13 @@ struct OpBiB {
 #pragma unroll
     for (int k = 0; k < B; ++k) xv[k] = pt[(size_t)c * B + k];
   }
-  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[1]) const {
-    if (!valid) return;
+  __device__ void epilogue(int, bool valid, const T (&acc)[B], double (&part)[1], const WaveRows<T, B>& io) const {
+    io.store(v, acc);
+    T rtv[B];
+    io.load(rt, rtv);
+    if (valid) {
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const size_t j = (size_t)row * B + k;
-      v[j] = acc[k];
-      part[0] += (double)rt[j] * (double)acc[k];
+      for (int k = 0; k < B; ++k) part[0] += (double)rtv[k] * (double)acc[k];
     }
   }
   __device__ void finalize(const double (&tot)[1]) const {
@@ -256,6 +262,8 @@ struct OpBiB {
 // c: s = r - alpha v; st = M^-1 s
 template <typename T, int B, bool PREC>
 struct OpBiC {
+  using VT = T;
+  static constexpr int BS = B;
   static constexpr int NV = 0;
   const T* __restrict__ r;
   const T* __restrict__ v;
@@ -267,18 +275,17 @@ struct OpBiC {
   T alpha;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() { alpha = (T)st->alpha; }
-  __device__ void row(int i, double (&)[1]) const {
-    T w[B];
+  __device__ void row(int i, bool valid, double (&)[1], const WaveRows<T, B>& io) const {
+    T rv[B], vv[B];
+    io.load(r, rv);
+    io.load(v, vv);
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const size_t j = (size_t)i * B + k;
-      const T sv = fma(-alpha, v[j], r[j]);  // S = -alpha*V + R  (BiCGSTAB.f90:118)
-      s[j] = sv;
-      w[k] = sv;
+    for (int k = 0; k < B; ++k) rv[k] = fma(-alpha, vv[k], rv[k]);  // S = -alpha*V + R  (BiCGSTAB.f90:118)
+    io.store(s, rv);
+    if constexpr (PREC) {
+      if (valid) diag_apply<T, T, B>(D, n, i, rv);
     }
-    if constexpr (PREC) diag_apply<T, T, B>(D, n, i, w);
-#pragma unroll
-    for (int k = 0; k < B; ++k) st_[(size_t)i * B + k] = w[k];
+    io.store(st_, rv);
   }
   __device__ void finalize(const double (&)[1]) const {}
 };
@@ -297,14 +304,16 @@ struct OpBiD {
 #pragma unroll
     for (int k = 0; k < B; ++k) xv[k] = stv[(size_t)c * B + k];
   }
-  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[2]) const {
-    if (!valid) return;
+  __device__ void epilogue(int, bool valid, const T (&acc)[B], double (&part)[2], const WaveRows<T, B>& io) const {
+    io.store(t, acc);
+    T sv[B];
+    io.load(s, sv);
+    if (valid) {
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const size_t j = (size_t)row * B + k;
-      t[j] = acc[k];
-      part[0] += (double)acc[k] * (double)s[j];
-      part[1] += (double)acc[k] * (double)acc[k];
+      for (int k = 0; k < B; ++k) {
+        part[0] += (double)acc[k] * (double)sv[k];
+        part[1] += (double)acc[k] * (double)acc[k];
+      }
     }
   }
   __device__ void finalize(const double (&tot)[2]) const {
@@ -322,6 +331,8 @@ struct OpBiD {
 // e: x += alpha pt + omega st; r = s - omega t; test; rho = (r, rt); beta
 template <typename T, int B>
 struct OpBiE {
+  using VT = T;
+  static constexpr int BS = B;
   static constexpr int NV = 2;
   T* __restrict__ x;
   T* __restrict__ r;
@@ -340,21 +351,36 @@ struct OpBiE {
     alpha = (T)st->alpha;
     omega = (T)st->omega;
   }
-  __device__ void row(int i, double (&part)[2]) const {
+  __device__ void row(int, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
+    if (flags & (F_UPD | F_XONLY)) {
+      T xv[B], pv[B], sv[B];
+      io.load(x, xv);
+      io.load(pt, pv);
+      io.load(stv, sv);
 #pragma unroll
-    for (int k = 0; k < B; ++k) {
-      const size_t j = (size_t)i * B + k;
-      if (flags & (F_UPD | F_XONLY)) x[j] = x[j] + alpha * pt[j] + omega * stv[j];  // BiCGSTAB.f90:153
-      if (flags & F_XONLY) continue;
-      T rv;
-      if (flags & F_UPD) {
-        rv = fma(-omega, t[j], s[j]);  // R = -omega*T + S  (BiCGSTAB.f90:166)
-        r[j] = rv;
-      } else {
-        rv = r[j];
+      for (int k = 0; k < B; ++k) xv[k] = xv[k] + alpha * pv[k] + omega * sv[k];  // BiCGSTAB.f90:153
+      io.store(x, xv);
+    }
+    if (flags & F_XONLY) return;
+    T rv[B];
+    if (flags & F_UPD) {
+      T sv[B], tv[B];
+      io.load(s, sv);
+      io.load(t, tv);
+#pragma unroll
+      for (int k = 0; k < B; ++k) rv[k] = fma(-omega, tv[k], sv[k]);  // R = -omega*T + S  (BiCGSTAB.f90:166)
+      io.store(r, rv);
+    } else {
+      io.load(r, rv);
+    }
+    T rtv[B];
+    io.load(rt, rtv);
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        part[0] += (double)rv[k] * (double)rv[k];
+        part[1] += (double)rv[k] * (double)rtv[k];
       }
-      part[0] += (double)rv * (double)rv;
-      part[1] += (double)rv * (double)rt[j];
     }
   }
   __device__ void finalize(const double (&tot)[2]) const {
@@ -488,8 +514,8 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
   for (int it = 1; it <= P.max_iter && !done;) {
     const int end = std::min(it + P.chunk - 1, P.max_iter);
     for (; it <= end; ++it) {
-      OpCgP<T, B> pu{z, p, st, (T)0};
-      MH_TRY(launch_rows(n, pu, parts, s));
+      OpCgP<T> pu{z, p, st, (T)0};
+      MH_TRY(launch_rows((int)m, pu, parts, s));
       OpCgSpmv<T, B> sp{p, q, st};
       MH_TRY((launch_sell<T, B>(M, sp, parts, var, s)));
       if (it % kIterRR == 0) {

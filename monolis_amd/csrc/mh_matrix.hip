@@ -341,11 +341,8 @@ struct OpSpmv {
 #pragma unroll
     for (int k = 0; k < B; ++k) xv[k] = x[(size_t)c * B + k];
   }
-  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&)[1]) const {
-    if (valid) {
-#pragma unroll
-      for (int k = 0; k < B; ++k) y[(size_t)row * B + k] = acc[k];
-    }
+  __device__ void epilogue(int, bool, const T (&acc)[B], double (&)[1], const WaveRows<T, B>& io) const {
+    io.store(y, acc);
   }
   __device__ void finalize(const double (&)[1]) const {}
 };
@@ -363,16 +360,15 @@ struct OpResid {
 #pragma unroll
     for (int k = 0; k < B; ++k) xv[k] = x[(size_t)c * B + k];
   }
-  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[1]) const {
-    if (valid) {
+  __device__ void epilogue(int, bool valid, const T (&acc)[B], double (&part)[1], const WaveRows<T, B>& io) const {
+    T v[B];
+    io.load(b, v);
 #pragma unroll
-      for (int k = 0; k < B; ++k) {
-        // monolis_residual_main_R: R = 1*B + (-1)*(A X)   (matvec.f90:901)
-        const T v = b[(size_t)row * B + k] - acc[k];
-        r[(size_t)row * B + k] = v;
-        part[0] += (double)v * (double)v;
-      }
+    for (int k = 0; k < B; ++k) {
+      v[k] = v[k] - acc[k];  // monolis_residual_main_R: R = 1*B + (-1)*(A X)   (matvec.f90:901)
+      if (valid) part[0] += (double)v[k] * (double)v[k];
     }
+    io.store(r, v);
   }
   __device__ void finalize(const double (&tot)[1]) const {
     if (out) *out = tot[0];
@@ -422,13 +418,17 @@ template int residual<float>(const Matrix&, const float*, const float*, float*, 
 // ---------------------------------------------------------------------------
 template <typename T>
 struct OpDot {
+  using VT = T;
+  static constexpr int BS = 1;
   static constexpr int NV = 1;
   const T* __restrict__ x;
   const T* __restrict__ y;
   double* out;
   __device__ bool skip() const { return false; }
   __device__ void begin() {}
-  __device__ void row(int i, double (&part)[1]) const { part[0] += (double)x[i] * (double)y[i]; }
+  __device__ void row(int i, bool valid, double (&part)[1], const WaveRows<T, 1>&) const {
+    if (valid) part[0] += (double)x[i] * (double)y[i];
+  }
   __device__ void finalize(const double (&tot)[1]) const { *out = tot[0]; }
 };
 
@@ -526,17 +526,25 @@ int diag_setup(Matrix& M, hipStream_t s) {
 template int diag_setup<double>(Matrix&, hipStream_t);
 template int diag_setup<float>(Matrix&, hipStream_t);
 
-template <typename T, typename TD, int B>
-__global__ void k_diag_apply(int n, const TD* __restrict__ D, const T* __restrict__ x, T* __restrict__ y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  T v[B];
-#pragma unroll
-  for (int k = 0; k < B; ++k) v[k] = x[(size_t)i * B + k];
-  diag_apply<T, TD, B>(D, n, i, v);
-#pragma unroll
-  for (int k = 0; k < B; ++k) y[(size_t)i * B + k] = v[k];
-}
+template <typename T, int B>
+struct OpDiagApply {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 0;
+  const T* __restrict__ D;
+  const T* __restrict__ x;
+  T* __restrict__ y;
+  int n;
+  __device__ bool skip() const { return false; }
+  __device__ void begin() {}
+  __device__ void row(int i, bool valid, double (&)[1], const WaveRows<T, B>& io) const {
+    T v[B];
+    io.load(x, v);
+    if (valid) diag_apply<T, T, B>(D, n, i, v);
+    io.store(y, v);
+  }
+  __device__ void finalize(const double (&)[1]) const {}
+};
 
 template <typename T>
 int diag_apply(const Matrix& M, const T* x, T* y, hipStream_t s) {
@@ -545,8 +553,8 @@ int diag_apply(const Matrix& M, const T* x, T* y, hipStream_t s) {
   if constexpr (sizeof(T) == 8) D = (const T*)M.D64.p;
   else D = (const T*)M.D32.p;
   MH_DISPATCH_B(M.b, {
-    k_diag_apply<T, T, B><<<(M.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(M.n, D, x, y);
-    MH_HIP(hipGetLastError());
+    OpDiagApply<T, B> op{D, x, y, M.n};
+    MH_TRY(launch_rows(M.n, op, nullptr, s));
   });
   return 0;
 }
