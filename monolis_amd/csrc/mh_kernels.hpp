@@ -152,9 +152,12 @@ struct WaveRows {
       wave_sync();
     }
   }
+  // Outputs are written non-temporally: they are consumed by a later kernel,
+  // and streaming them out during the kernel avoids leaving megabytes of dirty
+  // L2 lines for the end-of-kernel write-back.
   __device__ __forceinline__ void store(T* __restrict__ dst, const T (&v)[B]) const {
     if constexpr (B == 1) {
-      if (row0 + lane < nrows) dst[(size_t)row0 + lane] = v[0];
+      if (row0 + lane < nrows) __builtin_nontemporal_store(v[0], dst + (size_t)row0 + lane);
     } else {
       const int cnt = count();
       T* base = dst + (size_t)row0 * B;
@@ -164,7 +167,7 @@ struct WaveRows {
 #pragma unroll
       for (int j = 0; j < B; ++j) {
         const int e = j * kWave + lane;
-        if (e < cnt) base[e] = lds[e];
+        if (e < cnt) __builtin_nontemporal_store(lds[e], base + e);
       }
       wave_sync();
     }
