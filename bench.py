@@ -31,6 +31,20 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_bcsr3_stencil27_100cube_hbm_traffic.md")
+
+
+def measured_traffic(grid: int):
+    """HBM bytes per FP64 SpMV launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (calibrated with the copy kernel) of the 100^3 workload."""
+    if grid != 100 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":")[1]))
+    return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,7 +149,7 @@ def main() -> int:
                                f"({m.nndof} DOF, {m.nz} 3x3 blocks), FP64 BCSR SpMV",
                    "block": 3, "n_block_rows": m.n, "nnzb": m.nz, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": metrics.HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / metrics.HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs / metrics.HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.grid),
                      "kernel": "k_sell<double,double,3,OpSpmv>", "algorithmic_bytes": B64},
         "spmv_fp32": {"ms": round(ms32, 5), "gbs": round(gbs32, 1), "algorithmic_bytes": B32},
         "stream_copy": cp,

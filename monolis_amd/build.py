@@ -42,6 +42,41 @@ def _compile(src: str) -> str:
     return obj
 
 
+BINDIR = os.path.join(HERE, "bin")
+AMDFLANG = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+CC = shutil.which("gcc") or "cc"
+
+
+def build_callers(verbose: bool = True) -> list:
+    """C and Fortran programs that call the library through the flat ABI
+    (monolis_amd/c, monolis_amd/fortran); run by the GPU tests."""
+    os.makedirs(BINDIR, exist_ok=True)
+    os.makedirs(BUILD, exist_ok=True)
+    rpath = "-Wl,-rpath,$ORIGIN/../lib"
+    built = []
+    c_exe = os.path.join(BINDIR, "test_c_abi")
+    cmd = [CC, "-O2", "-std=c11", "-I", os.path.join(os.path.dirname(HERE), "include"),
+           os.path.join(HERE, "c", "test_c_abi.c"), "-L", LIBDIR, "-lmonolis_hip", rpath, "-lm", "-o", c_exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C caller build failed:\n{r.stderr}")
+    built.append(c_exe)
+    if os.path.exists(AMDFLANG):
+        mod_obj = os.path.join(BUILD, "mod_monolis_hip.o")
+        f_exe = os.path.join(BINDIR, "test_mtx_dat")
+        for cmd in ([AMDFLANG, "-O2", "-c", os.path.join(HERE, "fortran", "mod_monolis_hip.f90"),
+                     "-J", BUILD, "-o", mod_obj],
+                    [AMDFLANG, "-O2", "-J", BUILD, os.path.join(HERE, "fortran", "test_mtx_dat.f90"), mod_obj,
+                     "-L", LIBDIR, "-lmonolis_hip", rpath, "-o", f_exe]):
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"Fortran caller build failed:\n{r.stderr}")
+        built.append(f_exe)
+    if verbose:
+        print(f"[monolis_amd] built callers {', '.join(os.path.basename(b) for b in built)}")
+    return built
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
@@ -56,6 +91,7 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"[monolis_amd] built {LIB}")
+    build_callers(verbose)
     return LIB
 
 

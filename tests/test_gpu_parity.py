@@ -325,3 +325,19 @@ def test_invalid_csr_is_rejected_without_launching():
     mb = mm.Matrix(m.n, m.np_, 2, idx, m.item.copy(), m.A.copy()).ensure_dof_index()
     with pytest.raises(api._lib.MonolisHipError, match="non-decreasing"):
         api.DeviceMatrix(mb)
+
+
+# --- C and Fortran callers of the flat ABI ----------------------------------------------
+@pytest.mark.parametrize("exe,token", [("test_c_abi", "C_ABI_OK"), ("test_mtx_dat", "FORTRAN_OK")])
+def test_compiled_callers(exe, token):
+    """Programs compiled against include/monolis_hip.h (C) and mod_monolis_hip
+    (amdflang, ISO_C_BINDING) run the reference's own test cases on the GPU."""
+    import os
+    import subprocess
+    from monolis_amd import build as mb
+    path = os.path.join(mb.BINDIR, exe)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (run __graft_entry__.build())")
+    r = subprocess.run([path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert token in r.stdout
