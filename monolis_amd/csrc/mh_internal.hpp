@@ -68,7 +68,9 @@ struct KState {
   int converged;
   int done;
   int breakdown;    // bit mask of exact-zero dot products seen (BiCGSTAB)
-  int pad[3];
+  int bd_stage;     // != 0: the loop stopped for the sorted-sum fallback at this stage
+  int bd_iter;      // ... in this iteration
+  int pad;
 };
 
 // ---------------------------------------------------------------------------
@@ -124,6 +126,10 @@ template <typename T>
 int dot(long long m, const T* x, const T* y, double* out, double* partials, hipStream_t s);
 template <typename T>
 int axpby(long long m, double alpha, const T* x, double beta, const T* y, T* z, hipStream_t s);
+// monolis_global_sorted_inner_product_main_R on the device: products sorted by
+// magnitude, summed left to right; result written to *out (device).
+template <typename T>
+int sorted_dot(long long n, const T* x, const T* y, double* out, hipStream_t s);
 
 // Number of reduction partials a launch over `rows` rows (or `slices`) may need.
 size_t partials_needed(long long n);
@@ -146,7 +152,8 @@ struct SolveResult {
   double resid = 0.0;
   int converged = 0;
   int zero_rhs = 0;
-  int breakdown = 0;
+  int breakdown = 0;     // bit mask of exact-zero dots seen
+  int fallbacks = 0;     // sorted-sum recomputations performed
   double t_solve = 0.0;  // seconds, device-resident part
   std::vector<double> hist;
 };

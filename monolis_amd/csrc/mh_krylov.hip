@@ -236,6 +236,7 @@ struct OpBiB {
   T* __restrict__ v;
   const T* __restrict__ rt;
   KState* st;
+  int it;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() {}
   __device__ void gather(int c, T (&xv)[B]) const {
@@ -253,8 +254,14 @@ struct OpBiB {
   }
   __device__ void finalize(const double (&tot)[1]) const {
     const double c2 = tot[0];
-    if (c2 == 0.0) st->breakdown |= 2;
     st->pq = c2;
+    if (c2 == 0.0) {  // BiCGSTAB.f90:110-114: recompute with the sorted sum
+      st->breakdown |= 2;
+      st->bd_stage = 2;
+      st->bd_iter = it;
+      st->done = 1;
+      return;
+    }
     st->alpha = st->rho / c2;
   }
 };
@@ -298,6 +305,7 @@ struct OpBiD {
   T* __restrict__ t;
   const T* __restrict__ s;
   KState* st;
+  int it;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() {}
   __device__ void gather(int c, T (&xv)[B]) const {
@@ -319,12 +327,14 @@ struct OpBiD {
   __device__ void finalize(const double (&tot)[2]) const {
     st->ts = tot[0];
     st->tt = tot[1];
-    if (tot[1] == 0.0) {
+    if (tot[1] == 0.0) {  // BiCGSTAB.f90:132-137: recompute (t, t) with the sorted sum
       st->breakdown |= 4;
-      st->omega = 0.0;  // BiCGSTAB.f90:139-145
-    } else {
-      st->omega = tot[0] / tot[1];
+      st->bd_stage = 4;
+      st->bd_iter = it;
+      st->done = 1;
+      return;
     }
+    st->omega = tot[0] / tot[1];
   }
 };
 
@@ -387,7 +397,13 @@ struct OpBiE {
     if (flags & F_XONLY) return;
     if (finalize_check(st, hist, it, tot[0])) return;
     const double rho = tot[1];
-    if (rho == 0.0) st->breakdown |= 1;
+    if (rho == 0.0) {  // BiCGSTAB.f90:89-93 of the next iteration: sorted sum
+      st->breakdown |= 1;
+      st->bd_stage = 5;
+      st->bd_iter = it;
+      st->done = 1;
+      return;
+    }
     st->beta = (rho / st->rho) * (st->alpha / st->omega);  // BiCGSTAB.f90:96
     st->rho_prev = st->rho;
     st->rho = rho;
@@ -598,38 +614,82 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   }
   // rho of iteration 1 is (R, RT) with RT == R: the same sum as B2.
   MH_HIP(hipMemcpyAsync(&st->rho, &st->B2, sizeof(double), hipMemcpyDeviceToDevice, s));
-  Poller poll{c, st};
-  int done = 0;
   double* hp = P.keep_hist ? hist : nullptr;
-  for (int it = 1; it <= P.max_iter && !done;) {
-    const int end = std::min(it + P.chunk - 1, P.max_iter);
-    for (; it <= end; ++it) {
-      OpBiA<T, B, PREC> a{r, p, v, pt, D, n, st, it == 1, (T)0, (T)0};
-      MH_TRY(launch_rows(n, a, parts, s));
-      OpBiB<T, B> bb{pt, v, rt, st};
-      MH_TRY((launch_sell<T, B>(M, bb, parts, var, s)));
-      OpBiC<T, B, PREC> cc{r, v, sv, stv, D, n, st, (T)0};
-      MH_TRY(launch_rows(n, cc, parts, s));
-      OpBiD<T, B> dd{stv, t, sv, st};
-      MH_TRY((launch_sell<T, B>(M, dd, parts, var, s)));
-      if (it % kIterRR == 0) {
-        OpBiE<T, B> e1{x, r, pt, stv, sv, t, rt, st, hp, it, F_XONLY, (T)0, (T)0};
-        MH_TRY(launch_rows(n, e1, parts, s));
-        OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
-        MH_TRY((launch_sell<T, B>(M, rs, parts, var, s)));
-        OpBiE<T, B> e2{x, r, pt, stv, sv, t, rt, st, hp, it, F_CHECK, (T)0, (T)0};
-        MH_TRY(launch_rows(n, e2, parts, s));
-      } else {
-        OpBiE<T, B> e{x, r, pt, stv, sv, t, rt, st, hp, it, F_UPD | F_CHECK, (T)0, (T)0};
-        MH_TRY(launch_rows(n, e, parts, s));
-      }
-    }
-    MH_TRY(poll.issue(s));
-    MH_TRY(poll.check(1, &done));
-  }
-  MH_HIP(hipStreamSynchronize(s));
+  double* dsum = parts;  // sorted-sum result slot (partials are free between launches)
+  // Stage cursor: an iteration is A (p, pt), B (v, (rt, v) -> alpha), C (s, st),
+  // D (t, (t, s), (t, t) -> omega), E (x, r, test, (r, rt) -> beta).  A breakdown
+  // stops the device loop after B, D or E; the host then recomputes that dot with
+  // the sorted sum and resumes from the next stage.
+  int it = 1, stage = 0;
   KState hs;
-  MH_HIP(hipMemcpy(&hs, st, sizeof(KState), hipMemcpyDeviceToHost));
+  out->fallbacks = 0;
+  for (;;) {
+    Poller poll{c, st};
+    int done = 0;
+    while (it <= P.max_iter && !done) {
+      const int end = std::min(it + P.chunk - 1, P.max_iter);
+      for (; it <= end; ++it, stage = 0) {
+        if (stage <= 0) {
+          OpBiA<T, B, PREC> a{r, p, v, pt, D, n, st, it == 1, (T)0, (T)0};
+          MH_TRY(launch_rows(n, a, parts, s));
+          OpBiB<T, B> bb{pt, v, rt, st, it};
+          MH_TRY((launch_sell<T, B>(M, bb, parts, var, s)));
+        }
+        if (stage <= 2) {
+          OpBiC<T, B, PREC> cc{r, v, sv, stv, D, n, st, (T)0};
+          MH_TRY(launch_rows(n, cc, parts, s));
+          OpBiD<T, B> dd{stv, t, sv, st, it};
+          MH_TRY((launch_sell<T, B>(M, dd, parts, var, s)));
+        }
+        if (it % kIterRR == 0) {
+          OpBiE<T, B> e1{x, r, pt, stv, sv, t, rt, st, hp, it, F_XONLY, (T)0, (T)0};
+          MH_TRY(launch_rows(n, e1, parts, s));
+          OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
+          MH_TRY((launch_sell<T, B>(M, rs, parts, var, s)));
+          OpBiE<T, B> e2{x, r, pt, stv, sv, t, rt, st, hp, it, F_CHECK, (T)0, (T)0};
+          MH_TRY(launch_rows(n, e2, parts, s));
+        } else {
+          OpBiE<T, B> e{x, r, pt, stv, sv, t, rt, st, hp, it, F_UPD | F_CHECK, (T)0, (T)0};
+          MH_TRY(launch_rows(n, e, parts, s));
+        }
+      }
+      MH_TRY(poll.issue(s));
+      MH_TRY(poll.check(1, &done));
+    }
+    MH_HIP(hipStreamSynchronize(s));
+    MH_HIP(hipMemcpy(&hs, st, sizeof(KState), hipMemcpyDeviceToHost));
+    if (hs.bd_stage == 0 || hs.converged) break;
+    // sorted-sum fallback (monolis_global_sorted_inner_product_main_R), then resume
+    double val = 0.0;
+    out->fallbacks++;
+    if (hs.bd_stage == 2) {
+      MH_TRY(sorted_dot<T>((long long)m, rt, v, dsum, s));
+      MH_HIP(hipMemcpy(&val, dsum, sizeof(double), hipMemcpyDeviceToHost));
+      hs.pq = val;
+      hs.alpha = hs.rho / val;
+      it = hs.bd_iter;
+      stage = 2;
+    } else if (hs.bd_stage == 4) {
+      MH_TRY(sorted_dot<T>((long long)m, t, t, dsum, s));
+      MH_HIP(hipMemcpy(&val, dsum, sizeof(double), hipMemcpyDeviceToHost));
+      hs.tt = val;
+      hs.omega = (val == 0.0) ? 0.0 : hs.ts / val;  // BiCGSTAB.f90:139-145
+      it = hs.bd_iter;
+      stage = 4;
+    } else {
+      MH_TRY(sorted_dot<T>((long long)m, r, rt, dsum, s));
+      MH_HIP(hipMemcpy(&val, dsum, sizeof(double), hipMemcpyDeviceToHost));
+      hs.beta = (val / hs.rho) * (hs.alpha / hs.omega);
+      hs.rho_prev = hs.rho;
+      hs.rho = val;
+      it = hs.bd_iter + 1;
+      stage = 0;
+    }
+    hs.bd_stage = 0;
+    hs.done = 0;
+    MH_HIP(hipMemcpy(st, &hs, sizeof(KState), hipMemcpyHostToDevice));
+    if (it > P.max_iter) break;
+  }
   out->iter = hs.iter;
   out->resid = hs.resid;
   out->converged = hs.converged;

@@ -69,6 +69,7 @@ template struct DevArray<unsigned>;
 template struct DevArray<double>;
 template struct DevArray<float>;
 template struct DevArray<unsigned char>;
+template struct DevArray<unsigned long long>;
 
 int is_device_pointer(const void* p) {
   if (!p) return 0;

@@ -341,3 +341,33 @@ def test_compiled_callers(exe, token):
     r = subprocess.run([path], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert token in r.stdout
+
+
+# --- BiCGSTAB exact-zero dots: sorted-sum fallback and resume ---------------------------
+def test_bicgstab_breakdown_stage_d_diagonal_system():
+    """A = 2I with DIAG: s = 0 exactly, so (t, t) = 0 -> sorted sum -> omega = 0
+    (BiCGSTAB.f90:132-145) and x = b/2 after one iteration."""
+    from monolis_amd import matrix as mm
+    n = 100
+    m = mm.Matrix(n, n, 1, np.arange(n + 1, dtype=np.int32), np.arange(1, n + 1, dtype=np.int32),
+                  np.full(n, 2.0)).ensure_dof_index()
+    b = np.random.default_rng(0).uniform(-1, 1, n)
+    x, mon = _solve_flat(m, b, method=5, precond=1)
+    xo, st, I, R = oracle.solve(m, b, method=5, precond=1)
+    assert st == 0 and api.monolis_get_error_tag(mon) == 0
+    assert api.monolis_get_converge_iter(mon) == int(I[4]) == 1
+    assert np.array_equal(x, b / 2) and np.array_equal(xo, b / 2)
+
+
+def test_bicgstab_breakdown_stage_b_orthogonal_shadow():
+    """(rt, v) = 0 exactly at the first iteration: both paths recompute it with the
+    sorted sum, get 0 again and follow monolis into a non-converged solve."""
+    from monolis_amd import matrix as mm
+    m = mm.Matrix(2, 2, 1, np.array([0, 1, 2], np.int32), np.array([2, 1], np.int32),
+                  np.array([1.0, 1.0])).ensure_dof_index()
+    b = np.array([1.0, 0.0])
+    x, mon = _solve_flat(m, b, method=5, precond=0, max_iter=5)
+    xo, st, I, R = oracle.solve(m, b, method=5, precond=0, max_iter=5)
+    assert st == 1 and api.monolis_get_error_tag(mon) == 1
+    assert api.monolis_get_converge_iter(mon) == int(I[4])
+    assert np.array_equal(np.isfinite(x), np.isfinite(xo))
