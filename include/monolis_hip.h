@@ -128,6 +128,32 @@ int monolis_hip_matrix_autotune(monolis_hip_matrix* M, int fp32, int reps, int* 
 /* Forces a variant (0 = default) for one precision (fp32 = 0 or 1). */
 int monolis_hip_matrix_set_variant(monolis_hip_matrix* M, int fp32, int variant);
 
+/* Batched solves (no monolis counterpart; equivalent to one monolis_solve_R
+ * per system).  A batch is a block-diagonal scalar CSR matrix in monolis's
+ * format whose system k owns rows sys_row[k] .. sys_row[k+1]-1 (sys_row has
+ * nsys+1 entries, host memory) and couples only to its own rows.  Every system
+ * is solved by CG (Iarray[1] = 1) with NONE / DIAG preconditioning
+ * (Iarray[2]), its own scalars, convergence test (Rarray[1]) and iteration
+ * count, in one persistent launch captured in a hipGraph and replayed. */
+typedef struct monolis_hip_batch monolis_hip_batch;
+int monolis_hip_batch_create(monolis_hip_batch** out, int nsys, const int* sys_row, int n, int nz,
+                             const int* index, const int* item, const double* A);
+int monolis_hip_batch_destroy(monolis_hip_batch* B);
+/* b, x: device vectors of length n.  iters / resid / status: host arrays of
+ * nsys entries (status 0 converged, 1 not converged, 2 zero right-hand side). */
+int monolis_hip_batch_solve(monolis_hip_batch* B, const double* b, double* x, int* Iarray, double* Rarray,
+                            int* iters, double* resid, int* status);
+/* Autotuner of the persistent launch: workgroup size x slots per step.  Runs
+ * the solve `reps` times per configuration, keeps the fastest; ms (if not
+ * NULL) gets monolis_hip_batch_num_configs() average times. */
+int monolis_hip_batch_num_configs(void);
+int monolis_hip_batch_config(int cfg, int* threads, int* unroll);
+int monolis_hip_batch_autotune(monolis_hip_batch* B, const double* b, double* x, int* Iarray, double* Rarray,
+                               int reps, int* best, double* ms);
+/* Average time of `reps` graph replays of the batched solve (ms). */
+int monolis_hip_batch_time(monolis_hip_batch* B, const double* b, double* x, int* Iarray, double* Rarray, int reps,
+                           double* ms);
+
 /* Bandwidth probe (no monolis counterpart): a plain streaming copy kernel
  * dst[i] = src[i] over n doubles, `reps` timed launches on the library stream;
  * *ms = average per launch.  Used to calibrate HBM byte counters and to state

@@ -50,6 +50,16 @@ SIGNATURES = [
     ("monolis_hip_num_variants", c_int, []),
     ("monolis_hip_matrix_autotune", c_int, [c_void_p, c_int, c_int, c_int_p, c_double_p]),
     ("monolis_hip_matrix_set_variant", c_int, [c_void_p, c_int, c_int]),
+    ("monolis_hip_batch_create", c_int, [ctypes.POINTER(c_void_p), c_int, c_void_p, c_int, c_int,
+                                         c_void_p, c_void_p, c_void_p]),
+    ("monolis_hip_batch_destroy", c_int, [c_void_p]),
+    ("monolis_hip_batch_solve", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
+    ("monolis_hip_batch_num_configs", c_int, []),
+    ("monolis_hip_batch_config", c_int, [c_int, c_int_p, c_int_p]),
+    ("monolis_hip_batch_autotune", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int_p,
+                                           c_double_p]),
+    ("monolis_hip_batch_time", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double_p]),
 ]
 
 

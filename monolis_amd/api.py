@@ -278,3 +278,57 @@ def inner_product_device(x: DeviceBuffer, y: DeviceBuffer, m: Optional[int] = No
     _lib.check(_lib.load().monolis_hip_inner_product_R(n, ctypes.c_void_p(x.ptr), ctypes.c_void_p(y.ptr),
                                                        ctypes.byref(s)), "inner_product")
     return s.value
+
+
+class DeviceBatch:
+    """Many independent systems (block-diagonal scalar CSR, system k owns rows
+    sys_row[k]..sys_row[k+1]-1) solved by one persistent, graph-captured launch."""
+
+    def __init__(self, mat: Matrix, sys_row):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        self.sys_row = np.ascontiguousarray(sys_row, np.int32)
+        self.nsys = len(self.sys_row) - 1
+        _lib.check(L.monolis_hip_batch_create(ctypes.byref(h), self.nsys, _p(self.sys_row), mat.n, mat.nz,
+                                              _p(mat.index), _p(mat.item), _p(mat.A)), "monolis_hip_batch_create")
+        self.h = h.value
+        self.n = mat.n
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib.load().monolis_hip_batch_destroy(ctypes.c_void_p(self.h))
+                self.h = None
+        except Exception:
+            pass
+
+    def solve(self, b: DeviceBuffer, x: DeviceBuffer, prm: Prm):
+        iters = np.zeros(self.nsys, np.int32)
+        resid = np.zeros(self.nsys, np.float64)
+        status = np.zeros(self.nsys, np.int32)
+        _lib.check(_lib.load().monolis_hip_batch_solve(ctypes.c_void_p(self.h), ctypes.c_void_p(b.ptr),
+                                                       ctypes.c_void_p(x.ptr), _p(prm.Iarray), _p(prm.Rarray),
+                                                       _p(iters), _p(resid), _p(status)), "batch_solve")
+        return iters, resid, status
+
+    def autotune(self, b: DeviceBuffer, x: DeviceBuffer, prm: Prm, reps: int = 2) -> dict:
+        L = _lib.load()
+        nc = L.monolis_hip_batch_num_configs()
+        best = ctypes.c_int()
+        ms = (ctypes.c_double * nc)()
+        _lib.check(L.monolis_hip_batch_autotune(ctypes.c_void_p(self.h), ctypes.c_void_p(b.ptr), ctypes.c_void_p(x.ptr),
+                                                _p(prm.Iarray), _p(prm.Rarray), int(reps), ctypes.byref(best), ms),
+                   "batch_autotune")
+        cfgs = []
+        for c in range(nc):
+            t, u = ctypes.c_int(), ctypes.c_int()
+            L.monolis_hip_batch_config(c, ctypes.byref(t), ctypes.byref(u))
+            cfgs.append({"threads": t.value, "unroll": u.value})
+        return {"best": best.value, "ms": list(ms), "configs": cfgs}
+
+    def time(self, b: DeviceBuffer, x: DeviceBuffer, prm: Prm, reps: int = 5) -> float:
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().monolis_hip_batch_time(ctypes.c_void_p(self.h), ctypes.c_void_p(b.ptr),
+                                                      ctypes.c_void_p(x.ptr), _p(prm.Iarray), _p(prm.Rarray),
+                                                      int(reps), ctypes.byref(ms)), "batch_time")
+        return ms.value

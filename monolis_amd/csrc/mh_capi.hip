@@ -10,9 +10,15 @@
 #include "../../include/monolis_hip.h"
 #include "mh_internal.hpp"
 #include "mh_kernels.hpp"
+#include "mh_batch.hpp"
 
 using mh::Context;
 using mh::Matrix;
+
+struct monolis_hip_batch {
+  mh::Batch* B = nullptr;
+  ~monolis_hip_batch() { delete B; }
+};
 
 struct monolis_hip_matrix {
   Matrix* M = nullptr;
@@ -415,6 +421,139 @@ int monolis_hip_matrix_autotune(monolis_hip_matrix* H, int fp32, int reps, int* 
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
+}
+
+static mh::BatchParams batch_params(const int* Iarray, const double* Rarray) {
+  mh::BatchParams P;
+  P.max_iter = Iarray[I_MAX_ITER];
+  P.tol = Rarray[R_TOL];
+  P.init_x = Iarray[I_IS_INIT_X] != 0;
+  P.precond = Iarray[I_PRECOND];
+  return P;
+}
+
+static int batch_check(const int* Iarray) {
+  if (Iarray[I_METHOD] != 1) {
+    mh::set_error("batched solves implement CG (method 1), got %d", Iarray[I_METHOD]);
+    return ST_METHOD;
+  }
+  if (Iarray[I_PRECOND] != 0 && Iarray[I_PRECOND] != 1) {
+    mh::set_error("batched solves implement NONE / DIAG, got %d", Iarray[I_PRECOND]);
+    return ST_PRECOND;
+  }
+  if (Iarray[I_MAX_ITER] < 1) {
+    mh::set_error("max_iter must be >= 1");
+    return ST_ARGS;
+  }
+  return 0;
+}
+
+int monolis_hip_batch_create(monolis_hip_batch** out, int nsys, const int* sys_row, int n, int nz, const int* index,
+                             const int* item, const double* A) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  *out = nullptr;
+  mh::Batch* B = nullptr;
+  MH_TRY(mh::batch_create(&B, nsys, sys_row, n, nz, index, item, A, c->stream));
+  auto* H = new monolis_hip_batch();
+  H->B = B;
+  *out = H;
+  return 0;
+}
+
+int monolis_hip_batch_destroy(monolis_hip_batch* H) {
+  if (!H) return 0;
+  Context* c = nullptr;
+  if (mh::context(&c) == 0) (void)hipStreamSynchronize(c->stream);
+  delete H;
+  return 0;
+}
+
+int monolis_hip_batch_num_configs(void) { return mh::batch_num_configs(); }
+
+int monolis_hip_batch_config(int cfg, int* threads, int* unroll) {
+  if (cfg < 0 || cfg >= mh::batch_num_configs()) {
+    mh::set_error("batch config %d out of range", cfg);
+    return ST_ARGS;
+  }
+  mh::batch_config_info(cfg, threads, unroll);
+  return 0;
+}
+
+int monolis_hip_batch_solve(monolis_hip_batch* H, const double* b, double* x, int* Iarray, double* Rarray, int* iters,
+                            double* resid, int* status) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  const int rc0 = batch_check(Iarray);
+  if (rc0) return rc0;
+  const double t0 = now();
+  mh::Batch& B = *H->B;
+  MH_TRY(mh::batch_solve(B, batch_params(Iarray, Rarray), b, x, B.cfg, 1, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  if (iters) MH_HIP(hipMemcpy(iters, B.iters.p, sizeof(int) * B.nsys, hipMemcpyDeviceToHost));
+  if (resid) MH_HIP(hipMemcpy(resid, B.resid.p, sizeof(double) * B.nsys, hipMemcpyDeviceToHost));
+  if (status) MH_HIP(hipMemcpy(status, B.status.p, sizeof(int) * B.nsys, hipMemcpyDeviceToHost));
+  Rarray[R_TIME_SOL] = now() - t0;
+  return 0;
+}
+
+int monolis_hip_batch_time(monolis_hip_batch* H, const double* b, double* x, int* Iarray, double* Rarray, int reps,
+                           double* ms) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  const int rc0 = batch_check(Iarray);
+  if (rc0) return rc0;
+  mh::Batch& B = *H->B;
+  const mh::BatchParams P = batch_params(Iarray, Rarray);
+  MH_TRY(mh::batch_solve(B, P, b, x, B.cfg, 1, c->stream));  // capture + warm-up
+  hipEvent_t e0, e1;
+  MH_HIP(hipEventCreate(&e0));
+  MH_HIP(hipEventCreate(&e1));
+  MH_HIP(hipEventRecord(e0, c->stream));
+  for (int i = 0; i < reps; ++i) MH_TRY(mh::batch_solve(B, P, b, x, B.cfg, 1, c->stream));
+  MH_HIP(hipEventRecord(e1, c->stream));
+  MH_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  MH_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = reps > 0 ? (double)t / reps : 0.0;
+  return 0;
+}
+
+int monolis_hip_batch_autotune(monolis_hip_batch* H, const double* b, double* x, int* Iarray, double* Rarray, int reps,
+                               int* best, double* ms) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  const int rc0 = batch_check(Iarray);
+  if (rc0) return rc0;
+  mh::Batch& B = *H->B;
+  const mh::BatchParams P = batch_params(Iarray, Rarray);
+  double best_ms = 1e300;
+  int best_cfg = 0;
+  for (int cfg = 0; cfg < mh::batch_num_configs(); ++cfg) {
+    MH_TRY(mh::batch_solve(B, P, b, x, cfg, 0, c->stream));  // warm-up
+    hipEvent_t e0, e1;
+    MH_HIP(hipEventCreate(&e0));
+    MH_HIP(hipEventCreate(&e1));
+    MH_HIP(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i) MH_TRY(mh::batch_solve(B, P, b, x, cfg, 0, c->stream));
+    MH_HIP(hipEventRecord(e1, c->stream));
+    MH_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    MH_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const double avg = reps > 0 ? (double)t / reps : 0.0;
+    if (ms) ms[cfg] = avg;
+    if (avg < best_ms) {
+      best_ms = avg;
+      best_cfg = cfg;
+    }
+  }
+  B.cfg = best_cfg;
+  if (best) *best = best_cfg;
+  return 0;
 }
 
 __global__ void __launch_bounds__(256) k_stream_copy(long long n, const double* __restrict__ src,

@@ -1,0 +1,78 @@
+"""Batched solves (many independent systems, one persistent graph-captured
+launch) against one oracle solve per system.
+
+Bars: status identical, iteration count within +-2 % (+-1 below 50) of the
+oracle's solve of that system alone, ||x_k - x_k,oracle|| / ||x_k,oracle|| <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from monolis_amd import api, workloads as W
+from monolis_amd.matrix import Matrix
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init(hip):
+    oracle.set_threads(1)
+    yield
+
+
+def _system(mat, sys_row, k):
+    r0, r1 = int(sys_row[k]), int(sys_row[k + 1])
+    j0, j1 = int(mat.index[r0]), int(mat.index[r1])
+    index = (mat.index[r0:r1 + 1] - j0).astype(np.int32)
+    item = (mat.item[j0:j1] - r0).astype(np.int32)
+    return Matrix(r1 - r0, r1 - r0, 1, index, item, mat.A[j0:j1].copy()).ensure_dof_index()
+
+
+def _prm(tol=1e-8, precond=1, max_iter=1000):
+    prm = api.Prm().quiet()
+    prm.Iarray[2] = precond
+    prm.Iarray[3] = max_iter
+    prm.Rarray[1] = tol
+    return prm
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_batch_matches_individual_solves(precond):
+    mat, sys_row = W.poisson_batch(count=40, lo=3, hi=13, seed=7)
+    b = W.uniform_vector(mat.n, seed=8)
+    b[sys_row[5]:sys_row[6]] = 0.0                 # one zero right-hand side
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(mat.n)
+    B = api.DeviceBatch(mat, sys_row)
+    iters, resid, status = B.solve(db, dx, _prm(precond=precond))
+    x = dx.numpy()
+    for k in range(B.nsys):
+        sk = _system(mat, sys_row, k)
+        bk = b[sys_row[k]:sys_row[k + 1]]
+        xo, st, I, R = oracle.solve(sk, bk, precond=precond)
+        xk = x[sys_row[k]:sys_row[k + 1]]
+        if k == 5:
+            assert status[k] == 2 and np.all(xk == 0.0)
+            continue
+        assert status[k] == 0 and st == 0
+        assert abs(int(iters[k]) - int(I[4])) <= max(1, int(0.02 * I[4])), (k, iters[k], I[4])
+        assert np.linalg.norm(xk - xo) / np.linalg.norm(xo) <= 1e-6
+        assert resid[k] < 1e-8
+
+
+def test_batch_configs_graph_replay_and_autotune():
+    mat, sys_row = W.poisson_batch(count=24, lo=4, hi=20, seed=3)
+    b = W.uniform_vector(mat.n, seed=4)
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(mat.n)
+    B = api.DeviceBatch(mat, sys_row)
+    prm = _prm()
+    it0, r0, s0 = B.solve(db, dx, prm)
+    x0 = dx.numpy()
+    it1, r1, s1 = B.solve(db, dx, prm)          # graph replay: same bits
+    assert np.array_equal(x0, dx.numpy()) and np.array_equal(it0, it1)
+    tune = B.autotune(db, dx, prm, reps=1)
+    assert 0 <= tune["best"] < len(tune["ms"])
+    it2, r2, s2 = B.solve(db, dx, prm)          # tuned configuration
+    assert np.all(s2 == 0) and np.all(np.abs(it2 - it0) <= 1)
+    assert np.linalg.norm(dx.numpy() - x0) / np.linalg.norm(x0) <= 1e-9
