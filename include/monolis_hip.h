@@ -154,6 +154,43 @@ int monolis_hip_batch_autotune(monolis_hip_batch* B, const double* b, double* x,
 int monolis_hip_batch_time(monolis_hip_batch* B, const double* b, double* x, int* Iarray, double* Rarray, int reps,
                            double* ms);
 
+/* Multi-right-hand-side solves (no monolis counterpart; equivalent to one
+ * monolis_solve_R per column).  A scalar matrix (n_dof = 1) and nrhs (1..64)
+ * right-hand sides stored row-major: b and x are device arrays of NP x nrhs
+ * doubles with entry (i, j) at i*nrhs + j.  Every column runs monolis's CG
+ * (Iarray[1] = 1, NONE / DIAG preconditioning) with its own scalars,
+ * convergence test (Rarray[1]) and iteration count; the SpMM that applies A to
+ * all columns reads the matrix once.  iters / resid / status: host arrays of
+ * nrhs entries (status 0 converged, 1 not converged, 2 zero right-hand side,
+ * x = 0).  Iarray[4] receives the largest iteration count, Rarray[2] the
+ * largest relative residual, Iarray[5] the status. */
+int monolis_hip_solve_multi_R(monolis_hip_matrix* M, int nrhs, const double* b, double* x, int* Iarray,
+                              double* Rarray, int* iters, double* resid, int* status);
+/* FP32-inner / FP64-refined variant: FP32 multi-RHS CG corrections to relative
+ * residual inner_tol; FP64 residual B - A X, update and norms; a column stops
+ * when its ||b - A x|| / ||b - A x0|| < Rarray[1].  iters = inner iterations
+ * per column summed over corrections; stats[0] = outer iterations. */
+int monolis_hip_solve_multi_refined_R(monolis_hip_matrix* M, int nrhs, const double* b, double* x, int* Iarray,
+                                      double* Rarray, double inner_tol, int max_outer, int* iters, double* resid,
+                                      int* status, double* stats);
+
+/* Y = A X for nrhs columns (SpMM, scalar matrices): x, y device arrays of
+ * NP x nrhs (row-major, right-hand sides innermost), FP64 / FP32.  Equivalent
+ * to nrhs monolis_matvec_product_R calls; the matrix is read once. */
+int monolis_hip_matvec_multi_R(monolis_hip_matrix* M, int nrhs, const double* x, double* y);
+int monolis_hip_matvec_multi_S(monolis_hip_matrix* M, int nrhs, const float* x, float* y);
+
+/* Autotuner of the multi-RHS SpMM: row batches whose gathers are issued
+ * together x workgroups per CU.  Times `reps` SpMMs with nrhs columns per
+ * configuration (fp32 = 1: the FP32 copy used by the refined solve's inner CG)
+ * and keeps the fastest on the handle; ms (if not NULL) gets
+ * monolis_hip_multi_num_configs() averages. */
+int monolis_hip_multi_num_configs(void);
+int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu);
+int monolis_hip_multi_autotune(monolis_hip_matrix* M, int nrhs, int fp32, int reps, int* best, double* ms);
+/* Forces a multi-RHS SpMM configuration for one precision (fp32 = 0 or 1). */
+int monolis_hip_multi_set_config(monolis_hip_matrix* M, int fp32, int cfg);
+
 /* Bandwidth probe (no monolis counterpart): a plain streaming copy kernel
  * dst[i] = src[i] over n doubles, `reps` timed launches on the library stream;
  * *ms = average per launch.  Used to calibrate HBM byte counters and to state

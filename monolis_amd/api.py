@@ -250,6 +250,55 @@ class DeviceMatrix:
         _lib.check(rc, "monolis_hip_solve_refined_R")
         return {"outer": int(st[0]), "inner_total": int(st[1]), "resid": float(st[2]), "converged": bool(st[3])}
 
+    def solve_multi(self, b: DeviceBuffer, x: DeviceBuffer, nrhs: int, prm: Prm) -> dict:
+        """nrhs CG solves sharing each SpMM; b, x hold NP x nrhs doubles, row-major."""
+        it = np.zeros(nrhs, np.int32)
+        res = np.zeros(nrhs, np.float64)
+        st = np.zeros(nrhs, np.int32)
+        rc = _lib.load().monolis_hip_solve_multi_R(ctypes.c_void_p(self.h), int(nrhs), ctypes.c_void_p(b.ptr),
+                                                   ctypes.c_void_p(x.ptr), _p(prm.Iarray), _p(prm.Rarray),
+                                                   _p(it), _p(res), _p(st))
+        _lib.check(rc, "monolis_hip_solve_multi_R")
+        return {"iters": it, "resid": res, "status": st}
+
+    def autotune_multi(self, nrhs: int, fp32: bool = False, reps: int = 5) -> dict:
+        """Times every multi-RHS SpMM configuration and keeps the fastest."""
+        L = _lib.load()
+        nc = L.monolis_hip_multi_num_configs()
+        best = ctypes.c_int()
+        ms = (ctypes.c_double * nc)()
+        _lib.check(L.monolis_hip_multi_autotune(ctypes.c_void_p(self.h), int(nrhs), int(fp32), int(reps),
+                                                ctypes.byref(best), ms), "multi_autotune")
+        cfgs = []
+        for c in range(nc):
+            sb, wpc = ctypes.c_int(), ctypes.c_int()
+            L.monolis_hip_multi_config(c, ctypes.byref(sb), ctypes.byref(wpc))
+            cfgs.append((sb.value, wpc.value))
+        return {"best": best.value, "ms": list(ms), "configs": cfgs}
+
+    def matvec_multi(self, x: DeviceBuffer, y: DeviceBuffer, nrhs: int) -> None:
+        """Y = A X for nrhs columns (NP x nrhs row-major device buffers)."""
+        L = _lib.load()
+        fn = L.monolis_hip_matvec_multi_S if x.dtype == np.float32 else L.monolis_hip_matvec_multi_R
+        _lib.check(fn(ctypes.c_void_p(self.h), int(nrhs), ctypes.c_void_p(x.ptr), ctypes.c_void_p(y.ptr)),
+                   "matvec_multi")
+
+    def set_multi_config(self, cfg: int, fp32: bool = False) -> None:
+        _lib.check(_lib.load().monolis_hip_multi_set_config(ctypes.c_void_p(self.h), int(fp32), int(cfg)),
+                   "multi_set_config")
+
+    def solve_multi_refined(self, b: DeviceBuffer, x: DeviceBuffer, nrhs: int, prm: Prm,
+                            inner_tol: float = 1e-4, max_outer: int = 50) -> dict:
+        it = np.zeros(nrhs, np.int32)
+        res = np.zeros(nrhs, np.float64)
+        st = np.zeros(nrhs, np.int32)
+        stats = np.zeros(4, np.float64)
+        rc = _lib.load().monolis_hip_solve_multi_refined_R(
+            ctypes.c_void_p(self.h), int(nrhs), ctypes.c_void_p(b.ptr), ctypes.c_void_p(x.ptr), _p(prm.Iarray),
+            _p(prm.Rarray), float(inner_tol), int(max_outer), _p(it), _p(res), _p(st), _p(stats))
+        _lib.check(rc, "monolis_hip_solve_multi_refined_R")
+        return {"inner_iters": it, "resid": res, "status": st, "outer": int(stats[0])}
+
     def autotune(self, fp32: bool = True, reps: int = 20) -> dict:
         L = _lib.load()
         nv = L.monolis_hip_num_variants()

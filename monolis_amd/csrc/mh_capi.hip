@@ -1,4 +1,5 @@
 // extern "C" entry points of libmonolis_hip.so (declared in include/monolis_hip.h).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -11,6 +12,7 @@
 #include "mh_internal.hpp"
 #include "mh_kernels.hpp"
 #include "mh_batch.hpp"
+#include "mh_multirhs.hpp"
 
 using mh::Context;
 using mh::Matrix;
@@ -23,6 +25,7 @@ struct monolis_hip_batch {
 struct monolis_hip_matrix {
   Matrix* M = nullptr;
   mh::Workspace W;
+  mh::Workspace W_outer;  // FP64 residual / FP32 correction of multi-RHS refinement
   ~monolis_hip_matrix() { delete M; }
 };
 
@@ -553,6 +556,134 @@ int monolis_hip_batch_autotune(monolis_hip_batch* H, const double* b, double* x,
   }
   B.cfg = best_cfg;
   if (best) *best = best_cfg;
+  return 0;
+}
+
+static int multi_params(const int* Iarray, const double* Rarray, mh::MultiParams* P) {
+  if (Iarray[I_METHOD] != 1) {
+    mh::set_error("multi-RHS solves implement CG (method 1), got %d", Iarray[I_METHOD]);
+    return ST_METHOD;
+  }
+  if (Iarray[I_PRECOND] != 0 && Iarray[I_PRECOND] != 1) {
+    mh::set_error("multi-RHS solves implement NONE / DIAG, got %d", Iarray[I_PRECOND]);
+    return ST_PRECOND;
+  }
+  P->max_iter = Iarray[I_MAX_ITER];
+  P->precond = Iarray[I_PRECOND];
+  P->init_x = Iarray[I_IS_INIT_X] != 0;
+  P->tol = Rarray[R_TOL];
+  return 0;
+}
+
+// Iarray[4] / Rarray[2] / Iarray[5] summarise the columns: largest iteration
+// count, largest relative residual, not converged if any column is not.
+static void multi_summary(int nrhs, const std::vector<int>& it, const std::vector<double>& res,
+                          const std::vector<int>& st, int* Iarray, double* Rarray, int* iters, double* resid,
+                          int* status) {
+  int imax = 0, any_nc = 0;
+  double rmax = 0.0;
+  for (int j = 0; j < nrhs; ++j) {
+    imax = std::max(imax, it[j]);
+    if (st[j] != 2) rmax = std::max(rmax, res[j]);
+    any_nc |= st[j] == 1;
+    if (iters) iters[j] = it[j];
+    if (resid) resid[j] = res[j];
+    if (status) status[j] = st[j];
+  }
+  Iarray[I_CUR_ITER] = imax;
+  Rarray[R_CUR_RESID] = rmax;
+  Iarray[I_IERR] = any_nc ? ST_NOT_CONVERGED : ST_OK;
+}
+
+int monolis_hip_solve_multi_R(monolis_hip_matrix* H, int nrhs, const double* b, double* x, int* Iarray,
+                              double* Rarray, int* iters, double* resid, int* status) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  mh::MultiParams P;
+  int rc = multi_params(Iarray, Rarray, &P);
+  if (rc) {
+    Iarray[I_IERR] = rc;
+    return rc;
+  }
+  const double t0 = now();
+  mh::MultiResult R;
+  rc = mh::multi_cg<double>(*H->M, P, nrhs, b, x, H->W, &R, c->stream);
+  if (rc) {
+    Iarray[I_IERR] = rc == -3 ? ST_PRECOND : rc == -2 ? ST_ARGS : ST_HIP;
+    return Iarray[I_IERR];
+  }
+  multi_summary(nrhs, R.iters, R.resid, R.status, Iarray, Rarray, iters, resid, status);
+  Rarray[R_TIME_SOL] = now() - t0;
+  return 0;
+}
+
+int monolis_hip_solve_multi_refined_R(monolis_hip_matrix* H, int nrhs, const double* b, double* x, int* Iarray,
+                                      double* Rarray, double inner_tol, int max_outer, int* iters, double* resid,
+                                      int* status, double* stats) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  mh::MultiParams P;
+  int rc = multi_params(Iarray, Rarray, &P);
+  if (rc) {
+    Iarray[I_IERR] = rc;
+    return rc;
+  }
+  const double t0 = now();
+  mh::MultiRefineResult R;
+  rc = mh::multi_refine(*H->M, P, inner_tol, max_outer, nrhs, b, x, H->W, H->W_outer, &R, c->stream);
+  if (rc) {
+    Iarray[I_IERR] = rc == -3 ? ST_PRECOND : rc == -2 ? ST_ARGS : ST_HIP;
+    return Iarray[I_IERR];
+  }
+  multi_summary(nrhs, R.inner_iters, R.resid, R.status, Iarray, Rarray, iters, resid, status);
+  if (stats) stats[0] = R.outer;
+  Rarray[R_TIME_SOL] = now() - t0;
+  return 0;
+}
+
+int monolis_hip_matvec_multi_R(monolis_hip_matrix* H, int nrhs, const double* x, double* y) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_TRY(mh::multi_matvec<double>(*H->M, nrhs, x, y, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_matvec_multi_S(monolis_hip_matrix* H, int nrhs, const float* x, float* y) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_TRY(mh::multi_matvec<float>(*H->M, nrhs, x, y, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_multi_num_configs(void) { return mh::multi_num_configs(); }
+
+int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu) {
+  if (cfg < 0 || cfg >= mh::multi_num_configs()) {
+    mh::set_error("multi-RHS config %d out of range", cfg);
+    return ST_ARGS;
+  }
+  mh::multi_config_info(cfg, steps_per_batch, wg_per_cu);
+  return 0;
+}
+
+int monolis_hip_multi_set_config(monolis_hip_matrix* H, int fp32, int cfg) {
+  if (cfg < 0 || cfg >= mh::multi_num_configs()) {
+    mh::set_error("multi-RHS config %d out of range", cfg);
+    return ST_ARGS;
+  }
+  if (fp32) H->M->mrhs_cfg32 = cfg;
+  else H->M->mrhs_cfg64 = cfg;
+  return 0;
+}
+
+int monolis_hip_multi_autotune(monolis_hip_matrix* H, int nrhs, int fp32, int reps, int* best, double* ms) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  int b = 0;
+  MH_TRY(mh::multi_autotune(*H->M, nrhs, fp32, reps, &b, ms, c->stream));
+  if (best) *best = b;
   return 0;
 }
 

@@ -92,6 +92,8 @@ struct Matrix {
   bool diag_ready = false, diag32_ready = false, val32_ready = false;
   // SELL kernel variant per precision (0 = default); set by the autotuner
   int variant64 = 0, variant32 = 0;
+  // multi-right-hand-side SpMM launch configuration per precision (-1 = default)
+  int mrhs_cfg64 = -1, mrhs_cfg32 = -1;
   // transpose (for adjoint solves of nonsymmetric systems), built lazily
   Matrix* transpose = nullptr;
   ~Matrix();
