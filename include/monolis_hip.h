@@ -191,6 +191,36 @@ int monolis_hip_multi_autotune(monolis_hip_matrix* M, int nrhs, int fp32, int re
 /* Forces a multi-RHS SpMM configuration for one precision (fp32 = 0 or 1). */
 int monolis_hip_multi_set_config(monolis_hip_matrix* M, int fp32, int cfg);
 
+/* Device-side scalar access to a stored entry, block (i, j), component
+ * (sub_i, sub_j), all 0-based.  Counterparts of
+ * monolis_{set,add}_scalar_to_sparse_matrix_R and
+ * monolis_get_scalar_from_sparse_matrix_R (wrapper/matrix/monolis_spmat_handler_c.h:19-61,
+ * src/matrix/spmat_handler_util.f90:32-74, 207-252, 306-348) on the handle's
+ * device copy.  A block outside the pattern is an error (monolis stops the
+ * process there); get returns *is_find = 0 and *val = 0 for it. */
+int monolis_hip_matrix_set_scalar_R(monolis_hip_matrix* M, int i, int j, int sub_i, int sub_j, double val);
+int monolis_hip_matrix_add_scalar_R(monolis_hip_matrix* M, int i, int j, int sub_i, int sub_j, double val);
+int monolis_hip_matrix_get_scalar_R(monolis_hip_matrix* M, int i, int j, int sub_i, int sub_j, double* val,
+                                    int* is_find);
+
+/* Adjoint extension (no monolis counterpart).  For x = A^{-1} b and a loss
+ * L(x) with g = dL/dx:
+ *   A^T lambda = g,  dL/db = lambda,  dL/dA(i,j)[r][c] = -lambda[i*b+r] * x[j*b+c]
+ * on the stored blocks only, in monolis's A layout (block e at e*b*b, row-major).
+ * All vectors are device arrays; square local matrices (n == np) only.
+ *
+ * monolis_hip_solve_transpose_R: A^T x = b with the method / preconditioner /
+ * tolerance of Iarray / Rarray; the transpose is built on the device on first
+ * use (a radix sort of (column, row) keys) and refreshed after value changes. */
+int monolis_hip_solve_transpose_R(monolis_hip_matrix* M, const double* b, double* x, int* Iarray, double* Rarray);
+/* dA = -lambda x^T restricted to the pattern (nz * n_dof^2 values). */
+int monolis_hip_outer_grad_R(monolis_hip_matrix* M, const double* lambda, const double* x, double* dA);
+/* Adjoint solve plus gradient: lambda = A^{-T} g (A itself when symmetric != 0),
+ * then dA as monolis_hip_outer_grad_R when dA != NULL.  Iarray / Rarray as in
+ * monolis_hip_solve_R, for the adjoint solve. */
+int monolis_hip_adjoint_R(monolis_hip_matrix* M, const double* x, const double* g, double* lambda, double* dA,
+                          int symmetric, int* Iarray, double* Rarray);
+
 /* Bandwidth probe (no monolis counterpart): a plain streaming copy kernel
  * dst[i] = src[i] over n doubles, `reps` timed launches on the library stream;
  * *ms = average per launch.  Used to calibrate HBM byte counters and to state

@@ -62,6 +62,13 @@ SIGNATURES = [
     ("monolis_hip_batch_time", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double_p]),
     ("monolis_hip_solve_multi_R", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p]),
+    ("monolis_hip_matrix_set_scalar_R", c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double]),
+    ("monolis_hip_matrix_add_scalar_R", c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double]),
+    ("monolis_hip_matrix_get_scalar_R", c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double_p, c_int_p]),
+    ("monolis_hip_solve_transpose_R", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("monolis_hip_outer_grad_R", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("monolis_hip_adjoint_R", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p,
+                                      c_void_p]),
     ("monolis_hip_matvec_multi_R", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("monolis_hip_matvec_multi_S", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("monolis_hip_multi_num_configs", c_int, []),

@@ -276,6 +276,36 @@ class DeviceMatrix:
             cfgs.append((sb.value, wpc.value))
         return {"best": best.value, "ms": list(ms), "configs": cfgs}
 
+    # device-side scalar access (monolis_{set,add,get}_scalar_*_sparse_matrix_R, 0-based)
+    def set_scalar(self, i: int, j: int, sub_i: int, sub_j: int, val: float) -> None:
+        _lib.check(_lib.load().monolis_hip_matrix_set_scalar_R(ctypes.c_void_p(self.h), i, j, sub_i, sub_j,
+                                                               float(val)), "set_scalar")
+
+    def add_scalar(self, i: int, j: int, sub_i: int, sub_j: int, val: float) -> None:
+        _lib.check(_lib.load().monolis_hip_matrix_add_scalar_R(ctypes.c_void_p(self.h), i, j, sub_i, sub_j,
+                                                               float(val)), "add_scalar")
+
+    def get_scalar(self, i: int, j: int, sub_i: int, sub_j: int):
+        v, f = ctypes.c_double(), ctypes.c_int()
+        _lib.check(_lib.load().monolis_hip_matrix_get_scalar_R(ctypes.c_void_p(self.h), i, j, sub_i, sub_j,
+                                                               ctypes.byref(v), ctypes.byref(f)), "get_scalar")
+        return v.value, bool(f.value)
+
+    def solve_transpose(self, b: DeviceBuffer, x: DeviceBuffer, prm: Prm) -> None:
+        """A^T x = b (device transpose built on first use)."""
+        rc = _lib.load().monolis_hip_solve_transpose_R(ctypes.c_void_p(self.h), ctypes.c_void_p(b.ptr),
+                                                       ctypes.c_void_p(x.ptr), _p(prm.Iarray), _p(prm.Rarray))
+        _lib.check(rc, "monolis_hip_solve_transpose_R")
+
+    def adjoint(self, x: DeviceBuffer, g: DeviceBuffer, lam: DeviceBuffer, dA: Optional[DeviceBuffer], prm: Prm,
+                symmetric: bool = False) -> None:
+        """lam = A^{-T} g, dA = -lam x^T on the pattern (if dA is given)."""
+        rc = _lib.load().monolis_hip_adjoint_R(ctypes.c_void_p(self.h), ctypes.c_void_p(x.ptr),
+                                               ctypes.c_void_p(g.ptr), ctypes.c_void_p(lam.ptr),
+                                               ctypes.c_void_p(dA.ptr if dA is not None else None),
+                                               int(symmetric), _p(prm.Iarray), _p(prm.Rarray))
+        _lib.check(rc, "monolis_hip_adjoint_R")
+
     def matvec_multi(self, x: DeviceBuffer, y: DeviceBuffer, nrhs: int) -> None:
         """Y = A X for nrhs columns (NP x nrhs row-major device buffers)."""
         L = _lib.load()

@@ -13,6 +13,7 @@
 #include "mh_kernels.hpp"
 #include "mh_batch.hpp"
 #include "mh_multirhs.hpp"
+#include "mh_adjoint.hpp"
 
 using mh::Context;
 using mh::Matrix;
@@ -115,8 +116,9 @@ void print_after(const int* Iarray, const double* Rarray, const std::vector<doub
   exit(EXIT_FAILURE);
 }
 
-// Common device-resident solve.  Fills Iarray/Rarray; returns a status code.
-int solve_device(monolis_hip_matrix* H, const double* b, double* x, int* Iarray, double* Rarray,
+// Common device-resident solve with matrix A (the handle's matrix or its
+// transpose).  Fills Iarray/Rarray; returns a status code.
+int solve_device(Matrix& A, mh::Workspace& W, const double* b, double* x, int* Iarray, double* Rarray,
                  std::vector<double>* hist_out) {
   Context* c = nullptr;
   if (mh::context(&c)) return ST_HIP;
@@ -124,7 +126,7 @@ int solve_device(monolis_hip_matrix* H, const double* b, double* x, int* Iarray,
   int rc = check_method(P.method, P.precond);
   if (rc) return rc;
   mh::SolveResult res;
-  rc = mh::krylov_solve<double>(*H->M, P, b, x, H->W, &res, c->stream);
+  rc = mh::krylov_solve<double>(A, P, b, x, W, &res, c->stream);
   if (rc) return rc == -3 ? ST_PRECOND : rc == -4 ? ST_METHOD : rc == -2 ? ST_ARGS : ST_HIP;
   if (!res.zero_rhs) {
     Iarray[I_CUR_ITER] = res.iter;
@@ -290,7 +292,7 @@ int monolis_hip_solve_R(monolis_hip_matrix* H, const double* b, double* x, int* 
                         double* hist, int hist_len) {
   const double t0 = now();
   std::vector<double> h;
-  const int st = solve_device(H, b, x, Iarray, Rarray, &h);
+  const int st = solve_device(*H->M, H->W, b, x, Iarray, Rarray, &h);
   Iarray[I_IERR] = st;
   Rarray[R_TIME_SOL] = now() - t0;
   if (hist && hist_len > 0) {
@@ -657,6 +659,87 @@ int monolis_hip_matvec_multi_S(monolis_hip_matrix* H, int nrhs, const float* x, 
   return 0;
 }
 
+static int scalar_op(monolis_hip_matrix* H, int i, int j, int sub_i, int sub_j, double v, int mode, double* out,
+                     int* is_find) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  int found = 0;
+  double val = 0.0;
+  MH_TRY(mh::matrix_scalar(H->M, i, j, sub_i, sub_j, v, mode, &val, &found, c->stream));
+  if (out) *out = found ? val : 0.0;
+  if (is_find) *is_find = found;
+  if (!found && mode != 0) {
+    // monolis stops with monolis_stop_by_matrix_assemble (spmat_handler_util.f90:72)
+    mh::set_error("block (%d, %d) is not in the sparsity pattern", i, j);
+    return ST_ARGS;
+  }
+  return 0;
+}
+
+int monolis_hip_matrix_set_scalar_R(monolis_hip_matrix* H, int i, int j, int sub_i, int sub_j, double val) {
+  return scalar_op(H, i, j, sub_i, sub_j, val, 1, nullptr, nullptr);
+}
+
+int monolis_hip_matrix_add_scalar_R(monolis_hip_matrix* H, int i, int j, int sub_i, int sub_j, double val) {
+  return scalar_op(H, i, j, sub_i, sub_j, val, 2, nullptr, nullptr);
+}
+
+int monolis_hip_matrix_get_scalar_R(monolis_hip_matrix* H, int i, int j, int sub_i, int sub_j, double* val,
+                                    int* is_find) {
+  return scalar_op(H, i, j, sub_i, sub_j, 0.0, 0, val, is_find);
+}
+
+static int square_check(const Matrix& M) {
+  if (M.n != M.np) {
+    mh::set_error("adjoint operations need a square local matrix (n == np), got n=%d np=%d", M.n, M.np);
+    return ST_ARGS;
+  }
+  return 0;
+}
+
+int monolis_hip_solve_transpose_R(monolis_hip_matrix* H, const double* b, double* x, int* Iarray, double* Rarray) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_TRY(square_check(*H->M));
+  const double t0 = now();
+  MH_TRY(mh::matrix_transpose(H->M, c->stream));
+  const int st = solve_device(*H->M->transpose, H->W, b, x, Iarray, Rarray, nullptr);
+  Iarray[I_IERR] = st;
+  Rarray[R_TIME_SOL] = now() - t0;
+  return st < 0 ? st : 0;
+}
+
+int monolis_hip_outer_grad_R(monolis_hip_matrix* H, const double* lambda, const double* x, double* dA) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_TRY(square_check(*H->M));
+  MH_TRY(mh::outer_grad(*H->M, lambda, x, dA, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_adjoint_R(monolis_hip_matrix* H, const double* x, const double* g, double* lambda, double* dA,
+                          int symmetric, int* Iarray, double* Rarray) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_TRY(square_check(*H->M));
+  const double t0 = now();
+  Matrix* A = H->M;
+  if (!symmetric) {
+    MH_TRY(mh::matrix_transpose(H->M, c->stream));
+    A = H->M->transpose;
+  }
+  const int st = solve_device(*A, H->W, g, lambda, Iarray, Rarray, nullptr);
+  Iarray[I_IERR] = st;
+  if (st < 0) return st;
+  if (dA) {
+    MH_TRY(mh::outer_grad(*H->M, lambda, x, dA, c->stream));
+    MH_HIP(hipStreamSynchronize(c->stream));
+  }
+  Rarray[R_TIME_SOL] = now() - t0;
+  return 0;
+}
+
 int monolis_hip_multi_num_configs(void) { return mh::multi_num_configs(); }
 
 int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu) {
@@ -801,7 +884,7 @@ void monolis_solve_R_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZ
   }
   Rarray[R_TIME_PREP] = now() - tp0;
   std::vector<double> hist;
-  const int st = solve_device(H, db, dx, Iarray, Rarray, &hist);
+  const int st = solve_device(*H->M, H->W, db, dx, Iarray, Rarray, &hist);
   if (st < 0) {
     fail(st);
     return;

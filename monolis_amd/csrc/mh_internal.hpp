@@ -94,8 +94,11 @@ struct Matrix {
   int variant64 = 0, variant32 = 0;
   // multi-right-hand-side SpMM launch configuration per precision (-1 = default)
   int mrhs_cfg64 = -1, mrhs_cfg32 = -1;
-  // transpose (for adjoint solves of nonsymmetric systems), built lazily
+  // transpose (for adjoint solves of nonsymmetric systems), built lazily;
+  // tperm[p] = block of A that becomes block p of A^T, tiperm its inverse
   Matrix* transpose = nullptr;
+  DevArray<int> tperm, tiperm;
+  bool transpose_stale = false;   // values of A changed since the transpose was filled
   ~Matrix();
 };
 

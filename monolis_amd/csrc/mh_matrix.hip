@@ -210,6 +210,7 @@ static int fill_values(Matrix* M, const double* A, hipStream_t s, int fill_col) 
   MH_HIP(hipGetLastError());
   if (tmp.p) MH_HIP(hipStreamSynchronize(s));  // tmp is released on return
   M->diag_ready = M->diag32_ready = false;
+  M->transpose_stale = true;
   if (M->val32_ready) {
     M->val32_ready = false;
     MH_TRY(matrix_make_fp32(M, s));

@@ -176,3 +176,49 @@ def time_matvec(mat, x, reps):
     y = np.zeros(mat.n * mat.ndof, np.float64)
     x = np.ascontiguousarray(x, np.float64)
     return float(lib().orc_time_matvec(int(reps), mat.n, mat.ndof, mat.index, mat.item, mat.A, x, y))
+
+
+# ---------------------------------------------------------------------------
+# FP64 CPU adjoint of x = A^{-1} b (test infrastructure, like the rest).
+# monolis has no adjoint; this restates the textbook reverse-mode rule
+#   A^T lambda = dL/dx,  dL/db = lambda,  dL/dA_ij = -lambda_i x_j
+# on monolis's BCSR layout (src/matrix/def_mat.f90 arrays index / item / A).
+# Pinned in tests/test_oracle_adjoint.py against dense solves, scipy's
+# transpose and central finite differences of the loss.
+# ---------------------------------------------------------------------------
+def _rows_of_entries(mat):
+    return np.repeat(np.arange(mat.n, dtype=np.int64), np.diff(np.asarray(mat.index, np.int64)))
+
+
+def transpose(mat):
+    """BCSR transpose (square, one rank): block (j, i) of A^T = (block (i, j) of A)^T,
+    block columns ascending per row."""
+    assert mat.ndof > 0 and mat.n == mat.np_
+    n, b = mat.n, mat.ndof
+    rows = _rows_of_entries(mat)
+    cols = np.asarray(mat.item, np.int64) - 1
+    order = np.lexsort((rows, cols))               # by column, then row
+    counts = np.bincount(cols, minlength=n)
+    index = np.zeros(n + 1, np.int32)
+    index[1:] = np.cumsum(counts)
+    item = (rows[order] + 1).astype(np.int32)
+    A = np.asarray(mat.A, np.float64).reshape(-1, b, b)[order].transpose(0, 2, 1).reshape(-1).copy()
+    return type(mat)(n, n, b, index, item, A).ensure_dof_index()
+
+
+def outer_grad(mat, lam, x):
+    """dA[e*b*b + r*b + c] = -lam[i*b + r] * x[j*b + c] for every stored block e = (i, j)."""
+    b = mat.ndof
+    rows = _rows_of_entries(mat)
+    cols = np.asarray(mat.item, np.int64) - 1
+    lb = np.asarray(lam, np.float64).reshape(-1, b)[rows]
+    xb = np.asarray(x, np.float64).reshape(-1, b)[cols]
+    return (-(lb[:, :, None] * xb[:, None, :])).reshape(-1)
+
+
+def adjoint(mat, x, g, symmetric=False, **kw):
+    """lambda = A^{-T} g by the oracle's monolis solver (A itself when symmetric),
+    and dL/dA on the pattern.  Returns (lambda, dA, status, Iarray)."""
+    At = mat if symmetric else transpose(mat)
+    lam, st, I, R = solve(At, g, **kw)
+    return lam, outer_grad(mat, lam, x), st, I
