@@ -11,6 +11,17 @@ FP64 versus the FP32-inner / FP64-refined CG, with iteration counts, the error
 of the refined solution against the FP64 one, and the CPU baseline (the
 oracle's OpenMP FP64 SpMV and CG on the host cores, a bounded sample).
 
+Further workloads (extra keys, skipped with --no-extras):
+  precision        the same system scaled D A D (D log-uniform over [1, 1e3]),
+                   FP64 CG vs FP32-refined CG to 1e-10; 128^3 convection-
+                   diffusion (cell Peclet 0.5), FP64 vs FP32-refined BiCGSTAB;
+  batching         128^3 Poisson with 64 seeded right-hand sides as one
+                   multi-RHS CG (FP64 and FP32-refined) vs 64 single solves;
+                   512 Poisson systems with edges 10..40 in one persistent,
+                   graph-captured launch (autotuned);
+  differentiation  64^3 BCSR3 SPD and 64^3 convection-diffusion: forward solve
+                   and adjoint (A^T lambda = dL/dx plus the -lambda x^T kernel).
+
     python bench.py [--gpus 1] [--steps 200] [--warmup 20]
 """
 from __future__ import annotations
@@ -58,6 +69,8 @@ def main() -> int:
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--variant", type=int, default=-1,
                     help="force this SpMV kernel variant instead of running the autotuner")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the precision / batching / differentiation workloads")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the streaming-copy probe (1 GiB read + 1 GiB write per launch)")
     args = ap.parse_args()
@@ -222,8 +235,216 @@ def main() -> int:
             out["cg"]["fp64_iters_oracle"] = int(I[4])
             log(f"CPU CG: {int(I[4])} iterations in {tcg:.2f}s")
 
+    if not args.no_extras and not args.no_solve:
+        dm = dx = dy = dx32 = dy32 = None             # release the headline buffers first
+        out["precision"] = bench_precision(m, xt, args)
+        m = None
+        out["batching"] = bench_batching(args)
+        out["differentiation"] = bench_differentiation(args)
+
     print(json.dumps(out), flush=True)
     return 0
+
+
+def _prm(method=1, precond=1, tol=1e-8, max_iter=5000):
+    from monolis_amd import api
+    prm = api.Prm().quiet()
+    prm.Iarray[1], prm.Iarray[2], prm.Iarray[3] = method, precond, max_iter
+    prm.Rarray[1] = tol
+    return prm
+
+
+def _timed(fn, reps=1):
+    """Wall time of the last of `reps` calls (the first ones warm up)."""
+    t = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = fn()
+        t = time.perf_counter() - t0
+    return t, r
+
+
+def bench_precision(m, xt, args):
+    """FP32-inner / FP64-refined solves on an ill-conditioned twin of the headline
+    system and on the nonsymmetric convection-diffusion problem."""
+    import numpy as np
+    from monolis_amd import api, workloads as W
+    res = {}
+    t0 = time.time()
+    ms_ = W.scale_symmetric(m, 1e3, seed=args.seed + 5)
+    b = W.rhs_from_solution(ms_, xt)
+    log(f"built D A D twin in {time.time() - t0:.1f}s")
+    dm = api.DeviceMatrix(ms_)
+    db = api.DeviceBuffer.from_numpy(b)
+    x64, xr = api.DeviceBuffer(ms_.npndof), api.DeviceBuffer(ms_.npndof)
+    p64 = _prm(tol=1e-10)
+    t64, _ = _timed(lambda: dm.solve(db, x64, p64), reps=2)
+    it64 = int(p64.Iarray[4])
+    pr = _prm(tol=1e-10)
+    tr, st = _timed(lambda: dm.solve_refined(db, xr, pr, inner_tol=args.inner_tol), reps=2)
+    A = ms_.to_scipy()
+    a64, ar = x64.numpy(), xr.numpy()
+    res["ill_conditioned_bcsr3"] = {
+        "workload": f"D A D of the headline system, D log-uniform [1, 1e3], block-Jacobi CG to 1e-10",
+        "fp64_ms": round(t64 * 1e3, 3), "fp64_iters": it64,
+        "refined_ms": round(tr * 1e3, 3), "refined_outer": st["outer"], "refined_inner_total": st["inner_total"],
+        "speedup": round(t64 / tr, 3),
+        "fp64_true_resid": float(np.linalg.norm(b - A @ a64) / np.linalg.norm(b)),
+        "refined_true_resid": float(np.linalg.norm(b - A @ ar) / np.linalg.norm(b)),
+        "refined_rel_err_vs_fp64": float(np.linalg.norm(ar - a64) / np.linalg.norm(a64)),
+        "refined_rel_err_vs_xtrue": float(np.linalg.norm(ar - xt) / np.linalg.norm(xt)),
+    }
+    log(f"precision/D A D: FP64 {t64 * 1e3:.1f} ms ({it64} it), refined {tr * 1e3:.1f} ms "
+        f"({st['outer']} outer / {st['inner_total']} inner), err vs FP64 "
+        f"{res['ill_conditioned_bcsr3']['refined_rel_err_vs_fp64']:.2e}")
+    del dm, db, x64, xr, ms_, A
+    cd = W.convdiff7(128, peclet=0.5)
+    b = W.uniform_vector(cd.nndof, seed=args.seed + 6)
+    dm = api.DeviceMatrix(cd)
+    db = api.DeviceBuffer.from_numpy(b)
+    x64, xr = api.DeviceBuffer(cd.npndof), api.DeviceBuffer(cd.npndof)
+    p64 = _prm(method=5, tol=1e-8)
+    t64, _ = _timed(lambda: dm.solve(db, x64, p64), reps=2)
+    pr = _prm(method=5, tol=1e-8)
+    tr, st = _timed(lambda: dm.solve_refined(db, xr, pr, inner_tol=args.inner_tol), reps=2)
+    A = cd.to_scipy()
+    a64, ar = x64.numpy(), xr.numpy()
+    res["convdiff_128_bicgstab"] = {
+        "workload": "7-point convection-diffusion 128^3 (cell Peclet 0.5), diagonal-scaled BiCGSTAB to 1e-8",
+        "fp64_ms": round(t64 * 1e3, 3), "fp64_iters": int(p64.Iarray[4]),
+        "refined_ms": round(tr * 1e3, 3), "refined_outer": st["outer"], "refined_inner_total": st["inner_total"],
+        "speedup": round(t64 / tr, 3),
+        "fp64_true_resid": float(np.linalg.norm(b - A @ a64) / np.linalg.norm(b)),
+        "refined_true_resid": float(np.linalg.norm(b - A @ ar) / np.linalg.norm(b)),
+        "refined_rel_err_vs_fp64": float(np.linalg.norm(ar - a64) / np.linalg.norm(a64)),
+    }
+    log(f"precision/convdiff: FP64 {t64 * 1e3:.1f} ms, refined {tr * 1e3:.1f} ms")
+    return res
+
+
+def bench_batching(args):
+    """Many right-hand sides (SpMM) and many small systems (one persistent launch)."""
+    import numpy as np
+    from monolis_amd import api, workloads as W
+    res = {}
+    k = 64
+    pm = W.poisson7(128)
+    B = np.random.default_rng(args.seed + 7).uniform(-1, 1, size=(pm.n, k))
+    dm = api.DeviceMatrix(pm)
+    tune64 = dm.autotune_multi(k, fp32=False, reps=5)
+    tune32 = dm.autotune_multi(k, fp32=True, reps=5)
+    db = api.DeviceBuffer.from_numpy(B)
+    dx = api.DeviceBuffer(pm.n * k)
+    p = _prm(tol=1e-8)
+    t64, r64 = _timed(lambda: dm.solve_multi(db, dx, k, p), reps=2)
+    X64 = dx.numpy().reshape(pm.n, k)
+    pr = _prm(tol=1e-8)
+    tr, rr = _timed(lambda: dm.solve_multi_refined(db, dx, k, pr, inner_tol=1e-5), reps=2)
+    Xr = dx.numpy().reshape(pm.n, k)
+    # the same 64 columns as independent single-RHS solves
+    b1, x1 = api.DeviceBuffer(pm.n), api.DeviceBuffer(pm.n)
+    ps = _prm(tol=1e-8)
+    b1.upload(np.ascontiguousarray(B[:, 0]))
+    dm.solve(b1, x1, ps)                              # warm-up
+    t = time.perf_counter()
+    iters1 = []
+    for j in range(k):
+        b1.upload(np.ascontiguousarray(B[:, j]))
+        dm.solve(b1, x1, ps)
+        iters1.append(int(ps.Iarray[4]))
+    tseq = time.perf_counter() - t
+    spmm_bytes = (pm.nz * 12 + (pm.n + 1) * 4) + 2 * pm.n * k * 8
+    best_ms = tune64["ms"][tune64["best"]]
+    res["multi_rhs_poisson_128"] = {
+        "workload": "7-point Poisson 128^3 CSR, 64 seeded uniform RHS, DIAG CG to 1e-8",
+        "fp64_ms": round(t64 * 1e3, 2), "fp64_iters_max": int(r64["iters"].max()),
+        "fp64_iters_min": int(r64["iters"].min()),
+        "refined_ms": round(tr * 1e3, 2), "refined_outer": rr["outer"],
+        "refined_inner_max": int(rr["inner_iters"].max()), "refined_inner_tol": 1e-5,
+        "refined_rel_err_vs_fp64": float(np.linalg.norm(Xr - X64) / np.linalg.norm(X64)),
+        "sequential_64_single_solves_ms": round(tseq * 1e3, 2),
+        "single_iters_max": max(iters1),
+        "speedup_fp64_vs_sequential": round(tseq / t64, 3),
+        "speedup_refined_vs_sequential": round(tseq / tr, 3),
+        "spmm_ms": round(best_ms, 4), "spmm_algorithmic_gbs": round(spmm_bytes / (best_ms * 1e-3) / 1e9, 1),
+        "spmm_config_fp64": tune64["configs"][tune64["best"]], "spmm_config_fp32": tune32["configs"][tune32["best"]],
+    }
+    log(f"batching/multi-RHS: FP64 {t64 * 1e3:.0f} ms, refined {tr * 1e3:.0f} ms, 64 single solves "
+        f"{tseq * 1e3:.0f} ms, SpMM {best_ms:.3f} ms")
+    del dm, db, dx, pm, B
+    bm, sys_row = W.poisson_batch(512, 10, 40, seed=args.seed + 8)
+    bb = W.uniform_vector(bm.n, seed=args.seed + 9)
+    dbat = api.DeviceBatch(bm, sys_row)
+    db = api.DeviceBuffer.from_numpy(bb)
+    dx = api.DeviceBuffer(bm.n)
+    p = _prm(tol=1e-8)
+    tune = dbat.autotune(db, dx, p, reps=2)
+    ms = dbat.time(db, dx, p, reps=5)
+    iters, resid, status = dbat.solve(db, dx, p)
+    # a sample of the same systems solved one at a time through the single-system path
+    sample = list(range(0, 512, 32))
+    tsmp = 0.0
+    for s_ in sample:
+        r0, r1 = int(sys_row[s_]), int(sys_row[s_ + 1])
+        j0, j1 = int(bm.index[r0]), int(bm.index[r1])
+        from monolis_amd.matrix import Matrix
+        sm = Matrix(r1 - r0, r1 - r0, 1, (bm.index[r0:r1 + 1] - j0).astype(np.int32),
+                    (bm.item[j0:j1] - r0).astype(np.int32), bm.A[j0:j1].copy()).ensure_dof_index()
+        sdm = api.DeviceMatrix(sm)
+        sb, sx = api.DeviceBuffer.from_numpy(bb[r0:r1]), api.DeviceBuffer(r1 - r0)
+        ps = _prm(tol=1e-8)
+        sdm.solve(sb, sx, ps)
+        t = time.perf_counter()
+        sdm.solve(sb, sx, ps)
+        tsmp += time.perf_counter() - t
+    est_seq = tsmp * 512 / len(sample)
+    res["batch_512_poisson"] = {
+        "workload": f"512 independent 7-point Poisson systems, edges U{{10..40}}, {bm.n} DOF total, DIAG CG to 1e-8",
+        "ms": round(ms, 3), "all_converged": bool(np.all(status == 0)),
+        "iters_min": int(iters.min()), "iters_max": int(iters.max()),
+        "config": tune["configs"][tune["best"]], "autotune_ms": [round(v, 3) for v in tune["ms"]],
+        "sequential_est_ms": round(est_seq * 1e3, 1),
+        "sequential_sample": f"{len(sample)} systems solved one at a time, scaled to 512",
+        "speedup_vs_sequential_est": round(est_seq * 1e3 / ms, 2),
+    }
+    log(f"batching/512 systems: {ms:.2f} ms per graph replay (sequential est {est_seq * 1e3:.0f} ms)")
+    return res
+
+
+def bench_differentiation(args):
+    """Forward solve and adjoint (A^T lambda = dL/dx, dL/dA = -lambda x^T) at 64^3."""
+    import numpy as np
+    from monolis_amd import api, workloads as W
+    res = {}
+    for name, mat, method, sym in (("bcsr3_spd_64", W.stencil27_bcsr3(64, seed=args.seed + 10), 1, True),
+                                   ("convdiff_64", W.convdiff7(64, peclet=0.5), 5, False)):
+        mlen = mat.nndof
+        rng = np.random.default_rng(args.seed + 11)
+        b = rng.uniform(-1, 1, mlen)
+        xtg = rng.uniform(-1, 1, mlen)
+        dm = api.DeviceMatrix(mat)
+        db, dx = api.DeviceBuffer.from_numpy(b), api.DeviceBuffer(mat.npndof)
+        p = _prm(method=method, tol=1e-10)
+        tf, _ = _timed(lambda: dm.solve(db, dx, p), reps=2)
+        itf = int(p.Iarray[4])
+        x = dx.numpy()
+        dg = api.DeviceBuffer.from_numpy(2.0 * (x - xtg))
+        dl, dA = api.DeviceBuffer(mat.npndof), api.DeviceBuffer(len(mat.A))
+        pa = _prm(method=method, tol=1e-10)
+        t = time.perf_counter()
+        dm.adjoint(dx, dg, dl, dA, pa, symmetric=sym)   # first call builds the transpose (nonsymmetric)
+        t_first = time.perf_counter() - t
+        ta, _ = _timed(lambda: dm.adjoint(dx, dg, dl, dA, pa, symmetric=sym), reps=2)
+        res[name] = {
+            "forward_ms": round(tf * 1e3, 3), "forward_iters": itf,
+            "adjoint_ms": round(ta * 1e3, 3), "adjoint_iters": int(pa.Iarray[4]),
+            "adjoint_first_call_ms": round(t_first * 1e3, 3),
+            "method": "CG" if method == 1 else "BiCGSTAB", "transpose": "A itself (SPD)" if sym else "device transpose",
+            "grad_values": int(len(mat.A)),
+        }
+        log(f"differentiation/{name}: forward {tf * 1e3:.1f} ms, adjoint {ta * 1e3:.1f} ms "
+            f"(first {t_first * 1e3:.1f} ms)")
+    return res
 
 
 if __name__ == "__main__":
