@@ -83,6 +83,15 @@ int monolis_hip_memset(void* dst, int value, size_t bytes);
  * (src/solver/solver.f90:102-156) done once instead of per solve. */
 int monolis_hip_matrix_create(monolis_hip_matrix** out, int n, int np, int nz, int n_dof, const int* index,
                               const int* item, const double* A);
+/* Variable block sizes (monolis n_dof = -1): n_dof_list[NP], n_dof_index[NP+1]
+ * (0-based DOF offsets) and n_dof_index2[NZ+1] (0-based value offsets) as
+ * monolis builds them (src/matrix/spmat_nzpattern_util.f90:62-90); block (i, j)
+ * is n_i x n_j row-major at A[n_dof_index2[e]].  Stored as the scalar expansion
+ * (one row per DOF) with the diag_V block-Jacobi factors (blocks up to 16).
+ * The flat entry points take this path when n_dof = -1. */
+int monolis_hip_matrix_create_V(monolis_hip_matrix** out, int n, int np, int nz, const int* n_dof_list,
+                                const int* n_dof_index, const int* n_dof_index2, const int* index, const int* item,
+                                const double* A);
 int monolis_hip_matrix_destroy(monolis_hip_matrix* M);
 /* New values, same pattern (A(theta) updates). */
 int monolis_hip_matrix_set_values(monolis_hip_matrix* M, const double* A);

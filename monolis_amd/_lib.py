@@ -62,6 +62,8 @@ SIGNATURES = [
     ("monolis_hip_batch_time", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double_p]),
     ("monolis_hip_solve_multi_R", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p]),
+    ("monolis_hip_matrix_create_V", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_void_p, c_void_p,
+                                            c_void_p, c_void_p, c_void_p, c_void_p]),
     ("monolis_hip_matrix_set_scalar_R", c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double]),
     ("monolis_hip_matrix_add_scalar_R", c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double]),
     ("monolis_hip_matrix_get_scalar_R", c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double_p, c_int_p]),
@@ -89,11 +91,14 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    # MONOLIS_HIP_LIBRARY selects another build of the same library (e.g. one
+    # compiled with different kernel options); it must export the same symbols.
+    path = os.environ.get("MONOLIS_HIP_LIBRARY", LIB_PATH)
+    if not os.path.exists(path):
         raise MonolisHipError(
-            f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+            f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
             "or `python -m monolis_amd.build` (the HIP path has no fallback)")
-    L = ctypes.CDLL(LIB_PATH)
+    L = ctypes.CDLL(path)
     for name, res, args in SIGNATURES:
         fn = getattr(L, name)
         fn.restype = res

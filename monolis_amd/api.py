@@ -196,9 +196,16 @@ class DeviceMatrix:
         L = _lib.load()
         h = ctypes.c_void_p()
         mat.ensure_dof_index()
-        _lib.check(L.monolis_hip_matrix_create(ctypes.byref(h), mat.n, mat.np_, mat.nz, mat.ndof,
-                                               _p(mat.index), _p(mat.item), _p(mat.A)),
-                   "monolis_hip_matrix_create")
+        if mat.ndof == -1:
+            _lib.check(L.monolis_hip_matrix_create_V(
+                ctypes.byref(h), mat.n, mat.np_, mat.nz, _p(np.ascontiguousarray(mat.n_dof_list, np.int32)),
+                _p(np.ascontiguousarray(mat.n_dof_index, np.int32)),
+                _p(np.ascontiguousarray(mat.n_dof_index2, np.int32)), _p(mat.index), _p(mat.item), _p(mat.A)),
+                "monolis_hip_matrix_create_V")
+        else:
+            _lib.check(L.monolis_hip_matrix_create(ctypes.byref(h), mat.n, mat.np_, mat.nz, mat.ndof,
+                                                   _p(mat.index), _p(mat.item), _p(mat.A)),
+                       "monolis_hip_matrix_create")
         self.h = h.value
         self.mat = mat
         self.m = mat.nndof

@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmonolis_hip.so")
 
-SOURCES = ["mh_matrix.hip", "mh_krylov.hip", "mh_capi.hip", "mh_sorted.hip", "mh_batch.hip", "mh_multirhs.hip", "mh_adjoint.hip"]
+SOURCES = ["mh_matrix.hip", "mh_krylov.hip", "mh_capi.hip", "mh_sorted.hip", "mh_batch.hip", "mh_multirhs.hip", "mh_adjoint.hip", "mh_vblock.hip"]
 
 ARCH = os.environ.get("MONOLIS_HIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -32,9 +32,12 @@ CFLAGS = [
 ]
 
 
+EXTRA = os.environ.get("MONOLIS_HIP_CFLAGS", "").split()   # e.g. -DMH_STORE_CPOL=2 for kernel-option builds
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *CFLAGS, *EXTRA, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         msg = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
@@ -77,22 +80,25 @@ def build_callers(verbose: bool = True) -> list:
     return built
 
 
-def build(verbose: bool = True) -> str:
+def build(verbose: bool = True, out: str = None) -> str:
+    """Compile and link the library (to `out` instead of the in-tree path if given)."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
+    target = out or LIB
+    tmp = target + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, target)
     if verbose:
-        print(f"[monolis_amd] built {LIB}")
-    build_callers(verbose)
-    return LIB
+        print(f"[monolis_amd] built {target}")
+    if out is None:
+        build_callers(verbose)
+    return target
 
 
 if __name__ == "__main__":

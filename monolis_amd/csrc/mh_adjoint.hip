@@ -125,6 +125,10 @@ __global__ void k_scalar_access(int ci, int cj, int si, int sj, int b, double v,
 }  // namespace
 
 int matrix_transpose(Matrix* M, hipStream_t s) {
+  if (M->vb) {
+    set_error("transposes of variable-block matrices are not supported");
+    return -2;
+  }
   if (M->np != M->n) {
     set_error("transpose needs a square local matrix (np == n), got n=%d np=%d", M->n, M->np);
     return -2;
@@ -189,6 +193,10 @@ int matrix_transpose(Matrix* M, hipStream_t s) {
 }
 
 int outer_grad(const Matrix& M, const double* lambda, const double* x, double* dA, hipStream_t s) {
+  if (M.vb) {
+    set_error("the pattern gradient of variable-block matrices is not supported");
+    return -2;
+  }
   const size_t total = (size_t)M.nz * M.b * M.b;
   if (total == 0) return 0;
   size_t g = (total + kBlock - 1) / kBlock;
@@ -200,6 +208,10 @@ int outer_grad(const Matrix& M, const double* lambda, const double* x, double* d
 
 int matrix_scalar(Matrix* M, int ci, int cj, int si, int sj, double v, int mode, double* value, int* found,
                   hipStream_t s) {
+  if (M->vb) {
+    set_error("scalar access to variable-block matrices is not supported (use set_values)");
+    return -2;
+  }
   if (ci < 0 || ci >= M->n || cj < 0 || cj >= M->np || si < 0 || si >= M->b || sj < 0 || sj >= M->b) {
     set_error("scalar access (%d, %d, %d, %d) outside the matrix (n=%d np=%d n_dof=%d)", ci, cj, si, sj, M->n,
               M->np, M->b);

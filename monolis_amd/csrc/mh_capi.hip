@@ -206,11 +206,29 @@ int monolis_hip_matrix_create(monolis_hip_matrix** out, int n, int np, int nz, i
   }
   *out = nullptr;
   if (n_dof == -1) {
-    mh::set_error("variable block size (n_dof = -1) is not supported by the GPU path");
+    mh::set_error("variable block sizes (n_dof = -1) need monolis_hip_matrix_create_V");
     return ST_ARGS;
   }
   Matrix* M = nullptr;
   MH_TRY(mh::matrix_create(&M, n, np, nz, n_dof, index, item, A, c->stream));
+  auto* H = new monolis_hip_matrix();
+  H->M = M;
+  *out = H;
+  return 0;
+}
+
+int monolis_hip_matrix_create_V(monolis_hip_matrix** out, int n, int np, int nz, const int* n_dof_list,
+                                const int* n_dof_index, const int* n_dof_index2, const int* index, const int* item,
+                                const double* A) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (!out) {
+    mh::set_error("null output handle");
+    return ST_ARGS;
+  }
+  *out = nullptr;
+  Matrix* M = nullptr;
+  MH_TRY(mh::matrix_create_vblock(&M, n, np, nz, n_dof_list, n_dof_index, n_dof_index2, index, item, A, c->stream));
   auto* H = new monolis_hip_matrix();
   H->M = M;
   *out = H;
@@ -228,7 +246,8 @@ int monolis_hip_matrix_destroy(monolis_hip_matrix* H) {
 int monolis_hip_matrix_set_values(monolis_hip_matrix* H, const double* A) {
   Context* c = nullptr;
   MH_TRY(mh::context(&c));
-  MH_TRY(mh::matrix_set_values(H->M, A, c->stream));
+  if (H->M->vb) MH_TRY(mh::matrix_set_values_vblock(H->M, A, c->stream));
+  else MH_TRY(mh::matrix_set_values(H->M, A, c->stream));
   MH_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -238,7 +257,7 @@ int monolis_hip_matrix_info(const monolis_hip_matrix* H, long long* info) {
   info[0] = M->n;
   info[1] = M->np;
   info[2] = M->nz;
-  info[3] = M->b;
+  info[3] = M->vb ? -1 : M->b;
   info[4] = M->nslices;
   info[5] = (long long)M->nslots;
   info[6] = (long long)(M->nslots * mh::kWave * (size_t)M->b * M->b * 8 + M->nslots * mh::kWave * 4);
@@ -816,7 +835,7 @@ void monolis_solve_R_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZ
                             int* item, int my_rank, int comm, int comm_size, int recv_n_neib, int recv_nitem,
                             int* recv_neib_pe, int* recv_index, int* recv_item, int send_n_neib, int send_nitem,
                             int* send_neib_pe, int* send_index, int* send_item, int* Iarray, double* Rarray) {
-  (void)n_dof_list; (void)n_dof_index; (void)n_dof_index2; (void)NZNDOF2; (void)comm;
+  (void)NZNDOF2; (void)comm;
   (void)recv_n_neib; (void)recv_nitem; (void)recv_neib_pe; (void)recv_index; (void)recv_item;
   (void)send_n_neib; (void)send_nitem; (void)send_neib_pe; (void)send_index; (void)send_item;
   const double t0 = now();
@@ -848,7 +867,10 @@ void monolis_solve_R_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZ
   // staging: matrix (SELL build) and vectors, then the preconditioner
   const double tp0 = now();
   monolis_hip_matrix* H = nullptr;
-  rc = monolis_hip_matrix_create(&H, n, np, nz, n_dof, index, item, A);
+  if (n_dof == -1)
+    rc = monolis_hip_matrix_create_V(&H, n, np, nz, n_dof_list, n_dof_index, n_dof_index2, index, item, A);
+  else
+    rc = monolis_hip_matrix_create(&H, n, np, nz, n_dof, index, item, A);
   if (rc) {
     fail(rc == -2 ? ST_ARGS : ST_HIP);
     return;
@@ -910,17 +932,21 @@ void monolis_matvec_product_R_c_main(int n, int np, int nz, int n_dof, int NPNDO
                                      int recv_n_neib, int recv_nitem, int* recv_neib_pe, int* recv_index,
                                      int* recv_item, int send_n_neib, int send_nitem, int* send_neib_pe,
                                      int* send_index, int* send_item) {
-  (void)n_dof_list; (void)n_dof_index; (void)n_dof_index2; (void)NZNDOF2; (void)my_rank; (void)comm;
+  (void)NZNDOF2; (void)my_rank; (void)comm;
   (void)recv_n_neib; (void)recv_nitem; (void)recv_neib_pe; (void)recv_index; (void)recv_item;
   (void)send_n_neib; (void)send_nitem; (void)send_neib_pe; (void)send_index; (void)send_item;
   if (comm_size > 1) error_stop("monolis_matvec_product_R (GPU path): comm_size > 1 is not supported");
   Context* c = nullptr;
   if (mh::context(&c)) error_stop(mh::last_error());
   monolis_hip_matrix* H = nullptr;
-  if (monolis_hip_matrix_create(&H, n, np, nz, n_dof, index, item, A)) error_stop(mh::last_error());
+  const int rc = n_dof == -1 ? monolis_hip_matrix_create_V(&H, n, np, nz, n_dof_list, n_dof_index, n_dof_index2,
+                                                           index, item, A)
+                             : monolis_hip_matrix_create(&H, n, np, nz, n_dof, index, item, A);
+  if (rc) error_stop(mh::last_error());
   std::unique_ptr<monolis_hip_matrix> guard(H);
   const size_t vec = (size_t)NPNDOF;
-  const size_t m = (size_t)n * n_dof;
+  // internal entries: N * n_dof, or n_dof_index(N) for variable blocks (the scalar rows)
+  const size_t m = (size_t)H->M->n * (H->M->vb ? 1 : H->M->b);
   const double* dx = x;
   double* dy = y;
   mh::DevArray<double> bx, by;

@@ -74,6 +74,23 @@ struct KState {
 };
 
 // ---------------------------------------------------------------------------
+// Variable block sizes (monolis n_dof = -1): the matrix is stored as the scalar
+// expansion (one SELL row per DOF, columns in monolis's block-then-entry order)
+// and keeps the block structure for the diag_V preconditioner.
+// ---------------------------------------------------------------------------
+constexpr int kMaxVBlock = 16;  // largest variable block the diag_V kernels take
+struct VBlock {
+  int nb = 0, npb = 0, nzb = 0, maxb = 0;  // block rows (internal / total), stored blocks, largest block
+  long long nzs = 0;                       // scalar entries = n_dof_index2[nz]
+  DevArray<int> list, dof_index, index, item, index2;  // monolis arrays, device copies
+  DevArray<long long> doff;                // offset of block row i's factor (prefix of n_i^2)
+  DevArray<int> srowptr;                   // scalar row offsets of the expansion
+  DevArray<double> D64;                    // diag_V factors (no-pivot LU, reciprocal pivots)
+  DevArray<float> D32;
+  bool d32_ready = false;
+};
+
+// ---------------------------------------------------------------------------
 // Device matrix: SELL-64 pattern + FP64 (and optionally FP32) values, plus the
 // CSR arrays kept for the preconditioner setup, transposes and gradients.
 // ---------------------------------------------------------------------------
@@ -99,12 +116,24 @@ struct Matrix {
   Matrix* transpose = nullptr;
   DevArray<int> tperm, tiperm;
   bool transpose_stale = false;   // values of A changed since the transpose was filled
+  VBlock* vb = nullptr;           // non-null for variable block sizes
   ~Matrix();
 };
 
 int matrix_create(Matrix** out, int n, int np, int nz, int b, const int* index, const int* item,
                   const double* A, hipStream_t s);
 int matrix_set_values(Matrix* M, const double* A, hipStream_t s);
+// Variable block sizes (monolis n_dof = -1): arrays as monolis stores them
+// (host or device pointers); see VBlock.
+int matrix_create_vblock(Matrix** out, int n, int np, int nz, const int* n_dof_list, const int* n_dof_index,
+                         const int* n_dof_index2, const int* index, const int* item, const double* A,
+                         hipStream_t s);
+int matrix_set_values_vblock(Matrix* M, const double* A, hipStream_t s);
+// FP32 copy of the diag_V factors (the FP64 ones are built with the values).
+int vdiag_make_fp32(Matrix& M, hipStream_t s);
+// y = M_V^-1 x (diag_V apply) over all block rows.
+template <typename T>
+int vdiag_apply(const Matrix& M, const T* x, T* y, hipStream_t s);
 int matrix_make_fp32(Matrix* M, hipStream_t s);
 int matrix_transpose(Matrix* M, hipStream_t s);
 

@@ -88,21 +88,111 @@ __device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane,
   }
 }
 
+// Software-pipelined form of sell_row: the column ids and values of step j+1
+// are issued right after step j's gathers and before step j's products, so a
+// wavefront keeps the next step of the matrix stream in flight while it waits
+// on the (L2-resident) x gathers.  The wait counter of vector loads is in
+// order, so the gathers must be issued before the prefetch (scheduling
+// barriers keep that order), and the two step buffers alternate by name
+// (ping-pong) instead of being copied, which would wait for the whole
+// prefetch at the top of every step.  Same arithmetic order as sell_row.
+template <typename TV, int B, int U>
+struct SellStep {
+  int c[U];
+  TV a[U][B * B];
+};
+
+template <typename TV, typename TA, int B, int U, bool NT, class XG>
+__device__ __forceinline__ void sell_row_pipe(const SellView<TV>& M, int s, int lane, int len, const XG& xg,
+                                              TA (&acc)[B]) {
+  const int k0 = M.slice_ptr[s];
+  const int w = M.slice_ptr[s + 1] - k0;
+  const int* colp = M.col + (size_t)k0 * kWave + lane;
+  const TV* valp = M.val + (size_t)k0 * kWave * (B * B) + lane;
+  const int ns = w / U;  // full steps
+  auto fetch = [&](SellStep<TV, B, U>& st, int j) {
+    const int k = j * U;
+#pragma unroll
+    for (int u = 0; u < U; ++u) st.c[u] = ld_val<NT>(colp + (size_t)(k + u) * kWave);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int e = 0; e < B * B; ++e) st.a[u][e] = ld_val<NT>(valp + ((size_t)(k + u) * (B * B) + e) * kWave);
+  };
+  // step j from buffer `cur`; if nxt != nullptr, prefetch step jn into it
+  auto step = [&](const SellStep<TV, B, U>& cur, int j, SellStep<TV, B, U>* nxt, int jn) {
+    TA xv[U][B];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xg(cur.c[u], xv[u]);
+    __builtin_amdgcn_sched_barrier(0);
+    if (nxt) fetch(*nxt, jn);
+    __builtin_amdgcn_sched_barrier(0);
+    const int k = j * U;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool on = (k + u) < len;
+#pragma unroll
+      for (int r = 0; r < B; ++r) {
+        TA t = acc[r];
+#pragma unroll
+        for (int cc = 0; cc < B; ++cc) t = fma((TA)cur.a[u][r * B + cc], xv[u][cc], t);
+        acc[r] = on ? t : acc[r];
+      }
+    }
+  };
+  SellStep<TV, B, U> b0, b1;
+  int j = 0;
+  if (ns > 0) {
+    fetch(b0, 0);
+    for (; j + 2 < ns; j += 2) {
+      step(b0, j, &b1, j + 1);
+      step(b1, j + 1, &b0, j + 2);
+    }
+    if (ns - j == 2) {
+      step(b0, j, &b1, j + 1);
+      step(b1, j + 1, nullptr, 0);
+    } else {
+      step(b0, j, nullptr, 0);
+    }
+  }
+  for (int k = ns * U; k < w; ++k) {
+    const int c = ld_val<NT>(colp + (size_t)k * kWave);
+    TV a[B * B];
+#pragma unroll
+    for (int e = 0; e < B * B; ++e) a[e] = ld_val<NT>(valp + ((size_t)k * (B * B) + e) * kWave);
+    TA xv[B];
+    xg(c, xv);
+    const bool on = k < len;
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+      TA t = acc[r];
+#pragma unroll
+      for (int cc = 0; cc < B; ++cc) t = fma((TA)a[r * B + cc], xv[cc], t);
+      acc[r] = on ? t : acc[r];
+    }
+  }
+}
+
 // Kernel variants of the SELL SpMV.  A variant fixes
 //   unroll: slots per step in sell_row (loads in flight per wave vs. VGPRs), and
 //   wpc:    0 = one wavefront per slice over a grid covering every slice, or
 //           w > 0 = a persistent grid of w wavefronts per CU, each wavefront
 //           walking a contiguous range of slices (fewer, longer HBM streams).
+//   pipe:   1 = software-pipelined slot loop (sell_row_pipe), 0 = sell_row.
 // Matrix streams are always non-temporal.  Variant 0 is the default before
 // tuning; monolis_hip_matrix_autotune times all of them per matrix.
 struct SellVariant {
-  int unroll;
+  int unroll;  // slots per step, plus kSellPipe when pipelined
   int wpc;
 };
-constexpr int kNumVariants = 16;
+constexpr int kSellPipe = 16;
+constexpr int kNumVariants = 22;
 __host__ __device__ constexpr SellVariant sell_variant(int v) {
-  constexpr int u[kNumVariants] = {3, 2, 4, 1, 3, 2, 4, 1, 3, 2, 4, 3, 2, 4, 3, 4};
-  constexpr int w[kNumVariants] = {8, 0, 0, 0, 0, 8, 8, 8, 4, 4, 4, 12, 12, 12, 16, 16};
+  constexpr int P = kSellPipe;
+  constexpr int u[kNumVariants] = {3, 2, 4, 1, 3, 2, 4, 1, 3, 2, 4, 3, 2, 4, 3, 4,
+                                   P + 1, P + 1, P + 1, P + 1, P + 1, P + 1};
+  constexpr int w[kNumVariants] = {8, 0, 0, 0, 0, 8, 8, 8, 4, 4, 4, 12, 12, 12, 16, 16,
+                                   0, 4, 8, 12, 16, 24};
   return SellVariant{u[v], w[v]};
 }
 // B > 3 blocks carry B*B values per slot: cap the unroll to bound VGPRs.
@@ -155,19 +245,33 @@ struct WaveRows {
   // Outputs are written non-temporally: they are consumed by a later kernel,
   // and streaming them out during the kernel avoids leaving megabytes of dirty
   // L2 lines for the end-of-kernel write-back.
+  __device__ __forceinline__ void put(T* __restrict__ dst, size_t i, T v) const {
+#if defined(MH_STORE_CPOL) && MH_STORE_CPOL >= 0
+    // buffer store with explicit cache-policy bits (bit 0 sc0, bit 1 nt, bit 4 sc1)
+    using U2 = unsigned int __attribute__((ext_vector_type(2)));
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(dst, 0, (int)(sizeof(T) * (size_t)nrows * B), 0x00020000);
+    if constexpr (sizeof(T) == 8)
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), rs, (int)(i * sizeof(T)), 0, MH_STORE_CPOL);
+    else
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), rs, (int)(i * sizeof(T)), 0,
+                                            MH_STORE_CPOL);
+#else
+    __builtin_nontemporal_store(v, dst + i);
+#endif
+  }
   __device__ __forceinline__ void store(T* __restrict__ dst, const T (&v)[B]) const {
     if constexpr (B == 1) {
-      if (row0 + lane < nrows) __builtin_nontemporal_store(v[0], dst + (size_t)row0 + lane);
+      if (row0 + lane < nrows) put(dst, (size_t)row0 + lane, v[0]);
     } else {
       const int cnt = count();
-      T* base = dst + (size_t)row0 * B;
 #pragma unroll
       for (int k = 0; k < B; ++k) lds[lane * B + k] = v[k];
       wave_sync();
 #pragma unroll
       for (int j = 0; j < B; ++j) {
         const int e = j * kWave + lane;
-        if (e < cnt) __builtin_nontemporal_store(lds[e], base + e);
+        if (e < cnt) put(dst, (size_t)row0 * B + e, lds[e]);
       }
       wave_sync();
     }
@@ -223,8 +327,12 @@ __device__ __forceinline__ void diag_apply(const TD* __restrict__ D, int nrows, 
 //                 const WaveRows<TA, B>& io)     (every lane calls it; vectors via io)
 //   void finalize(const double (&tot)[NV])       run once by k_finalize (thread 0)
 // and NV >= 0 partial sums per workgroup, stored for the finalize kernel.
+#ifndef MH_SELL_WAVES_PER_EU
+#define MH_SELL_WAVES_PER_EU 1
+#endif
 template <typename TV, typename TA, int B, int U, class Op>
-__global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op0, double* partials, int nwg, int spw) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MH_SELL_WAVES_PER_EU)))
+k_sell(SellView<TV> M, Op op0, double* partials, int nwg, int spw) {
   const int wg = xcd_tile(nwg);
   if (wg < 0) return;
   Op op = op0;
@@ -253,7 +361,10 @@ __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op0, double*
 #pragma unroll
     for (int r = 0; r < B; ++r) acc[r] = (TA)0;
     auto xg = [&](int c, TA(&xv)[B]) { op.gather(c, xv); };
-    sell_row<TV, TA, B, SellCfg<B, U>::U, true>(M, s, lane, len, xg, acc);
+    if constexpr (U >= kSellPipe)
+      sell_row_pipe<TV, TA, B, SellCfg<B, U - kSellPipe>::U, true>(M, s, lane, len, xg, acc);
+    else
+      sell_row<TV, TA, B, SellCfg<B, U>::U, true>(M, s, lane, len, xg, acc);
     const WaveRows<TA, B> io{lds + (threadIdx.x / kWave) * kWave * B, s * kWave, M.nrows, lane};
     op.epilogue(row, valid, acc, part, io);
   }
@@ -269,6 +380,7 @@ __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op0, double*
     case 1: { constexpr int U = 1; __VA_ARGS__; } break;            \
     case 2: { constexpr int U = 2; __VA_ARGS__; } break;            \
     case 4: { constexpr int U = 4; __VA_ARGS__; } break;            \
+    case kSellPipe + 1: { constexpr int U = kSellPipe + 1; __VA_ARGS__; } break; \
     default: { constexpr int U = 3; __VA_ARGS__; } break;           \
   }
 

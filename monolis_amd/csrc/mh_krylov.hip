@@ -18,6 +18,7 @@
 
 #include "mh_internal.hpp"
 #include "mh_launch.hpp"
+#include "mh_vblock.hpp"
 
 namespace mh {
 
@@ -25,7 +26,9 @@ namespace {
 
 constexpr int kIterRR = 200;
 
-enum : int { F_UPD = 1, F_CHECK = 2, F_FIRST = 4, F_XONLY = 8 };
+// F_EXTPREC: the preconditioner runs as its own kernel afterwards (diag_V for
+// variable block sizes); this step only updates x, r and tests (r, r).
+enum : int { F_UPD = 1, F_CHECK = 2, F_FIRST = 4, F_XONLY = 8, F_EXTPREC = 16 };
 
 // ---- shared finalize: monolis_check_converge_R --------------------------------
 __device__ __forceinline__ bool finalize_check(KState* st, double* hist, int it, double rr) {
@@ -164,6 +167,13 @@ struct OpCgUpdate {
       for (int k = 0; k < B; ++k) v[k] = fma(-alpha, qv[k], v[k]);  // R = -alpha*Q + R  (CG.f90:94)
       io.store(r, v);
     }
+    if (flags & F_EXTPREC) {
+      if (valid) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) part[0] += (double)v[k] * (double)v[k];
+      }
+      return;
+    }
     T w[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) w[k] = v[k];
@@ -182,6 +192,7 @@ struct OpCgUpdate {
     if (flags & F_CHECK) {
       if (finalize_check(st, hist, it, tot[0])) return;
     }
+    if (flags & F_EXTPREC) return;  // rho and beta come from the preconditioner kernel
     const double rho = tot[1];
     st->beta = (flags & F_FIRST) ? 0.0 : rho / st->rho;
     st->rho = rho;
@@ -520,7 +531,21 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
     return 0;
   }
   // first z = M^-1 r, rho_1
+  // variable block sizes: diag_V runs after the update as its own kernel
+  const bool vprec = PREC && B == 1 && M.vb != nullptr;
   auto upd = [&](int it, int flags) -> int {
+    if constexpr (PREC && B == 1) {
+      if (vprec) {
+        OpCgUpdate<T, B, false> op{x, r, z, p, q, nullptr, n, st, P.keep_hist ? hist : nullptr, it,
+                                   flags | F_EXTPREC, (T)0};
+        MH_TRY(launch_rows(n, op, parts, s));
+        if (flags & F_XONLY) return 0;
+        const VBlock& V = *M.vb;
+        const T* DV = sizeof(T) == 8 ? (const T*)V.D64.p : (const T*)V.D32.p;
+        OpVPrec<T, true> vp{r, z, V.list.p, V.dof_index.p, V.doff.p, DV, st, (flags & F_FIRST) ? 1 : 0};
+        return launch_rows(V.nb, vp, parts, s);
+      }
+    }
     OpCgUpdate<T, B, PREC> op{x, r, z, p, q, D, n, st, P.keep_hist ? hist : nullptr, it, flags, (T)0};
     return launch_rows(n, op, parts, s);
   };
@@ -614,6 +639,19 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   }
   // rho of iteration 1 is (R, RT) with RT == R: the same sum as B2.
   MH_HIP(hipMemcpyAsync(&st->rho, &st->B2, sizeof(double), hipMemcpyDeviceToDevice, s));
+  // variable block sizes: diag_V runs as its own kernel after steps A and C
+  const bool vprec = PREC && B == 1 && M.vb != nullptr;
+  auto vapply = [&](const T* src, T* dst) -> int {
+    if constexpr (PREC && B == 1) {
+      const VBlock& V = *M.vb;
+      const T* DV = sizeof(T) == 8 ? (const T*)V.D64.p : (const T*)V.D32.p;
+      OpVPrec<T, false> vp{src, dst, V.list.p, V.dof_index.p, V.doff.p, DV, st, 0};
+      return launch_rows(V.nb, vp, nullptr, s);
+    }
+    (void)src;
+    (void)dst;
+    return 0;
+  };
   double* hp = P.keep_hist ? hist : nullptr;
   double* dsum = parts;  // sorted-sum result slot (partials are free between launches)
   // Stage cursor: an iteration is A (p, pt), B (v, (rt, v) -> alpha), C (s, st),
@@ -630,14 +668,26 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
       const int end = std::min(it + P.chunk - 1, P.max_iter);
       for (; it <= end; ++it, stage = 0) {
         if (stage <= 0) {
-          OpBiA<T, B, PREC> a{r, p, v, pt, D, n, st, it == 1, (T)0, (T)0};
-          MH_TRY(launch_rows(n, a, parts, s));
+          if (vprec) {
+            OpBiA<T, B, false> a{r, p, v, pt, nullptr, n, st, it == 1, (T)0, (T)0};
+            MH_TRY(launch_rows(n, a, parts, s));
+            MH_TRY(vapply(p, pt));
+          } else {
+            OpBiA<T, B, PREC> a{r, p, v, pt, D, n, st, it == 1, (T)0, (T)0};
+            MH_TRY(launch_rows(n, a, parts, s));
+          }
           OpBiB<T, B> bb{pt, v, rt, st, it};
           MH_TRY((launch_sell<T, B>(M, bb, parts, var, s)));
         }
         if (stage <= 2) {
-          OpBiC<T, B, PREC> cc{r, v, sv, stv, D, n, st, (T)0};
-          MH_TRY(launch_rows(n, cc, parts, s));
+          if (vprec) {
+            OpBiC<T, B, false> cc{r, v, sv, stv, nullptr, n, st, (T)0};
+            MH_TRY(launch_rows(n, cc, parts, s));
+            MH_TRY(vapply(sv, stv));
+          } else {
+            OpBiC<T, B, PREC> cc{r, v, sv, stv, D, n, st, (T)0};
+            MH_TRY(launch_rows(n, cc, parts, s));
+          }
           OpBiD<T, B> dd{stv, t, sv, st, it};
           MH_TRY((launch_sell<T, B>(M, dd, parts, var, s)));
         }

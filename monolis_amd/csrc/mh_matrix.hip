@@ -70,6 +70,7 @@ template struct DevArray<double>;
 template struct DevArray<float>;
 template struct DevArray<unsigned char>;
 template struct DevArray<unsigned long long>;
+template struct DevArray<long long>;
 
 int is_device_pointer(const void* p) {
   if (!p) return 0;
@@ -176,7 +177,10 @@ static int grid_1d(size_t m) {
   return (int)(g > 0 ? g : 1);
 }
 
-Matrix::~Matrix() { delete transpose; }
+Matrix::~Matrix() {
+  delete transpose;
+  delete vb;
+}
 
 template <typename TV>
 SellView<TV> view(const Matrix& M) {
@@ -503,6 +507,10 @@ __global__ void k_diag_setup(int n, int b, const int* __restrict__ slice_ptr, co
 
 template <typename T>
 int diag_setup(Matrix& M, hipStream_t s) {
+  if (M.vb) {  // diag_V factors are built with the values
+    if (sizeof(T) == 4) MH_TRY(vdiag_make_fp32(M, s));
+    return 0;
+  }
   if (!M.diag_ready) {
     const size_t need = (size_t)M.n * M.b * M.b;
     if (M.D64.n < need) MH_TRY(M.D64.alloc(need));
@@ -550,6 +558,7 @@ struct OpDiagApply {
 
 template <typename T>
 int diag_apply(const Matrix& M, const T* x, T* y, hipStream_t s) {
+  if (M.vb) return vdiag_apply<T>(M, x, y, s);
   if (M.n == 0) return 0;
   const T* D = nullptr;
   if constexpr (sizeof(T) == 8) D = (const T*)M.D64.p;

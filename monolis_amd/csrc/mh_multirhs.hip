@@ -507,8 +507,8 @@ void spmm(const Matrix& M, int k, int ks, int cfg, const T* val, const T* x, con
 template <typename T>
 int multi_cg(Matrix& M, const MultiParams& P, int k, const T* b, T* x, Workspace& W, MultiResult* out,
              hipStream_t s) {
-  if (M.b != 1) {
-    set_error("multi-RHS CG takes scalar matrices (n_dof = 1), got n_dof = %d", M.b);
+  if (M.b != 1 || M.vb) {
+    set_error("multi-RHS CG takes scalar matrices (n_dof = 1), got n_dof = %d", M.vb ? -1 : M.b);
     return -2;
   }
   if (k < 1 || k > kMaxRhs) {
@@ -620,7 +620,7 @@ template int multi_cg<float>(Matrix&, const MultiParams&, int, const float*, flo
 
 int multi_refine(Matrix& M, const MultiParams& P, double inner_tol, int max_outer, int k, const double* b, double* x,
                  Workspace& Wi, Workspace& Wo, MultiRefineResult* out, hipStream_t s) {
-  if (M.b != 1 || k < 1 || k > kMaxRhs) {
+  if (M.b != 1 || M.vb || k < 1 || k > kMaxRhs) {
     set_error("multi-RHS refinement takes scalar matrices and 1..%d right-hand sides", kMaxRhs);
     return -2;
   }
@@ -701,7 +701,7 @@ int multi_refine(Matrix& M, const MultiParams& P, double inner_tol, int max_oute
 
 template <typename T>
 int multi_matvec(Matrix& M, int k, const T* x, T* y, hipStream_t s) {
-  if (M.b != 1 || k < 1 || k > kMaxRhs) {
+  if (M.b != 1 || M.vb || k < 1 || k > kMaxRhs) {
     set_error("multi-RHS matvec takes scalar matrices and 1..%d right-hand sides", kMaxRhs);
     return -2;
   }
@@ -728,7 +728,7 @@ void multi_config_info(int cfg, int* steps_per_batch, int* wg_per_cu) {
 }
 
 int multi_autotune(Matrix& M, int k, int fp32, int reps, int* best, double* ms, hipStream_t s) {
-  if (M.b != 1 || k < 1 || k > kMaxRhs) {
+  if (M.b != 1 || M.vb || k < 1 || k > kMaxRhs) {
     set_error("multi-RHS autotune takes scalar matrices and 1..%d right-hand sides", kMaxRhs);
     return -2;
   }

@@ -130,6 +130,23 @@ def nonzero_pattern_by_simple_mesh(n_node: int, ndof: int, elems) -> Matrix:
     return Matrix(n_node, n_node, ndof, index, item, A).ensure_dof_index()
 
 
+def nonzero_pattern_by_simple_mesh_V(n_node: int, n_dof_list, elems) -> Matrix:
+    """monolis_get_nonzero_pattern_by_simple_mesh_V_R: the pattern of
+    nonzero_pattern_by_simple_mesh with a block size per node; n_dof_index and
+    n_dof_index2 are the prefix sums of monolis_set_n_dof_index
+    (src/matrix/spmat_nzpattern_util.f90:62-90)."""
+    m = nonzero_pattern_by_simple_mesh(n_node, 1, elems)
+    lst = np.ascontiguousarray(n_dof_list, np.int32)
+    dofi = np.zeros(n_node + 1, np.int32)
+    dofi[1:] = np.cumsum(lst)
+    rows = np.repeat(np.arange(n_node), np.diff(m.index))
+    sizes = lst[rows].astype(np.int64) * lst[m.item - 1]
+    idx2 = np.zeros(len(m.item) + 1, np.int32)
+    idx2[1:] = np.cumsum(sizes)
+    return Matrix(n_node, n_node, -1, m.index, m.item, np.zeros(int(idx2[-1]), np.float64),
+                  n_dof_list=lst, n_dof_index=dofi, n_dof_index2=idx2)
+
+
 def _find_block(mat: Matrix, i: int, j: int) -> int:
     """Binary search of 1-based column j+1 inside row i (spmat_handler_util.f90:339)."""
     lo, hi = int(mat.index[i]), int(mat.index[i + 1])

@@ -128,8 +128,17 @@ def check_converge(r, B2, it, tol):
 
 
 def diag_setup(mat):
-    """Factored diagonal blocks, the layout of PREC%R%D (diag_33.f90 / diag_nn.f90)."""
+    """Factored diagonal blocks, the layout of PREC%R%D (diag_33.f90 / diag_nn.f90;
+    diag_V.f90 for variable blocks, returned as (doff, D))."""
     n, nd = mat.n, mat.ndof
+    if nd == -1:
+        lst = np.ascontiguousarray(mat.n_dof_list, np.int32)
+        tot = int(np.sum(lst[:n].astype(np.int64) ** 2))
+        doff = np.zeros(n + 1, np.int64)
+        D = np.zeros(max(tot, 1), np.float64)
+        lib().orc_diag_V_setup(n, lst, np.ascontiguousarray(mat.n_dof_index2, np.int32), mat.index, mat.item,
+                               mat.A, doff, D)
+        return doff, D
     D = np.zeros(max(nd * nd * n, 1), np.float64)
     if nd == 3:
         lib().orc_diag_33_setup(n, mat.index, mat.item, mat.A, D)
@@ -142,6 +151,11 @@ def diag_apply(mat, D, x):
     n, nd = mat.n, mat.ndof
     x = np.ascontiguousarray(x, np.float64)
     y = np.zeros_like(x)
+    if nd == -1:
+        doff, DD = D
+        lib().orc_diag_V_apply(n, np.ascontiguousarray(mat.n_dof_list, np.int32),
+                               np.ascontiguousarray(mat.n_dof_index, np.int32), doff, DD, x, y)
+        return y
     if nd == 3:
         lib().orc_diag_33_apply(n, D, x, y)
     else:

@@ -87,3 +87,30 @@ def random_bcsr(n, ndof, density, seed, spd=False, diag_missing=()):
                 c = item[k] - 1
                 m.A[k * b2:(k + 1) * b2] = D[i * ndof:(i + 1) * ndof, c * ndof:(c + 1) * ndof].reshape(-1)
     return m
+
+
+def random_vblock(n, density, seed, maxb=4, spd=True, sizes=None):
+    """Random variable-block matrix (monolis n_dof = -1): block sizes 1..maxb,
+    symmetric pattern with the diagonal, values uniform; SPD (symmetric,
+    diagonally dominant) when spd."""
+    rng = np.random.default_rng(seed)
+    lst = np.asarray(sizes if sizes is not None else rng.integers(1, maxb + 1, size=n), np.int32)
+    elems = []
+    for i in range(n):
+        for j in np.nonzero(rng.random(n) < density)[0]:
+            if j != i:
+                elems.append((i, int(j)))
+    m = mm.nonzero_pattern_by_simple_mesh_V(n, lst, elems)
+    m.A = rng.uniform(-1, 1, len(m.A))
+    if spd:
+        D = m.dense()
+        D = 0.5 * (D + D.T)
+        D += np.eye(len(D)) * (np.abs(D).sum(1).max() + 1.0)
+        for i in range(n):
+            r0, n1 = int(m.n_dof_index[i]), int(lst[i])
+            for e in range(m.index[i], m.index[i + 1]):
+                j = int(m.item[e]) - 1
+                c0, n2 = int(m.n_dof_index[j]), int(lst[j])
+                a0 = int(m.n_dof_index2[e])
+                m.A[a0:a0 + n1 * n2] = D[r0:r0 + n1, c0:c0 + n2].reshape(-1)
+    return m
