@@ -33,8 +33,8 @@ extern "C" {
 /* Replaces monolis_solve_R_c_main, wrapper/solver/monolis_solver_c.h:51-80,
  * defined as the Fortran bind(c) routine wrapper/solver/solver_wrapper.f90:11-92.
  * Runs monolis_solve_main_R (src/solver/solver.f90:77-185) with the method in
- * Iarray[1] (CG = 1, BiCGSTAB = 5) and preconditioner in Iarray[2] (NONE = 0,
- * DIAG = 1) on the GPU.  Fills Iarray[4] (cur_iter), Iarray[5] (status: 0 ok,
+ * Iarray[1] (CG = 1, GropCG = 2, PipeCG = 3, BiCGSTAB = 5, BiCGSTAB_noprec = 7)
+ * and preconditioner in Iarray[2] (NONE = 0, DIAG = 1) on the GPU.  Fills Iarray[4] (cur_iter), Iarray[5] (status: 0 ok,
  * 1 not converged, < 0 error), Rarray[2] (cur_resid) and Rarray[3..4]
  * (solution / preparation time, s).  Non-convergence stops the process when
  * Iarray[16] (error_abort) is set and Iarray[13] (measurement) is not,

@@ -69,8 +69,9 @@ mh::SolveParams params_from(const int* Iarray, const double* Rarray) {
 }
 
 int check_method(int method, int precond) {
-  if (method != 1 && method != 5) {
-    mh::set_error("method %d (%s) is not implemented on the GPU path (CG = 1, BiCGSTAB = 5)", method,
+  if (method < 1 || method > 8 || method == 4) {
+    mh::set_error("method %d (%s) is not implemented on the GPU path (CG = 1, GropCG = 2, PipeCG = 3, "
+                  "BiCGSTAB = 5, PipeBiCGSTAB = 6, BiCGSTAB_noprec = 7, PipeBiCGSTAB_noprec = 8)", method,
                   str_iter(method));
     return ST_METHOD;
   }

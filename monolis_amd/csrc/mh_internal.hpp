@@ -62,6 +62,7 @@ struct KState {
   double resid;     // sqrt(rr / B2)
   double tol;
   double ts, tt;    // BiCGSTAB (t, s), (t, t)
+  double gamma, delta;  // PipeCG (r, u), (v, u)
   double aux[4];
   int iter;         // last iteration whose convergence test ran
   int max_iter;
@@ -172,7 +173,7 @@ size_t partials_needed(long long n);
 // Solver driver.
 // ---------------------------------------------------------------------------
 struct SolveParams {
-  int method = 1;        // 1 CG, 5 BiCGSTAB
+  int method = 1;        // 1 CG, 3 PipeCG, 5 BiCGSTAB
   int precond = 1;       // 0 NONE, 1 DIAG
   int max_iter = 1000;
   int init_x = 1;

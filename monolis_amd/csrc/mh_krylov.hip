@@ -421,12 +421,521 @@ struct OpBiE {
   }
 };
 
+// ---- PipeCG (src/iterative/PipeCG.f90) ---------------------------------------
+// a: gamma = (r, u), delta = (v, u); m = M^-1 v; then the scalar recurrences
+//    gamma1 = 1/gamma_old, alpha1 = 1/alpha_old, beta = gamma*gamma1,
+//    alpha = gamma / (delta - beta*gamma*alpha1) (first: alpha = gamma/delta, beta = 0).
+template <typename T, int B, bool PREC>
+struct OpPipeA {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 2;
+  const T* __restrict__ r;
+  const T* __restrict__ u;
+  const T* __restrict__ v;
+  T* __restrict__ m;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  int first;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void row(int i, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
+    T rv[B], uv[B], vv[B];
+    io.load(r, rv);
+    io.load(u, uv);
+    io.load(v, vv);
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        part[0] += (double)rv[k] * (double)uv[k];
+        part[1] += (double)vv[k] * (double)uv[k];
+      }
+      if constexpr (PREC) diag_apply<T, T, B>(D, n, i, vv);
+    }
+    io.store(m, vv);
+  }
+  __device__ void finalize(const double (&tot)[2]) const {
+    const double gamma1 = 1.0 / st->gamma;
+    const double alpha1 = 1.0 / st->alpha;
+    const double gamma = tot[0], delta = tot[1];
+    st->gamma = gamma;
+    st->delta = delta;
+    if (!first) {
+      st->beta = gamma * gamma1;
+      st->alpha = gamma / (delta - st->beta * gamma * alpha1);
+    } else {
+      st->alpha = gamma / delta;
+      st->beta = 0.0;
+    }
+  }
+};
+
+// y = A x (no reduction), skipped once the solve is done.
+template <typename T, int B>
+struct OpSpmvSolve {
+  static constexpr int NV = 0;
+  const T* __restrict__ x;
+  T* __restrict__ y;
+  const KState* st;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void gather(int c, T (&xv)[B]) const {
+#pragma unroll
+    for (int k = 0; k < B; ++k) xv[k] = x[(size_t)c * B + k];
+  }
+  __device__ void epilogue(int, bool, const T (&acc)[B], double (&)[1], const WaveRows<T, B>& io) const {
+    io.store(y, acc);
+  }
+  __device__ void finalize(const double (&)[1]) const {}
+};
+
+// b: z = beta z + l; q = beta q + m; s = beta s + v; p = beta p + u; then
+//    r -= alpha s; u -= alpha q; v -= alpha z; x += alpha p and the test on
+//    (r, r) (PipeCG.f90 loop body), or x += alpha p only (F_XONLY: the
+//    residual-replacement iteration).
+template <typename T, int B>
+struct OpPipeB {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 1;
+  T *__restrict__ z, *__restrict__ q, *__restrict__ s, *__restrict__ p;
+  T *__restrict__ r, *__restrict__ u, *__restrict__ v, *__restrict__ x;
+  const T* __restrict__ l;
+  const T* __restrict__ m;
+  KState* st;
+  double* hist;
+  int it;
+  int flags;
+  T alpha, beta;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {
+    alpha = (T)st->alpha;
+    beta = (T)st->beta;
+  }
+  __device__ void row(int, bool valid, double (&part)[1], const WaveRows<T, B>& io) const {
+    T zv[B], lv[B], qv[B], mv[B], sv[B], vv[B], pv[B], uv[B];
+    io.load(z, zv);
+    io.load(l, lv);
+    io.load(q, qv);
+    io.load(m, mv);
+    io.load(s, sv);
+    io.load(v, vv);
+    io.load(p, pv);
+    io.load(u, uv);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      zv[k] = fma(beta, zv[k], lv[k]);  // Z = beta*Z + L
+      qv[k] = fma(beta, qv[k], mv[k]);  // Q = beta*Q + M
+      sv[k] = fma(beta, sv[k], vv[k]);  // S = beta*S + V
+      pv[k] = fma(beta, pv[k], uv[k]);  // P = beta*P + U
+    }
+    io.store(z, zv);
+    io.store(q, qv);
+    io.store(s, sv);
+    io.store(p, pv);
+    T xv[B];
+    io.load(x, xv);
+#pragma unroll
+    for (int k = 0; k < B; ++k) xv[k] = fma(alpha, pv[k], xv[k]);  // X = alpha*P + X
+    io.store(x, xv);
+    if (flags & F_XONLY) return;
+    T rv[B];
+    io.load(r, rv);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      rv[k] = fma(-alpha, sv[k], rv[k]);  // R = -alpha*S + R
+      uv[k] = fma(-alpha, qv[k], uv[k]);  // U = -alpha*Q + U
+      vv[k] = fma(-alpha, zv[k], vv[k]);  // V = -alpha*Z + V
+    }
+    io.store(r, rv);
+    io.store(u, uv);
+    io.store(v, vv);
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) part[0] += (double)rv[k] * (double)rv[k];
+    }
+  }
+  __device__ void finalize(const double (&tot)[1]) const {
+    if (flags & F_XONLY) return;
+    finalize_check(st, hist, it, tot[0]);
+  }
+};
+
+// u = M^-1 r, with the test on (r, r) when F_CHECK (after a true residual).
+template <typename T, int B, bool PREC>
+struct OpPrecCheck {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 1;
+  const T* __restrict__ r;
+  T* __restrict__ u;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  double* hist;
+  int it;
+  int flags;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void row(int i, bool valid, double (&part)[1], const WaveRows<T, B>& io) const {
+    T rv[B];
+    io.load(r, rv);
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) part[0] += (double)rv[k] * (double)rv[k];
+      if constexpr (PREC) diag_apply<T, T, B>(D, n, i, rv);
+    }
+    io.store(u, rv);
+  }
+  __device__ void finalize(const double (&tot)[1]) const {
+    if (flags & F_CHECK) finalize_check(st, hist, it, tot[0]);
+  }
+};
+
+// ---- GropCG (src/iterative/GropCG.f90) ---------------------------------------
+// init: u = M^-1 r; gamma = (r, u)
+template <typename T, int B, bool PREC>
+struct OpGropInit {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 1;
+  const T* __restrict__ r;
+  T* __restrict__ u;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void row(int i, bool valid, double (&part)[1], const WaveRows<T, B>& io) const {
+    T rv[B], w[B];
+    io.load(r, rv);
+#pragma unroll
+    for (int k = 0; k < B; ++k) w[k] = rv[k];
+    if (valid) {
+      if constexpr (PREC) diag_apply<T, T, B>(D, n, i, w);
+#pragma unroll
+      for (int k = 0; k < B; ++k) part[0] += (double)rv[k] * (double)w[k];
+    }
+    io.store(u, w);
+  }
+  __device__ void finalize(const double (&tot)[1]) const { st->gamma = tot[0]; }
+};
+
+// x += alpha p; r -= alpha s; u -= alpha q; test on (r, r); beta = (r, u) / gamma,
+// gamma = (r, u)  (GropCG.f90 loop body)
+template <typename T, int B>
+struct OpGropUpd {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 2;
+  T *__restrict__ x, *__restrict__ r, *__restrict__ u;
+  const T *__restrict__ p, *__restrict__ s, *__restrict__ q;
+  KState* st;
+  double* hist;
+  int it;
+  T alpha;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() { alpha = (T)st->alpha; }
+  __device__ void row(int, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
+    T xv[B], pv[B], rv[B], sv[B], uv[B], qv[B];
+    io.load(x, xv);
+    io.load(p, pv);
+    io.load(r, rv);
+    io.load(s, sv);
+    io.load(u, uv);
+    io.load(q, qv);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      xv[k] = fma(alpha, pv[k], xv[k]);   // X = alpha*P + X
+      rv[k] = fma(-alpha, sv[k], rv[k]);  // R = -alpha*S + R
+      uv[k] = fma(-alpha, qv[k], uv[k]);  // U = -alpha*Q + U
+    }
+    io.store(x, xv);
+    io.store(r, rv);
+    io.store(u, uv);
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        part[0] += (double)rv[k] * (double)uv[k];
+        part[1] += (double)rv[k] * (double)rv[k];
+      }
+    }
+  }
+  __device__ void finalize(const double (&tot)[2]) const {
+    if (finalize_check(st, hist, it, tot[1])) return;
+    st->beta = tot[0] / st->gamma;
+    st->gamma = tot[0];
+  }
+};
+
+// v = A u fused with p = beta p + u; s = beta s + v; q = M^-1 s; delta = (p, s),
+// alpha = gamma / delta for the next iteration.  v itself is never stored.
+template <typename T, int B, bool PREC>
+struct OpGropSpmv {
+  static constexpr int NV = 1;
+  const T* __restrict__ u;
+  T* __restrict__ p;
+  T* __restrict__ s;
+  T* __restrict__ q;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  T beta;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() { beta = (T)st->beta; }
+  __device__ void gather(int c, T (&xv)[B]) const {
+#pragma unroll
+    for (int k = 0; k < B; ++k) xv[k] = u[(size_t)c * B + k];
+  }
+  __device__ void epilogue(int row, bool valid, const T (&acc)[B], double (&part)[1],
+                           const WaveRows<T, B>& io) const {
+    T pv[B], uv[B], sv[B];
+    io.load(p, pv);
+    io.load(u, uv);
+    io.load(s, sv);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      pv[k] = fma(beta, pv[k], uv[k]);   // P = beta*P + U
+      sv[k] = fma(beta, sv[k], acc[k]);  // S = beta*S + V
+    }
+    io.store(p, pv);
+    io.store(s, sv);
+    T w[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) w[k] = sv[k];
+    if (valid) {
+      if constexpr (PREC) diag_apply<T, T, B>(D, n, row, w);
+#pragma unroll
+      for (int k = 0; k < B; ++k) part[0] += (double)pv[k] * (double)sv[k];
+    }
+    io.store(q, w);
+  }
+  __device__ void finalize(const double (&tot)[1]) const {
+    st->delta = tot[0];
+    st->alpha = st->gamma / tot[0];
+  }
+};
+
+// ---- PipeBiCGSTAB (src/iterative/PipeBiCGSTAB.f90) ----------------------------
+// a: the seven recurrences of the loop head, (Q, Y), (Y, Y) and ZT = M^-1 Z;
+//    omega1 = QY / YY (0 when YY = 0).
+template <typename T, int B, bool PREC>
+struct OpPbA {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 2;
+  T *__restrict__ pt, *__restrict__ s_, *__restrict__ st_, *__restrict__ z, *__restrict__ zt;
+  T *__restrict__ q, *__restrict__ qt, *__restrict__ y;
+  const T *__restrict__ rt, *__restrict__ w0, *__restrict__ wt, *__restrict__ t, *__restrict__ v,
+      *__restrict__ r;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  T alpha, beta, omega;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {
+    alpha = (T)st->alpha;
+    beta = (T)st->beta;
+    omega = (T)st->omega;
+  }
+  __device__ void row(int i, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
+    T ptv[B], sv[B], stv[B], zv[B], ztv[B], rtv[B], w0v[B], wtv[B], tv[B], vv[B], rv[B];
+    io.load(pt, ptv);
+    io.load(s_, sv);
+    io.load(st_, stv);
+    io.load(z, zv);
+    io.load(zt, ztv);
+    io.load(rt, rtv);
+    io.load(w0, w0v);
+    io.load(wt, wtv);
+    io.load(t, tv);
+    io.load(v, vv);
+    io.load(r, rv);
+    T qv[B], qtv[B], yv[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      ptv[k] = rtv[k] + beta * (ptv[k] - omega * stv[k]);
+      sv[k] = w0v[k] + beta * (sv[k] - omega * zv[k]);
+      stv[k] = wtv[k] + beta * (stv[k] - omega * ztv[k]);
+      zv[k] = tv[k] + beta * (zv[k] - omega * vv[k]);
+      qv[k] = rv[k] - alpha * sv[k];
+      qtv[k] = rtv[k] - alpha * stv[k];
+      yv[k] = w0v[k] - alpha * zv[k];
+    }
+    io.store(pt, ptv);
+    io.store(s_, sv);
+    io.store(st_, stv);
+    io.store(z, zv);
+    io.store(q, qv);
+    io.store(qt, qtv);
+    io.store(y, yv);
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        part[0] += (double)qv[k] * (double)yv[k];
+        part[1] += (double)yv[k] * (double)yv[k];
+      }
+      if constexpr (PREC) diag_apply<T, T, B>(D, n, i, zv);
+    }
+    io.store(zt, zv);
+  }
+  __device__ void finalize(const double (&tot)[2]) const {
+    st->aux[0] = (tot[1] != 0.0) ? tot[0] / tot[1] : 0.0;  // omega1 (PipeBiCGSTAB.f90 YY guard)
+  }
+};
+
+// b: (mode 0) X += alpha PT + omega1 QT; R = Q - omega1 Y; RT = QT - omega1 (WT - alpha ZT);
+//    W0 = Y - omega1 (T - alpha V); then (mode 0 and 2) WT = M^-1 W0 and the dots
+//    (R0,R), (R0,W0), (R0,S), (R0,Z), (R,R): test, beta, alpha, omega, RR.
+//    mode 1: X += alpha PT + omega1 T only (the residual-replacement iteration,
+//    as the reference writes it).
+template <typename T, int B, bool PREC>
+struct OpPbB {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 5;
+  T *__restrict__ x, *__restrict__ r, *__restrict__ rt, *__restrict__ w0, *__restrict__ wt;
+  const T *__restrict__ pt, *__restrict__ qt, *__restrict__ q, *__restrict__ y, *__restrict__ zt,
+      *__restrict__ t, *__restrict__ v, *__restrict__ r0, *__restrict__ s_, *__restrict__ z;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  double* hist;
+  int it;
+  int mode;
+  T alpha, omega1;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {
+    alpha = (T)st->alpha;
+    omega1 = (T)st->aux[0];
+  }
+  __device__ void row(int i, bool valid, double (&part)[5], const WaveRows<T, B>& io) const {
+    if (mode != 2) {
+      T xv[B], ptv[B], ov[B];
+      io.load(x, xv);
+      io.load(pt, ptv);
+      io.load(mode == 1 ? t : qt, ov);
+#pragma unroll
+      for (int k = 0; k < B; ++k) xv[k] = xv[k] + alpha * ptv[k] + omega1 * ov[k];
+      io.store(x, xv);
+      if (mode == 1) return;
+    }
+    T rv[B], w0v[B];
+    if (mode == 0) {
+      T qv[B], yv[B], qtv[B], wtv[B], ztv[B], tv[B], vv[B], rtv[B];
+      io.load(q, qv);
+      io.load(y, yv);
+      io.load(qt, qtv);
+      io.load(wt, wtv);
+      io.load(zt, ztv);
+      io.load(t, tv);
+      io.load(v, vv);
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        rv[k] = qv[k] - omega1 * yv[k];
+        rtv[k] = qtv[k] - omega1 * (wtv[k] - alpha * ztv[k]);
+        w0v[k] = yv[k] - omega1 * (tv[k] - alpha * vv[k]);
+      }
+      io.store(r, rv);
+      io.store(rt, rtv);
+      io.store(w0, w0v);
+    } else {
+      io.load(r, rv);
+      io.load(w0, w0v);
+    }
+    T r0v[B], sv[B], zv[B];
+    io.load(r0, r0v);
+    io.load(s_, sv);
+    io.load(z, zv);
+    T wn[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) wn[k] = w0v[k];
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        part[0] += (double)r0v[k] * (double)rv[k];
+        part[1] += (double)r0v[k] * (double)w0v[k];
+        part[2] += (double)r0v[k] * (double)sv[k];
+        part[3] += (double)r0v[k] * (double)zv[k];
+        part[4] += (double)rv[k] * (double)rv[k];
+      }
+      if constexpr (PREC) diag_apply<T, T, B>(D, n, i, wn);
+    }
+    io.store(wt, wn);
+  }
+  __device__ void finalize(const double (&tot)[5]) const {
+    if (mode == 1) return;
+    if (finalize_check(st, hist, it, tot[4])) return;
+    const double RR1 = tot[0], RW = tot[1], RS = tot[2], RZ = tot[3];
+    const double alpha_ = st->alpha, omega1_ = st->aux[0];
+    const double beta_ = (alpha_ * RR1) / (st->rho * omega1_);
+    st->beta = beta_;
+    st->alpha = RR1 / (RW + beta_ * RS - beta_ * omega1_ * RZ);
+    st->omega = omega1_;
+    st->rho = RR1;
+  }
+};
+
+// setup: R0 = R and RT = M^-1 R (mode 0); WT = M^-1 W0 with (R, R), (R, W0) ->
+// alpha = RR / RW, beta = omega = 0 (mode 1); RT = M^-1 R only (mode 2, after a
+// true residual).
+template <typename T, int B, bool PREC>
+struct OpPbInit {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 2;
+  const T* __restrict__ r;
+  T* __restrict__ r0;
+  T* __restrict__ rt;
+  const T* __restrict__ w0;
+  T* __restrict__ wt;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  int mode;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void row(int i, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
+    T rv[B];
+    io.load(r, rv);
+    if (mode != 1) {
+      if (mode == 0) io.store(r0, rv);
+      if constexpr (PREC) {
+        if (valid) diag_apply<T, T, B>(D, n, i, rv);
+      }
+      io.store(rt, rv);
+      return;
+    }
+    T wv[B];
+    io.load(w0, wv);
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        part[0] += (double)rv[k] * (double)rv[k];
+        part[1] += (double)rv[k] * (double)wv[k];
+      }
+      if constexpr (PREC) diag_apply<T, T, B>(D, n, i, wv);
+    }
+    io.store(wt, wv);
+  }
+  __device__ void finalize(const double (&tot)[2]) const {
+    if (mode != 1) return;
+    st->rho = tot[0];
+    st->alpha = tot[0] / tot[1];
+    st->beta = 0.0;
+    st->omega = 0.0;
+  }
+};
+
 __global__ void k_state_init(KState* st, double tol, int max_iter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     KState z;
     memset(&z, 0, sizeof(z));
     z.tol = tol;
     z.max_iter = max_iter;
+    z.gamma = 1.0;  // PipeCG.f90 starts with gamma = alpha = 1
+    z.alpha = 1.0;
     *st = z;
   }
 }
@@ -585,9 +1094,376 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
   return 0;
 }
 
+// GropCG (monolis method 2, src/iterative/GropCG.f90): two fused launches per
+// iteration -- the x / r / u update with (r, u), (r, r) and the test; the SpMV
+// A u with the p, s and q = M^-1 s updates and (p, s) in its epilogue.  No
+// residual replacement (the reference has none for this method).
+template <typename T, int B, bool PREC>
+int gropcg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, SolveResult* out,
+                hipStream_t s) {
+  Context* c = nullptr;
+  MH_TRY(context(&c));
+  const size_t vec = (size_t)M.np * B;
+  MH_TRY(W.ensure(cg_bytes<T>(M, 5, P.max_iter)));
+  Carve cv{W.raw.p};
+  T* r = cv.take<T>(vec);
+  T* u = cv.take<T>(vec);
+  T* q = cv.take<T>(vec);
+  T* p = cv.take<T>(vec);
+  T* sv = cv.take<T>(vec);
+  const size_t nparts = partials_needed((long long)M.n) * 4 + partials_needed((long long)M.nslices * kWave) * 4;
+  double* parts = cv.take<double>(nparts);
+  KState* st = cv.take<KState>(1);
+  double* hist = cv.take<double>((size_t)P.max_iter + 2);
+  const T* D = nullptr;
+  if constexpr (PREC) D = (sizeof(T) == 8) ? (const T*)M.D64.p : (const T*)M.D32.p;
+  const int n = M.n;
+  const size_t m = (size_t)n * B;
+  const int var = sizeof(T) == 8 ? M.variant64 : M.variant32;
+  double* hp = P.keep_hist ? hist : nullptr;
+  const bool vprec = PREC && B == 1 && M.vb != nullptr;
+  auto vop = [&](const T* src, T* dst, bool dot) -> int {
+    if constexpr (PREC && B == 1) {
+      const VBlock& V = *M.vb;
+      const T* DV = sizeof(T) == 8 ? (const T*)V.D64.p : (const T*)V.D32.p;
+      if (dot) {
+        OpVPrec<T, true> vp{src, dst, V.list.p, V.dof_index.p, V.doff.p, DV, st, 2};
+        return launch_rows(V.nb, vp, parts, s);
+      }
+      OpVPrec<T, false> vp{src, dst, V.list.p, V.dof_index.p, V.doff.p, DV, st, 0};
+      return launch_rows(V.nb, vp, nullptr, s);
+    }
+    (void)src;
+    (void)dst;
+    (void)dot;
+    return 0;
+  };
+  auto spmv_step = [&]() -> int {
+    if (vprec) {
+      OpGropSpmv<T, B, false> op{u, p, sv, q, nullptr, n, st, (T)0};
+      MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
+      return vop(sv, q, false);
+    }
+    OpGropSpmv<T, B, PREC> op{u, p, sv, q, D, n, st, (T)0};
+    return launch_sell<T, B>(M, op, parts, var, s);
+  };
+
+  k_state_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
+  MH_HIP(hipGetLastError());
+  if (P.init_x) MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+  for (T* w : {u, q, p, sv}) MH_HIP(hipMemsetAsync(w, 0, sizeof(T) * vec, s));
+  {
+    OpResidSolve<T, B> op{x, b, r, nullptr, &st->B2, st};
+    MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
+  }
+  double B2 = 0.0;
+  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  out->hist.clear();
+  if (B2 == 0.0) {  // mat_converge.f90:40-48
+    MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+    MH_HIP(hipStreamSynchronize(s));
+    out->zero_rhs = 1;
+    out->converged = 1;
+    out->iter = -1;
+    out->resid = 0.0;
+    return 0;
+  }
+  // U = M^-1 R, gamma = (R, U); P = U, S = A P, Q = M^-1 S, alpha = gamma / (P, S)
+  if (vprec) {
+    MH_TRY(vop(r, u, true));
+  } else {
+    OpGropInit<T, B, PREC> op{r, u, D, n, st};
+    MH_TRY(launch_rows(n, op, parts, s));
+  }
+  MH_TRY(spmv_step());  // beta = 0 from the state init, p = s = 0
+  Poller poll{c, st};
+  int done = 0;
+  for (int it = 1; it <= P.max_iter && !done;) {
+    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    for (; it <= end; ++it) {
+      OpGropUpd<T, B> up{x, r, u, p, sv, q, st, hp, it, (T)0};
+      MH_TRY(launch_rows(n, up, parts, s));
+      MH_TRY(spmv_step());
+    }
+    MH_TRY(poll.issue(s));
+    MH_TRY(poll.check(1, &done));
+  }
+  MH_HIP(hipStreamSynchronize(s));
+  KState hs;
+  MH_HIP(hipMemcpy(&hs, st, sizeof(KState), hipMemcpyDeviceToHost));
+  out->iter = hs.iter;
+  out->resid = hs.resid;
+  out->converged = hs.converged;
+  if (P.keep_hist && hs.iter > 0) {
+    out->hist.resize((size_t)hs.iter + 1);
+    MH_HIP(hipMemcpy(out->hist.data(), hist, sizeof(double) * ((size_t)hs.iter + 1), hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+// PipeCG (monolis method 3, src/iterative/PipeCG.f90): three fused launches per
+// iteration -- (r, u), (v, u) and m = M^-1 v; l = A m; the eight vector
+// recurrences with (r, r) and the convergence test -- and the true residual
+// every 50 iterations (PipeCG.f90 iter_RR = 50).
+template <typename T, int B, bool PREC>
+int pipecg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, SolveResult* out,
+                hipStream_t s) {
+  constexpr int kPipeRR = 50;
+  Context* c = nullptr;
+  MH_TRY(context(&c));
+  const size_t vec = (size_t)M.np * B;
+  MH_TRY(W.ensure(cg_bytes<T>(M, 9, P.max_iter)));
+  Carve cv{W.raw.p};
+  T* r = cv.take<T>(vec);
+  T* u = cv.take<T>(vec);
+  T* v = cv.take<T>(vec);
+  T* q = cv.take<T>(vec);
+  T* p = cv.take<T>(vec);
+  T* z = cv.take<T>(vec);
+  T* l = cv.take<T>(vec);
+  T* mm = cv.take<T>(vec);
+  T* sv = cv.take<T>(vec);
+  const size_t nparts = partials_needed((long long)M.n) * 4 + partials_needed((long long)M.nslices * kWave) * 4;
+  double* parts = cv.take<double>(nparts);
+  KState* st = cv.take<KState>(1);
+  double* hist = cv.take<double>((size_t)P.max_iter + 2);
+  const T* D = nullptr;
+  if constexpr (PREC) D = (sizeof(T) == 8) ? (const T*)M.D64.p : (const T*)M.D32.p;
+  const int n = M.n;
+  const size_t m = (size_t)n * B;
+  const int var = sizeof(T) == 8 ? M.variant64 : M.variant32;
+  double* hp = P.keep_hist ? hist : nullptr;
+  const bool vprec = PREC && B == 1 && M.vb != nullptr;
+  auto vapply = [&](const T* src, T* dst) -> int {
+    if constexpr (PREC && B == 1) {
+      const VBlock& V = *M.vb;
+      const T* DV = sizeof(T) == 8 ? (const T*)V.D64.p : (const T*)V.D32.p;
+      OpVPrec<T, false> vp{src, dst, V.list.p, V.dof_index.p, V.doff.p, DV, st, 0};
+      return launch_rows(V.nb, vp, nullptr, s);
+    }
+    (void)src;
+    (void)dst;
+    return 0;
+  };
+  // u = M^-1 r (+ test when flags has F_CHECK)
+  auto prec_check = [&](int it, int flags) -> int {
+    if (vprec) {
+      OpPrecCheck<T, B, false> op{r, u, nullptr, n, st, hp, it, flags};
+      MH_TRY(launch_rows(n, op, parts, s));
+      return vapply(r, u);
+    }
+    OpPrecCheck<T, B, PREC> op{r, u, D, n, st, hp, it, flags};
+    return launch_rows(n, op, parts, s);
+  };
+
+  k_state_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
+  MH_HIP(hipGetLastError());
+  if (P.init_x) MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+  for (T* w : {q, p, z, sv, l, mm, u, v}) MH_HIP(hipMemsetAsync(w, 0, sizeof(T) * vec, s));
+  {
+    OpResidSolve<T, B> op{x, b, r, nullptr, &st->B2, st};
+    MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
+  }
+  double B2 = 0.0;
+  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  out->hist.clear();
+  if (B2 == 0.0) {  // mat_converge.f90:40-48
+    MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+    MH_HIP(hipStreamSynchronize(s));
+    out->zero_rhs = 1;
+    out->converged = 1;
+    out->iter = -1;
+    out->resid = 0.0;
+    return 0;
+  }
+  // U = M^-1 R; V = A U
+  MH_TRY(prec_check(0, 0));
+  {
+    OpSpmvSolve<T, B> sp{u, v, st};
+    MH_TRY((launch_sell<T, B>(M, sp, parts, var, s)));
+  }
+  Poller poll{c, st};
+  int done = 0;
+  for (int it = 1; it <= P.max_iter && !done;) {
+    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    for (; it <= end; ++it) {
+      if (vprec) {
+        OpPipeA<T, B, false> a{r, u, v, mm, nullptr, n, st, it == 1};
+        MH_TRY(launch_rows(n, a, parts, s));
+        MH_TRY(vapply(v, mm));
+      } else {
+        OpPipeA<T, B, PREC> a{r, u, v, mm, D, n, st, it == 1};
+        MH_TRY(launch_rows(n, a, parts, s));
+      }
+      OpSpmvSolve<T, B> sl{mm, l, st};
+      MH_TRY((launch_sell<T, B>(M, sl, parts, var, s)));
+      const bool rr = it % kPipeRR == 0;
+      OpPipeB<T, B> ub{z, q, sv, p, r, u, v, x, l, mm, st, hp, it, rr ? F_XONLY : 0, (T)0, (T)0};
+      MH_TRY(launch_rows(n, ub, parts, s));
+      if (rr) {
+        OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
+        MH_TRY((launch_sell<T, B>(M, rs, parts, var, s)));
+        MH_TRY(prec_check(it, F_CHECK));
+        OpSpmvSolve<T, B> sp{u, v, st};
+        MH_TRY((launch_sell<T, B>(M, sp, parts, var, s)));
+      }
+    }
+    MH_TRY(poll.issue(s));
+    MH_TRY(poll.check(1, &done));
+  }
+  MH_HIP(hipStreamSynchronize(s));
+  KState hs;
+  MH_HIP(hipMemcpy(&hs, st, sizeof(KState), hipMemcpyDeviceToHost));
+  out->iter = hs.iter;
+  out->resid = hs.resid;
+  out->converged = hs.converged;
+  if (P.keep_hist && hs.iter > 0) {
+    out->hist.resize((size_t)hs.iter + 1);
+    MH_HIP(hipMemcpy(out->hist.data(), hist, sizeof(double) * ((size_t)hs.iter + 1), hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+// PipeBiCGSTAB (monolis method 6; method 8 = no preconditioner): per iteration
+// the recurrence pass with (Q, Y), (Y, Y) and ZT = M^-1 Z; V = A ZT; the update
+// pass with five dots, WT = M^-1 W0 and the test; T = A WT.  True residual
+// every 200 iterations (PipeBiCGSTAB.f90 iter_RR).
+template <typename T, int B, bool PREC>
+int pipebicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, SolveResult* out,
+                      hipStream_t s) {
+  constexpr int kPbRR = 200;
+  Context* c = nullptr;
+  MH_TRY(context(&c));
+  const size_t vec = (size_t)M.np * B;
+  MH_TRY(W.ensure(cg_bytes<T>(M, 15, P.max_iter)));
+  Carve cv{W.raw.p};
+  T *r = cv.take<T>(vec), *r0 = cv.take<T>(vec), *rt = cv.take<T>(vec), *w0 = cv.take<T>(vec);
+  T *wt = cv.take<T>(vec), *t = cv.take<T>(vec), *pt = cv.take<T>(vec), *sv = cv.take<T>(vec);
+  T *stv = cv.take<T>(vec), *z = cv.take<T>(vec), *zt = cv.take<T>(vec), *q = cv.take<T>(vec);
+  T *qt = cv.take<T>(vec), *y = cv.take<T>(vec), *v = cv.take<T>(vec);
+  const size_t nparts = partials_needed((long long)M.n) * 4 + partials_needed((long long)M.nslices * kWave) * 4;
+  double* parts = cv.take<double>(nparts);
+  KState* st = cv.take<KState>(1);
+  double* hist = cv.take<double>((size_t)P.max_iter + 2);
+  const T* D = nullptr;
+  if constexpr (PREC) D = (sizeof(T) == 8) ? (const T*)M.D64.p : (const T*)M.D32.p;
+  const int n = M.n;
+  const size_t m = (size_t)n * B;
+  const int var = sizeof(T) == 8 ? M.variant64 : M.variant32;
+  double* hp = P.keep_hist ? hist : nullptr;
+  const bool vprec = PREC && B == 1 && M.vb != nullptr;
+  auto vapply = [&](const T* src, T* dst) -> int {
+    if constexpr (PREC && B == 1) {
+      if (vprec) {
+        const VBlock& V = *M.vb;
+        const T* DV = sizeof(T) == 8 ? (const T*)V.D64.p : (const T*)V.D32.p;
+        OpVPrec<T, false> vp{src, dst, V.list.p, V.dof_index.p, V.doff.p, DV, st, 0};
+        return launch_rows(V.nb, vp, nullptr, s);
+      }
+    }
+    (void)src;
+    (void)dst;
+    return 0;
+  };
+  // the fused block-Jacobi runs only for fixed blocks; variable blocks take vapply
+  auto rows_init = [&](int mode) -> int {
+    if (vprec) {
+      OpPbInit<T, B, false> op{r, r0, rt, w0, wt, nullptr, n, st, mode};
+      MH_TRY(launch_rows(n, op, parts, s));
+      return mode == 1 ? vapply(w0, wt) : vapply(r, rt);
+    }
+    OpPbInit<T, B, PREC> op{r, r0, rt, w0, wt, D, n, st, mode};
+    return launch_rows(n, op, parts, s);
+  };
+  auto rows_b = [&](int it, int mode) -> int {
+    if (vprec) {
+      OpPbB<T, B, false> op{x, r, rt, w0, wt, pt, qt, q, y, zt, t, v, r0, sv, z, nullptr, n, st, hp, it, mode,
+                            (T)0, (T)0};
+      MH_TRY(launch_rows(n, op, parts, s));
+      return mode == 1 ? 0 : vapply(w0, wt);
+    }
+    OpPbB<T, B, PREC> op{x, r, rt, w0, wt, pt, qt, q, y, zt, t, v, r0, sv, z, D, n, st, hp, it, mode, (T)0, (T)0};
+    return launch_rows(n, op, parts, s);
+  };
+  auto spmv = [&](const T* in, T* outv) -> int {
+    OpSpmvSolve<T, B> op{in, outv, st};
+    return launch_sell<T, B>(M, op, parts, var, s);
+  };
+
+  k_state_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
+  MH_HIP(hipGetLastError());
+  if (P.init_x) MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+  for (T* w : {r0, rt, w0, wt, t, pt, sv, stv, z, zt, q, qt, y, v}) MH_HIP(hipMemsetAsync(w, 0, sizeof(T) * vec, s));
+  {
+    OpResidSolve<T, B> op{x, b, r, nullptr, &st->B2, st};
+    MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
+  }
+  double B2 = 0.0;
+  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  out->hist.clear();
+  if (B2 == 0.0) {  // mat_converge.f90:40-48
+    MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+    MH_HIP(hipStreamSynchronize(s));
+    out->zero_rhs = 1;
+    out->converged = 1;
+    out->iter = -1;
+    out->resid = 0.0;
+    return 0;
+  }
+  MH_TRY(rows_init(0));     // R0 = R, RT = M^-1 R
+  MH_TRY(spmv(rt, w0));     // W0 = A RT
+  MH_TRY(rows_init(1));     // WT = M^-1 W0, RR, RW, alpha
+  MH_TRY(spmv(wt, t));      // T = A WT
+  Poller poll{c, st};
+  int done = 0;
+  for (int it = 1; it <= P.max_iter && !done;) {
+    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    for (; it <= end; ++it) {
+      if (vprec) {
+        OpPbA<T, B, false> a{pt, sv, stv, z, zt, q, qt, y, rt, w0, wt, t, v, r, nullptr, n, st, (T)0, (T)0, (T)0};
+        MH_TRY(launch_rows(n, a, parts, s));
+        MH_TRY(vapply(z, zt));
+      } else {
+        OpPbA<T, B, PREC> a{pt, sv, stv, z, zt, q, qt, y, rt, w0, wt, t, v, r, D, n, st, (T)0, (T)0, (T)0};
+        MH_TRY(launch_rows(n, a, parts, s));
+      }
+      MH_TRY(spmv(zt, v));  // V = A ZT
+      if (it % kPbRR == 0) {
+        MH_TRY(rows_b(it, 1));  // X += alpha PT + omega1 T
+        OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
+        MH_TRY((launch_sell<T, B>(M, rs, parts, var, s)));
+        MH_TRY(rows_init(2));  // RT = M^-1 R (mode 2 = mode 0 without touching R0)
+        MH_TRY(spmv(rt, w0));
+        MH_TRY(rows_b(it, 2));  // dots, WT, test, scalars
+      } else {
+        MH_TRY(rows_b(it, 0));
+      }
+      MH_TRY(spmv(wt, t));  // T = A WT
+    }
+    MH_TRY(poll.issue(s));
+    MH_TRY(poll.check(1, &done));
+  }
+  MH_HIP(hipStreamSynchronize(s));
+  KState hs;
+  MH_HIP(hipMemcpy(&hs, st, sizeof(KState), hipMemcpyDeviceToHost));
+  out->iter = hs.iter;
+  out->resid = hs.resid;
+  out->converged = hs.converged;
+  if (P.keep_hist && hs.iter > 0) {
+    out->hist.resize((size_t)hs.iter + 1);
+    MH_HIP(hipMemcpy(out->hist.data(), hist, sizeof(double) * ((size_t)hs.iter + 1), hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+// rr_period: true-residual replacement period (200 for BiCGSTAB.f90, 50 for
+// BiCGSTAB_noprec.f90, which also runs without a preconditioner).
 template <typename T, int B, bool PREC>
 int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, SolveResult* out,
-                  hipStream_t s) {
+                  hipStream_t s, int rr_period) {
   Context* c = nullptr;
   MH_TRY(context(&c));
   const size_t vec = (size_t)M.np * B;
@@ -691,7 +1567,7 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
           OpBiD<T, B> dd{stv, t, sv, st, it};
           MH_TRY((launch_sell<T, B>(M, dd, parts, var, s)));
         }
-        if (it % kIterRR == 0) {
+        if (it % rr_period == 0) {
           OpBiE<T, B> e1{x, r, pt, stv, sv, t, rt, st, hp, it, F_XONLY, (T)0, (T)0};
           MH_TRY(launch_rows(n, e1, parts, s));
           OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
@@ -760,8 +1636,9 @@ int krylov_solve(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W
     set_error("preconditioner %d is not implemented on the GPU path (NONE=0, DIAG=1)", P.precond);
     return -3;
   }
-  if (P.method != 1 && P.method != 5) {
-    set_error("method %d is not implemented on the GPU path (CG=1, BiCGSTAB=5)", P.method);
+  if (P.method < 1 || P.method > 8 || P.method == 4) {
+    set_error("method %d is not implemented on the GPU path (CG=1, GropCG=2, PipeCG=3, BiCGSTAB=5, "
+              "PipeBiCGSTAB=6, BiCGSTAB_noprec=7, PipeBiCGSTAB_noprec=8)", P.method);
     return -4;
   }
   if (P.max_iter < 0) {
@@ -769,7 +1646,7 @@ int krylov_solve(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W
     return -2;
   }
   if (sizeof(T) == 4) MH_TRY(matrix_make_fp32(&M, s));
-  if (P.precond == 1) MH_TRY(diag_setup<T>(M, s));
+  if (P.precond == 1 && P.method != 7 && P.method != 8) MH_TRY(diag_setup<T>(M, s));
   const auto t0 = std::chrono::steady_clock::now();
   int rc = 0;
   if (P.method == 1) {
@@ -778,11 +1655,33 @@ int krylov_solve(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W
     } else {
       MH_DISPATCH_B(M.b, rc = (cg_impl<T, B, false>(M, P, b, x, W, out, s)));
     }
+  } else if (P.method == 2) {
+    if (P.precond == 1) {
+      MH_DISPATCH_B(M.b, rc = (gropcg_impl<T, B, true>(M, P, b, x, W, out, s)));
+    } else {
+      MH_DISPATCH_B(M.b, rc = (gropcg_impl<T, B, false>(M, P, b, x, W, out, s)));
+    }
+  } else if (P.method == 3) {
+    if (P.precond == 1) {
+      MH_DISPATCH_B(M.b, rc = (pipecg_impl<T, B, true>(M, P, b, x, W, out, s)));
+    } else {
+      MH_DISPATCH_B(M.b, rc = (pipecg_impl<T, B, false>(M, P, b, x, W, out, s)));
+    }
+  } else if (P.method == 6) {
+    if (P.precond == 1) {
+      MH_DISPATCH_B(M.b, rc = (pipebicgstab_impl<T, B, true>(M, P, b, x, W, out, s)));
+    } else {
+      MH_DISPATCH_B(M.b, rc = (pipebicgstab_impl<T, B, false>(M, P, b, x, W, out, s)));
+    }
+  } else if (P.method == 8) {  // PipeBiCGSTAB_noprec.f90
+    MH_DISPATCH_B(M.b, rc = (pipebicgstab_impl<T, B, false>(M, P, b, x, W, out, s)));
+  } else if (P.method == 7) {  // BiCGSTAB_noprec.f90: no preconditioner whatever Iarray[2] says
+    MH_DISPATCH_B(M.b, rc = (bicgstab_impl<T, B, false>(M, P, b, x, W, out, s, 50)));
   } else {
     if (P.precond == 1) {
-      MH_DISPATCH_B(M.b, rc = (bicgstab_impl<T, B, true>(M, P, b, x, W, out, s)));
+      MH_DISPATCH_B(M.b, rc = (bicgstab_impl<T, B, true>(M, P, b, x, W, out, s, kIterRR)));
     } else {
-      MH_DISPATCH_B(M.b, rc = (bicgstab_impl<T, B, false>(M, P, b, x, W, out, s)));
+      MH_DISPATCH_B(M.b, rc = (bicgstab_impl<T, B, false>(M, P, b, x, W, out, s, kIterRR)));
     }
   }
   out->t_solve = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

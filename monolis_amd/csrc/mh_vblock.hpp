@@ -11,7 +11,7 @@ namespace mh {
 // substitution with the unit-lower factor, then backward substitution with
 // the reciprocal pivots.  With DOT, also the partial (src, dst) and, in
 // finalize, CG's rho = (r, z) and beta = rho / rho_prev (0 when `first`),
-// CG.f90:71-84.
+// CG.f90:71-84, or with `first` = 2 GropCG's gamma = (r, u).
 template <typename T, bool DOT>
 struct OpVPrec {
   using VT = T;
@@ -47,6 +47,10 @@ struct OpVPrec {
   }
   __device__ void finalize(const double (&tot)[1]) const {
     if constexpr (DOT) {
+      if (first == 2) {
+        st->gamma = tot[0];
+        return;
+      }
       const double rho = tot[0];
       st->beta = first ? 0.0 : rho / st->rho;
       st->rho = rho;

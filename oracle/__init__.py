@@ -32,7 +32,8 @@ _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 I_METHOD, I_PRECOND, I_MAX_ITER, I_CUR_ITER, I_IERR = 1, 2, 3, 4, 5
 I_IS_INIT_X, I_IS_ERROR_ABORT = 10, 16
 R_TOL, R_CUR_RESID = 1, 2
-ITER_CG, ITER_BICGSTAB = 1, 5
+ITER_CG, ITER_GROPCG, ITER_PIPECG, ITER_BICGSTAB, ITER_PIPEBICGSTAB, ITER_BICGSTAB_NOPREC = 1, 2, 3, 5, 6, 7
+ITER_PIPEBICGSTAB_NOPREC = 8
 PREC_NONE, PREC_DIAG = 0, 1
 
 

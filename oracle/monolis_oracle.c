@@ -37,7 +37,12 @@
 #endif
 
 #define ORC_ITER_CG 1
+#define ORC_ITER_GROPCG 2
+#define ORC_ITER_PIPECG 3
 #define ORC_ITER_BICGSTAB 5
+#define ORC_ITER_PIPEBICGSTAB 6
+#define ORC_ITER_BICGSTAB_NOPREC 7
+#define ORC_ITER_PIPEBICGSTAB_NOPREC 8
 #define ORC_PREC_NONE 0
 #define ORC_PREC_DIAG 1
 
@@ -470,11 +475,126 @@ done:
   return status;
 }
 
-/* src/iterative/BiCGSTAB.f90:18-192 (right preconditioned). */
-static int orc_bicgstab(const orc_ctx* c, const double* B, double* X,
-                        int* Iarray, double* Rarray) {
+/* src/iterative/GropCG.f90 (monolis method 2): Gropp's CG, the precondition
+ * of A p overlapped with the dot products; no residual replacement. */
+static int orc_gropcg(const orc_ctx* c, const double* B, double* X, int* Iarray,
+                      double* Rarray) {
   const int m = c->m;
-  const int iter_RR = 200;
+  double B2;
+  if (Iarray[I_IS_INIT_X]) memset(X, 0, sizeof(double) * (size_t)m);
+  double* V6[6];
+  for (int k = 0; k < 6; ++k) V6[k] = (double*)calloc((size_t)m + 1, sizeof(double));
+  double *R = V6[0], *U = V6[1], *V = V6[2], *Q = V6[3], *P = V6[4], *S = V6[5];
+  int status = ORC_NOT_CONVERGED;
+  ctx_residual(c, X, B, R);
+  B2 = orc_inner_product(m, R, R);
+  if (B2 == 0.0) {
+    memset(X, 0, sizeof(double) * (size_t)m);
+    status = ORC_OK;
+    goto done;
+  }
+  ctx_prec_apply(c, R, U);
+  orc_vec_copy(m, U, P);
+  ctx_matvec(c, P, S);
+  double gamma = orc_inner_product(m, R, U);
+  for (int iter = 1; iter <= Iarray[I_MAX_ITER]; ++iter) {
+    const double delta = orc_inner_product(m, P, S);
+    ctx_prec_apply(c, S, Q);
+    const double alpha = gamma / delta;
+    orc_vec_axpby(m, alpha, P, 1.0, X, X);
+    orc_vec_axpby(m, -alpha, S, 1.0, R, R);
+    orc_vec_axpby(m, -alpha, Q, 1.0, U, U);
+    const double gamma1 = orc_inner_product(m, R, U);
+    ctx_matvec(c, U, V);
+    const double beta = gamma1 / gamma;
+    gamma = gamma1;
+    orc_vec_axpby(m, beta, P, 1.0, U, P);
+    orc_vec_axpby(m, beta, S, 1.0, V, S);
+    if (orc_check_converge(m, R, B2, iter, Iarray, Rarray)) {
+      status = ORC_OK;
+      break;
+    }
+  }
+done:
+  for (int k = 0; k < 6; ++k) free(V6[k]);
+  return status;
+}
+
+/* src/iterative/PipeCG.f90 (monolis method 3): pipelined PCG (Ghysels and
+ * Vanroose), three dots per iteration, true residual every 50 iterations.
+ * The scalars follow the reference literally: gamma1 = 1/gamma,
+ * alpha1 = 1/alpha, beta = gamma*gamma1, alpha = gamma/(delta - beta*gamma*alpha1). */
+static int orc_pipecg(const orc_ctx* c, const double* B, double* X, int* Iarray,
+                      double* Rarray) {
+  const int m = c->m;
+  const int iter_RR = 50;
+  double B2;
+  if (Iarray[I_IS_INIT_X]) memset(X, 0, sizeof(double) * (size_t)m);
+  double* V9[9];
+  for (int k = 0; k < 9; ++k) V9[k] = (double*)calloc((size_t)m + 1, sizeof(double));
+  double *R = V9[0], *U = V9[1], *V = V9[2], *Q = V9[3], *P = V9[4], *Z = V9[5], *L = V9[6],
+         *M = V9[7], *S = V9[8];
+  int status = ORC_NOT_CONVERGED;
+  ctx_residual(c, X, B, R);
+  B2 = orc_inner_product(m, R, R);
+  if (B2 == 0.0) {
+    memset(X, 0, sizeof(double) * (size_t)m);
+    status = ORC_OK;
+    goto done;
+  }
+  ctx_prec_apply(c, R, U);
+  ctx_matvec(c, U, V);
+  double gamma = 1.0, alpha = 1.0;
+  for (int iter = 1; iter <= Iarray[I_MAX_ITER]; ++iter) {
+    const double g = orc_inner_product(m, R, U);
+    const double d = orc_inner_product(m, V, U);
+    ctx_prec_apply(c, V, M);
+    ctx_matvec(c, M, L);
+    const double gamma1 = 1.0 / gamma;
+    const double alpha1 = 1.0 / alpha;
+    gamma = g;
+    const double delta = d;
+    double beta;
+    if (1 < iter) {
+      beta = gamma * gamma1;
+      alpha = gamma / (delta - beta * gamma * alpha1);
+    } else {
+      alpha = gamma / delta;
+      beta = 0.0;
+    }
+    orc_vec_axpby(m, beta, Z, 1.0, L, Z);
+    orc_vec_axpby(m, beta, Q, 1.0, M, Q);
+    orc_vec_axpby(m, beta, S, 1.0, V, S);
+    orc_vec_axpby(m, beta, P, 1.0, U, P);
+    if (iter % iter_RR == 0) {
+      orc_vec_axpby(m, alpha, P, 1.0, X, X);
+      ctx_residual(c, X, B, R);
+      ctx_prec_apply(c, R, U);
+      ctx_matvec(c, U, V);
+    } else {
+      orc_vec_axpby(m, -alpha, S, 1.0, R, R);
+      orc_vec_axpby(m, -alpha, Q, 1.0, U, U);
+      orc_vec_axpby(m, -alpha, Z, 1.0, V, V);
+      orc_vec_axpby(m, alpha, P, 1.0, X, X);
+    }
+    if (orc_check_converge(m, R, B2, iter, Iarray, Rarray)) {
+      status = ORC_OK;
+      break;
+    }
+  }
+done:
+  for (int k = 0; k < 9; ++k) free(V9[k]);
+  return status;
+}
+
+/* src/iterative/BiCGSTAB.f90:18-192 (right preconditioned). */
+/* noprec = 1: src/iterative/BiCGSTAB_noprec.f90 (method 7): no preconditioner
+ * (whatever Iarray[2] says), no sorted-sum fallback, omega = (t,s)/(t,t) as is,
+ * true residual every 50 iterations. */
+static int orc_bicgstab(const orc_ctx* c, const double* B, double* X,
+                        int* Iarray, double* Rarray, int noprec) {
+  const int m = c->m;
+  const int iter_RR = noprec ? 50 : 200;
   double alpha = 0.0, beta, rho = 0.0, rho1 = 0.0, c2, omega = 0.0, B2;
   double CG1, CG2;
   if (Iarray[I_IS_INIT_X]) memset(X, 0, sizeof(double) * (size_t)m);
@@ -498,7 +618,7 @@ static int orc_bicgstab(const orc_ctx* c, const double* B, double* X,
   orc_vec_copy(m, R, RT);
   for (int iter = 1; iter <= Iarray[I_MAX_ITER]; ++iter) {
     rho = orc_inner_product(m, R, RT);
-    if (fabs(rho) == 0.0) rho = orc_sorted_inner_product(m, R, RT);
+    if (!noprec && fabs(rho) == 0.0) rho = orc_sorted_inner_product(m, R, RT);
     if (iter > 1) {
       beta = (rho / rho1) * (alpha / omega);
       int i;
@@ -507,18 +627,24 @@ static int orc_bicgstab(const orc_ctx* c, const double* B, double* X,
     } else {
       orc_vec_copy(m, R, P);
     }
-    ctx_prec_apply(c, P, PT);
+    if (noprec) orc_vec_copy(m, P, PT);
+    else ctx_prec_apply(c, P, PT);
     ctx_matvec(c, PT, V);
     c2 = orc_inner_product(m, RT, V);
-    if (fabs(c2) == 0.0) c2 = orc_sorted_inner_product(m, RT, V);
+    if (!noprec && fabs(c2) == 0.0) c2 = orc_sorted_inner_product(m, RT, V);
     alpha = rho / c2;
     orc_vec_axpby(m, -alpha, V, 1.0, R, S);
-    ctx_prec_apply(c, S, ST);
+    if (noprec) orc_vec_copy(m, S, ST);
+    else ctx_prec_apply(c, S, ST);
     ctx_matvec(c, ST, T);
     CG1 = orc_inner_product(m, T, S);
     CG2 = orc_inner_product(m, T, T);
-    if (fabs(CG2) == 0.0) CG2 = orc_sorted_inner_product(m, T, T);
-    omega = (fabs(CG2) == 0.0) ? 0.0 : CG1 / CG2;
+    if (noprec) {
+      omega = CG1 / CG2;
+    } else {
+      if (fabs(CG2) == 0.0) CG2 = orc_sorted_inner_product(m, T, T);
+      omega = (fabs(CG2) == 0.0) ? 0.0 : CG1 / CG2;
+    }
     {
       int i;
 #pragma omp parallel for schedule(static)
@@ -553,6 +679,98 @@ done:
  * Instead of stopping the process on non-convergence it returns
  * ORC_NOT_CONVERGED and writes it to Iarray[5].
  * ------------------------------------------------------------------------- */
+/* src/iterative/PipeBiCGSTAB.f90 (method 6) and, with noprec = 1,
+ * PipeBiCGSTAB_noprec.f90 (method 8: M = I and omega1 = QY/YY without the
+ * zero guard).  Pipelined BiCGSTAB with one fused reduction block per half
+ * step; true residual every 200 iterations.  The residual-replacement branch
+ * updates X with omega1*T as the reference does. */
+static int orc_pipebicgstab(const orc_ctx* c, const double* B, double* X, int* Iarray,
+                            double* Rarray, int noprec) {
+  const int m = c->m;
+  const int iter_RR = 200;
+  double B2;
+  if (Iarray[I_IS_INIT_X]) memset(X, 0, sizeof(double) * (size_t)m);
+  double* W15[15];
+  for (int k = 0; k < 15; ++k) W15[k] = (double*)calloc((size_t)m + 1, sizeof(double));
+  double *R = W15[0], *R0 = W15[1], *RT = W15[2], *W0 = W15[3], *WT = W15[4], *T = W15[5],
+         *PT = W15[6], *S = W15[7], *ST = W15[8], *Z = W15[9], *ZT = W15[10], *Q = W15[11],
+         *QT = W15[12], *Y = W15[13], *V = W15[14];
+  int status = ORC_NOT_CONVERGED;
+#define PB_PREC(src, dst) \
+  do {                    \
+    if (noprec)           \
+      orc_vec_copy(m, src, dst); \
+    else                  \
+      ctx_prec_apply(c, src, dst); \
+  } while (0)
+  ctx_residual(c, X, B, R);
+  B2 = orc_inner_product(m, R, R);
+  if (B2 == 0.0) {
+    memset(X, 0, sizeof(double) * (size_t)m);
+    status = ORC_OK;
+    goto done;
+  }
+  orc_vec_copy(m, R, R0);
+  PB_PREC(R, RT);
+  ctx_matvec(c, RT, W0);
+  PB_PREC(W0, WT);
+  ctx_matvec(c, WT, T);
+  double RR = orc_inner_product(m, R, R);
+  const double RW0 = orc_inner_product(m, R, W0);
+  double alpha = RR / RW0, beta = 0.0, omega = 0.0;
+  for (int iter = 1; iter <= Iarray[I_MAX_ITER]; ++iter) {
+    int i;
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < m; ++i) {
+      PT[i] = RT[i] + beta * (PT[i] - omega * ST[i]);
+      S[i] = W0[i] + beta * (S[i] - omega * Z[i]);
+      ST[i] = WT[i] + beta * (ST[i] - omega * ZT[i]);
+      Z[i] = T[i] + beta * (Z[i] - omega * V[i]);
+      Q[i] = R[i] - alpha * S[i];
+      QT[i] = RT[i] - alpha * ST[i];
+      Y[i] = W0[i] - alpha * Z[i];
+    }
+    const double QY = orc_inner_product(m, Q, Y);
+    const double YY = orc_inner_product(m, Y, Y);
+    PB_PREC(Z, ZT);
+    ctx_matvec(c, ZT, V);
+    const double omega1 = (noprec || YY != 0.0) ? QY / YY : 0.0;
+    if (iter % iter_RR == 0) {
+#pragma omp parallel for schedule(static)
+      for (i = 0; i < m; ++i) X[i] = X[i] + alpha * PT[i] + omega1 * T[i];
+      ctx_residual(c, X, B, R);
+      PB_PREC(R, RT);
+      ctx_matvec(c, RT, W0);
+    } else {
+#pragma omp parallel for schedule(static)
+      for (i = 0; i < m; ++i) {
+        X[i] = X[i] + alpha * PT[i] + omega1 * QT[i];
+        R[i] = Q[i] - omega1 * Y[i];
+        RT[i] = QT[i] - omega1 * (WT[i] - alpha * ZT[i]);
+        W0[i] = Y[i] - omega1 * (T[i] - alpha * V[i]);
+      }
+    }
+    const double RR1 = orc_inner_product(m, R0, R);
+    const double RW = orc_inner_product(m, R0, W0);
+    const double RS = orc_inner_product(m, R0, S);
+    const double RZ = orc_inner_product(m, R0, Z);
+    PB_PREC(W0, WT);
+    ctx_matvec(c, WT, T);
+    if (orc_check_converge(m, R, B2, iter, Iarray, Rarray)) {
+      status = ORC_OK;
+      break;
+    }
+    beta = (alpha * RR1) / (RR * omega1);
+    alpha = RR1 / (RW + beta * RS - beta * omega1 * RZ);
+    omega = omega1;
+    RR = RR1;
+  }
+#undef PB_PREC
+done:
+  for (int k = 0; k < 15; ++k) free(W15[k]);
+  return status;
+}
+
 int orc_solve(int n, int np, int nz, int n_dof, const int* n_dof_list,
               const int* n_dof_index, const int* n_dof_index2, const double* A,
               double* x, const double* b, const int* index, const int* item,
@@ -597,8 +815,18 @@ int orc_solve(int n, int np, int nz, int n_dof, const int* n_dof_list,
   int st;
   if (Iarray[I_METHOD] == ORC_ITER_CG)
     st = orc_cg(&c, b, x, Iarray, Rarray);
+  else if (Iarray[I_METHOD] == ORC_ITER_GROPCG)
+    st = orc_gropcg(&c, b, x, Iarray, Rarray);
+  else if (Iarray[I_METHOD] == ORC_ITER_PIPECG)
+    st = orc_pipecg(&c, b, x, Iarray, Rarray);
   else if (Iarray[I_METHOD] == ORC_ITER_BICGSTAB)
-    st = orc_bicgstab(&c, b, x, Iarray, Rarray);
+    st = orc_bicgstab(&c, b, x, Iarray, Rarray, 0);
+  else if (Iarray[I_METHOD] == ORC_ITER_BICGSTAB_NOPREC)
+    st = orc_bicgstab(&c, b, x, Iarray, Rarray, 1);
+  else if (Iarray[I_METHOD] == ORC_ITER_PIPEBICGSTAB)
+    st = orc_pipebicgstab(&c, b, x, Iarray, Rarray, 0);
+  else if (Iarray[I_METHOD] == ORC_ITER_PIPEBICGSTAB_NOPREC)
+    st = orc_pipebicgstab(&c, b, x, Iarray, Rarray, 1);
   else
     st = ORC_BAD_METHOD;
   free(c.D);
