@@ -71,6 +71,9 @@ def main() -> int:
                     help="force this SpMV kernel variant instead of running the autotuner")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the precision / batching / differentiation workloads")
+    ap.add_argument("--multi-spmm", action="store_true",
+                    help="only time the 64-RHS SpMM of the 128^3 Poisson workload (profiling runs)")
+    ap.add_argument("--fp32", action="store_true", help="with --multi-spmm: the FP32 SpMM")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the streaming-copy probe (1 GiB read + 1 GiB write per launch)")
     args = ap.parse_args()
@@ -98,6 +101,23 @@ def main() -> int:
                                              ctypes.byref(ms_c)), "stream_copy")
         return {"bytes_per_launch": 16 * n, "ms": round(ms_c.value, 5),
                 "gbs": round(16 * n / (ms_c.value * 1e-3) / 1e9, 1)}
+
+    if args.multi_spmm:
+        pm = W.poisson7(128)
+        k = 64
+        dt = np.float32 if args.fp32 else np.float64
+        dmm = api.DeviceMatrix(pm)
+        X = np.random.default_rng(args.seed).uniform(-1, 1, size=(pm.n, k)).astype(dt)
+        dxm, dym = api.DeviceBuffer.from_numpy(X), api.DeviceBuffer(pm.n * k, dt)
+        for _ in range(args.warmup):
+            dmm.matvec_multi(dxm, dym, k)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            dmm.matvec_multi(dxm, dym, k)
+        ms_m = (time.perf_counter() - t) * 1e3 / max(args.steps, 1)
+        log(f"multi-RHS SpMM ({'FP32' if args.fp32 else 'FP64'}, 64 RHS, 128^3): {ms_m:.3f} ms per call")
+        print(json.dumps({"multi_spmm_ms_per_call": round(ms_m, 4), "steps": args.steps}), flush=True)
+        return 0
 
     if args.copy_only:
         cp = copy_probe(args.steps)
