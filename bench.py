@@ -20,7 +20,10 @@ Further workloads (extra keys, skipped with --no-extras):
                    512 Poisson systems with edges 10..40 in one persistent,
                    graph-captured launch (autotuned);
   differentiation  64^3 BCSR3 SPD and 64^3 convection-diffusion: forward solve
-                   and adjoint (A^T lambda = dL/dx plus the -lambda x^T kernel).
+                   and adjoint (A^T lambda = dL/dx plus the -lambda x^T kernel);
+  complex          the headline pattern with complex values K + i 0.5 D (D the
+                   diagonal blocks): complex SpMV GB/s and COCG + complex
+                   block-Jacobi to 1e-8 (monolis method 13).
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 20]
 """
@@ -74,6 +77,8 @@ def main() -> int:
     ap.add_argument("--multi-spmm", action="store_true",
                     help="only time the 64-RHS SpMM of the 128^3 Poisson workload (profiling runs)")
     ap.add_argument("--fp32", action="store_true", help="with --multi-spmm: the FP32 SpMM")
+    ap.add_argument("--complex-spmv", action="store_true",
+                    help="only time the complex SpMV of the headline pattern (profiling runs)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the streaming-copy probe (1 GiB read + 1 GiB write per launch)")
     args = ap.parse_args()
@@ -117,6 +122,10 @@ def main() -> int:
         ms_m = (time.perf_counter() - t) * 1e3 / max(args.steps, 1)
         log(f"multi-RHS SpMM ({'FP32' if args.fp32 else 'FP64'}, 64 RHS, 128^3): {ms_m:.3f} ms per call")
         print(json.dumps({"multi_spmm_ms_per_call": round(ms_m, 4), "steps": args.steps}), flush=True)
+        return 0
+
+    if args.complex_spmv:
+        print(json.dumps({"complex": bench_complex(args, solve=False)}), flush=True)
         return 0
 
     if args.copy_only:
@@ -261,6 +270,7 @@ def main() -> int:
         m = None
         out["batching"] = bench_batching(args)
         out["differentiation"] = bench_differentiation(args)
+        out["complex"] = bench_complex(args, cpu=not args.no_cpu)
 
     print(json.dumps(out), flush=True)
     return 0
@@ -464,6 +474,55 @@ def bench_differentiation(args):
         }
         log(f"differentiation/{name}: forward {tf * 1e3:.1f} ms, adjoint {ta * 1e3:.1f} ms "
             f"(first {t_first * 1e3:.1f} ms)")
+    return res
+
+
+def bench_complex(args, solve=True, cpu=False):
+    """Complex SpMV and COCG on the headline pattern with complex values."""
+    import numpy as np
+    from monolis_amd import _lib, api, metrics, workloads as W
+    t0 = time.time()
+    m = W.stencil27_bcsr3_complex(args.grid, seed=args.seed)
+    rng = np.random.default_rng(args.seed + 20)
+    log(f"built {m.meta['workload']} in {time.time() - t0:.1f}s")
+    dm = api.DeviceMatrixC(m)
+    x = rng.uniform(-1, 1, m.nndof) + 1j * rng.uniform(-1, 1, m.nndof)
+    dx, dy = api.DeviceBuffer.from_numpy(x), api.DeviceBuffer(m.nndof, np.complex128)
+    ms = dm.time_matvec(dx, dy, warmup=args.warmup, reps=args.steps)
+    nbytes = metrics.spmv_bytes(m.n, m.nz, 3, 16)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    log(f"complex SpMV: {ms * 1e3:.1f} us/launch, {gbs:.0f} GB/s ({100 * gbs / metrics.HBM_PEAK_GBS:.1f} % of peak)")
+    res = {"workload": m.meta["workload"], "spmv_ms": round(ms, 5), "spmv_gbs": round(gbs, 1),
+           "spmv_frac_hbm_peak": round(gbs / metrics.HBM_PEAK_GBS, 4), "algorithmic_bytes": nbytes,
+           "kernel": "k_sell<zd,zd,3,OpZSpmv>"}
+    if solve:
+        xt = rng.uniform(-1, 1, m.nndof) + 1j * rng.uniform(-1, 1, m.nndof)
+        dxt = api.DeviceBuffer.from_numpy(xt)
+        db = api.DeviceBuffer(m.nndof, np.complex128)
+        dm.matvec(dxt, db)
+        xs = api.DeviceBuffer(m.npndof, np.complex128)
+        p = _prm(method=13, tol=1e-8)
+        t, _ = _timed(lambda: dm.solve(db, xs, p), reps=2)   # first call builds the complex block-Jacobi
+        it = int(p.Iarray[4])
+        err = float(np.linalg.norm(xs.numpy() - xt) / np.linalg.norm(xt))
+        ib = metrics.cg_iter_bytes(m.n, m.nz, 3, 16)
+        res.update({"cocg_ms": round(t * 1e3, 3), "cocg_iters": it, "cocg_status": int(p.Iarray[5]),
+                    "cocg_ms_per_iter": round(t * 1e3 / max(it, 1), 4),
+                    "cocg_iter_bytes": ib, "cocg_gbs": round(ib * it / t / 1e9, 1),
+                    "cocg_rel_err_vs_xtrue": err, "tol": 1e-8, "precond": "complex block-Jacobi 3x3"})
+        log(f"COCG: {it} iterations, {t * 1e3:.1f} ms, rel. err {err:.2e}")
+    if cpu:
+        import oracle
+        oracle.set_threads(os.cpu_count() or 1)
+        t = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            oracle.matvec_C(m, x)
+        tc = (time.perf_counter() - t) / reps
+        res["cpu_spmv_ms"] = round(tc * 1e3, 2)
+        res["cpu_spmv_gbs"] = round(nbytes / tc / 1e9, 2)
+        res["cpu_threads"] = oracle.get_threads()
+        log(f"CPU complex SpMV: {tc * 1e3:.1f} ms")
     return res
 
 

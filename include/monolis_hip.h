@@ -58,6 +58,40 @@ void monolis_matvec_product_R_c_main(int n, int np, int nz, int n_dof, int NPNDO
  * defined at wrapper/linalg/inner_product_wrap.f90:35-55: *sum = sum_i x_i y_i. */
 void monolis_inner_product_R_c_main(int m, double* x, double* y, double* sum, int comm);
 
+/* Complex path (double _Complex = two doubles, real part first; the layout of
+ * Fortran complex(c_double) and C99 double _Complex).  Fixed block sizes
+ * 1..8; variable blocks (n_dof = -1) are rejected on this path. */
+
+/* Replaces monolis_solve_C_c_main, wrapper/solver/monolis_solver_c.h:124-153,
+ * defined as the Fortran bind(c) routine wrapper/solver/solver_wrapper.f90:94-175.
+ * Runs monolis_solve_main_C (src/solver/solver.f90:189-213): the method in
+ * Iarray[1] must be COCG = 13 (monolis_solver_select_C accepts nothing else),
+ * preconditioner NONE = 0 or DIAG = 1.  Fills Iarray[4] (cur_iter), Iarray[5]
+ * (status as the real entry point; -6 = zero pivot in a diagonal block, where
+ * monolis stops), Rarray[2] = real part of the complex sqrt(R2/B2) (what
+ * monolis stores; convergence tests its modulus) and Rarray[3..4]. */
+void monolis_solve_C_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZNDOF2, int* n_dof_list,
+                            int* n_dof_index, int* n_dof_index2, double _Complex* A, double _Complex* x,
+                            double _Complex* b, int* index, int* item, int my_rank, int comm, int comm_size,
+                            int recv_n_neib, int recv_nitem, int* recv_neib_pe, int* recv_index, int* recv_item,
+                            int send_n_neib, int send_nitem, int* send_neib_pe, int* send_index, int* send_item,
+                            int* Iarray, double* Rarray);
+
+/* Replaces monolis_matvec_product_C_c_main, wrapper/linalg/monolis_matvec_c.h:68-95,
+ * defined at wrapper/linalg/matvec_wrap.f90:112-207:
+ * y[0 .. n*n_dof) = A x in complex arithmetic. */
+void monolis_matvec_product_C_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZNDOF2, int* n_dof_list,
+                                     int* n_dof_index, int* n_dof_index2, double _Complex* A, double _Complex* x,
+                                     double _Complex* y, int* index, int* item, int my_rank, int comm,
+                                     int comm_size, int recv_n_neib, int recv_nitem, int* recv_neib_pe,
+                                     int* recv_index, int* recv_item, int send_n_neib, int send_nitem,
+                                     int* send_neib_pe, int* send_index, int* send_item);
+
+/* Replaces monolis_inner_product_C_c_main, wrapper/linalg/monolis_inner_product_c.h:155-160,
+ * defined at wrapper/linalg/inner_product_wrap.f90:59-77:
+ * *sum = sum_i x_i y_i, unconjugated (src/linalg/inner_product.f90:457-499). */
+void monolis_inner_product_C_c_main(int m, double _Complex* x, double _Complex* y, double _Complex* sum, int comm);
+
 /* ======================================================================== */
 /* Part 2: device-resident interface and extensions (no monolis counterpart  */
 /* except where cited).  Every function returns 0 on success, < 0 on error;  */
@@ -229,6 +263,28 @@ int monolis_hip_outer_grad_R(monolis_hip_matrix* M, const double* lambda, const 
  * monolis_hip_solve_R, for the adjoint solve. */
 int monolis_hip_adjoint_R(monolis_hip_matrix* M, const double* x, const double* g, double* lambda, double* dA,
                           int symmetric, int* Iarray, double* Rarray);
+
+/* Device-resident complex matrix (the complex counterpart of monolis_hip_matrix;
+ * same BCSR arrays, A holds nz * n_dof^2 double _Complex values).  A separate
+ * handle type keeps real and complex matrices apart at compile time. */
+typedef struct monolis_hip_matrix_C monolis_hip_matrix_C;
+int monolis_hip_matrix_create_C(monolis_hip_matrix_C** out, int n, int np, int nz, int n_dof, const int* index,
+                                const int* item, const double _Complex* A);
+int monolis_hip_matrix_destroy_C(monolis_hip_matrix_C* M);
+int monolis_hip_matrix_set_values_C(monolis_hip_matrix_C* M, const double _Complex* A);
+/* info[0..7] as monolis_hip_matrix_info, value bytes at 16 per entry (info[7] = 0). */
+int monolis_hip_matrix_info_C(const monolis_hip_matrix_C* M, long long* info);
+/* y = A x (monolis_matvec_product_main_C, src/linalg/matvec.f90:192-227). */
+int monolis_hip_matvec_C(monolis_hip_matrix_C* M, const double _Complex* x, double _Complex* y);
+/* *sum = sum_i x_i y_i, unconjugated (monolis_inner_product_main_C). */
+int monolis_hip_inner_product_C(int m, const double _Complex* x, const double _Complex* y, double _Complex* sum);
+/* COCG (src/iterative/COCG.f90:18-105) on device vectors; Iarray / Rarray as in
+ * monolis_solve_C_c_main; hist[k] = |sqrt(R2/B2)| after iteration k. */
+int monolis_hip_solve_C(monolis_hip_matrix_C* M, const double _Complex* b, double _Complex* x, int* Iarray,
+                        double* Rarray, double* hist, int hist_len);
+/* Average time of one complex SpMV over `reps` launches after `warmup`. */
+int monolis_hip_time_matvec_C(monolis_hip_matrix_C* M, const double _Complex* x, double _Complex* y, int warmup,
+                              int reps, double* ms);
 
 /* Bandwidth probe (no monolis counterpart): a plain streaming copy kernel
  * dst[i] = src[i] over n doubles, `reps` timed launches on the library stream;

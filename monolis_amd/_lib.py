@@ -26,6 +26,18 @@ SIGNATURES = [
     ("monolis_solve_R_c_main", None, _FLAT + [c_void_p, c_void_p]),
     ("monolis_matvec_product_R_c_main", None, _FLAT),
     ("monolis_inner_product_R_c_main", None, [c_int, c_void_p, c_void_p, c_void_p, c_int]),
+    ("monolis_solve_C_c_main", None, _FLAT + [c_void_p, c_void_p]),
+    ("monolis_matvec_product_C_c_main", None, _FLAT),
+    ("monolis_inner_product_C_c_main", None, [c_int, c_void_p, c_void_p, c_void_p, c_int]),
+    ("monolis_hip_matrix_create_C", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int,
+                                            c_void_p, c_void_p, c_void_p]),
+    ("monolis_hip_matrix_destroy_C", c_int, [c_void_p]),
+    ("monolis_hip_matrix_set_values_C", c_int, [c_void_p, c_void_p]),
+    ("monolis_hip_matrix_info_C", c_int, [c_void_p, c_ll_p]),
+    ("monolis_hip_matvec_C", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("monolis_hip_inner_product_C", c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    ("monolis_hip_solve_C", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
+    ("monolis_hip_time_matvec_C", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_double_p]),
     ("monolis_hip_last_error", ctypes.c_char_p, []),
     ("monolis_hip_init", c_int, []),
     ("monolis_hip_stream", c_void_p, []),

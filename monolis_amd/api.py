@@ -21,6 +21,7 @@ from .matrix import Matrix
 
 # ids, src/define/def_solver_prm.f90:7-58 / wrapper/define/monolis_def_solver_prm_c.h
 MONOLIS_ITER_CG, MONOLIS_ITER_BICGSTAB = 1, 5
+MONOLIS_ITER_COCG = 13
 MONOLIS_PREC_NONE, MONOLIS_PREC_DIAG, MONOLIS_PREC_SOR, MONOLIS_PREC_LU = 0, 1, 2, 3
 MONOLIS_PRM_I_METHOD, MONOLIS_PRM_I_PRECOND, MONOLIS_PRM_I_MAX_ITER = 1, 2, 3
 MONOLIS_PRM_I_CUR_ITER, MONOLIS_PRM_I_IERR = 4, 5
@@ -142,6 +143,52 @@ def monolis_inner_product_R(monolis: Monolis, n_dof: int, x: np.ndarray, y: np.n
     s = np.zeros(1, np.float64)
     L.monolis_inner_product_R_c_main(m, _p(x), _p(y), _p(s), 0)
     return float(s[0])
+
+
+# complex path (wrapper/solver/monolis_solver_c.c monolis_solve_C,
+# wrapper/linalg/monolis_matvec_c.c monolis_matvec_product_C,
+# wrapper/linalg/monolis_inner_product_c.c monolis_inner_product_C)
+def _flat_mat_args_C(mat: Matrix):
+    mat.ensure_dof_index()
+    if mat.A.dtype != np.complex128:
+        raise TypeError("complex entry points take complex128 matrix values")
+    return _flat_mat_args(mat)
+
+
+def monolis_solve_C(monolis: Monolis, b: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """C monolis_solve_C: x (complex128) is overwritten in place."""
+    L = _lib.load()
+    mat = monolis.mat
+    assert x.dtype == np.complex128 and x.flags.c_contiguous and len(x) >= mat.npndof
+    b = np.ascontiguousarray(b, np.complex128)
+    args = _flat_mat_args_C(mat)
+    L.monolis_solve_C_c_main(*args[:10], _p(x), _p(b), _p(mat.index), _p(mat.item),
+                             *_self_comm_args(), _p(monolis.prm.Iarray), _p(monolis.prm.Rarray))
+    return x
+
+
+def monolis_matvec_product_C(monolis: Monolis, x: np.ndarray, y: Optional[np.ndarray] = None) -> np.ndarray:
+    """C monolis_matvec_product_C: y = A x in complex arithmetic."""
+    L = _lib.load()
+    mat = monolis.mat
+    x = np.ascontiguousarray(x, np.complex128)
+    if y is None:
+        y = np.zeros(mat.npndof, np.complex128)
+    args = _flat_mat_args_C(mat)
+    L.monolis_matvec_product_C_c_main(*args[:10], _p(x), _p(y), _p(mat.index), _p(mat.item),
+                                      *_self_comm_args())
+    return y
+
+
+def monolis_inner_product_C(monolis: Monolis, n_dof: int, x: np.ndarray, y: np.ndarray) -> complex:
+    """C monolis_inner_product_C: unconjugated sum over N*n_dof entries."""
+    L = _lib.load()
+    m = monolis.mat.n * n_dof
+    x = np.ascontiguousarray(x, np.complex128)
+    y = np.ascontiguousarray(y, np.complex128)
+    s = np.zeros(1, np.complex128)
+    L.monolis_inner_product_C_c_main(m, _p(x), _p(y), _p(s), 0)
+    return complex(s[0])
 
 
 # ---------------------------------------------------------------------------
@@ -356,6 +403,72 @@ class DeviceMatrix:
                                                        ctypes.c_void_p(y.ptr), warmup, reps, ctypes.byref(ms)),
                    "time_matvec")
         return ms.value
+
+
+class DeviceMatrixC:
+    """A complex monolis BCSR matrix (complex128 values) resident in HBM."""
+
+    def __init__(self, mat: Matrix):
+        L = _lib.load()
+        h = ctypes.c_void_p()
+        mat.ensure_dof_index()
+        A = np.ascontiguousarray(mat.A, np.complex128)
+        _lib.check(L.monolis_hip_matrix_create_C(ctypes.byref(h), mat.n, mat.np_, mat.nz, mat.ndof,
+                                                 _p(mat.index), _p(mat.item), _p(A)),
+                   "monolis_hip_matrix_create_C")
+        self.h = h.value
+        self.mat = mat
+        self.m = mat.nndof
+        self.vec = mat.npndof
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                _lib.load().monolis_hip_matrix_destroy_C(ctypes.c_void_p(self.h))
+                self.h = None
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        a = (ctypes.c_longlong * 8)()
+        _lib.check(_lib.load().monolis_hip_matrix_info_C(ctypes.c_void_p(self.h), a), "info_C")
+        keys = ["n", "np", "nz", "ndof", "slices", "slots", "bytes", "unused"]
+        return dict(zip(keys, list(a)))
+
+    def set_values(self, A) -> None:
+        if isinstance(A, DeviceBuffer):
+            ptr = ctypes.c_void_p(A.ptr)
+        else:
+            A = np.ascontiguousarray(A, np.complex128)
+            ptr = _p(A)
+        _lib.check(_lib.load().monolis_hip_matrix_set_values_C(ctypes.c_void_p(self.h), ptr), "set_values_C")
+
+    def matvec(self, x: DeviceBuffer, y: DeviceBuffer) -> None:
+        _lib.check(_lib.load().monolis_hip_matvec_C(ctypes.c_void_p(self.h), ctypes.c_void_p(x.ptr),
+                                                    ctypes.c_void_p(y.ptr)), "matvec_C")
+
+    def solve(self, b: DeviceBuffer, x: DeviceBuffer, prm: Prm, hist_len: int = 0):
+        hist = np.zeros(max(hist_len, 1), np.float64)
+        rc = _lib.load().monolis_hip_solve_C(ctypes.c_void_p(self.h), ctypes.c_void_p(b.ptr), ctypes.c_void_p(x.ptr),
+                                             _p(prm.Iarray), _p(prm.Rarray), _p(hist), hist_len)
+        _lib.check(rc, "monolis_hip_solve_C")
+        return hist[:hist_len] if hist_len else None
+
+    def time_matvec(self, x: DeviceBuffer, y: DeviceBuffer, warmup: int = 5, reps: int = 50) -> float:
+        ms = ctypes.c_double()
+        _lib.check(_lib.load().monolis_hip_time_matvec_C(ctypes.c_void_p(self.h), ctypes.c_void_p(x.ptr),
+                                                         ctypes.c_void_p(y.ptr), warmup, reps, ctypes.byref(ms)),
+                   "time_matvec_C")
+        return ms.value
+
+
+def inner_product_device_C(x: DeviceBuffer, y: DeviceBuffer, m: Optional[int] = None) -> complex:
+    """Unconjugated (x, y) of two complex device buffers."""
+    n = x.count if m is None else m
+    s = np.zeros(1, np.complex128)
+    _lib.check(_lib.load().monolis_hip_inner_product_C(n, ctypes.c_void_p(x.ptr), ctypes.c_void_p(y.ptr), _p(s)),
+               "inner_product_C")
+    return complex(s[0])
 
 
 def inner_product_device(x: DeviceBuffer, y: DeviceBuffer, m: Optional[int] = None) -> float:

@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmonolis_hip.so")
 
-SOURCES = ["mh_matrix.hip", "mh_krylov.hip", "mh_capi.hip", "mh_sorted.hip", "mh_batch.hip", "mh_multirhs.hip", "mh_adjoint.hip", "mh_vblock.hip"]
+SOURCES = ["mh_matrix.hip", "mh_krylov.hip", "mh_capi.hip", "mh_sorted.hip", "mh_batch.hip", "mh_multirhs.hip", "mh_adjoint.hip", "mh_vblock.hip", "mh_complex.hip"]
 
 ARCH = os.environ.get("MONOLIS_HIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -66,15 +66,17 @@ def build_callers(verbose: bool = True) -> list:
     built.append(c_exe)
     if os.path.exists(AMDFLANG):
         mod_obj = os.path.join(BUILD, "mod_monolis_hip.o")
-        f_exe = os.path.join(BINDIR, "test_mtx_dat")
-        for cmd in ([AMDFLANG, "-O2", "-c", os.path.join(HERE, "fortran", "mod_monolis_hip.f90"),
-                     "-J", BUILD, "-o", mod_obj],
-                    [AMDFLANG, "-O2", "-J", BUILD, os.path.join(HERE, "fortran", "test_mtx_dat.f90"), mod_obj,
-                     "-L", LIBDIR, "-lmonolis_hip", rpath, "-o", f_exe]):
-            r = subprocess.run(cmd, capture_output=True, text=True)
+        r = subprocess.run([AMDFLANG, "-O2", "-c", os.path.join(HERE, "fortran", "mod_monolis_hip.f90"),
+                            "-J", BUILD, "-o", mod_obj], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran module build failed:\n{r.stderr}")
+        for prog in ("test_mtx_dat", "test_cocg_chain"):
+            f_exe = os.path.join(BINDIR, prog)
+            r = subprocess.run([AMDFLANG, "-O2", "-J", BUILD, os.path.join(HERE, "fortran", prog + ".f90"), mod_obj,
+                                "-L", LIBDIR, "-lmonolis_hip", rpath, "-o", f_exe], capture_output=True, text=True)
             if r.returncode != 0:
                 raise RuntimeError(f"Fortran caller build failed:\n{r.stderr}")
-        built.append(f_exe)
+            built.append(f_exe)
     if verbose:
         print(f"[monolis_amd] built callers {', '.join(os.path.basename(b) for b in built)}")
     return built

@@ -4,9 +4,14 @@
  *   - the R33 matvec known answer (wrapper_test/linalg/monolis_matvec_c_test.c:161-252),
  *   - the 2-DOF chain solve with diagonal rand()%1001+5000 and symmetric
  *     off-diagonal blocks rand()%1001, b = A 1, tol 1e-10, x = 1 expected
- *     (wrapper_test/solver/monolis_solver_c_test.c:8-81).
- * Prints "C_ABI_OK" when both pass, exits 1 otherwise.
+ *     (wrapper_test/solver/monolis_solver_c_test.c:8-81),
+ *   - its complex twin solved by COCG: diagonal v + 0.1 v i with v = rand()%1001+5000,
+ *     off-diagonal blocks v + 0.1 v i with v = rand()%1001 set on both sides,
+ *     b = A (1 + i), tol 1e-10, x = 1 + i expected
+ *     (wrapper_test/solver/monolis_solver_c_test.c:83-158).
+ * Prints "C_ABI_OK" when all pass, exits 1 otherwise.
  */
+#include <complex.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -40,11 +45,11 @@ static void set_entry(double* A, const int* index, const int* item, int nd, int 
   else A[nd * nd * k + nd * si + sj] = v;
 }
 
-static void prm_default(int* I, double* R) {
-  memset(I, 0, 100 * sizeof(int));
-  memset(R, 0, 100 * sizeof(double));
-  I[1] = 1; I[2] = 1; I[3] = 1000; I[5] = -1; I[10] = 1; I[16] = 1; I[24] = 4;
-  R[1] = 1.0e-8;
+static void prm_default(int* Ip, double* Rp) {
+  memset(Ip, 0, 100 * sizeof(int));
+  memset(Rp, 0, 100 * sizeof(double));
+  Ip[1] = 1; Ip[2] = 1; Ip[3] = 1000; Ip[5] = -1; Ip[10] = 1; Ip[16] = 1; Ip[24] = 4;
+  Rp[1] = 1.0e-8;
 }
 
 int main(void) {
@@ -91,19 +96,60 @@ int main(void) {
     monolis_matvec_product_R_c_main(n, n, nz, nd, n * nd, nz * 4, empty, empty, empty, A, a, b, index, item, 0, 0,
                                     1, 0, 0, empty, empty, empty, 0, 0, empty, empty, empty);
     for (int k = 0; k < 20; ++k) a[k] = 0.0;
-    int I[100];
-    double R[100];
-    prm_default(I, R);
-    R[1] = 1.0e-10;
+    int Ir[100];
+    double Rr[100];
+    prm_default(Ir, Rr);
+    Rr[1] = 1.0e-10;
     monolis_solve_R_c_main(n, n, nz, nd, n * nd, nz * 4, empty, empty, empty, A, a, b, index, item, 0, 0, 1, 0, 0,
-                           empty, empty, empty, 0, 0, empty, empty, empty, I, R);
+                           empty, empty, empty, 0, 0, empty, empty, empty, Ir, Rr);
     double err = 0.0;
     for (int k = 0; k < 20; ++k) err = fmax(err, fabs(a[k] - 1.0));
-    printf("chain solve: iter %d resid %.3e max|x-1| %.3e status %d\n", I[4], R[2], err, I[5]);
-    if (err > 1e-8 || I[5] != 0) ok = 0;
+    printf("chain solve: iter %d resid %.3e max|x-1| %.3e status %d\n", Ir[4], Rr[2], err, Ir[5]);
+    if (err > 1e-8 || Ir[5] != 0) ok = 0;
     double s = 0.0;
     monolis_inner_product_R_c_main(20, a, a, &s, 0);
     if (fabs(s - 20.0) > 1e-6) ok = 0;
+    free(A);
+  }
+  /* --- complex 2-DOF chain, COCG (method 13), x = 1 + i --- */
+  {
+    const int n = 10, nd = 2;
+    int index[11], item[30];
+    const int nz = chain_pattern(n, index, item);
+    double _Complex* A = calloc((size_t)nz * 4, sizeof(double _Complex));
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < nd; ++j) {
+        const double v = rand() % 1001 + 5000.0;
+        A[4 * find_block(index, item, i, i) + 2 * j + j] = v + 0.1 * v * I;
+      }
+    for (int i = 0; i < n - 1; ++i)
+      for (int j = 0; j < nd; ++j)
+        for (int k = 0; k < nd; ++k) {
+          const double v = rand() % 1001;
+          A[4 * find_block(index, item, i, i + 1) + 2 * j + k] = v + 0.1 * v * I;
+          A[4 * find_block(index, item, i + 1, i) + 2 * j + k] = v + 0.1 * v * I;
+        }
+    double _Complex a[20], b[20];
+    for (int k = 0; k < 20; ++k) a[k] = 1.0 + 1.0 * I;
+    monolis_matvec_product_C_c_main(n, n, nz, nd, n * nd, nz * 4, empty, empty, empty, A, a, b, index, item, 0, 0,
+                                    1, 0, 0, empty, empty, empty, 0, 0, empty, empty, empty);
+    for (int k = 0; k < 20; ++k) a[k] = 0.0;
+    int Ia[100];
+    double Ra[100];
+    prm_default(Ia, Ra);
+    Ia[1] = 13;  /* MONOLIS_ITER_COCG */
+    Ra[1] = 1.0e-10;
+    monolis_solve_C_c_main(n, n, nz, nd, n * nd, nz * 4, empty, empty, empty, A, a, b, index, item, 0, 0, 1, 0, 0,
+                           empty, empty, empty, 0, 0, empty, empty, empty, Ia, Ra);
+    double err = 0.0;
+    for (int k = 0; k < 20; ++k) err = fmax(err, cabs(a[k] - (1.0 + 1.0 * I)));
+    printf("complex chain solve (COCG): iter %d resid %.3e max|x-(1+i)| %.3e status %d\n", Ia[4], Ra[2], err,
+           Ia[5]);
+    if (err > 1e-8 || Ia[5] != 0) ok = 0;
+    /* unconjugated: (x, x) with x = 1 + i is 2i per entry */
+    double _Complex s = 0.0;
+    monolis_inner_product_C_c_main(20, a, a, &s, 0);
+    if (cabs(s - 40.0 * I) > 1e-6) ok = 0;
     free(A);
   }
   if (!ok) return 1;

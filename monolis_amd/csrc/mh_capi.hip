@@ -14,6 +14,7 @@
 #include "mh_batch.hpp"
 #include "mh_multirhs.hpp"
 #include "mh_adjoint.hpp"
+#include "mh_complex.hpp"
 
 using mh::Context;
 using mh::Matrix;
@@ -21,6 +22,12 @@ using mh::Matrix;
 struct monolis_hip_batch {
   mh::Batch* B = nullptr;
   ~monolis_hip_batch() { delete B; }
+};
+
+struct monolis_hip_matrix_C {
+  Matrix* M = nullptr;  // is_complex
+  mh::Workspace W;
+  ~monolis_hip_matrix_C() { delete M; }
 };
 
 struct monolis_hip_matrix {
@@ -42,7 +49,7 @@ enum { R_TOL = 1, R_CUR_RESID = 2, R_TIME_SOL = 3, R_TIME_PREP = 4, R_TIME_SPMV 
 
 // Status codes written to Iarray[5].
 enum { ST_OK = 0, ST_NOT_CONVERGED = 1, ST_HIP = -1, ST_ARGS = -2, ST_PRECOND = -3, ST_METHOD = -4,
-       ST_MULTI_RANK = -5 };
+       ST_MULTI_RANK = -5, ST_ZERO_PIVOT = -6 };
 
 const char* str_iter(int m) {
   static const char* s[] = {"CG", "GropCG", "PipeCG", "PipeCR", "BiCGSTAB", "PipeBiCGSTAB", "BiCGSTAB_noprec",
@@ -134,6 +141,25 @@ int solve_device(Matrix& A, mh::Workspace& W, const double* b, double* x, int* I
     Rarray[R_CUR_RESID] = res.resid;
   } else {
     fprintf(stderr, "** monolis error: norm of RHS vector B is 0.0\n");  // mat_converge.f90:41-43
+  }
+  if (hist_out) *hist_out = std::move(res.hist);
+  return res.converged ? ST_OK : ST_NOT_CONVERGED;
+}
+
+// Complex counterpart of solve_device: COCG on the handle's complex matrix.
+int solve_device_C(Matrix& A, mh::Workspace& W, const mh::zd* b, mh::zd* x, int* Iarray, double* Rarray,
+                   std::vector<double>* hist_out) {
+  Context* c = nullptr;
+  if (mh::context(&c)) return ST_HIP;
+  const mh::SolveParams P = params_from(Iarray, Rarray);
+  mh::ZSolveResult res;
+  const int rc = mh::cocg_solve(A, P, b, x, W, &res, c->stream);
+  if (rc) return rc == -3 ? ST_PRECOND : rc == -4 ? ST_METHOD : rc == -2 ? ST_ARGS : rc == -6 ? ST_ZERO_PIVOT : ST_HIP;
+  if (!res.zero_rhs) {
+    Iarray[I_CUR_ITER] = res.iter;
+    Rarray[R_CUR_RESID] = res.resid_re;  // mat_converge.f90:178: complex resid stored in a real slot
+  } else {
+    fprintf(stderr, "** monolis error: norm of RHS vector B is 0.0\n");  // mat_converge.f90:79-82
   }
   if (hist_out) *hist_out = std::move(res.hist);
   return res.converged ? ST_OK : ST_NOT_CONVERGED;
@@ -971,6 +997,280 @@ void monolis_matvec_product_R_c_main(int n, int np, int nz, int n_dof, int NPNDO
 void monolis_inner_product_R_c_main(int m, double* x, double* y, double* sum, int comm) {
   (void)comm;
   if (monolis_hip_inner_product_R(m, x, y, sum)) error_stop(mh::last_error());
+}
+
+// ---------------------------------------------------------------------------
+// Complex path
+// ---------------------------------------------------------------------------
+int monolis_hip_matrix_create_C(monolis_hip_matrix_C** out, int n, int np, int nz, int n_dof, const int* index,
+                                const int* item, const double _Complex* A) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (!out) {
+    mh::set_error("null output handle");
+    return ST_ARGS;
+  }
+  *out = nullptr;
+  if (n_dof == -1) {
+    mh::set_error("variable block sizes (n_dof = -1) are not supported on the complex GPU path");
+    return ST_ARGS;
+  }
+  Matrix* M = nullptr;
+  MH_TRY(mh::matrix_create_complex(&M, n, np, nz, n_dof, index, item, reinterpret_cast<const double*>(A),
+                                   c->stream));
+  auto* H = new monolis_hip_matrix_C();
+  H->M = M;
+  *out = H;
+  return 0;
+}
+
+int monolis_hip_matrix_destroy_C(monolis_hip_matrix_C* H) {
+  if (!H) return 0;
+  Context* c = nullptr;
+  if (mh::context(&c) == 0) (void)hipStreamSynchronize(c->stream);
+  delete H;
+  return 0;
+}
+
+int monolis_hip_matrix_set_values_C(monolis_hip_matrix_C* H, const double _Complex* A) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_TRY(mh::matrix_set_values(H->M, reinterpret_cast<const double*>(A), c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_matrix_info_C(const monolis_hip_matrix_C* H, long long* info) {
+  const Matrix* M = H->M;
+  info[0] = M->n;
+  info[1] = M->np;
+  info[2] = M->nz;
+  info[3] = M->b;
+  info[4] = M->nslices;
+  info[5] = (long long)M->nslots;
+  info[6] = (long long)(M->nslots * mh::kWave * (size_t)M->b * M->b * 16 + M->nslots * mh::kWave * 4);
+  info[7] = 0;
+  return 0;
+}
+
+int monolis_hip_matvec_C(monolis_hip_matrix_C* H, const double _Complex* x, double _Complex* y) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  MH_TRY(mh::zspmv(*H->M, reinterpret_cast<const mh::zd*>(x), reinterpret_cast<mh::zd*>(y), c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_time_matvec_C(monolis_hip_matrix_C* H, const double _Complex* x, double _Complex* y, int warmup,
+                              int reps, double* ms) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  const auto* dx = reinterpret_cast<const mh::zd*>(x);
+  auto* dy = reinterpret_cast<mh::zd*>(y);
+  for (int i = 0; i < warmup; ++i) MH_TRY(mh::zspmv(*H->M, dx, dy, c->stream));
+  hipEvent_t e0, e1;
+  MH_HIP(hipEventCreate(&e0));
+  MH_HIP(hipEventCreate(&e1));
+  MH_HIP(hipEventRecord(e0, c->stream));
+  int rc = 0;
+  for (int i = 0; i < reps && !rc; ++i) rc = mh::zspmv(*H->M, dx, dy, c->stream);
+  MH_HIP(hipEventRecord(e1, c->stream));
+  MH_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  MH_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *ms = reps > 0 ? (double)t / reps : 0.0;
+  return 0;
+}
+
+int monolis_hip_inner_product_C(int m, const double _Complex* x, const double _Complex* y, double _Complex* sum) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (m < 0) {
+    mh::set_error("negative length");
+    return ST_ARGS;
+  }
+  const size_t np = mh::partials_needed(m) * 2;
+  mh::DevArray<double> scratch;
+  MH_TRY(scratch.alloc(np + 16));
+  const mh::zd* dx = reinterpret_cast<const mh::zd*>(x);
+  const mh::zd* dy = reinterpret_cast<const mh::zd*>(y);
+  mh::DevArray<mh::zd> hx, hy;
+  if (!mh::is_device_pointer(x)) {
+    MH_TRY(hx.alloc((size_t)m));
+    MH_TRY(mh::upload(hx.p, x, sizeof(mh::zd) * (size_t)m, c->stream));
+    dx = hx.p;
+  }
+  if (!mh::is_device_pointer(y)) {
+    MH_TRY(hy.alloc((size_t)m));
+    MH_TRY(mh::upload(hy.p, y, sizeof(mh::zd) * (size_t)m, c->stream));
+    dy = hy.p;
+  }
+  MH_TRY(mh::zdot(m, dx, dy, scratch.p + np, scratch.p, c->stream));
+  MH_HIP(hipMemcpyAsync(sum, scratch.p + np, sizeof(mh::zd), hipMemcpyDefault, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_solve_C(monolis_hip_matrix_C* H, const double _Complex* b, double _Complex* x, int* Iarray,
+                        double* Rarray, double* hist, int hist_len) {
+  const double t0 = now();
+  std::vector<double> h;
+  const int st = solve_device_C(*H->M, H->W, reinterpret_cast<const mh::zd*>(b), reinterpret_cast<mh::zd*>(x),
+                                Iarray, Rarray, &h);
+  Iarray[I_IERR] = st;
+  Rarray[R_TIME_SOL] = now() - t0;
+  if (hist && hist_len > 0) {
+    for (int k = 0; k < hist_len; ++k) hist[k] = (k < (int)h.size()) ? h[k] : 0.0;
+  }
+  return st < 0 ? st : 0;
+}
+
+void monolis_solve_C_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZNDOF2, int* n_dof_list,
+                            int* n_dof_index, int* n_dof_index2, double _Complex* A, double _Complex* x,
+                            double _Complex* b, int* index, int* item, int my_rank, int comm, int comm_size,
+                            int recv_n_neib, int recv_nitem, int* recv_neib_pe, int* recv_index, int* recv_item,
+                            int send_n_neib, int send_nitem, int* send_neib_pe, int* send_index, int* send_item,
+                            int* Iarray, double* Rarray) {
+  (void)NZNDOF2; (void)comm; (void)n_dof_list; (void)n_dof_index; (void)n_dof_index2;
+  (void)recv_n_neib; (void)recv_nitem; (void)recv_neib_pe; (void)recv_index; (void)recv_item;
+  (void)send_n_neib; (void)send_nitem; (void)send_neib_pe; (void)send_index; (void)send_item;
+  const double t0 = now();
+  for (int k = R_TIME_SOL; k <= R_TIME_COMM_SPMV; ++k) Rarray[k] = 0.0;
+  const int abort_on_error = Iarray[I_IS_ERROR_ABORT] != 0;
+  auto fail = [&](int code) {
+    Iarray[I_IERR] = code;
+    fprintf(stderr, "** monolis error: %s\n", mh::last_error());
+    fflush(stderr);
+    if (abort_on_error) error_stop("monolis_solve_C (GPU path) failed");
+  };
+  if (comm_size > 1) {
+    mh::set_error("comm_size %d > 1: the GPU path runs on one rank", comm_size);
+    fail(ST_MULTI_RANK);
+    return;
+  }
+  if (Iarray[I_METHOD] != 13) {
+    // src/solver/solver.f90 monolis_solver_select_C: COCG only
+    mh::set_error("monolis_solver_select_C: please select a linear solver for complex numbers (COCG = 13, got %d)",
+                  Iarray[I_METHOD]);
+    fail(ST_METHOD);
+    return;
+  }
+  if (Iarray[I_PRECOND] != 0 && Iarray[I_PRECOND] != 1) {
+    mh::set_error("preconditioner %d (%s) is not implemented on the complex GPU path (NONE = 0, DIAG = 1)",
+                  Iarray[I_PRECOND], str_prec(Iarray[I_PRECOND]));
+    fail(ST_PRECOND);
+    return;
+  }
+  print_banner(Iarray, my_rank);
+  Context* c = nullptr;
+  if (mh::context(&c)) {
+    fail(ST_HIP);
+    return;
+  }
+  const double tp0 = now();
+  monolis_hip_matrix_C* H = nullptr;
+  int rc = monolis_hip_matrix_create_C(&H, n, np, nz, n_dof, index, item, A);
+  if (rc) {
+    fail(rc == -2 ? ST_ARGS : ST_HIP);
+    return;
+  }
+  std::unique_ptr<monolis_hip_matrix_C> guard(H);
+  const size_t vec = (size_t)NPNDOF;
+  mh::zd* dx = reinterpret_cast<mh::zd*>(x);
+  const mh::zd* db = reinterpret_cast<const mh::zd*>(b);
+  mh::DevArray<mh::zd> bx, bb;
+  const int x_dev = mh::is_device_pointer(x), b_dev = mh::is_device_pointer(b);
+  if (!x_dev) {
+    if (bx.alloc(vec) || mh::upload(bx.p, x, sizeof(mh::zd) * vec, c->stream)) {
+      fail(ST_HIP);
+      return;
+    }
+    dx = bx.p;
+  }
+  if (!b_dev) {
+    if (bb.alloc(vec) || mh::upload(bb.p, b, sizeof(mh::zd) * vec, c->stream)) {
+      fail(ST_HIP);
+      return;
+    }
+    db = bb.p;
+  }
+  if (Iarray[I_PRECOND] == 1) {
+    rc = mh::zdiag_setup(*H->M, c->stream);
+    if (rc) {
+      fail(rc == -6 ? ST_ZERO_PIVOT : ST_HIP);
+      return;
+    }
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    mh::set_error("staging failed");
+    fail(ST_HIP);
+    return;
+  }
+  Rarray[R_TIME_PREP] = now() - tp0;
+  std::vector<double> hist;
+  const int st = solve_device_C(*H->M, H->W, db, dx, Iarray, Rarray, &hist);
+  if (st < 0) {
+    fail(st);
+    return;
+  }
+  if (!x_dev) {
+    if (hipMemcpyAsync(x, dx, sizeof(mh::zd) * vec, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      mh::set_error("copy of the solution failed");
+      fail(ST_HIP);
+      return;
+    }
+  }
+  Iarray[I_IERR] = st;
+  Rarray[R_TIME_SOL] = now() - t0;
+  print_after(Iarray, Rarray, hist, my_rank);
+  if (st == ST_NOT_CONVERGED && !Iarray[I_IS_MEASUREMENT] && abort_on_error)
+    error_stop("reached the maximum number of iterations");  // mat_converge.f90:189-194
+}
+
+void monolis_matvec_product_C_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZNDOF2, int* n_dof_list,
+                                     int* n_dof_index, int* n_dof_index2, double _Complex* A, double _Complex* x,
+                                     double _Complex* y, int* index, int* item, int my_rank, int comm,
+                                     int comm_size, int recv_n_neib, int recv_nitem, int* recv_neib_pe,
+                                     int* recv_index, int* recv_item, int send_n_neib, int send_nitem,
+                                     int* send_neib_pe, int* send_index, int* send_item) {
+  (void)NZNDOF2; (void)my_rank; (void)comm; (void)n_dof_list; (void)n_dof_index; (void)n_dof_index2;
+  (void)recv_n_neib; (void)recv_nitem; (void)recv_neib_pe; (void)recv_index; (void)recv_item;
+  (void)send_n_neib; (void)send_nitem; (void)send_neib_pe; (void)send_index; (void)send_item;
+  if (comm_size > 1) error_stop("monolis_matvec_product_C (GPU path): comm_size > 1 is not supported");
+  Context* c = nullptr;
+  if (mh::context(&c)) error_stop(mh::last_error());
+  monolis_hip_matrix_C* H = nullptr;
+  if (monolis_hip_matrix_create_C(&H, n, np, nz, n_dof, index, item, A)) error_stop(mh::last_error());
+  std::unique_ptr<monolis_hip_matrix_C> guard(H);
+  const size_t vec = (size_t)NPNDOF;
+  const size_t m = (size_t)H->M->n * H->M->b;
+  const mh::zd* dx = reinterpret_cast<const mh::zd*>(x);
+  mh::zd* dy = reinterpret_cast<mh::zd*>(y);
+  mh::DevArray<mh::zd> bx, by;
+  const int x_dev = mh::is_device_pointer(x), y_dev = mh::is_device_pointer(y);
+  if (!x_dev) {
+    if (bx.alloc(vec) || mh::upload(bx.p, x, sizeof(mh::zd) * vec, c->stream)) error_stop(mh::last_error());
+    dx = bx.p;
+  }
+  if (!y_dev) {
+    if (by.alloc(vec)) error_stop(mh::last_error());
+    dy = by.p;
+  }
+  if (mh::zspmv(*H->M, dx, dy, c->stream)) error_stop(mh::last_error());
+  if (!y_dev && m > 0) {
+    if (hipMemcpyAsync(y, dy, sizeof(mh::zd) * m, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+      error_stop("copy of y failed");
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) error_stop("monolis_matvec_product_C (GPU path) failed");
+}
+
+void monolis_inner_product_C_c_main(int m, double _Complex* x, double _Complex* y, double _Complex* sum, int comm) {
+  (void)comm;
+  if (monolis_hip_inner_product_C(m, x, y, sum)) error_stop(mh::last_error());
 }
 
 }  // extern "C"

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 
@@ -133,6 +134,48 @@ __device__ __forceinline__ void sum_partials(const double* partials, int ntiles,
   for (int k = 0; k < NV; ++k) total[k] = tot[k];
 }
 
+// ---------------------------------------------------------------------------
+// Complex FP64 value (monolis complex(kdouble): real part, then imaginary).
+// Trivially default-constructible so it can live in LDS; the constructor from
+// a real value is explicit so no real code converts into it by accident.
+// ---------------------------------------------------------------------------
+struct alignas(16) zd {
+  double re, im;
+  zd() = default;
+  __host__ __device__ constexpr explicit zd(double r, double i = 0.0) : re(r), im(i) {}
+};
+__host__ __device__ __forceinline__ zd operator+(zd a, zd b) { return zd(a.re + b.re, a.im + b.im); }
+__host__ __device__ __forceinline__ zd operator-(zd a, zd b) { return zd(a.re - b.re, a.im - b.im); }
+__host__ __device__ __forceinline__ zd operator-(zd a) { return zd(-a.re, -a.im); }
+__host__ __device__ __forceinline__ zd operator*(zd a, zd b) {
+  return zd(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+// Smith's algorithm for a / b (no intermediate overflow for |b| near the range ends).
+__host__ __device__ __forceinline__ zd zdiv(zd a, zd b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, d = b.re + b.im * r;
+    return zd((a.re + a.im * r) / d, (a.im - a.re * r) / d);
+  }
+  const double r = b.re / b.im, d = b.im + b.re * r;
+  return zd((a.re * r + a.im) / d, (a.im * r - a.re) / d);
+}
+
+// on ? a : b.  The complex form selects each part: a select of two structs
+// becomes a select of their addresses, which keeps them in scratch memory.
+template <typename T>
+__device__ __forceinline__ T pick(bool on, T a, T b) {
+  return on ? a : b;
+}
+__device__ __forceinline__ zd pick(bool on, zd a, zd b) { return zd(on ? a.re : b.re, on ? a.im : b.im); }
+
+// t + a * x, the multiply-add of the SpMV row loops.  The complex form keeps
+// the real and imaginary parts as two fused chains.
+__device__ __forceinline__ double madd(double a, double x, double t) { return fma(a, x, t); }
+__device__ __forceinline__ float madd(float a, float x, float t) { return fmaf(a, x, t); }
+__device__ __forceinline__ zd madd(zd a, zd x, zd t) {
+  return zd(fma(a.re, x.re, fma(-a.im, x.im, t.re)), fma(a.re, x.im, fma(a.im, x.re, t.im)));
+}
+
 // Streaming loads: matrix values and column ids are read once per SpMV, so
 // they are fetched non-temporally to keep L2 / MALL for the gathered vector.
 template <typename T>
@@ -143,6 +186,27 @@ template <bool NT, typename T>
 __device__ __forceinline__ T ld_val(const T* p) {
   if constexpr (NT) return __builtin_nontemporal_load(p);
   else return *p;
+}
+using d2v = double __attribute__((ext_vector_type(2)));
+template <bool NT>
+__device__ __forceinline__ zd ld_val(const zd* p) {
+  d2v v;
+  if constexpr (NT) v = __builtin_nontemporal_load((const d2v*)p);
+  else v = *(const d2v*)p;
+  return zd(v.x, v.y);
+}
+
+// Non-temporal store of one element (16-byte complex values as one vector store).
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+  if constexpr (sizeof(T) == 16) {
+    d2v w;
+    w.x = v.re;
+    w.y = v.im;
+    __builtin_nontemporal_store(w, (d2v*)p);
+  } else {
+    __builtin_nontemporal_store(v, p);
+  }
 }
 
 }  // namespace mh

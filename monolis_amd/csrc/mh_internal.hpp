@@ -118,12 +118,22 @@ struct Matrix {
   DevArray<int> tperm, tiperm;
   bool transpose_stale = false;   // values of A changed since the transpose was filled
   VBlock* vb = nullptr;           // non-null for variable block sizes
+  // complex matrices (monolis's *_C path): values in valz instead of val64,
+  // complex block-Jacobi factors in Dz (same structure-of-arrays layout as D64)
+  bool is_complex = false;
+  DevArray<zd> valz;
+  DevArray<zd> Dz;
+  bool diagz_ready = false;
   ~Matrix();
 };
 
 int matrix_create(Matrix** out, int n, int np, int nz, int b, const int* index, const int* item,
                   const double* A, hipStream_t s);
 int matrix_set_values(Matrix* M, const double* A, hipStream_t s);
+// Complex matrix: A holds NZ*b*b complex values, interleaved (re, im) doubles;
+// matrix_set_values refills it from the same layout.
+int matrix_create_complex(Matrix** out, int n, int np, int nz, int b, const int* index, const int* item,
+                          const double* A, hipStream_t s);
 // Variable block sizes (monolis n_dof = -1): arrays as monolis stores them
 // (host or device pointers); see VBlock.
 int matrix_create_vblock(Matrix** out, int n, int np, int nz, const int* n_dof_list, const int* n_dof_index,

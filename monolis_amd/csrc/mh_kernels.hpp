@@ -65,8 +65,8 @@ __device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane,
       for (int r = 0; r < B; ++r) {
         TA t = acc[r];
 #pragma unroll
-        for (int cc = 0; cc < B; ++cc) t = fma((TA)a[u][r * B + cc], xv[u][cc], t);
-        acc[r] = on ? t : acc[r];
+        for (int cc = 0; cc < B; ++cc) t = madd((TA)a[u][r * B + cc], xv[u][cc], t);
+        acc[r] = pick(on, t, acc[r]);
       }
     }
   }
@@ -82,8 +82,8 @@ __device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane,
     for (int r = 0; r < B; ++r) {
       TA t = acc[r];
 #pragma unroll
-      for (int cc = 0; cc < B; ++cc) t = fma((TA)a[r * B + cc], xv[cc], t);
-      acc[r] = on ? t : acc[r];
+      for (int cc = 0; cc < B; ++cc) t = madd((TA)a[r * B + cc], xv[cc], t);
+      acc[r] = pick(on, t, acc[r]);
     }
   }
 }
@@ -135,8 +135,8 @@ __device__ __forceinline__ void sell_row_pipe(const SellView<TV>& M, int s, int 
       for (int r = 0; r < B; ++r) {
         TA t = acc[r];
 #pragma unroll
-        for (int cc = 0; cc < B; ++cc) t = fma((TA)cur.a[u][r * B + cc], xv[u][cc], t);
-        acc[r] = on ? t : acc[r];
+        for (int cc = 0; cc < B; ++cc) t = madd((TA)cur.a[u][r * B + cc], xv[u][cc], t);
+        acc[r] = pick(on, t, acc[r]);
       }
     }
   };
@@ -167,8 +167,8 @@ __device__ __forceinline__ void sell_row_pipe(const SellView<TV>& M, int s, int 
     for (int r = 0; r < B; ++r) {
       TA t = acc[r];
 #pragma unroll
-      for (int cc = 0; cc < B; ++cc) t = fma((TA)a[r * B + cc], xv[cc], t);
-      acc[r] = on ? t : acc[r];
+      for (int cc = 0; cc < B; ++cc) t = madd((TA)a[r * B + cc], xv[cc], t);
+      acc[r] = pick(on, t, acc[r]);
     }
   }
 }
@@ -227,7 +227,8 @@ struct WaveRows {
   }
   __device__ __forceinline__ void load(const T* __restrict__ src, T (&v)[B]) const {
     if constexpr (B == 1) {
-      v[0] = (row0 + lane < nrows) ? src[(size_t)row0 + lane] : (T)0;
+      v[0] = (T)0;
+      if (row0 + lane < nrows) v[0] = src[(size_t)row0 + lane];
     } else {
       const int cnt = count();
       const T* base = src + (size_t)row0 * B;
@@ -247,6 +248,7 @@ struct WaveRows {
   // L2 lines for the end-of-kernel write-back.
   __device__ __forceinline__ void put(T* __restrict__ dst, size_t i, T v) const {
 #if defined(MH_STORE_CPOL) && MH_STORE_CPOL >= 0
+    static_assert(sizeof(T) <= 8, "cache-policy store covers 4- and 8-byte values");
     // buffer store with explicit cache-policy bits (bit 0 sc0, bit 1 nt, bit 4 sc1)
     using U2 = unsigned int __attribute__((ext_vector_type(2)));
     const __amdgpu_buffer_rsrc_t rs =
@@ -257,7 +259,7 @@ struct WaveRows {
       __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), rs, (int)(i * sizeof(T)), 0,
                                             MH_STORE_CPOL);
 #else
-    __builtin_nontemporal_store(v, dst + i);
+    st_stream(dst + i, v);
 #endif
   }
   __device__ __forceinline__ void store(T* __restrict__ dst, const T (&v)[B]) const {

@@ -940,18 +940,6 @@ __global__ void k_state_init(KState* st, double tol, int max_iter) {
   }
 }
 
-// ---- workspace carving --------------------------------------------------------
-struct Carve {
-  unsigned char* base;
-  size_t off = 0;
-  template <typename U>
-  U* take(size_t count) {
-    off = (off + 255) / 256 * 256;
-    U* p = (U*)(base + off);
-    off += count * sizeof(U);
-    return p;
-  }
-};
 
 template <typename T>
 size_t cg_bytes(const Matrix& M, int nvec, int max_iter) {
@@ -961,37 +949,6 @@ size_t cg_bytes(const Matrix& M, int nvec, int max_iter) {
          ((size_t)max_iter + 2) * sizeof(double) + 4096;
 }
 
-// Polls `done` one chunk behind the enqueue front.
-struct Poller {
-  Context* c;
-  const KState* st;
-  std::deque<int> pending;
-  int next = 0;
-  int issue(hipStream_t s) {
-    const int slot = next;
-    next = (next + 1) % kFlagSlots;
-    c->h_flags[slot] = 0;
-    MH_HIP(hipMemcpyAsync(&c->h_flags[slot], &st->done, sizeof(int), hipMemcpyDeviceToHost, s));
-    MH_HIP(hipEventRecord(c->ev[slot], s));
-    pending.push_back(slot);
-    return 0;
-  }
-  // returns 1 when the oldest outstanding poll reports done
-  int check(size_t keep, int* done) {
-    *done = 0;
-    while (pending.size() > keep) {
-      const int slot = pending.front();
-      pending.pop_front();
-      MH_HIP(hipEventSynchronize(c->ev[slot]));
-      if (c->h_flags[slot]) {
-        *done = 1;
-        pending.clear();
-        return 0;
-      }
-    }
-    return 0;
-  }
-};
 
 template <typename T, int B, bool PREC>
 int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, SolveResult* out, hipStream_t s) {
@@ -1059,7 +1016,7 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
     return launch_rows(n, op, parts, s);
   };
   MH_TRY(upd(0, F_FIRST));
-  Poller poll{c, st};
+  Poller poll{c, &st->done};
   int done = 0;
   for (int it = 1; it <= P.max_iter && !done;) {
     const int end = std::min(it + P.chunk - 1, P.max_iter);
@@ -1177,7 +1134,7 @@ int gropcg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W,
     MH_TRY(launch_rows(n, op, parts, s));
   }
   MH_TRY(spmv_step());  // beta = 0 from the state init, p = s = 0
-  Poller poll{c, st};
+  Poller poll{c, &st->done};
   int done = 0;
   for (int it = 1; it <= P.max_iter && !done;) {
     const int end = std::min(it + P.chunk - 1, P.max_iter);
@@ -1284,7 +1241,7 @@ int pipecg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W,
     OpSpmvSolve<T, B> sp{u, v, st};
     MH_TRY((launch_sell<T, B>(M, sp, parts, var, s)));
   }
-  Poller poll{c, st};
+  Poller poll{c, &st->done};
   int done = 0;
   for (int it = 1; it <= P.max_iter && !done;) {
     const int end = std::min(it + P.chunk - 1, P.max_iter);
@@ -1417,7 +1374,7 @@ int pipebicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspa
   MH_TRY(spmv(rt, w0));     // W0 = A RT
   MH_TRY(rows_init(1));     // WT = M^-1 W0, RR, RW, alpha
   MH_TRY(spmv(wt, t));      // T = A WT
-  Poller poll{c, st};
+  Poller poll{c, &st->done};
   int done = 0;
   for (int it = 1; it <= P.max_iter && !done;) {
     const int end = std::min(it + P.chunk - 1, P.max_iter);
@@ -1538,7 +1495,7 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   KState hs;
   out->fallbacks = 0;
   for (;;) {
-    Poller poll{c, st};
+    Poller poll{c, &st->done};
     int done = 0;
     while (it <= P.max_iter && !done) {
       const int end = std::min(it + P.chunk - 1, P.max_iter);

@@ -2,6 +2,8 @@
 // row-parallel kernels, each followed by its reduction finalize kernel.
 #pragma once
 
+#include <deque>
+
 #include "mh_internal.hpp"
 #include "mh_kernels.hpp"
 
@@ -65,6 +67,52 @@ int launch_rows(int nrows, const Op& op, double* partials, hipStream_t s) {
   }
   return 0;
 }
+
+// Sub-allocation of a solver workspace, 256-byte aligned pieces.
+struct Carve {
+  unsigned char* base;
+  size_t off = 0;
+  template <typename U>
+  U* take(size_t count) {
+    off = (off + 255) / 256 * 256;
+    U* p = (U*)(base + off);
+    off += count * sizeof(U);
+    return p;
+  }
+};
+
+// Polls a device `done` flag one chunk behind the enqueue front: each issue()
+// copies the flag into a pinned host slot and records an event after it.
+struct Poller {
+  Context* c;
+  const int* done_flag;
+  std::deque<int> pending;
+  int next = 0;
+  int issue(hipStream_t s) {
+    const int slot = next;
+    next = (next + 1) % kFlagSlots;
+    c->h_flags[slot] = 0;
+    MH_HIP(hipMemcpyAsync(&c->h_flags[slot], done_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    MH_HIP(hipEventRecord(c->ev[slot], s));
+    pending.push_back(slot);
+    return 0;
+  }
+  // waits for the polls beyond the newest `keep`; *done = 1 when one reports done
+  int check(size_t keep, int* done) {
+    *done = 0;
+    while (pending.size() > keep) {
+      const int slot = pending.front();
+      pending.pop_front();
+      MH_HIP(hipEventSynchronize(c->ev[slot]));
+      if (c->h_flags[slot]) {
+        *done = 1;
+        pending.clear();
+        return 0;
+      }
+    }
+    return 0;
+  }
+};
 
 #define MH_DISPATCH_B(b, ...)                                            \
   switch (b) {                                                           \

@@ -71,6 +71,7 @@ template struct DevArray<float>;
 template struct DevArray<unsigned char>;
 template struct DevArray<unsigned long long>;
 template struct DevArray<long long>;
+template struct DevArray<zd>;
 
 int is_device_pointer(const void* p) {
   if (!p) return 0;
@@ -136,9 +137,9 @@ __global__ void k_validate(int np, int nz, const int* __restrict__ rowptr, const
 }
 
 // One thread per (slice, lane); writes every slot of that lane, padding included.
-template <typename TV>
+template <typename TS, typename TV>
 __global__ void k_sell_fill(int n, int nslices, int b, const int* __restrict__ rowptr, const int* __restrict__ item,
-                            const double* __restrict__ A, const int* __restrict__ slice_ptr,
+                            const TS* __restrict__ A, const int* __restrict__ slice_ptr,
                             int* __restrict__ col, TV* __restrict__ val, int fill_col) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nslices * kWave) return;
@@ -190,7 +191,9 @@ SellView<TV> view(const Matrix& M) {
   v.slice_ptr = M.slice_ptr.p;
   v.rowptr = M.rowptr.p;
   v.col = M.col.p;
-  if constexpr (sizeof(TV) == 8)
+  if constexpr (sizeof(TV) == 16)
+    v.val = (const TV*)M.valz.p;
+  else if constexpr (sizeof(TV) == 8)
     v.val = (const TV*)M.val64.p;
   else
     v.val = (const TV*)M.val32.p;
@@ -198,22 +201,29 @@ SellView<TV> view(const Matrix& M) {
 }
 template SellView<double> view<double>(const Matrix&);
 template SellView<float> view<float>(const Matrix&);
+template SellView<zd> view<zd>(const Matrix&);
 
 static int fill_values(Matrix* M, const double* A, hipStream_t s, int fill_col) {
-  const size_t nvals = (size_t)M->nz * M->b * M->b;
+  // complex values are two doubles each
+  const size_t ndbl = (size_t)M->nz * M->b * M->b * (M->is_complex ? 2 : 1);
   const double* dA = A;
   DevArray<double> tmp;
   if (!is_device_pointer(A)) {
-    MH_TRY(tmp.alloc(nvals));
-    MH_TRY(upload(tmp.p, A, nvals * sizeof(double), s));
+    MH_TRY(tmp.alloc(ndbl));
+    MH_TRY(upload(tmp.p, A, ndbl * sizeof(double), s));
     dA = tmp.p;
   }
   const long long threads = (long long)M->nslices * kWave;
-  k_sell_fill<double><<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-      M->n, M->nslices, M->b, M->rowptr.p, M->csr_item.p, dA, M->slice_ptr.p, M->col.p, M->val64.p, fill_col);
+  const unsigned grid = (unsigned)((threads + kBlock - 1) / kBlock);
+  if (M->is_complex)
+    k_sell_fill<zd, zd><<<grid, kBlock, 0, s>>>(M->n, M->nslices, M->b, M->rowptr.p, M->csr_item.p,
+                                               (const zd*)dA, M->slice_ptr.p, M->col.p, M->valz.p, fill_col);
+  else
+    k_sell_fill<double, double><<<grid, kBlock, 0, s>>>(M->n, M->nslices, M->b, M->rowptr.p, M->csr_item.p, dA,
+                                                       M->slice_ptr.p, M->col.p, M->val64.p, fill_col);
   MH_HIP(hipGetLastError());
   if (tmp.p) MH_HIP(hipStreamSynchronize(s));  // tmp is released on return
-  M->diag_ready = M->diag32_ready = false;
+  M->diag_ready = M->diag32_ready = M->diagz_ready = false;
   M->transpose_stale = true;
   if (M->val32_ready) {
     M->val32_ready = false;
@@ -222,8 +232,8 @@ static int fill_values(Matrix* M, const double* A, hipStream_t s, int fill_col) 
   return 0;
 }
 
-int matrix_create(Matrix** out, int n, int np, int nz, int b, const int* index, const int* item,
-                  const double* A, hipStream_t s) {
+static int matrix_build(Matrix** out, int n, int np, int nz, int b, const int* index, const int* item,
+                        const double* A, bool cplx, hipStream_t s) {
   *out = nullptr;
   if (n < 0 || np < n || nz < 0 || b < 1) {
     set_error("invalid matrix sizes n=%d np=%d nz=%d ndof=%d", n, np, nz, b);
@@ -238,6 +248,7 @@ int matrix_create(Matrix** out, int n, int np, int nz, int b, const int* index, 
   M->np = np;
   M->nz = nz;
   M->b = b;
+  M->is_complex = cplx;
   M->nslices = (n + kWave - 1) / kWave;
   int rc = 0;
   do {
@@ -306,7 +317,11 @@ int matrix_create(Matrix** out, int n, int np, int nz, int b, const int* index, 
     if ((rc = M->slice_ptr.alloc((size_t)M->nslices + 1))) break;
     if ((rc = upload(M->slice_ptr.p, hp.data(), sizeof(int) * ((size_t)M->nslices + 1), s))) break;
     if ((rc = M->col.alloc(acc * kWave))) break;
-    if ((rc = M->val64.alloc(acc * kWave * (size_t)b * b))) break;
+    if (cplx) {
+      if ((rc = M->valz.alloc(acc * kWave * (size_t)b * b))) break;
+    } else {
+      if ((rc = M->val64.alloc(acc * kWave * (size_t)b * b))) break;
+    }
     if ((rc = fill_values(M, A, s, 1))) break;
     if (hipStreamSynchronize(s) != hipSuccess) {
       set_error("matrix build failed");
@@ -322,9 +337,23 @@ int matrix_create(Matrix** out, int n, int np, int nz, int b, const int* index, 
   return 0;
 }
 
+int matrix_create(Matrix** out, int n, int np, int nz, int b, const int* index, const int* item,
+                  const double* A, hipStream_t s) {
+  return matrix_build(out, n, np, nz, b, index, item, A, false, s);
+}
+
+int matrix_create_complex(Matrix** out, int n, int np, int nz, int b, const int* index, const int* item,
+                          const double* A, hipStream_t s) {
+  return matrix_build(out, n, np, nz, b, index, item, A, true, s);
+}
+
 int matrix_set_values(Matrix* M, const double* A, hipStream_t s) { return fill_values(M, A, s, 0); }
 
 int matrix_make_fp32(Matrix* M, hipStream_t s) {
+  if (M->is_complex) {
+    set_error("complex matrices have no FP32 copy");
+    return -2;
+  }
   if (M->val32_ready) return 0;
   if (M->val32.n < M->val64.n) MH_TRY(M->val32.alloc(M->val64.n));
   k_to_fp32<<<grid_1d(M->val64.n), kBlock, 0, s>>>(M->val64.n, M->val64.p, M->val32.p);

@@ -15,6 +15,7 @@ module mod_monolis_hip
 
   integer(c_int), parameter :: monolis_hip_iter_CG = 1
   integer(c_int), parameter :: monolis_hip_iter_BiCGSTAB = 5
+  integer(c_int), parameter :: monolis_hip_iter_COCG = 13
   integer(c_int), parameter :: monolis_hip_prec_NONE = 0
   integer(c_int), parameter :: monolis_hip_prec_DIAG = 1
 
@@ -65,6 +66,55 @@ module mod_monolis_hip
       real(c_double) :: sum
       integer(c_int), value :: comm
     end subroutine monolis_inner_product_R_c_main
+
+    ! complex path (wrapper/solver/solver_wrapper.f90:94-175, wrapper/linalg/matvec_wrap.f90:112-207,
+    ! wrapper/linalg/inner_product_wrap.f90:59-77)
+    subroutine monolis_solve_C_c_main(n, np, nz, n_dof, NPNDOF, NZNDOF2, &
+        n_dof_list, n_dof_index, n_dof_index2, A, x, b, index, item, &
+        my_rank, comm, comm_size, &
+        recv_n_neib, recv_nitem, recv_neib_pe, recv_index, recv_item, &
+        send_n_neib, send_nitem, send_neib_pe, send_index, send_item, &
+        Iarray, Rarray) bind(c, name="monolis_solve_C_c_main")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: n, np, nz, n_dof, NPNDOF, NZNDOF2
+      integer(c_int) :: n_dof_list(*), n_dof_index(*), n_dof_index2(*)
+      complex(c_double_complex) :: A(*), x(*), b(*)
+      integer(c_int) :: index(*), item(*)
+      integer(c_int), value :: my_rank, comm, comm_size
+      integer(c_int), value :: recv_n_neib, recv_nitem
+      integer(c_int) :: recv_neib_pe(*), recv_index(*), recv_item(*)
+      integer(c_int), value :: send_n_neib, send_nitem
+      integer(c_int) :: send_neib_pe(*), send_index(*), send_item(*)
+      integer(c_int) :: Iarray(0:99)
+      real(c_double) :: Rarray(0:99)
+    end subroutine monolis_solve_C_c_main
+
+    subroutine monolis_matvec_product_C_c_main(n, np, nz, n_dof, NPNDOF, NZNDOF2, &
+        n_dof_list, n_dof_index, n_dof_index2, A, x, y, index, item, &
+        my_rank, comm, comm_size, &
+        recv_n_neib, recv_nitem, recv_neib_pe, recv_index, recv_item, &
+        send_n_neib, send_nitem, send_neib_pe, send_index, send_item) &
+        bind(c, name="monolis_matvec_product_C_c_main")
+      import :: c_int, c_double_complex
+      integer(c_int), value :: n, np, nz, n_dof, NPNDOF, NZNDOF2
+      integer(c_int) :: n_dof_list(*), n_dof_index(*), n_dof_index2(*)
+      complex(c_double_complex) :: A(*), x(*), y(*)
+      integer(c_int) :: index(*), item(*)
+      integer(c_int), value :: my_rank, comm, comm_size
+      integer(c_int), value :: recv_n_neib, recv_nitem
+      integer(c_int) :: recv_neib_pe(*), recv_index(*), recv_item(*)
+      integer(c_int), value :: send_n_neib, send_nitem
+      integer(c_int) :: send_neib_pe(*), send_index(*), send_item(*)
+    end subroutine monolis_matvec_product_C_c_main
+
+    subroutine monolis_inner_product_C_c_main(m, x, y, sum, comm) &
+        bind(c, name="monolis_inner_product_C_c_main")
+      import :: c_int, c_double_complex
+      integer(c_int), value :: m
+      complex(c_double_complex) :: x(*), y(*)
+      complex(c_double_complex) :: sum
+      integer(c_int), value :: comm
+    end subroutine monolis_inner_product_C_c_main
   end interface
 
 contains
@@ -138,4 +188,53 @@ contains
     real(c_double), intent(out) :: sum
     call monolis_inner_product_R_c_main(m, X, Y, sum, 0)
   end subroutine monolis_hip_inner_product_R
+  !> One-rank complex solve (COCG, Iarray(1) = 13): X (in/out, N*ndof), B (in).
+  subroutine monolis_hip_solve_C(N, ndof, index, item, A, B, X, Iarray, Rarray)
+    integer(c_int), intent(in) :: N, ndof
+    integer(c_int), intent(in) :: index(N + 1), item(*)
+    complex(c_double_complex), intent(in) :: A(*), B(*)
+    complex(c_double_complex), intent(inout) :: X(*)
+    integer(c_int), intent(inout) :: Iarray(100)
+    real(c_double), intent(inout) :: Rarray(100)
+    integer(c_int) :: empty(1), idx(N + 1), itm(max(1, index(N + 1)))
+    integer(c_int) :: Ic(0:99)
+    real(c_double) :: Rc(0:99)
+    integer(c_int) :: nz
+    nz = index(N + 1)
+    empty = 0
+    idx = index
+    itm(1:nz) = item(1:nz)
+    Ic(0) = 0
+    Ic(1:99) = Iarray(1:99)
+    Rc(0) = 0.0d0
+    Rc(1:99) = Rarray(1:99)
+    call monolis_solve_C_c_main(N, N, nz, ndof, N*ndof, nz*ndof*ndof, empty, empty, empty, &
+      A, X, B, idx, itm, 0, 0, 1, 0, 0, empty, empty, empty, 0, 0, empty, empty, empty, Ic, Rc)
+    Iarray(1:99) = Ic(1:99)
+    Rarray(1:99) = Rc(1:99)
+  end subroutine monolis_hip_solve_C
+
+  !> One-rank complex SpMV: Y(1:N*ndof) = A X.
+  subroutine monolis_hip_matvec_C(N, ndof, index, item, A, X, Y)
+    integer(c_int), intent(in) :: N, ndof
+    integer(c_int), intent(in) :: index(N + 1), item(*)
+    complex(c_double_complex), intent(in) :: A(*), X(*)
+    complex(c_double_complex), intent(out) :: Y(*)
+    integer(c_int) :: empty(1), idx(N + 1), itm(max(1, index(N + 1)))
+    integer(c_int) :: nz
+    nz = index(N + 1)
+    empty = 0
+    idx = index
+    itm(1:nz) = item(1:nz)
+    call monolis_matvec_product_C_c_main(N, N, nz, ndof, N*ndof, nz*ndof*ndof, empty, empty, empty, &
+      A, X, Y, idx, itm, 0, 0, 1, 0, 0, empty, empty, empty, 0, 0, empty, empty, empty)
+  end subroutine monolis_hip_matvec_C
+
+  !> sum = sum_i X(i) Y(i) over m entries, unconjugated.
+  subroutine monolis_hip_inner_product_C(m, X, Y, sum)
+    integer(c_int), intent(in) :: m
+    complex(c_double_complex), intent(in) :: X(*), Y(*)
+    complex(c_double_complex), intent(out) :: sum
+    call monolis_inner_product_C_c_main(m, X, Y, sum, 0)
+  end subroutine monolis_hip_inner_product_C
 end module mod_monolis_hip

@@ -33,7 +33,7 @@ class Matrix:
     ndof: int                   # block size b, or -1 for variable blocks
     index: np.ndarray           # int32 [NP+1]
     item: np.ndarray            # int32 [NZ], 1-based
-    A: np.ndarray               # float64
+    A: np.ndarray               # float64 (complex128 for the complex path)
     n_dof_list: Optional[np.ndarray] = None
     n_dof_index: Optional[np.ndarray] = None
     n_dof_index2: Optional[np.ndarray] = None
@@ -111,10 +111,11 @@ class Matrix:
         return self.A.astype(dtype)
 
 
-def nonzero_pattern_by_simple_mesh(n_node: int, ndof: int, elems) -> Matrix:
+def nonzero_pattern_by_simple_mesh(n_node: int, ndof: int, elems, dtype=np.float64) -> Matrix:
     """monolis_get_nonzero_pattern_by_simple_mesh_R for elements given as
     0-based node lists (the C API's convention; node ids become 1-based
-    columns in ``item``)."""
+    columns in ``item``).  ``dtype=np.complex128`` gives the _C variant
+    (src/matrix/spmat_nzpattern.f90:53-87)."""
     nbr = [set() for _ in range(n_node)]
     for e in elems:
         e = [int(v) for v in e]
@@ -126,7 +127,7 @@ def nonzero_pattern_by_simple_mesh(n_node: int, ndof: int, elems) -> Matrix:
     index = np.zeros(n_node + 1, np.int32)
     index[1:] = np.cumsum([len(r) for r in rows])
     item = np.array([c + 1 for r in rows for c in r], np.int32)
-    A = np.zeros(int(index[-1]) * ndof * ndof, np.float64)
+    A = np.zeros(int(index[-1]) * ndof * ndof, dtype)
     return Matrix(n_node, n_node, ndof, index, item, A).ensure_dof_index()
 
 

@@ -24,6 +24,11 @@ Workloads (sizes as used by the tests and bench.py):
                        neighbours -1, diagonal 6 (nonsymmetric).
 * ``scale_symmetric``  D A D with D log-uniform over [1, dmax] per DOF.
 * ``poisson_batch``    many independent 7-point systems with random edges.
+* ``helmholtz7``       complex symmetric 7-point damped Helmholtz operator
+                       L - s (1 + i eta) I (L the 7-point Laplacian), CSR.
+* ``stencil27_bcsr3_complex``  K + i g D on the stencil27_bcsr3 pattern: K the
+                       real SPD system, D its diagonal blocks (a damped
+                       dynamic stiffness; complex symmetric).
 """
 from __future__ import annotations
 
@@ -175,3 +180,30 @@ def poisson_batch(count: int = 512, lo: int = 10, hi: int = 40, seed: int = 0):
     m = Matrix(row_off, row_off, 1, index, item, A)
     m.meta = {"workload": f"poisson_batch_{count}x[{lo}..{hi}]^3", "edges": edges, "seed": seed}
     return m.ensure_dof_index(), row_ptr
+
+
+def helmholtz7(n: int, shift: float = 0.5, damping: float = 0.2) -> Matrix:
+    """Complex symmetric damped Helmholtz on n^3 nodes: 7-point Laplacian with
+    diagonal 6 - shift (1 + i damping), off-diagonals -1 (complex128 values)."""
+    m = poisson7(n)
+    diag = m.item == np.repeat(np.arange(1, n ** 3 + 1, dtype=np.int32), np.diff(m.index))
+    A = m.A.astype(np.complex128)
+    A[diag] -= shift * (1.0 + 1j * damping)
+    out = Matrix(m.n, m.np_, 1, m.index, m.item, A)
+    out.meta = {"workload": f"helmholtz7_{n}^3_s{shift:g}_eta{damping:g}", "grid": n}
+    return out.ensure_dof_index()
+
+
+def stencil27_bcsr3_complex(n: int, seed: int = 0, damping: float = 0.5) -> Matrix:
+    """K + i damping D with K = stencil27_bcsr3(n, seed) and D its diagonal blocks."""
+    m = stencil27_bcsr3(n, seed)
+    N = n ** 3
+    A = m.A.astype(np.complex128)
+    rows = np.repeat(np.arange(1, N + 1, dtype=np.int32), np.diff(m.index))
+    dpos = np.nonzero(m.item == rows)[0]
+    del rows
+    blk = A.reshape(-1, 9)
+    blk[dpos] += 1j * damping * m.A.reshape(-1, 9)[dpos]
+    out = Matrix(N, N, 3, m.index, m.item, A)
+    out.meta = {"workload": f"stencil27_bcsr3_complex_{n}^3", "grid": n, "seed": seed, "damping": damping}
+    return out.ensure_dof_index()

@@ -27,6 +27,7 @@ _lib = None
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_c128p = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
 
 # Iarray / Rarray slots (C index k == Fortran index k), src/define/def_solver_prm.f90:90-163
 I_METHOD, I_PRECOND, I_MAX_ITER, I_CUR_ITER, I_IERR = 1, 2, 3, 4, 5
@@ -34,6 +35,8 @@ I_IS_INIT_X, I_IS_ERROR_ABORT = 10, 16
 R_TOL, R_CUR_RESID = 1, 2
 ITER_CG, ITER_GROPCG, ITER_PIPECG, ITER_BICGSTAB, ITER_PIPEBICGSTAB, ITER_BICGSTAB_NOPREC = 1, 2, 3, 5, 6, 7
 ITER_PIPEBICGSTAB_NOPREC = 8
+ITER_COCG = 13
+ZERO_DIAG = -6   # status of orc_solve_C when a diagonal block has a zero pivot
 PREC_NONE, PREC_DIAG = 0, 1
 
 
@@ -73,6 +76,15 @@ def lib():
         L.orc_solve.restype = c_int
         L.orc_time_matvec.argtypes = [c_int, c_int, c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
         L.orc_time_matvec.restype = c_double
+        L.orc_matvec_C.argtypes = [c_int, c_int, _i32p, _i32p, _c128p, _c128p, _c128p]
+        L.orc_inner_product_C.argtypes = [c_int, _c128p, _c128p, _c128p]
+        L.orc_check_converge_C.argtypes = [c_int, _c128p, _c128p, c_int, _i32p, _f64p]
+        L.orc_check_converge_C.restype = c_int
+        L.orc_diag_nn_setup_C.argtypes = [c_int, c_int, _i32p, _i32p, _c128p, _c128p]
+        L.orc_diag_nn_setup_C.restype = c_int
+        L.orc_diag_nn_apply_C.argtypes = [c_int, c_int, _c128p, _c128p, _c128p]
+        L.orc_solve_C.argtypes = [c_int, c_int, _i32p, _i32p, _c128p, _c128p, _c128p, _i32p, _f64p]
+        L.orc_solve_C.restype = c_int
         _lib = L
     return _lib
 
@@ -237,3 +249,62 @@ def adjoint(mat, x, g, symmetric=False, **kw):
     At = mat if symmetric else transpose(mat)
     lam, st, I, R = solve(At, g, **kw)
     return lam, outer_grad(mat, lam, x), st, I
+
+
+# ---------------------------------------------------------------------------
+# Complex path (monolis's *_C routines and COCG), monolis_oracle_cplx.c.
+# Values are complex128 (Fortran complex(kdouble) layout).
+# ---------------------------------------------------------------------------
+def _z(a):
+    return np.ascontiguousarray(a, np.complex128)
+
+
+def matvec_C(mat, x):
+    """y = A x in complex arithmetic (src/linalg/matvec.f90:192-227, fixed block sizes)."""
+    y = np.zeros(max(mat.n * mat.ndof, 1), np.complex128)
+    lib().orc_matvec_C(mat.n, mat.ndof, mat.index, mat.item, _z(mat.A), _z(x), y)
+    return y[: mat.n * mat.ndof]
+
+
+def inner_product_C(x, y):
+    """Unconjugated sum_i x_i y_i (src/linalg/inner_product.f90:457-499)."""
+    x, y = _z(x), _z(y)
+    out = np.zeros(1, np.complex128)
+    lib().orc_inner_product_C(len(x), x, y, out)
+    return complex(out[0])
+
+
+def check_converge_C(r, B2, it, tol):
+    Iarray = np.zeros(100, np.int32)
+    Rarray = np.zeros(100, np.float64)
+    Rarray[R_TOL] = tol
+    r = _z(r)
+    b2 = np.array([B2], np.complex128)
+    conv = lib().orc_check_converge_C(len(r), r, b2, int(it), Iarray, Rarray)
+    return bool(conv), int(Iarray[I_CUR_ITER]), float(Rarray[R_CUR_RESID])
+
+
+def diag_setup_C(mat):
+    """Complex block-Jacobi factors (diag_nn.f90:90-159).  Returns (D, status);
+    status ZERO_DIAG where the reference stops on a zero pivot."""
+    D = np.zeros(max(mat.ndof * mat.ndof * mat.n, 1), np.complex128)
+    st = lib().orc_diag_nn_setup_C(mat.n, mat.ndof, mat.index, mat.item, _z(mat.A), D)
+    return D, int(st)
+
+
+def diag_apply_C(mat, D, x):
+    x = _z(x)
+    y = np.zeros_like(x)
+    lib().orc_diag_nn_apply_C(mat.n, mat.ndof, _z(D), x, y)
+    return y
+
+
+def solve_C(mat, b, x0=None, Iarray=None, Rarray=None, **kw):
+    """monolis_solve_main_C (COCG) on one rank.  Returns (x, status, Iarray, Rarray)."""
+    if Iarray is None:
+        kw.setdefault("method", ITER_COCG)
+        Iarray, Rarray = default_arrays(**kw)
+    m = mat.n * mat.ndof
+    x = np.zeros(max(m, 1), np.complex128) if x0 is None else np.array(x0, np.complex128, copy=True)
+    st = lib().orc_solve_C(mat.n, mat.ndof, mat.index, mat.item, _z(mat.A), x, _z(b), Iarray, Rarray)
+    return x[:m], int(st), Iarray, Rarray

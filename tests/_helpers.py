@@ -114,3 +114,88 @@ def random_vblock(n, density, seed, maxb=4, spd=True, sizes=None):
                 a0 = int(m.n_dof_index2[e])
                 m.A[a0:a0 + n1 * n2] = D[r0:r0 + n1, c0:c0 + n2].reshape(-1)
     return m
+
+
+def chain_matrix_C(n_node, ndof, seed):
+    """Shape of src_test/iterative/COCG_test.f90:41-70: 10-node chain, diagonal
+    (U[0,1) + i U[0,1)) + (2 + 2i) on the block diagonals, complex symmetric
+    (A_ij = A_ji, not conjugated) U[0,1) + i U[0,1) off-diagonal blocks."""
+    rng = np.random.default_rng(seed)
+    elems = [[i, i + 1] for i in range(n_node - 1)]
+    m = mm.nonzero_pattern_by_simple_mesh(n_node, ndof, elems, dtype=np.complex128)
+    for i in range(n_node):
+        for d in range(ndof):
+            mm.add_scalar(m, i, i, d, d, complex(rng.random(), rng.random()) + (2.0 + 2.0j))
+    for a, b in elems:
+        for i2 in range(ndof):
+            for j2 in range(ndof):
+                v = complex(rng.random(), rng.random())
+                mm.add_scalar(m, a, b, i2, j2, v)
+                mm.add_scalar(m, b, a, j2, i2, v)
+    return m
+
+
+def chain_matrix_C_wrapper(seed, n_node=10, ndof=2):
+    """Shape of wrapper_test/solver/monolis_solver_c_test.c:83-141: diagonal
+    v + 0.1 v i with v in 5000..6000, off-diagonal blocks v + 0.1 v i with
+    v in 0..1000 set at (e0, e1, j, k) and (e1, e0, j, k) (the same block on
+    both sides, so not symmetric for n_dof > 1)."""
+    rng = np.random.default_rng(seed)
+    elems = [[i, i + 1] for i in range(n_node - 1)]
+    m = mm.nonzero_pattern_by_simple_mesh(n_node, ndof, elems, dtype=np.complex128)
+    for i in range(n_node):
+        for j in range(ndof):
+            v = float(rng.integers(0, 1001)) + 5000.0
+            mm.set_scalar(m, i, i, j, j, v + 0.1 * v * 1j)
+    for a, b in elems:
+        for j in range(ndof):
+            for k in range(ndof):
+                v = float(rng.integers(0, 1001))
+                mm.set_scalar(m, a, b, j, k, v + 0.1 * v * 1j)
+                mm.set_scalar(m, b, a, j, k, v + 0.1 * v * 1j)
+    return m
+
+
+def random_bcsr_C(n, ndof, density, seed, symmetric=True, diag_missing=()):
+    """Random complex block matrix; with `symmetric` it is complex symmetric
+    (A = A^T, no conjugation), diagonally dominant with a positive imaginary
+    shift on the diagonal -- the damped-Helmholtz class COCG is meant for."""
+    mr = random_bcsr(n, ndof, density, seed, spd=False, diag_missing=diag_missing)
+    rng = np.random.default_rng(seed + 1000)
+    b2 = ndof * ndof
+    A = mr.A + 1j * rng.uniform(-1, 1, size=mr.A.size)
+    m = mm.Matrix(n, n, ndof, mr.index.copy(), mr.item.copy(), A).ensure_dof_index()
+    if symmetric:
+        sets = [set((m.item[m.index[i]:m.index[i + 1]] - 1).tolist()) for i in range(n)]
+        for i in range(n):
+            for c in list(sets[i]):
+                sets[c].add(i)
+        rows = [sorted(s) for s in sets]
+        index = np.zeros(n + 1, np.int32)
+        index[1:] = np.cumsum([len(r) for r in rows])
+        item = np.array([c + 1 for r in rows for c in r], np.int32)
+        D = np.zeros((n * ndof, n * ndof), np.complex128)
+        for i in range(n):
+            for k in range(m.index[i], m.index[i + 1]):
+                c = m.item[k] - 1
+                D[i * ndof:(i + 1) * ndof, c * ndof:(c + 1) * ndof] = m.A[k * b2:(k + 1) * b2].reshape(ndof, ndof)
+        D = 0.5 * (D + D.T)
+        D += np.eye(len(D)) * (np.abs(D).sum(1).max() + 1.0) * (1.0 + 0.5j)
+        A = np.zeros(len(item) * b2, np.complex128)
+        for i in range(n):
+            for k in range(index[i], index[i + 1]):
+                c = item[k] - 1
+                A[k * b2:(k + 1) * b2] = D[i * ndof:(i + 1) * ndof, c * ndof:(c + 1) * ndof].reshape(-1)
+        m = mm.Matrix(n, n, ndof, index, item, A).ensure_dof_index()
+    return m
+
+
+def dense_C(m):
+    """Dense complex copy of a fixed-block matrix (tiny sizes only)."""
+    b = m.ndof
+    D = np.zeros((m.n * b, m.np_ * b), np.complex128)
+    for i in range(m.n):
+        for k in range(m.index[i], m.index[i + 1]):
+            c = m.item[k] - 1
+            D[i * b:(i + 1) * b, c * b:(c + 1) * b] = m.A[k * b * b:(k + 1) * b * b].reshape(b, b)
+    return D

@@ -27,7 +27,8 @@ def declared_functions():
 
 def test_headers_declare_reference_entry_points():
     names = declared_functions()
-    for ref in ["monolis_solve_R_c_main", "monolis_matvec_product_R_c_main", "monolis_inner_product_R_c_main"]:
+    for ref in ["monolis_solve_R_c_main", "monolis_matvec_product_R_c_main", "monolis_inner_product_R_c_main",
+                "monolis_solve_C_c_main", "monolis_matvec_product_C_c_main", "monolis_inner_product_C_c_main"]:
         assert ref in names
 
 
