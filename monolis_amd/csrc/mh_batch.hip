@@ -78,6 +78,7 @@ __device__ __forceinline__ double sell_row_scalar(const BatchView& V, int s, int
   M.rowptr = nullptr;
   M.col = V.col;
   M.val = V.val;
+  M.col16 = nullptr;
   double acc[1] = {0.0};
   auto xg = [&](int c, double(&xv)[1]) { xv[0] = x[c]; };
   sell_row<double, double, 1, U, true>(M, s, lane, len, xg, acc);

@@ -100,6 +100,9 @@ struct Matrix {
   int nslices = 0;
   size_t nslots = 0;
   DevArray<int> slice_ptr, rowptr, col;     // SELL pattern (rowptr = monolis index)
+  // 16-bit column deltas (col - row), two slots per 32-bit word (SellView::col16);
+  // empty when some |col - row| exceeds 32767
+  DevArray<unsigned short> col16;
   DevArray<int> csr_item;                   // 1-based, as given
   DevArray<double> csr_val;                 // AoS blocks, as given (kept for setup / gradients)
   DevArray<double> val64;

@@ -10,8 +10,10 @@
 //     val[(t*B*B + e)*64 + l]       block entry e = r*B + c (row-major block)
 //   so every load of a slot is one contiguous 256 B (int) or 512 B (double)
 //   line-aligned request for the whole wavefront.  Slots past a row's length
-//   are padding (col = own row, val = 0) and are never loaded: the lane is
-//   masked off by the row length taken from monolis's `index` array.
+//   are padding (col = own row, val = 0); a padded slot is loaded but masked
+//   off by the row length taken from monolis's `index` array.  When every
+//   |col - row| fits in 16 bits the SpMV reads the columns from a 16-bit delta
+//   copy (SellView::col16), which halves the index stream.
 #pragma once
 
 #include "mh_common.hpp"
@@ -26,7 +28,28 @@ struct SellView {
   const int* rowptr;       // [nrows + 1] monolis `index` (0-based offsets)
   const int* col;          // [nslots * 64]
   const TV* val;           // [nslots * 64 * B * B]
+  // Optional 16-bit copy of `col` as deltas col - row, two slots per 32-bit
+  // word: word (t / 2) * 64 + l holds slot t of lane l in its low half when t
+  // is even, high half when odd.  Null when some |col - row| exceeds 32767.
+  const unsigned* col16;
 };
+
+// Column id of slot t (global slot index) for lane `lane` (block row `row`).
+// The 16-bit form halves the index stream; t's parity is wave-uniform, so the
+// half select is scalar.  Its words are loaded without the non-temporal hint:
+// each word serves two consecutive slots, and a streaming load would let the
+// line leave L2 before the second slot reads it (measured: the index stream
+// was then fetched twice).
+template <bool NT, bool C16, typename TV>
+__device__ __forceinline__ int sell_col(const SellView<TV>& M, size_t t, int lane, int row, bool valid) {
+  if constexpr (C16) {
+    const unsigned w = ld_val<false>(M.col16 + (t >> 1) * kWave + lane);
+    const int d = (t & 1) ? (int)(short)(w >> 16) : (int)(short)(w & 0xffffu);
+    return valid ? row + d : 0;
+  } else {
+    return ld_val<NT>(M.col + t * kWave + lane);
+  }
+}
 
 // acc[r] += sum_k sum_c val(k, r, c) * x(col_k, c) for the lane's row.  `xg`
 // gathers the B entries of the input vector at a block column.  Products are
@@ -38,12 +61,32 @@ struct SellView {
 // then U gathers.  Padding slots are always safe to load (col = own row,
 // val = 0), so no lane branches; a select keeps a padded slot from touching the
 // accumulator, which preserves IEEE semantics (0 * inf never enters a sum).
-template <typename TV, typename TA, int B, int U, bool NT, class XG>
-__device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane, int len, const XG& xg,
-                                         TA (&acc)[B]) {
+// Row length of a lane, either a plain int or the row's two offsets as
+// loaded from monolis's `index`.  The offsets pass through an empty asm
+// statement where the length is first needed (after the step's matrix loads
+// are issued): the wait for their load then overlaps those loads instead of
+// stalling the start of every slice (a loop-invariant length would otherwise
+// be computed, and waited for, before the slot loop).
+struct RowLen {
+  int r0, r1;
+  bool valid;
+};
+__device__ __forceinline__ int row_len(int len, int) { return len; }
+__device__ __forceinline__ int row_len(const RowLen& r, int k) {
+  // not volatile (no side effect, so uniform loads keep using the scalar
+  // cache); the slot index as an input keeps it inside the loop
+  int a = r.r0, b = r.r1;
+  asm("" : "+v"(a), "+v"(b) : "s"(k));
+  return r.valid ? b - a : 0;
+}
+
+template <typename TV, typename TA, int B, int U, bool NT, bool C16, class XG, class LEN>
+__device__ __forceinline__ void sell_row_impl(const SellView<TV>& M, int s, int lane, const LEN& len, const XG& xg,
+                                              TA (&acc)[B]) {
   const int k0 = M.slice_ptr[s];
   const int w = M.slice_ptr[s + 1] - k0;
-  const int* colp = M.col + (size_t)k0 * kWave + lane;
+  const int row = s * kWave + lane;
+  const bool valid = row < M.nrows;
   const TV* valp = M.val + (size_t)k0 * kWave * (B * B) + lane;
   int k = 0;
   for (; k + U <= w; k += U) {
@@ -51,16 +94,17 @@ __device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane,
     TV a[U][B * B];
     TA xv[U][B];
 #pragma unroll
-    for (int u = 0; u < U; ++u) c[u] = ld_val<NT>(colp + (size_t)(k + u) * kWave);
+    for (int u = 0; u < U; ++u) c[u] = sell_col<NT, C16>(M, (size_t)(k0 + k + u), lane, row, valid);
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int e = 0; e < B * B; ++e) a[u][e] = ld_val<NT>(valp + ((size_t)(k + u) * (B * B) + e) * kWave);
 #pragma unroll
     for (int u = 0; u < U; ++u) xg(c[u], xv[u]);
+    const int lim = row_len(len, k);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const bool on = (k + u) < len;
+      const bool on = (k + u) < lim;
 #pragma unroll
       for (int r = 0; r < B; ++r) {
         TA t = acc[r];
@@ -71,13 +115,13 @@ __device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane,
     }
   }
   for (; k < w; ++k) {
-    const int c = ld_val<NT>(colp + (size_t)k * kWave);
+    const int c = sell_col<NT, C16>(M, (size_t)(k0 + k), lane, row, valid);
     TV a[B * B];
 #pragma unroll
     for (int e = 0; e < B * B; ++e) a[e] = ld_val<NT>(valp + ((size_t)k * (B * B) + e) * kWave);
     TA xv[B];
     xg(c, xv);
-    const bool on = k < len;
+    const bool on = k < row_len(len, k);
 #pragma unroll
     for (int r = 0; r < B; ++r) {
       TA t = acc[r];
@@ -86,6 +130,15 @@ __device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane,
       acc[r] = pick(on, t, acc[r]);
     }
   }
+}
+
+template <typename TV, typename TA, int B, int U, bool NT, class XG, class LEN>
+__device__ __forceinline__ void sell_row(const SellView<TV>& M, int s, int lane, const LEN& len, const XG& xg,
+                                         TA (&acc)[B]) {
+  if (M.col16)
+    sell_row_impl<TV, TA, B, U, NT, true>(M, s, lane, len, xg, acc);
+  else
+    sell_row_impl<TV, TA, B, U, NT, false>(M, s, lane, len, xg, acc);
 }
 
 // Software-pipelined form of sell_row: the column ids and values of step j+1
@@ -102,18 +155,19 @@ struct SellStep {
   TV a[U][B * B];
 };
 
-template <typename TV, typename TA, int B, int U, bool NT, class XG>
-__device__ __forceinline__ void sell_row_pipe(const SellView<TV>& M, int s, int lane, int len, const XG& xg,
-                                              TA (&acc)[B]) {
+template <typename TV, typename TA, int B, int U, bool NT, bool C16, class XG, class LEN>
+__device__ __forceinline__ void sell_row_pipe_impl(const SellView<TV>& M, int s, int lane, const LEN& len,
+                                                   const XG& xg, TA (&acc)[B]) {
   const int k0 = M.slice_ptr[s];
   const int w = M.slice_ptr[s + 1] - k0;
-  const int* colp = M.col + (size_t)k0 * kWave + lane;
+  const int row = s * kWave + lane;
+  const bool valid = row < M.nrows;
   const TV* valp = M.val + (size_t)k0 * kWave * (B * B) + lane;
   const int ns = w / U;  // full steps
   auto fetch = [&](SellStep<TV, B, U>& st, int j) {
     const int k = j * U;
 #pragma unroll
-    for (int u = 0; u < U; ++u) st.c[u] = ld_val<NT>(colp + (size_t)(k + u) * kWave);
+    for (int u = 0; u < U; ++u) st.c[u] = sell_col<NT, C16>(M, (size_t)(k0 + k + u), lane, row, valid);
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -128,9 +182,10 @@ __device__ __forceinline__ void sell_row_pipe(const SellView<TV>& M, int s, int 
     if (nxt) fetch(*nxt, jn);
     __builtin_amdgcn_sched_barrier(0);
     const int k = j * U;
+    const int lim = row_len(len, k);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const bool on = (k + u) < len;
+      const bool on = (k + u) < lim;
 #pragma unroll
       for (int r = 0; r < B; ++r) {
         TA t = acc[r];
@@ -156,13 +211,13 @@ __device__ __forceinline__ void sell_row_pipe(const SellView<TV>& M, int s, int 
     }
   }
   for (int k = ns * U; k < w; ++k) {
-    const int c = ld_val<NT>(colp + (size_t)k * kWave);
+    const int c = sell_col<NT, C16>(M, (size_t)(k0 + k), lane, row, valid);
     TV a[B * B];
 #pragma unroll
     for (int e = 0; e < B * B; ++e) a[e] = ld_val<NT>(valp + ((size_t)k * (B * B) + e) * kWave);
     TA xv[B];
     xg(c, xv);
-    const bool on = k < len;
+    const bool on = k < row_len(len, k);
 #pragma unroll
     for (int r = 0; r < B; ++r) {
       TA t = acc[r];
@@ -171,6 +226,15 @@ __device__ __forceinline__ void sell_row_pipe(const SellView<TV>& M, int s, int 
       acc[r] = pick(on, t, acc[r]);
     }
   }
+}
+
+template <typename TV, typename TA, int B, int U, bool NT, class XG, class LEN>
+__device__ __forceinline__ void sell_row_pipe(const SellView<TV>& M, int s, int lane, const LEN& len, const XG& xg,
+                                              TA (&acc)[B]) {
+  if (M.col16)
+    sell_row_pipe_impl<TV, TA, B, U, NT, true>(M, s, lane, len, xg, acc);
+  else
+    sell_row_pipe_impl<TV, TA, B, U, NT, false>(M, s, lane, len, xg, acc);
 }
 
 // Kernel variants of the SELL SpMV.  A variant fixes
@@ -335,7 +399,7 @@ __device__ __forceinline__ void diag_apply(const TD* __restrict__ D, int nrows, 
 template <typename TV, typename TA, int B, int U, class Op>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MH_SELL_WAVES_PER_EU)))
 k_sell(SellView<TV> M, Op op0, double* partials, int nwg, int spw) {
-  const int wg = xcd_tile(nwg);
+  const int wg = xcd_tile<MH_XCD_SELL != 0>(nwg);
   if (wg < 0) return;
   Op op = op0;
   if (op.skip()) return;
@@ -357,8 +421,12 @@ k_sell(SellView<TV> M, Op op0, double* partials, int nwg, int spw) {
   for (int s = s0; s < s1; ++s) {
     const int row = s * kWave + lane;
     const bool valid = row < M.nrows;
-    int len = 0;
-    if (valid) len = M.rowptr[row + 1] - M.rowptr[row];
+    // The row length is loaded unconditionally (lanes past the last row read
+    // the last row's offsets and are masked): a load under a branch makes the
+    // compiler wait for it at the branch's end, which would stall every slice
+    // for a full memory latency before its first matrix load.
+    const int rowc = valid ? row : M.nrows - 1;
+    const RowLen len{M.rowptr[rowc], M.rowptr[rowc + 1], valid};
     TA acc[B];
 #pragma unroll
     for (int r = 0; r < B; ++r) acc[r] = (TA)0;
@@ -394,7 +462,7 @@ k_sell(SellView<TV> M, Op op0, double* partials, int nwg, int spw) {
 //   void finalize(...)
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_rows(int nrows, Op op0, double* partials, int ntiles) {
-  const int tile = xcd_tile(ntiles);
+  const int tile = xcd_tile<MH_XCD_ROWS != 0>(ntiles);
   if (tile < 0) return;
   Op op = op0;
   if (op.skip()) return;

@@ -66,6 +66,7 @@ void DevArray<T>::release() {
 }
 template struct DevArray<int>;
 template struct DevArray<unsigned>;
+template struct DevArray<unsigned short>;
 template struct DevArray<double>;
 template struct DevArray<float>;
 template struct DevArray<unsigned char>;
@@ -167,6 +168,38 @@ __global__ void k_sell_fill(int n, int nslices, int b, const int* __restrict__ r
   }
 }
 
+// Largest |col - row| over the stored slots of the valid rows (atomicMax).
+__global__ void k_col_span(int n, int nslices, const int* __restrict__ rowptr, const int* __restrict__ slice_ptr,
+                           const int* __restrict__ col, int* __restrict__ span) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nslices * kWave) return;
+  const int s = (int)(t / kWave), lane = (int)(t % kWave), row = (int)t;
+  if (row >= n) return;
+  const int k0 = slice_ptr[s];
+  const int len = rowptr[row + 1] - rowptr[row];
+  int m = 0;
+  for (int k = 0; k < len; ++k) {
+    const int d = col[((size_t)k0 + k) * kWave + lane] - row;
+    m = max(m, d < 0 ? -d : d);
+  }
+  if (m > 0) atomicMax(span, m);
+}
+
+// 16-bit delta copy of the column ids: slot t of lane l goes to half (t & 1)
+// of word (t / 2) * 64 + l (every slot, padding included; rows past n get 0).
+__global__ void k_col16_fill(int n, int nslices, const int* __restrict__ slice_ptr, const int* __restrict__ col,
+                             unsigned short* __restrict__ col16) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nslices * kWave) return;
+  const int s = (int)(t / kWave), lane = (int)(t % kWave), row = (int)t;
+  const int k0 = slice_ptr[s], w = slice_ptr[s + 1] - k0;
+  for (int k = 0; k < w; ++k) {
+    const size_t slot = (size_t)k0 + k;
+    const int d = row < n ? col[slot * kWave + lane] - row : 0;
+    col16[((slot >> 1) * kWave + lane) * 2 + (slot & 1)] = (unsigned short)(short)d;
+  }
+}
+
 __global__ void k_to_fp32(size_t m, const double* __restrict__ a, float* __restrict__ b) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
     b[i] = (float)a[i];
@@ -191,6 +224,7 @@ SellView<TV> view(const Matrix& M) {
   v.slice_ptr = M.slice_ptr.p;
   v.rowptr = M.rowptr.p;
   v.col = M.col.p;
+  v.col16 = M.col16.n ? (const unsigned*)M.col16.p : nullptr;
   if constexpr (sizeof(TV) == 16)
     v.val = (const TV*)M.valz.p;
   else if constexpr (sizeof(TV) == 8)
@@ -229,6 +263,27 @@ static int fill_values(Matrix* M, const double* A, hipStream_t s, int fill_col) 
     M->val32_ready = false;
     MH_TRY(matrix_make_fp32(M, s));
   }
+  return 0;
+}
+
+// Builds the 16-bit column stream when every |col - row| <= 32767.
+static int build_col16(Matrix* M, hipStream_t s) {
+  M->col16.release();
+  if (M->nslices == 0 || M->nslots == 0) return 0;
+  DevArray<int> span;
+  MH_TRY(span.alloc(1));
+  MH_HIP(hipMemsetAsync(span.p, 0, sizeof(int), s));
+  const long long threads = (long long)M->nslices * kWave;
+  const unsigned grid = (unsigned)((threads + kBlock - 1) / kBlock);
+  k_col_span<<<grid, kBlock, 0, s>>>(M->n, M->nslices, M->rowptr.p, M->slice_ptr.p, M->col.p, span.p);
+  MH_HIP(hipGetLastError());
+  int h = 0;
+  MH_HIP(hipMemcpyAsync(&h, span.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  if (h > 32767) return 0;
+  MH_TRY(M->col16.alloc(((M->nslots + 1) / 2) * kWave * 2));
+  k_col16_fill<<<grid, kBlock, 0, s>>>(M->n, M->nslices, M->slice_ptr.p, M->col.p, M->col16.p);
+  MH_HIP(hipGetLastError());
   return 0;
 }
 
@@ -323,6 +378,7 @@ static int matrix_build(Matrix** out, int n, int np, int nz, int b, const int* i
       if ((rc = M->val64.alloc(acc * kWave * (size_t)b * b))) break;
     }
     if ((rc = fill_values(M, A, s, 1))) break;
+    if ((rc = build_col16(M, s))) break;
     if (hipStreamSynchronize(s) != hipSuccess) {
       set_error("matrix build failed");
       rc = -1;

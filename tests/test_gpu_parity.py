@@ -117,6 +117,62 @@ def test_matvec_every_kernel_variant_is_bit_identical(gen):
     assert np.array_equal(dy.numpy(), outs[0])
 
 
+def _with_far_zero_block(m, far):
+    """Copy of m with one extra stored block (row 0, column `far`) holding zeros:
+    the same operator, but the column span exceeds the 16-bit delta range."""
+    from monolis_amd import matrix as mm
+    b2 = m.ndof * m.ndof
+    n = m.n
+    rows = [list(m.item[m.index[i]:m.index[i + 1]]) for i in range(n)]
+    vals = [m.A[m.index[i] * b2:m.index[i + 1] * b2] for i in range(n)]
+    cols0 = rows[0] + [far + 1]
+    order = np.argsort(cols0, kind="stable")
+    v0 = np.concatenate([vals[0].reshape(-1, b2), np.zeros((1, b2))])[order].reshape(-1)
+    rows[0] = [cols0[k] for k in order]
+    vals[0] = v0
+    index = np.zeros(n + 1, np.int32)
+    index[1:] = np.cumsum([len(r) for r in rows])
+    item = np.array([c for r in rows for c in r], np.int32)
+    return mm.Matrix(n, n, m.ndof, index, item, np.concatenate(vals)).ensure_dof_index()
+
+
+@pytest.mark.parametrize("gen", ["poisson7", "stencil27"])
+def test_matvec_16bit_column_stream_matches_32bit(gen):
+    """Matrices whose |col - row| fits in 16 bits read a 16-bit delta copy of
+    the column ids; a far zero block forces the 32-bit stream on the same
+    operator.  Both must give the same bits in every kernel variant, and the
+    same CG iterations."""
+    from monolis_amd import _lib
+    m = W.poisson7(42) if gen == "poisson7" else W.stencil27_bcsr3(33, seed=2)  # n = 74,088 / 35,937 block rows
+    wide = _with_far_zero_block(m, m.n - 1)
+    assert m.n - 1 > 32767
+    x = np.random.default_rng(7).uniform(-1, 1, m.nndof)
+    dx, dy = api.DeviceBuffer.from_numpy(x), api.DeviceBuffer(m.nndof)
+    outs = {}
+    for name, mat in (("narrow", m), ("wide", wide)):
+        dm = api.DeviceMatrix(mat)
+        res = []
+        for v in range(_lib.load().monolis_hip_num_variants()):
+            dm.set_variant(v)
+            dm.matvec(dx, dy)
+            res.append(dy.numpy())
+        outs[name] = res
+    ref = oracle.matvec(m, x)
+    assert _rel(outs["narrow"][0], ref) <= 1e-12
+    for a, b in zip(outs["narrow"], outs["wide"]):
+        assert np.array_equal(a, b)
+    b = oracle.matvec(m, np.ones(m.nndof))
+    its = []
+    for mat in (m, wide):
+        mo = api.Monolis(mat)
+        mo.prm.quiet()
+        xs = np.zeros(mat.npndof)
+        api.monolis_solve_R(mo, b, xs)
+        assert mo.prm.Iarray[5] == 0
+        its.append(int(mo.prm.Iarray[4]))
+    assert its[0] == its[1]
+
+
 def test_matvec_values_update():
     m = W.stencil27_bcsr3(10, seed=1)
     dm = api.DeviceMatrix(m)
