@@ -23,7 +23,11 @@ Further workloads (extra keys, skipped with --no-extras):
                    and adjoint (A^T lambda = dL/dx plus the -lambda x^T kernel);
   complex          the headline pattern with complex values K + i 0.5 D (D the
                    diagonal blocks): complex SpMV GB/s and COCG + complex
-                   block-Jacobi to 1e-8 (monolis method 13).
+                   block-Jacobi to 1e-8 (monolis method 13);
+  assembly         device assembly of a 64^3-element hexahedral mesh with
+                   3 DOF per node: pattern build, 262,144 element matrices
+                   (24 x 24) added in element order, Dirichlet conditions on
+                   one face; inputs resident in HBM.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 20]
 """
@@ -77,6 +81,8 @@ def main() -> int:
     ap.add_argument("--multi-spmm", action="store_true",
                     help="only time the 64-RHS SpMM of the 128^3 Poisson workload (profiling runs)")
     ap.add_argument("--fp32", action="store_true", help="with --multi-spmm: the FP32 SpMM")
+    ap.add_argument("--assembly", action="store_true",
+                    help="only run the device-assembly workload (profiling runs)")
     ap.add_argument("--complex-spmv", action="store_true",
                     help="only time the complex SpMV of the headline pattern (profiling runs)")
     ap.add_argument("--copy-only", action="store_true",
@@ -122,6 +128,10 @@ def main() -> int:
         ms_m = (time.perf_counter() - t) * 1e3 / max(args.steps, 1)
         log(f"multi-RHS SpMM ({'FP32' if args.fp32 else 'FP64'}, 64 RHS, 128^3): {ms_m:.3f} ms per call")
         print(json.dumps({"multi_spmm_ms_per_call": round(ms_m, 4), "steps": args.steps}), flush=True)
+        return 0
+
+    if args.assembly:
+        print(json.dumps({"assembly": bench_assembly(args)}), flush=True)
         return 0
 
     if args.complex_spmv:
@@ -271,6 +281,7 @@ def main() -> int:
         out["batching"] = bench_batching(args)
         out["differentiation"] = bench_differentiation(args)
         out["complex"] = bench_complex(args, cpu=not args.no_cpu)
+        out["assembly"] = bench_assembly(args, cpu=not args.no_cpu)
 
     print(json.dumps(out), flush=True)
     return 0
@@ -523,6 +534,90 @@ def bench_complex(args, solve=True, cpu=False):
         res["cpu_spmv_gbs"] = round(nbytes / tc / 1e9, 2)
         res["cpu_threads"] = oracle.get_threads()
         log(f"CPU complex SpMV: {tc * 1e3:.1f} ms")
+    return res
+
+
+def _hex_mesh(nx):
+    import numpy as np
+    ex, ey, ez = np.meshgrid(np.arange(nx), np.arange(nx), np.arange(nx), indexing="ij")
+    ex, ey, ez = ex.ravel(), ey.ravel(), ez.ravel()
+    nid = lambda x, y, z: (z * (nx + 1) + y) * (nx + 1) + x
+    corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    elems = np.stack([nid(ex + dx, ey + dy, ez + dz) for dx, dy, dz in corners], axis=1).astype(np.int32)
+    return (nx + 1) ** 3, elems
+
+
+def bench_assembly(args, cpu=False, nx=64, b=3):
+    """Device assembly of a hexahedral mesh (inputs resident in HBM)."""
+    import ctypes
+    import numpy as np
+    from monolis_amd import _lib, api
+    L = _lib.load()
+    n_node, elems = _hex_mesh(nx)
+    n_elem, nzm = len(elems), 8 * b
+    rng = np.random.default_rng(args.seed + 30)
+    R = rng.uniform(-1, 1, size=(n_elem, nzm, nzm))
+    Ke = (np.matmul(R, R.transpose(0, 2, 1)) / nzm + 0.01 * np.eye(nzm)).reshape(-1)
+    del R
+    de, dk = api.DeviceBuffer.from_numpy(elems), api.DeviceBuffer.from_numpy(Ke)
+    h = ctypes.c_void_p()
+
+    def build():
+        _lib.check(L.monolis_hip_matrix_create_by_simple_mesh(ctypes.byref(h), n_node, 8, b, n_elem,
+                                                              ctypes.c_void_p(de.ptr)), "create_by_simple_mesh")
+
+    build()                                          # warm-up (code objects, allocator)
+    L.monolis_hip_matrix_destroy(h)
+    t0 = time.perf_counter()
+    build()
+    t_pattern = time.perf_counter() - t0
+    info = (ctypes.c_longlong * 8)()
+    L.monolis_hip_matrix_info(h, info)
+    nz = int(info[2])
+
+    def add():
+        _lib.check(L.monolis_hip_matrix_add_elements_R(h, n_elem, 8, ctypes.c_void_p(de.ptr),
+                                                       ctypes.c_void_p(dk.ptr)), "add_elements")
+
+    add()                                            # warm-up
+    t0 = time.perf_counter()
+    add()
+    t_add = time.perf_counter() - t0
+    face = np.arange((nx + 1) ** 2, dtype=np.int32)
+    nodes = np.repeat(face, b)
+    dofs = np.tile(np.arange(b, dtype=np.int32), len(face))
+    dn, dd = api.DeviceBuffer.from_numpy(nodes), api.DeviceBuffer.from_numpy(dofs)
+    dv = api.DeviceBuffer.from_numpy(np.zeros(len(nodes)))
+    rhs = api.DeviceBuffer.from_numpy(np.ones(n_node * b))
+    t0 = time.perf_counter()
+    _lib.check(L.monolis_hip_set_Dirichlet_bc_R(h, ctypes.c_void_p(rhs.ptr), len(nodes), ctypes.c_void_p(dn.ptr),
+                                                ctypes.c_void_p(dd.ptr), ctypes.c_void_p(dv.ptr)), "dirichlet")
+    t_bc = time.perf_counter() - t0
+    L.monolis_hip_matrix_destroy(h)
+    contrib = n_elem * nzm * nzm
+    res = {"workload": f"hex8 mesh {nx}^3 elements, {n_node} nodes x {b} DOF, {n_elem} element matrices "
+                       f"{nzm}x{nzm}", "nnzb": nz,
+           "pattern_ms": round(t_pattern * 1e3, 3), "add_elements_ms": round(t_add * 1e3, 3),
+           "add_elements_contributions_per_s": round(contrib / t_add, 1),
+           "add_elements_ke_gbs": round(contrib * 8 / t_add / 1e9, 1),
+           "dirichlet_ms": round(t_bc * 1e3, 3), "dirichlet_conditions": len(nodes)}
+    log(f"assembly: pattern {t_pattern * 1e3:.1f} ms, {n_elem} elements {t_add * 1e3:.1f} ms "
+        f"({contrib / t_add / 1e9:.2f} G contributions/s), Dirichlet {t_bc * 1e3:.1f} ms")
+    if cpu:
+        import oracle
+        from monolis_amd import matrix as mm
+        t0 = time.perf_counter()
+        index, item = oracle.pattern_by_simple_mesh(n_node, elems)
+        tp = time.perf_counter() - t0
+        mat = mm.Matrix(n_node, n_node, b, index, item, np.zeros(len(item) * b * b))
+        sample = n_elem // 8
+        t0 = time.perf_counter()
+        oracle.add_elements(mat, elems[:sample], Ke[: sample * nzm * nzm])
+        ta = (time.perf_counter() - t0) * n_elem / sample
+        res["cpu_pattern_ms"] = round(tp * 1e3, 1)
+        res["cpu_add_elements_ms_est"] = round(ta * 1e3, 1)
+        res["cpu_sample"] = f"pattern of the whole mesh; element add timed on {sample} elements, scaled"
+        log(f"CPU assembly: pattern {tp * 1e3:.0f} ms, elements {ta * 1e3:.0f} ms (estimated)")
     return res
 
 

@@ -264,6 +264,44 @@ int monolis_hip_outer_grad_R(monolis_hip_matrix* M, const double* lambda, const 
 int monolis_hip_adjoint_R(monolis_hip_matrix* M, const double* x, const double* g, double* lambda, double* dA,
                           int symmetric, int* Iarray, double* Rarray);
 
+/* Device assembly: counterparts of monolis's matrix assembly routines that
+ * build and fill the handle's matrix on the GPU.  Node ids are 0-based, as in
+ * monolis's C API; real fixed-block square matrices (n == np).
+ *
+ * monolis_hip_matrix_create_by_simple_mesh replaces the pattern build of
+ * monolis_get_nonzero_pattern_by_simple_mesh_R (wrapper/matrix/monolis_spmat_nzpattern_c.h:21-27,
+ * src/matrix/spmat_nzpattern.f90:15-49): nodes that share an element are
+ * coupled, every row holds its diagonal, columns ascending; the matrix starts
+ * at zero.  elem holds n_elem * n_base node ids (element-major). */
+int monolis_hip_matrix_create_by_simple_mesh(monolis_hip_matrix** out, int n_node, int n_base, int n_dof, int n_elem,
+                                             const int* elem);
+/* index (np + 1, 0-based offsets) and item (nz, 1-based columns) of the handle; host or device. */
+int monolis_hip_matrix_get_pattern(const monolis_hip_matrix* M, int* index, int* item);
+/* The values in monolis's layout (nz * n_dof^2, row-major blocks); host or device. */
+int monolis_hip_matrix_get_values_R(const monolis_hip_matrix* M, double* A);
+/* monolis_add_matrix_to_sparse_matrix_R (wrapper/matrix/monolis_spmat_handler_c.h:73-77,
+ * src/matrix/spmat_handler_util.f90:398-497) for n_elem elements in one call:
+ * Ke holds n_elem row-major (n_base*n_dof)^2 element matrices; entry
+ * (a*n_dof + k1, c*n_dof + k2) of element e is added to component (k1, k2) of
+ * block (elem[e*n_base + a], elem[e*n_base + c]).  The sums are those of one
+ * call per element in element order, bit for bit.  Elements are processed
+ * in chunks of 2^24 node pairs (environment MONOLIS_HIP_ASSEMBLY_CHUNK_PAIRS
+ * overrides); a block outside the pattern is an error (monolis stops there),
+ * and the chunks before the failing one stay added. */
+int monolis_hip_matrix_add_elements_R(monolis_hip_matrix* M, int n_elem, int n_base, const int* elem,
+                                      const double* Ke);
+/* monolis_set_Dirichlet_bc_R (wrapper/matrix/monolis_spmat_handler_c.h:142-147,
+ * src/matrix/spmat_handler_util.f90:604-677) for n_bc conditions at once:
+ * x(node_id[i]*n_dof + n_dof_bc[i]) = val[i].  b (N*n_dof, host or device)
+ * receives b - val * A(:, dof) for the other rows and val for the condition
+ * rows; condition rows and columns are zeroed and their diagonals set to 1.
+ * Equal to applying the conditions one by one (the right-hand-side updates
+ * of one row are summed in column order, which is list order for a list
+ * sorted by node and DOF).  A DOF listed twice, or a node without a diagonal
+ * block (monolis stops there), is an error. */
+int monolis_hip_set_Dirichlet_bc_R(monolis_hip_matrix* M, double* b, int n_bc, const int* node_id,
+                                   const int* n_dof_bc, const double* val);
+
 /* Device-resident complex matrix (the complex counterpart of monolis_hip_matrix;
  * same BCSR arrays, A holds nz * n_dof^2 double _Complex values).  A separate
  * handle type keeps real and complex matrices apart at compile time. */

@@ -38,6 +38,12 @@ SIGNATURES = [
     ("monolis_hip_inner_product_C", c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     ("monolis_hip_solve_C", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     ("monolis_hip_time_matvec_C", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_double_p]),
+    ("monolis_hip_matrix_create_by_simple_mesh", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int,
+                                                         c_void_p]),
+    ("monolis_hip_matrix_get_pattern", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("monolis_hip_matrix_get_values_R", c_int, [c_void_p, c_void_p]),
+    ("monolis_hip_matrix_add_elements_R", c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    ("monolis_hip_set_Dirichlet_bc_R", c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("monolis_hip_last_error", ctypes.c_char_p, []),
     ("monolis_hip_init", c_int, []),
     ("monolis_hip_stream", c_void_p, []),

@@ -331,6 +331,30 @@ class DeviceMatrix:
         return {"best": best.value, "ms": list(ms), "configs": cfgs}
 
     # device-side scalar access (monolis_{set,add,get}_scalar_*_sparse_matrix_R, 0-based)
+    def values(self) -> np.ndarray:
+        """The device values in monolis's A layout."""
+        A = np.zeros(max(self.mat.nz * self.mat.ndof * self.mat.ndof, 1), np.float64)
+        _lib.check(_lib.load().monolis_hip_matrix_get_values_R(ctypes.c_void_p(self.h), _p(A)), "get_values")
+        return A[: self.mat.nz * self.mat.ndof * self.mat.ndof]
+
+    def add_elements(self, elems, Ke) -> None:
+        """monolis_add_matrix_to_sparse_matrix_R for every element (rows of `elems`,
+        0-based nodes); Ke: (n_elem, n_base*b, n_base*b) element matrices."""
+        e = np.ascontiguousarray(elems, np.int32)
+        k = np.ascontiguousarray(Ke, np.float64)
+        _lib.check(_lib.load().monolis_hip_matrix_add_elements_R(ctypes.c_void_p(self.h), int(e.shape[0]),
+                                                                  int(e.shape[1]), _p(e), _p(k)), "add_elements")
+
+    def set_dirichlet(self, b: np.ndarray, node_id, n_dof_bc, val) -> np.ndarray:
+        """monolis_set_Dirichlet_bc_R for every listed condition; b (N*b) updated in place."""
+        nd = np.ascontiguousarray(node_id, np.int32)
+        dd = np.ascontiguousarray(n_dof_bc, np.int32)
+        vv = np.ascontiguousarray(val, np.float64)
+        assert b.dtype == np.float64 and b.flags.c_contiguous
+        _lib.check(_lib.load().monolis_hip_set_Dirichlet_bc_R(ctypes.c_void_p(self.h), _p(b), int(len(nd)), _p(nd),
+                                                               _p(dd), _p(vv)), "set_Dirichlet_bc")
+        return b
+
     def set_scalar(self, i: int, j: int, sub_i: int, sub_j: int, val: float) -> None:
         _lib.check(_lib.load().monolis_hip_matrix_set_scalar_R(ctypes.c_void_p(self.h), i, j, sub_i, sub_j,
                                                                float(val)), "set_scalar")
@@ -403,6 +427,28 @@ class DeviceMatrix:
                                                        ctypes.c_void_p(y.ptr), warmup, reps, ctypes.byref(ms)),
                    "time_matvec")
         return ms.value
+
+
+def device_matrix_by_simple_mesh(n_node: int, n_dof: int, elems) -> "DeviceMatrix":
+    """monolis_get_nonzero_pattern_by_simple_mesh_R on the GPU: a DeviceMatrix
+    with the mesh's pattern and zero values (elems: 0-based node ids, one row
+    per element)."""
+    L = _lib.load()
+    e = np.ascontiguousarray(elems, np.int32)
+    n_elem, n_base = (e.shape[0], e.shape[1]) if e.ndim == 2 else (0, 1)
+    h = ctypes.c_void_p()
+    _lib.check(L.monolis_hip_matrix_create_by_simple_mesh(ctypes.byref(h), int(n_node), int(n_base), int(n_dof),
+                                                          int(n_elem), _p(e)), "create_by_simple_mesh")
+    info = (ctypes.c_longlong * 8)()
+    _lib.check(L.monolis_hip_matrix_info(h, info), "info")
+    nz = int(info[2])
+    index = np.zeros(n_node + 1, np.int32)
+    item = np.zeros(max(nz, 1), np.int32)
+    _lib.check(L.monolis_hip_matrix_get_pattern(h, _p(index), _p(item)), "get_pattern")
+    mat = Matrix(n_node, n_node, n_dof, index, item[:nz], np.zeros(nz * n_dof * n_dof)).ensure_dof_index()
+    dm = DeviceMatrix.__new__(DeviceMatrix)
+    dm.h, dm.mat, dm.m, dm.vec = h.value, mat, mat.nndof, mat.npndof
+    return dm
 
 
 class DeviceMatrixC:

@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmonolis_hip.so")
 
-SOURCES = ["mh_matrix.hip", "mh_krylov.hip", "mh_capi.hip", "mh_sorted.hip", "mh_batch.hip", "mh_multirhs.hip", "mh_adjoint.hip", "mh_vblock.hip", "mh_complex.hip"]
+SOURCES = ["mh_matrix.hip", "mh_krylov.hip", "mh_capi.hip", "mh_sorted.hip", "mh_batch.hip", "mh_multirhs.hip", "mh_adjoint.hip", "mh_vblock.hip", "mh_complex.hip", "mh_assembly.hip"]
 
 ARCH = os.environ.get("MONOLIS_HIP_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

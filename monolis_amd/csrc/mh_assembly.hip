@@ -1,0 +1,498 @@
+// Device-side matrix assembly, the GPU counterpart of monolis's host assembly:
+//   * the nonzero pattern of a simple mesh (monolis_get_nonzero_pattern_by_simple_mesh_R,
+//     src/matrix/spmat_nzpattern.f90:15-49, with the nodal graph of
+//     spmat_nzpattern_util.f90:13-58): (row, column) keys of every node pair of
+//     every element plus the diagonal, radix-sorted and made unique;
+//   * batched element matrices (monolis_add_matrix_to_sparse_matrix_main_R,
+//     src/matrix/spmat_handler_util.f90:398-497): the node pairs of all
+//     elements keyed by their block position, stably sorted (element order
+//     kept), then every block component summed left to right onto its current
+//     value, so the result equals one add-matrix call per element in element
+//     order, bit for bit;
+//   * Dirichlet conditions (monolis_set_Dirichlet_bc_main_R,
+//     src/matrix/spmat_handler_util.f90:604-677) for a whole list in one pass;
+//   * value and pattern read-back in monolis's layout.
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
+
+#include <hipcub/hipcub.hpp>
+
+#include "mh_assembly.hpp"
+#include "mh_launch.hpp"
+
+namespace mh {
+
+namespace {
+
+int grid_for(long long n) {
+  const long long g = (n + kBlock - 1) / kBlock;
+  return (int)(g > 0 ? g : 1);
+}
+
+int bits_for(unsigned long long v) {
+  int b = 1;
+  while (b < 64 && (v >> b) != 0) ++b;
+  return b;
+}
+
+// Device copy of a host array (device pointers are used in place).
+template <typename T>
+int stage(const T* src, size_t count, DevArray<T>& tmp, const T** out, hipStream_t s) {
+  if (count == 0 || is_device_pointer(src)) {
+    *out = src;
+    return 0;
+  }
+  MH_TRY(tmp.alloc(count));
+  MH_TRY(upload(tmp.p, src, count * sizeof(T), s));
+  *out = tmp.p;
+  return 0;
+}
+
+int read_flag(const int* d, int* h, hipStream_t s) {
+  MH_HIP(hipMemcpyAsync(h, d, sizeof(int), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+__global__ void k_check_range(long long m, const int* __restrict__ v, int hi, int* __restrict__ bad) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x) {
+    const int x = v[i];
+    if (x < 0 || x >= hi) atomicOr(bad, 1);
+  }
+}
+
+// keys of every ordered node pair (a != c) of every element, then the diagonal
+__global__ void k_mesh_keys(int n_node, int n_base, int n_elem, const int* __restrict__ elem,
+                            unsigned long long* __restrict__ keys) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long per = (long long)n_base * (n_base - 1);
+  const long long npair = (long long)n_elem * per;
+  if (t < npair) {
+    const long long e = t / per;
+    const int q = (int)(t % per);
+    const int a = q / (n_base - 1);
+    int c = q % (n_base - 1);
+    if (c >= a) ++c;
+    keys[t] = ((unsigned long long)elem[e * n_base + a] << 32) | (unsigned)elem[e * n_base + c];
+  } else if (t < npair + n_node) {
+    const unsigned i = (unsigned)(t - npair);
+    keys[t] = ((unsigned long long)i << 32) | i;
+  }
+}
+
+// index[r] = first key of row r, item = column + 1 (keys sorted and unique)
+__global__ void k_pattern_csr(int n_node, int nz, const unsigned long long* __restrict__ keys,
+                              int* __restrict__ index, int* __restrict__ item) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t <= n_node) {
+    const unsigned long long target = (unsigned long long)t << 32;
+    int lo = 0, hi = nz;
+    while (lo < hi) {
+      const int mid = lo + (hi - lo) / 2;
+      if (keys[mid] < target) lo = mid + 1;
+      else hi = mid;
+    }
+    index[t] = lo;
+  }
+  if (t < nz) item[t] = (int)(keys[t] & 0xffffffffull) + 1;
+}
+
+// Block position of (row, col) in the CSR, -1 when absent (item sorted per row).
+__device__ __forceinline__ int find_block(const int* __restrict__ rowptr, const int* __restrict__ item, int row,
+                                          int col) {
+  int lo = rowptr[row];
+  const int end = rowptr[row + 1];
+  int hi = end;
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (item[mid] < col + 1) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < end && item[lo] == col + 1) ? lo : -1;
+}
+
+// One thread per (element, a, c) node pair: key = block position of
+// (elem[a], elem[c]) or `none` (sorted last) when the block is absent; value =
+// the pair id e*n_base^2 + a*n_base + c.  A stable sort by key keeps the pairs
+// of one block in element order.
+__global__ void k_elem_targets(int ne, int n_base, const int* __restrict__ elem, const int* __restrict__ rowptr,
+                               const int* __restrict__ item, unsigned long long* __restrict__ keys,
+                               int* __restrict__ ids, int* __restrict__ bad, unsigned long long none) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long per = (long long)n_base * n_base;
+  if (t >= (long long)ne * per) return;
+  const int e = (int)(t / per);
+  const int q = (int)(t % per);
+  const int a = q / n_base, c = q % n_base;
+  const int blk = find_block(rowptr, item, elem[(size_t)e * n_base + a], elem[(size_t)e * n_base + c]);
+  if (blk < 0) atomicOr(bad, 1);
+  keys[t] = blk >= 0 ? (unsigned long long)blk : none;
+  ids[t] = (int)t;
+}
+
+// SELL position of block `blk`, component e (row found by bisection).
+__device__ __forceinline__ size_t sell_pos(int blk, int e, int bb, int n, const int* __restrict__ rowptr,
+                                           const int* __restrict__ slice_ptr) {
+  int lo = 0, hi = n - 1;  // first row r with rowptr[r + 1] > blk
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (rowptr[mid + 1] > blk) hi = mid;
+    else lo = mid + 1;
+  }
+  const int r = lo;
+  const size_t slot = (size_t)slice_ptr[r / kWave] + (size_t)(blk - rowptr[r]);
+  return (slot * bb + e) * kWave + (r % kWave);
+}
+
+// Thread (i, component): at the head i of each run of equal block keys, adds
+// the component of every pair in the run, in order, onto the stored value.
+__global__ void k_elem_accumulate(long long m, const unsigned long long* __restrict__ keys,
+                                  const int* __restrict__ ids, const double* __restrict__ Ke, int n_base, int b,
+                                  int n, const int* __restrict__ rowptr, const int* __restrict__ slice_ptr,
+                                  double* __restrict__ val, unsigned long long none) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int bb = b * b;
+  if (t >= m * bb) return;
+  const long long i = t / bb;
+  const int ee = (int)(t % bb);
+  const unsigned long long key = keys[i];
+  if (key == none || (i > 0 && keys[i - 1] == key)) return;
+  const int k1 = ee / b, k2 = ee % b;
+  const long long per = (long long)n_base * n_base;
+  const long long nzm = (long long)n_base * b;
+  const size_t pos = sell_pos((int)key, ee, bb, n, rowptr, slice_ptr);
+  double acc = val[pos];
+  for (long long j = i; j < m && keys[j] == key; ++j) {
+    const long long pid = ids[j];
+    const long long e = pid / per;
+    const int q = (int)(pid % per);
+    const int a = q / n_base, c = q % n_base;
+    acc = acc + Ke[e * nzm * nzm + (long long)(a * b + k1) * nzm + (c * b + k2)];
+  }
+  val[pos] = acc;
+}
+
+__global__ void k_bc_mark(int nbc, const int* __restrict__ node, const int* __restrict__ dof,
+                          const double* __restrict__ v, int b, int* __restrict__ isbc, double* __restrict__ bcval,
+                          int* __restrict__ dup) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbc) return;
+  const int d = node[i] * b + dof[i];
+  if (atomicAdd(&isbc[d], 1) != 0) atomicOr(dup, 1);
+  bcval[d] = v[i];
+}
+
+__global__ void k_bc_check_diag(int nbc, const int* __restrict__ node, const int* __restrict__ rowptr,
+                                const int* __restrict__ item, int* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbc) return;
+  if (find_block(rowptr, item, node[i], node[i]) < 0) atomicOr(bad, 1);
+}
+
+// One thread per block row: rows of condition DOFs become unit rows with
+// rhs = val; other rows move val * A(r, c) of condition columns to the
+// right-hand side (blocks in column order, then component order) and zero them.
+__global__ void k_bc_apply(int n, int b, const int* __restrict__ slice_ptr, const int* __restrict__ rowptr,
+                           const int* __restrict__ col, double* __restrict__ val, const int* __restrict__ isbc,
+                           const double* __restrict__ bcval, double* __restrict__ rhs) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int s = r / kWave, lane = r % kWave;
+  const int k0 = slice_ptr[s];
+  const int len = rowptr[r + 1] - rowptr[r];
+  const int bb = b * b;
+  for (int k1 = 0; k1 < b; ++k1) {
+    const bool rowbc = isbc[r * b + k1] != 0;
+    double acc = rhs[(size_t)r * b + k1];
+    for (int k = 0; k < len; ++k) {
+      const size_t slot = (size_t)k0 + k;
+      const int c = col[slot * kWave + lane];
+      for (int k2 = 0; k2 < b; ++k2) {
+        const bool colbc = isbc[c * b + k2] != 0;
+        const size_t pos = (slot * bb + k1 * b + k2) * kWave + lane;
+        if (colbc && !rowbc) {
+          const double t = bcval[c * b + k2] * val[pos];
+          acc = acc - t;  // B(n) = B(n) - val*A(m)  (spmat_handler_util.f90:636)
+        }
+        if (rowbc || colbc) val[pos] = (rowbc && c == r && k1 == k2) ? 1.0 : 0.0;
+      }
+    }
+    rhs[(size_t)r * b + k1] = rowbc ? bcval[r * b + k1] : acc;
+  }
+}
+
+__global__ void k_sell_gather(int n, int b, const int* __restrict__ slice_ptr, const int* __restrict__ rowptr,
+                              const double* __restrict__ val, double* __restrict__ A) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int s = r / kWave, lane = r % kWave;
+  const int k0 = slice_ptr[s];
+  const int j0 = rowptr[r], len = rowptr[r + 1] - j0;
+  const int bb = b * b;
+  for (int k = 0; k < len; ++k)
+    for (int e = 0; e < bb; ++e)
+      A[((size_t)j0 + k) * bb + e] = val[(((size_t)k0 + k) * bb + e) * kWave + lane];
+}
+
+int require_assemblable(const Matrix& M, const char* what) {
+  if (M.vb || M.is_complex) {
+    set_error("%s: real fixed-block matrices only", what);
+    return -2;
+  }
+  if (M.np != M.n) {
+    set_error("%s: square local matrices only (n == np)", what);
+    return -2;
+  }
+  return 0;
+}
+
+void mark_values_changed(Matrix* M) {
+  M->diag_ready = M->diag32_ready = M->diagz_ready = false;
+  M->val32_ready = false;  // the FP32 copy is rebuilt on its next use
+  M->transpose_stale = true;
+}
+
+}  // namespace
+
+int pattern_by_simple_mesh(int n_node, int n_base, int n_elem, const int* elem, DevArray<int>& index,
+                           DevArray<int>& item, int* nz_out, hipStream_t s) {
+  if (n_node < 1 || n_base < 1 || n_elem < 0) {
+    set_error("invalid mesh sizes n_node=%d n_base=%d n_elem=%d", n_node, n_base, n_elem);
+    return -2;
+  }
+  const long long nelem_ids = (long long)n_elem * n_base;
+  const long long npair = (long long)n_elem * n_base * (n_base - 1);
+  const long long K = npair + n_node;
+  if (K >= (1LL << 31)) {
+    set_error("mesh too large for one pattern build (%lld node pairs)", K);
+    return -2;
+  }
+  DevArray<int> etmp;
+  const int* de = nullptr;
+  MH_TRY(stage(elem, (size_t)nelem_ids, etmp, &de, s));
+  DevArray<int> flag;
+  MH_TRY(flag.alloc(1));
+  MH_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+  if (nelem_ids > 0) {
+    k_check_range<<<std::min(grid_for(nelem_ids), 65536), kBlock, 0, s>>>(nelem_ids, de, n_node, flag.p);
+    MH_HIP(hipGetLastError());
+  }
+  int bad = 0;
+  MH_TRY(read_flag(flag.p, &bad, s));
+  if (bad) {
+    set_error("element connectivity holds a node id outside 0..%d", n_node - 1);
+    return -2;
+  }
+  DevArray<unsigned long long> k0, k1;
+  MH_TRY(k0.alloc((size_t)K));
+  MH_TRY(k1.alloc((size_t)K));
+  k_mesh_keys<<<grid_for(K), kBlock, 0, s>>>(n_node, n_base, n_elem, de, k0.p);
+  MH_HIP(hipGetLastError());
+  const int end_bit = 32 + bits_for((unsigned long long)n_node);
+  size_t tmp_bytes = 0, tmp2 = 0;
+  MH_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, k0.p, k1.p, (int)K, 0, end_bit, s));
+  MH_HIP(hipcub::DeviceSelect::Unique(nullptr, tmp2, k1.p, k0.p, flag.p, (int)K, s));
+  DevArray<unsigned char> tmp;
+  MH_TRY(tmp.alloc(std::max(tmp_bytes, tmp2)));
+  MH_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tmp_bytes, k0.p, k1.p, (int)K, 0, end_bit, s));
+  MH_HIP(hipcub::DeviceSelect::Unique(tmp.p, tmp2, k1.p, k0.p, flag.p, (int)K, s));
+  int nz = 0;
+  MH_TRY(read_flag(flag.p, &nz, s));
+  MH_TRY(index.alloc((size_t)n_node + 1));
+  MH_TRY(item.alloc((size_t)nz));
+  k_pattern_csr<<<grid_for(std::max<long long>(nz, (long long)n_node + 1)), kBlock, 0, s>>>(n_node, nz, k0.p,
+                                                                                            index.p, item.p);
+  MH_HIP(hipGetLastError());
+  MH_HIP(hipStreamSynchronize(s));
+  *nz_out = nz;
+  return 0;
+}
+
+int matrix_add_elements(Matrix* M, int n_elem, int n_base, const int* elem, const double* Ke, hipStream_t s) {
+  MH_TRY(require_assemblable(*M, "element assembly"));
+  if (n_elem < 0 || n_base < 1) {
+    set_error("invalid element sizes n_elem=%d n_base=%d", n_elem, n_base);
+    return -2;
+  }
+  if (n_elem == 0 || M->n == 0) return 0;
+  const int b = M->b, bb = b * b;
+  const long long nzm = (long long)n_base * b;
+  const long long per_elem = nzm * nzm;       // values per element matrix
+  const long long pairs_elem = (long long)n_base * n_base;
+  if (per_elem > (1LL << 30)) {
+    set_error("element matrix too large (%lld entries)", per_elem);
+    return -2;
+  }
+  // node pairs per chunk: bounded temporaries, element order kept across chunks
+  // (MONOLIS_HIP_ASSEMBLY_CHUNK_PAIRS overrides the 2^24 default)
+  long long max_pairs = 1LL << 24;
+  if (const char* env = getenv("MONOLIS_HIP_ASSEMBLY_CHUNK_PAIRS")) {
+    const long long v = atoll(env);
+    if (v > 0) max_pairs = v;
+  }
+  const int chunk = (int)std::max<long long>(1, std::min<long long>(n_elem, max_pairs / pairs_elem));
+  DevArray<int> etmp;
+  const int* de = nullptr;
+  MH_TRY(stage(elem, (size_t)n_elem * n_base, etmp, &de, s));
+  DevArray<int> flag;
+  MH_TRY(flag.alloc(1));
+  MH_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+  k_check_range<<<std::min(grid_for((long long)n_elem * n_base), 65536), kBlock, 0, s>>>((long long)n_elem * n_base,
+                                                                                        de, M->n, flag.p);
+  MH_HIP(hipGetLastError());
+  int bad = 0;
+  MH_TRY(read_flag(flag.p, &bad, s));
+  if (bad) {
+    set_error("element connectivity holds a node id outside 0..%d", M->n - 1);
+    return -2;
+  }
+  const bool ke_dev = is_device_pointer(Ke);
+  const long long cap = (long long)chunk * pairs_elem;
+  DevArray<unsigned long long> k0, k1;
+  DevArray<int> i0, i1;
+  DevArray<double> kt;
+  MH_TRY(k0.alloc((size_t)cap));
+  MH_TRY(k1.alloc((size_t)cap));
+  MH_TRY(i0.alloc((size_t)cap));
+  MH_TRY(i1.alloc((size_t)cap));
+  if (!ke_dev) MH_TRY(kt.alloc((size_t)chunk * per_elem));
+  // keys are block positions < nz; `none` (an absent block) sorts last
+  const int end_bit = bits_for((unsigned long long)M->nz) + 1;
+  const unsigned long long none = 1ull << (end_bit - 1);
+  size_t tmp_bytes = 0;
+  MH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0.p, k1.p, i0.p, i1.p, (int)cap, 0, end_bit, s));
+  DevArray<unsigned char> tmp;
+  MH_TRY(tmp.alloc(tmp_bytes));
+  for (int e0 = 0; e0 < n_elem; e0 += chunk) {
+    const int ne = std::min(chunk, n_elem - e0);
+    const long long m = (long long)ne * pairs_elem;
+    const double* kc = nullptr;
+    if (ke_dev) {
+      kc = Ke + (size_t)e0 * per_elem;
+    } else {
+      MH_TRY(upload(kt.p, Ke + (size_t)e0 * per_elem, (size_t)ne * per_elem * sizeof(double), s));
+      kc = kt.p;
+    }
+    k_elem_targets<<<grid_for(m), kBlock, 0, s>>>(ne, n_base, de + (size_t)e0 * n_base, M->rowptr.p, M->csr_item.p,
+                                                   k0.p, i0.p, flag.p, none);
+    MH_HIP(hipGetLastError());
+    MH_TRY(read_flag(flag.p, &bad, s));
+    if (bad) {
+      // monolis stops here (monolis_stop_by_matrix_assemble); earlier chunks stay added
+      set_error("element %d..%d couples nodes whose block is not in the nonzero pattern", e0, e0 + ne - 1);
+      mark_values_changed(M);
+      return -2;
+    }
+    size_t tb = tmp_bytes;
+    MH_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0.p, k1.p, i0.p, i1.p, (int)m, 0, end_bit, s));
+    k_elem_accumulate<<<grid_for(m * bb), kBlock, 0, s>>>(m, k1.p, i1.p, kc, n_base, b, M->n, M->rowptr.p,
+                                                           M->slice_ptr.p, M->val64.p, none);
+    MH_HIP(hipGetLastError());
+  }
+  MH_HIP(hipStreamSynchronize(s));
+  mark_values_changed(M);
+  return 0;
+}
+
+int matrix_dirichlet(Matrix* M, double* rhs, int n_bc, const int* node, const int* dof, const double* val,
+                     hipStream_t s) {
+  MH_TRY(require_assemblable(*M, "Dirichlet conditions"));
+  if (n_bc < 0) {
+    set_error("negative number of conditions");
+    return -2;
+  }
+  if (n_bc == 0 || M->n == 0) return 0;
+  const int b = M->b;
+  DevArray<int> nt, dt;
+  DevArray<double> vt;
+  const int *dn = nullptr, *dd = nullptr;
+  const double* dv = nullptr;
+  MH_TRY(stage(node, (size_t)n_bc, nt, &dn, s));
+  MH_TRY(stage(dof, (size_t)n_bc, dt, &dd, s));
+  MH_TRY(stage(val, (size_t)n_bc, vt, &dv, s));
+  DevArray<int> flag;
+  MH_TRY(flag.alloc(2));
+  MH_HIP(hipMemsetAsync(flag.p, 0, 2 * sizeof(int), s));
+  k_check_range<<<std::min(grid_for(n_bc), 65536), kBlock, 0, s>>>(n_bc, dn, M->n, flag.p);
+  k_check_range<<<std::min(grid_for(n_bc), 65536), kBlock, 0, s>>>(n_bc, dd, b, flag.p + 1);
+  MH_HIP(hipGetLastError());
+  int bad[2] = {0, 0};
+  MH_HIP(hipMemcpyAsync(bad, flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  if (bad[0] || bad[1]) {
+    set_error("Dirichlet condition outside the matrix (%s)", bad[0] ? "node id" : "DOF index");
+    return -2;
+  }
+  MH_HIP(hipMemsetAsync(flag.p, 0, 2 * sizeof(int), s));
+  k_bc_check_diag<<<grid_for(n_bc), kBlock, 0, s>>>(n_bc, dn, M->rowptr.p, M->csr_item.p, flag.p);
+  MH_HIP(hipGetLastError());
+  DevArray<int> isbc;
+  DevArray<double> bcval;
+  const size_t m = (size_t)M->n * b;
+  MH_TRY(isbc.alloc(m));
+  MH_TRY(bcval.alloc(m));
+  MH_HIP(hipMemsetAsync(isbc.p, 0, m * sizeof(int), s));
+  k_bc_mark<<<grid_for(n_bc), kBlock, 0, s>>>(n_bc, dn, dd, dv, b, isbc.p, bcval.p, flag.p + 1);
+  MH_HIP(hipGetLastError());
+  MH_HIP(hipMemcpyAsync(bad, flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  if (bad[0]) {
+    // monolis stops (monolis_stop_by_set_DBC)
+    set_error("Dirichlet condition on a node without a diagonal block");
+    return -2;
+  }
+  if (bad[1]) {
+    set_error("the same DOF is listed twice among the Dirichlet conditions");
+    return -2;
+  }
+  DevArray<double> rtmp;
+  double* dr = rhs;
+  const bool rhs_dev = is_device_pointer(rhs);
+  if (!rhs_dev) {
+    MH_TRY(rtmp.alloc(m));
+    MH_TRY(upload(rtmp.p, rhs, m * sizeof(double), s));
+    dr = rtmp.p;
+  }
+  k_bc_apply<<<grid_for(M->n), kBlock, 0, s>>>(M->n, b, M->slice_ptr.p, M->rowptr.p, M->col.p, M->val64.p, isbc.p,
+                                               bcval.p, dr);
+  MH_HIP(hipGetLastError());
+  if (!rhs_dev) MH_HIP(hipMemcpyAsync(rhs, dr, m * sizeof(double), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  mark_values_changed(M);
+  return 0;
+}
+
+int matrix_get_values(const Matrix& M, double* A, hipStream_t s) {
+  if (M.vb || M.is_complex) {
+    set_error("value read-back: real fixed-block matrices only");
+    return -2;
+  }
+  const size_t nv = (size_t)M.nz * M.b * M.b;
+  if (nv == 0) return 0;
+  DevArray<double> tmp;
+  double* dA = A;
+  const bool dev = is_device_pointer(A);
+  if (!dev) {
+    MH_TRY(tmp.alloc(nv));
+    dA = tmp.p;
+    if (M.n < M.np) MH_HIP(hipMemsetAsync(dA, 0, nv * sizeof(double), s));
+  }
+  if (M.n > 0) {
+    k_sell_gather<<<grid_for(M.n), kBlock, 0, s>>>(M.n, M.b, M.slice_ptr.p, M.rowptr.p, M.val64.p, dA);
+    MH_HIP(hipGetLastError());
+  }
+  if (!dev) MH_HIP(hipMemcpyAsync(A, dA, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int matrix_get_pattern(const Matrix& M, int* index, int* item, hipStream_t s) {
+  if (index) MH_HIP(hipMemcpyAsync(index, M.rowptr.p, sizeof(int) * ((size_t)M.np + 1), hipMemcpyDefault, s));
+  if (item && M.nz > 0) MH_HIP(hipMemcpyAsync(item, M.csr_item.p, sizeof(int) * (size_t)M.nz, hipMemcpyDefault, s));
+  MH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace mh

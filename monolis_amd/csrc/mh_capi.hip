@@ -15,6 +15,7 @@
 #include "mh_multirhs.hpp"
 #include "mh_adjoint.hpp"
 #include "mh_complex.hpp"
+#include "mh_assembly.hpp"
 
 using mh::Context;
 using mh::Matrix;
@@ -997,6 +998,64 @@ void monolis_matvec_product_R_c_main(int n, int np, int nz, int n_dof, int NPNDO
 void monolis_inner_product_R_c_main(int m, double* x, double* y, double* sum, int comm) {
   (void)comm;
   if (monolis_hip_inner_product_R(m, x, y, sum)) error_stop(mh::last_error());
+}
+
+// ---------------------------------------------------------------------------
+// Device assembly
+// ---------------------------------------------------------------------------
+int monolis_hip_matrix_create_by_simple_mesh(monolis_hip_matrix** out, int n_node, int n_base, int n_dof,
+                                             int n_elem, const int* elem) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (!out) {
+    mh::set_error("null output handle");
+    return ST_ARGS;
+  }
+  *out = nullptr;
+  if (n_dof < 1 || n_dof > mh::kMaxB) {
+    mh::set_error("block size %d outside 1..%d", n_dof, mh::kMaxB);
+    return ST_ARGS;
+  }
+  mh::DevArray<int> index, item;
+  int nz = 0;
+  MH_TRY(mh::pattern_by_simple_mesh(n_node, n_base, n_elem, elem, index, item, &nz, c->stream));
+  // values start at zero, as monolis_alloc_nonzero_pattern_mat_val_R leaves them
+  mh::DevArray<double> zeros;
+  const size_t nv = (size_t)nz * n_dof * n_dof;
+  MH_TRY(zeros.alloc(nv));
+  MH_HIP(hipMemsetAsync(zeros.p, 0, sizeof(double) * (nv ? nv : 1), c->stream));
+  Matrix* M = nullptr;
+  MH_TRY(mh::matrix_create(&M, n_node, n_node, nz, n_dof, index.p, item.p, zeros.p, c->stream));
+  auto* H = new monolis_hip_matrix();
+  H->M = M;
+  *out = H;
+  return 0;
+}
+
+int monolis_hip_matrix_get_pattern(const monolis_hip_matrix* H, int* index, int* item) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  return mh::matrix_get_pattern(*H->M, index, item, c->stream);
+}
+
+int monolis_hip_matrix_get_values_R(const monolis_hip_matrix* H, double* A) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  return mh::matrix_get_values(*H->M, A, c->stream);
+}
+
+int monolis_hip_matrix_add_elements_R(monolis_hip_matrix* H, int n_elem, int n_base, const int* elem,
+                                      const double* Ke) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  return mh::matrix_add_elements(H->M, n_elem, n_base, elem, Ke, c->stream);
+}
+
+int monolis_hip_set_Dirichlet_bc_R(monolis_hip_matrix* H, double* b, int n_bc, const int* node_id,
+                                   const int* n_dof_bc, const double* val) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  return mh::matrix_dirichlet(H->M, b, n_bc, node_id, n_dof_bc, val, c->stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -85,6 +85,16 @@ def lib():
         L.orc_diag_nn_apply_C.argtypes = [c_int, c_int, _c128p, _c128p, _c128p]
         L.orc_solve_C.argtypes = [c_int, c_int, _i32p, _i32p, _c128p, _c128p, _c128p, _i32p, _f64p]
         L.orc_solve_C.restype = c_int
+        L.orc_pattern_simple_mesh.argtypes = [c_int, c_int, c_int, _i32p, _i32p, _i32p, ctypes.c_longlong]
+        L.orc_pattern_simple_mesh.restype = c_int
+        L.orc_add_matrix.argtypes = [c_int, _i32p, _i32p, _f64p, c_int, _i32p, _f64p]
+        L.orc_add_matrix.restype = c_int
+        L.orc_add_elements.argtypes = [c_int, _i32p, _i32p, _f64p, c_int, c_int, _i32p, _f64p]
+        L.orc_add_elements.restype = c_int
+        L.orc_dirichlet.argtypes = [c_int, c_int, _i32p, _i32p, _f64p, _f64p, c_int, c_int, c_double]
+        L.orc_dirichlet.restype = c_int
+        L.orc_dirichlet_list.argtypes = [c_int, c_int, _i32p, _i32p, _f64p, _f64p, c_int, _i32p, _i32p, _f64p]
+        L.orc_dirichlet_list.restype = c_int
         _lib = L
     return _lib
 
@@ -308,3 +318,44 @@ def solve_C(mat, b, x0=None, Iarray=None, Rarray=None, **kw):
     x = np.zeros(max(m, 1), np.complex128) if x0 is None else np.array(x0, np.complex128, copy=True)
     st = lib().orc_solve_C(mat.n, mat.ndof, mat.index, mat.item, _z(mat.A), x, _z(b), Iarray, Rarray)
     return x[:m], int(st), Iarray, Rarray
+
+
+# ---------------------------------------------------------------------------
+# Assembly (monolis_oracle_asm.c): pattern of a simple mesh, element matrices,
+# Dirichlet conditions.  Fixed block size, one rank, 0-based node ids.
+# ---------------------------------------------------------------------------
+def pattern_by_simple_mesh(n_node, elems):
+    """(index, item) of monolis_get_nonzero_pattern_by_simple_mesh_R; item 1-based."""
+    e = np.ascontiguousarray(elems, np.int32)
+    n_elem, n_base = (e.shape[0], e.shape[1]) if e.ndim == 2 and e.size else (0, 1)
+    cap = int(n_elem) * n_base * n_base + n_node
+    index = np.zeros(n_node + 1, np.int32)
+    item = np.zeros(max(cap, 1), np.int32)
+    nz = lib().orc_pattern_simple_mesh(int(n_node), int(n_base), int(n_elem), e.reshape(-1) if e.size else
+                                       np.zeros(1, np.int32), index, item, cap)
+    if nz < 0:
+        raise ValueError("invalid connectivity")
+    return index, item[:nz].copy()
+
+
+def add_elements(mat, elems, Ke):
+    """A += element matrices, one monolis_add_matrix_to_sparse_matrix_R call per
+    element in order (row-major (n_base*b)^2 element matrices).  In place on mat.A."""
+    e = np.ascontiguousarray(elems, np.int32)
+    k = np.ascontiguousarray(Ke, np.float64).reshape(-1)
+    st = lib().orc_add_elements(mat.ndof, mat.index, mat.item, mat.A, int(e.shape[0]), int(e.shape[1]),
+                                e.reshape(-1), k)
+    if st:
+        raise KeyError("element couples nodes outside the nonzero pattern")
+    return mat
+
+
+def dirichlet(mat, B, node, dof, val):
+    """Conditions applied one by one (monolis_set_Dirichlet_bc_R); mat.A and B in place."""
+    node = np.ascontiguousarray(node, np.int32)
+    dof = np.ascontiguousarray(dof, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    st = lib().orc_dirichlet_list(mat.n, mat.ndof, mat.index, mat.item, mat.A, B, len(node), node, dof, val)
+    if st:
+        raise KeyError("Dirichlet condition on a node without a diagonal block")
+    return mat, B

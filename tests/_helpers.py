@@ -199,3 +199,26 @@ def dense_C(m):
             c = m.item[k] - 1
             D[i * b:(i + 1) * b, c * b:(c + 1) * b] = m.A[k * b * b:(k + 1) * b * b].reshape(b, b)
     return D
+
+
+def hex_mesh(nx, ny=None, nz=None):
+    """Structured hexahedral mesh: (nx+1)(ny+1)(nz+1) nodes, nx*ny*nz 8-node
+    elements, 0-based node ids, element-major (n_elem, 8)."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    ex, ey, ez = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ex, ey, ez = ex.ravel(), ey.ravel(), ez.ravel()
+    nid = lambda x, y, z: (z * (ny + 1) + y) * (nx + 1) + x
+    corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    elems = np.stack([nid(ex + dx, ey + dy, ez + dz) for dx, dy, dz in corners], axis=1).astype(np.int32)
+    return (nx + 1) * (ny + 1) * (nz + 1), elems
+
+
+def element_matrices(n_elem, nzm, seed, spd=True):
+    """Random element matrices (n_elem, nzm, nzm); symmetric positive
+    semi-definite plus a small diagonal shift when spd."""
+    rng = np.random.default_rng(seed)
+    R = rng.uniform(-1, 1, size=(n_elem, nzm, nzm))
+    if not spd:
+        return R
+    return np.matmul(R, R.transpose(0, 2, 1)) / nzm + 0.01 * np.eye(nzm)

@@ -1,0 +1,188 @@
+"""Device assembly on the GPU against the oracle (monolis_oracle_asm.c): the
+pattern of a simple mesh, batched element matrices (bit-for-bit equal to one
+add-matrix call per element in order), Dirichlet conditions (bit-for-bit for a
+sorted list), value read-back, and an assemble -> condition -> solve pipeline."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from monolis_amd import _lib, api
+from monolis_amd import matrix as mm
+
+from _helpers import element_matrices, hex_mesh
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _init(hip):
+    oracle.set_threads(1)
+    yield
+
+
+def _oracle_mat(n_node, b, elems):
+    index, item = oracle.pattern_by_simple_mesh(n_node, elems)
+    return mm.Matrix(n_node, n_node, b, index, item, np.zeros(len(item) * b * b)).ensure_dof_index()
+
+
+def test_pattern_reference_unit_test():
+    """src_test/matrix/spmat_nzpattern_test.f90:38-73 built on the GPU."""
+    dm = api.device_matrix_by_simple_mesh(4, 2, [[0, 1], [1, 2], [2, 3]])
+    assert dm.mat.index.tolist() == [0, 2, 5, 8, 10]
+    assert dm.mat.item.tolist() == [1, 2, 1, 2, 3, 2, 3, 4, 3, 4]
+    assert not dm.values().any()
+
+
+@pytest.mark.parametrize("case", ["hex10", "tets", "isolated_nodes"])
+def test_pattern_matches_oracle(case):
+    rng = np.random.default_rng(4)
+    if case == "hex10":
+        n_node, elems = hex_mesh(10, 9, 8)
+    elif case == "tets":
+        n_node = 3000
+        elems = np.array([rng.choice(n_node, 4, replace=False) for _ in range(5000)], np.int32)
+        elems[7, 3] = elems[7, 0]                       # degenerate element
+    else:
+        n_node = 500                                     # nodes 400..499 in no element
+        elems = np.array([rng.choice(400, 3, replace=False) for _ in range(300)], np.int32)
+    dm = api.device_matrix_by_simple_mesh(n_node, 3, elems)
+    index, item = oracle.pattern_by_simple_mesh(n_node, elems)
+    assert np.array_equal(dm.mat.index, index) and np.array_equal(dm.mat.item, item)
+
+
+def test_pattern_rejects_bad_node_ids():
+    h = ctypes.c_void_p()
+    e = np.array([[0, 5]], np.int32)
+    rc = _lib.load().monolis_hip_matrix_create_by_simple_mesh(ctypes.byref(h), 4, 2, 1, 1, api._p(e))
+    assert rc == -2 and b"outside" in _lib.load().monolis_hip_last_error()
+
+
+@pytest.mark.parametrize("b", [1, 2, 3])
+def test_element_assembly_bit_identical(b):
+    n_node, elems = hex_mesh(6, 5, 4)
+    Ke = element_matrices(len(elems), 8 * b, seed=b, spd=False)
+    dm = api.device_matrix_by_simple_mesh(n_node, b, elems)
+    # two calls (elements split in two) equal one call per element in order
+    half = len(elems) // 2
+    dm.add_elements(elems[:half], Ke[:half])
+    dm.add_elements(elems[half:], Ke[half:])
+    mo = _oracle_mat(n_node, b, elems)
+    oracle.add_elements(mo, elems, Ke)
+    assert np.array_equal(dm.values(), mo.A)
+    # the assembled device matrix multiplies as the oracle's
+    x = np.random.default_rng(1).uniform(-1, 1, n_node * b)
+    dx, dy = api.DeviceBuffer.from_numpy(x), api.DeviceBuffer(n_node * b)
+    dm.matvec(dx, dy)
+    ref = oracle.matvec(mo, x)
+    assert np.max(np.abs(dy.numpy() - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_element_assembly_large_crosses_chunks_and_device_inputs(monkeypatch):
+    """35,937 hexahedra (2.3 M node pairs, 20.7 M values) in chunks of 2^20
+    node pairs; connectivity and element matrices passed as device buffers."""
+    monkeypatch.setenv("MONOLIS_HIP_ASSEMBLY_CHUNK_PAIRS", str(1 << 20))
+    b = 3
+    n_node, elems = hex_mesh(33)
+    Ke = element_matrices(len(elems), 8 * b, seed=11, spd=True)
+    dm = api.device_matrix_by_simple_mesh(n_node, b, elems)
+    de, dk = api.DeviceBuffer.from_numpy(elems), api.DeviceBuffer.from_numpy(Ke.reshape(-1))
+    rc = _lib.load().monolis_hip_matrix_add_elements_R(ctypes.c_void_p(dm.h), len(elems), 8, ctypes.c_void_p(de.ptr),
+                                                       ctypes.c_void_p(dk.ptr))
+    assert rc == 0, _lib.load().monolis_hip_last_error()
+    mo = _oracle_mat(n_node, b, elems)
+    oracle.add_elements(mo, elems, Ke)
+    assert np.array_equal(dm.values(), mo.A)
+
+
+def test_element_assembly_rejects_block_outside_pattern():
+    dm = api.device_matrix_by_simple_mesh(4, 1, [[0, 1], [1, 2], [2, 3]])
+    with pytest.raises(_lib.MonolisHipError, match="not in the nonzero pattern"):
+        dm.add_elements(np.array([[0, 3]], np.int32), np.ones((1, 2, 2)))
+    with pytest.raises(_lib.MonolisHipError, match="outside"):
+        dm.add_elements(np.array([[0, 9]], np.int32), np.ones((1, 2, 2)))
+
+
+def test_dirichlet_reference_unit_test():
+    """src_test/matrix/spmat_handler_test.f90:119-132 on the device matrix."""
+    dm = api.device_matrix_by_simple_mesh(4, 2, [[0, 1], [1, 2], [2, 3]])
+    dm.set_values(np.full(10 * 4, 2.0))
+    B = np.zeros(8)
+    dm.set_dirichlet(B, [1], [0], [1.0])
+    assert B.tolist() == [-2.0, -2.0, 1.0, -2.0, -2.0, -2.0, 0.0, 0.0]
+    mo = _oracle_mat(4, 2, [[0, 1], [1, 2], [2, 3]])
+    mo.A[:] = 2.0
+    oracle.dirichlet(mo, np.zeros(8), [1], [0], [1.0])
+    assert np.array_equal(dm.values(), mo.A)
+
+
+@pytest.mark.parametrize("b", [1, 3])
+def test_dirichlet_list_bit_identical_to_sequential(b):
+    n_node, elems = hex_mesh(8)
+    Ke = element_matrices(len(elems), 8 * b, seed=5)
+    dm = api.device_matrix_by_simple_mesh(n_node, b, elems)
+    dm.add_elements(elems, Ke)
+    mo = _oracle_mat(n_node, b, elems)
+    oracle.add_elements(mo, elems, Ke)
+    rng = np.random.default_rng(9)
+    f = rng.uniform(-1, 1, n_node * b)
+    nodes = np.sort(rng.choice(n_node, 60, replace=False)).astype(np.int32)
+    nodes = np.repeat(nodes, b)                      # every DOF of those nodes ...
+    dofs = np.tile(np.arange(b, dtype=np.int32), 60)
+    keep = rng.random(len(nodes)) < 0.7              # ... or most of them, sorted by (node, dof)
+    nodes, dofs = nodes[keep], dofs[keep]
+    val = rng.uniform(-1, 1, len(nodes))
+    Bg = f.copy()
+    dm.set_dirichlet(Bg, nodes, dofs, val)
+    Bo = f.copy()
+    oracle.dirichlet(mo, Bo, nodes, dofs, val)
+    assert np.array_equal(dm.values(), mo.A)
+    assert np.array_equal(Bg, Bo)
+
+
+def test_dirichlet_errors():
+    dm = api.device_matrix_by_simple_mesh(4, 2, [[0, 1], [1, 2], [2, 3]])
+    B = np.zeros(8)
+    with pytest.raises(_lib.MonolisHipError, match="twice"):
+        dm.set_dirichlet(B, [1, 1], [0, 0], [1.0, 2.0])
+    with pytest.raises(_lib.MonolisHipError, match="outside"):
+        dm.set_dirichlet(B, [4], [0], [1.0])
+    with pytest.raises(_lib.MonolisHipError, match="outside"):
+        dm.set_dirichlet(B, [0], [2], [1.0])
+    # a row without its diagonal block (user BCSR): monolis stops, here an error
+    m = mm.Matrix(2, 2, 1, np.array([0, 1, 2], np.int32), np.array([2, 1], np.int32),
+                  np.array([1.0, 1.0])).ensure_dof_index()
+    dm2 = api.DeviceMatrix(m)
+    with pytest.raises(_lib.MonolisHipError, match="diagonal block"):
+        dm2.set_dirichlet(np.zeros(2), [0], [0], [1.0])
+
+
+def test_assemble_condition_solve_pipeline():
+    """Hexahedral mesh, 3 DOF per node: pattern, element matrices and Dirichlet
+    conditions on the GPU, then the device CG; against the same pipeline in
+    the oracle."""
+    b = 3
+    n_node, elems = hex_mesh(12)
+    Ke = element_matrices(len(elems), 8 * b, seed=21)
+    dm = api.device_matrix_by_simple_mesh(n_node, b, elems)
+    dm.add_elements(elems, Ke)
+    bottom = np.arange(13 * 13, dtype=np.int32)          # z = 0 face, all DOFs fixed to 0
+    nodes, dofs = np.repeat(bottom, b), np.tile(np.arange(b, dtype=np.int32), len(bottom))
+    f = np.random.default_rng(2).uniform(-1, 1, n_node * b)
+    Bg = f.copy()
+    dm.set_dirichlet(Bg, nodes, dofs, np.zeros(len(nodes)))
+    db, dx = api.DeviceBuffer.from_numpy(Bg), api.DeviceBuffer(n_node * b)
+    prm = api.Prm().quiet()
+    prm.Rarray[1] = 1e-10
+    dm.solve(db, dx, prm)
+    mo = _oracle_mat(n_node, b, elems)
+    oracle.add_elements(mo, elems, Ke)
+    Bo = f.copy()
+    oracle.dirichlet(mo, Bo, nodes, dofs, np.zeros(len(nodes)))
+    xo, st, I, R = oracle.solve(mo, Bo, tol=1e-10)
+    assert prm.Iarray[5] == 0 and st == 0
+    assert abs(int(prm.Iarray[4]) - int(I[4])) <= max(1, int(0.02 * I[4]))
+    x = dx.numpy()
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert not x[bottom[0] * b:(bottom[-1] + 1) * b].any()
