@@ -250,6 +250,26 @@ def main() -> int:
             "cg_fp64_gbs": round(metrics.cg_iter_bytes(m.n, m.nz, 3, 8) * it64 / t64 / 1e9, 1),
         }
 
+        # every real Krylov method on the same system and right-hand side
+        names = {1: "CG", 2: "GropCG", 3: "PipeCG", 4: "PipeCR", 5: "BiCGSTAB", 6: "PipeBiCGSTAB",
+                 7: "BiCGSTAB_noprec", 8: "PipeBiCGSTAB_noprec"}
+        methods = {}
+        for meth, name in names.items():
+            xm = api.DeviceBuffer(m.npndof)
+            dm.solve(db, xm, _prm(method=meth, tol=1e-8))     # warm-up (workspace, factors)
+            pm = _prm(method=meth, tol=1e-8)
+            tm, _ = _timed(lambda: dm.solve(db, xm, pm), reps=1)
+            itm = int(pm.Iarray[4])
+            xmh = xm.numpy()
+            methods[name] = {"method": meth, "status": int(pm.Iarray[5]), "iters": itm, "ms": round(tm * 1e3, 3),
+                             "ms_per_iter": round(tm * 1e3 / max(itm, 1), 4),
+                             "true_resid": float(np.linalg.norm(b - A @ xmh) / np.linalg.norm(b))}
+            log(f"{name}: {itm} iterations, {tm * 1e3:.1f} ms ({tm * 1e3 / max(itm, 1):.3f} ms/iter), "
+                f"true resid {methods[name]['true_resid']:.2e}")
+            del xm
+        out["methods"] = {"workload": "headline system, block-Jacobi (methods 7 and 8: none) to 1e-8, FP64",
+                          **methods}
+
     # --- CPU baseline: the oracle (monolis's CPU loops, OpenMP) on host cores -----
     if not args.no_cpu:
         import oracle

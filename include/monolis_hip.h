@@ -33,12 +33,17 @@ extern "C" {
 /* Replaces monolis_solve_R_c_main, wrapper/solver/monolis_solver_c.h:51-80,
  * defined as the Fortran bind(c) routine wrapper/solver/solver_wrapper.f90:11-92.
  * Runs monolis_solve_main_R (src/solver/solver.f90:77-185) with the method in
- * Iarray[1] (CG = 1, GropCG = 2, PipeCG = 3, BiCGSTAB = 5, BiCGSTAB_noprec = 7)
- * and preconditioner in Iarray[2] (NONE = 0, DIAG = 1) on the GPU.  Fills Iarray[4] (cur_iter), Iarray[5] (status: 0 ok,
+ * Iarray[1] (CG = 1, GropCG = 2, PipeCG = 3, PipeCR = 4, BiCGSTAB = 5,
+ * PipeBiCGSTAB = 6, BiCGSTAB_noprec = 7, PipeBiCGSTAB_noprec = 8) and
+ * preconditioner in Iarray[2] (NONE = 0, DIAG = 1) on the GPU.  Fills Iarray[4] (cur_iter), Iarray[5] (status: 0 ok,
  * 1 not converged, < 0 error), Rarray[2] (cur_resid) and Rarray[3..4]
  * (solution / preparation time, s).  Non-convergence stops the process when
  * Iarray[16] (error_abort) is set and Iarray[13] (measurement) is not,
- * as src/linalg/mat_converge.f90:134-140 does. */
+ * as src/linalg/mat_converge.f90:134-140 does -- except for PipeCR, whose
+ * loop ends silently at max_iter (src/iterative/PipeCR.f90:81-123).  PipeCR
+ * reports cur_iter (the iteration whose test passed) and cur_resid
+ * (||M^-1 r|| / ||M^-1 r0||) although the reference leaves both slots
+ * unwritten for it. */
 void monolis_solve_R_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZNDOF2, int* n_dof_list,
                             int* n_dof_index, int* n_dof_index2, double* A, double* x, double* b, int* index,
                             int* item, int my_rank, int comm, int comm_size, int recv_n_neib, int recv_nitem,

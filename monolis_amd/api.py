@@ -20,7 +20,9 @@ from . import _lib
 from .matrix import Matrix
 
 # ids, src/define/def_solver_prm.f90:7-58 / wrapper/define/monolis_def_solver_prm_c.h
-MONOLIS_ITER_CG, MONOLIS_ITER_BICGSTAB = 1, 5
+MONOLIS_ITER_CG, MONOLIS_ITER_GROPCG, MONOLIS_ITER_PIPECG, MONOLIS_ITER_PIPECR = 1, 2, 3, 4
+MONOLIS_ITER_BICGSTAB, MONOLIS_ITER_PIPEBICGSTAB = 5, 6
+MONOLIS_ITER_BICGSTAB_NOPREC, MONOLIS_ITER_PIPEBICGSTAB_NOPREC = 7, 8
 MONOLIS_ITER_COCG = 13
 MONOLIS_PREC_NONE, MONOLIS_PREC_DIAG, MONOLIS_PREC_SOR, MONOLIS_PREC_LU = 0, 1, 2, 3
 MONOLIS_PRM_I_METHOD, MONOLIS_PRM_I_PRECOND, MONOLIS_PRM_I_MAX_ITER = 1, 2, 3

@@ -77,9 +77,9 @@ mh::SolveParams params_from(const int* Iarray, const double* Rarray) {
 }
 
 int check_method(int method, int precond) {
-  if (method < 1 || method > 8 || method == 4) {
+  if (method < 1 || method > 8) {
     mh::set_error("method %d (%s) is not implemented on the GPU path (CG = 1, GropCG = 2, PipeCG = 3, "
-                  "BiCGSTAB = 5, PipeBiCGSTAB = 6, BiCGSTAB_noprec = 7, PipeBiCGSTAB_noprec = 8)", method,
+                  "PipeCR = 4, BiCGSTAB = 5, PipeBiCGSTAB = 6, BiCGSTAB_noprec = 7, PipeBiCGSTAB_noprec = 8)", method,
                   str_iter(method));
     return ST_METHOD;
   }
@@ -950,8 +950,9 @@ void monolis_solve_R_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZ
   Iarray[I_IERR] = st;
   Rarray[R_TIME_SOL] = now() - t0;
   print_after(Iarray, Rarray, hist, my_rank);
-  if (st == ST_NOT_CONVERGED && !Iarray[I_IS_MEASUREMENT] && abort_on_error)
-    error_stop("reached the maximum number of iterations");  // mat_converge.f90:134-140
+  // mat_converge.f90:134-140; PipeCR has no such stop (PipeCR.f90:81-123 ends the loop silently)
+  if (st == ST_NOT_CONVERGED && !Iarray[I_IS_MEASUREMENT] && abort_on_error && Iarray[I_METHOD] != 4)
+    error_stop("reached the maximum number of iterations");
 }
 
 void monolis_matvec_product_R_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZNDOF2, int* n_dof_list,

@@ -593,6 +593,162 @@ struct OpPrecCheck {
   }
 };
 
+// ---- PipeCR (src/iterative/PipeCR.f90) ---------------------------------------
+// init: U2_0 = (u, u) of u = M^-1 r0; phi = sqrt(R2/U2_0), R2 = (r0, r0) = B2;
+// the test threshold utol = tol/phi (PipeCR.f90:72-79).  KState::aux holds
+// phi, utol, 1/gamma and 1/alpha.
+template <typename T, int B>
+struct OpCrPhi {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 1;
+  const T* __restrict__ u;
+  KState* st;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void row(int, bool valid, double (&part)[1], const WaveRows<T, B>& io) const {
+    T uv[B];
+    io.load(u, uv);
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) part[0] += (double)uv[k] * (double)uv[k];
+    }
+  }
+  __device__ void finalize(const double (&tot)[1]) const {
+    const double phi = sqrt(st->B2 / tot[0]);
+    st->aux[0] = phi;
+    st->aux[1] = st->tol / phi;
+  }
+};
+
+// a: m = M^-1 v (or m given: MGIVEN, the variable-block apply ran before);
+//    gamma = (v, u), delta = (v, m), U2 = (u, u); then the test
+//    sqrt(U2/B2) <= utol and, when it fails, beta = gamma/gamma_old and
+//    alpha = gamma/(delta - beta*gamma/alpha_old) (first: beta = 0,
+//    alpha = gamma/delta), PipeCR.f90:82-105.  cur_resid is reported as
+//    sqrt(U2/B2)*phi.
+template <typename T, int B, bool PREC, bool MGIVEN>
+struct OpCrA {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 3;
+  const T* __restrict__ u;
+  const T* __restrict__ v;
+  T* __restrict__ m;
+  const T* __restrict__ D;
+  int n;
+  KState* st;
+  double* hist;
+  int it;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void row(int i, bool valid, double (&part)[3], const WaveRows<T, B>& io) const {
+    T uv[B], vv[B], mv[B];
+    io.load(u, uv);
+    io.load(v, vv);
+    if constexpr (MGIVEN) {
+      io.load(m, mv);
+    } else {
+#pragma unroll
+      for (int k = 0; k < B; ++k) mv[k] = vv[k];
+      if constexpr (PREC) {
+        if (valid) diag_apply<T, T, B>(D, n, i, mv);
+      }
+      io.store(m, mv);
+    }
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < B; ++k) {
+        part[0] += (double)vv[k] * (double)uv[k];
+        part[1] += (double)vv[k] * (double)mv[k];
+        part[2] += (double)uv[k] * (double)uv[k];
+      }
+    }
+  }
+  __device__ void finalize(const double (&tot)[3]) const {
+    const double gamma = tot[0], delta = tot[1], U2 = tot[2];
+    const double resid = sqrt(U2 / st->B2);
+    const double phi = st->aux[0];
+    st->iter = it;
+    st->rr = U2;
+    st->resid = resid * phi;
+    if (hist) hist[it] = resid * phi;
+    if (resid <= st->aux[1]) {
+      st->converged = 1;
+      st->done = 1;
+      return;
+    }
+    double alpha, beta;
+    if (it > 1) {
+      beta = gamma * st->aux[2];
+      alpha = gamma / (delta - beta * gamma * st->aux[3]);
+    } else {
+      beta = 0.0;
+      alpha = gamma / delta;
+    }
+    st->gamma = gamma;
+    st->delta = delta;
+    st->alpha = alpha;
+    st->beta = beta;
+    st->aux[2] = 1.0 / gamma;  // gamma1, PipeCR.f90:121
+    st->aux[3] = 1.0 / alpha;  // alpha1, PipeCR.f90:122
+  }
+};
+
+// b: z = beta z + l; q = beta q + m; p = beta p + u; x += alpha p; then
+//    u -= alpha q; v -= alpha z, or x only (F_XONLY: the residual-replacement
+//    iteration), PipeCR.f90:107-119.
+template <typename T, int B>
+struct OpCrB {
+  using VT = T;
+  static constexpr int BS = B;
+  static constexpr int NV = 0;
+  T *__restrict__ z, *__restrict__ q, *__restrict__ p;
+  T *__restrict__ u, *__restrict__ v, *__restrict__ x;
+  const T* __restrict__ l;
+  const T* __restrict__ m;
+  KState* st;
+  int flags;
+  T alpha, beta;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {
+    alpha = (T)st->alpha;
+    beta = (T)st->beta;
+  }
+  __device__ void row(int, bool, double (&)[1], const WaveRows<T, B>& io) const {
+    T zv[B], lv[B], qv[B], mv[B], pv[B], uv[B], xv[B];
+    io.load(z, zv);
+    io.load(l, lv);
+    io.load(q, qv);
+    io.load(m, mv);
+    io.load(p, pv);
+    io.load(u, uv);
+    io.load(x, xv);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      zv[k] = fma(beta, zv[k], lv[k]);   // Z = beta*Z + L
+      qv[k] = fma(beta, qv[k], mv[k]);   // Q = beta*Q + M
+      pv[k] = fma(beta, pv[k], uv[k]);   // P = beta*P + U
+      xv[k] = fma(alpha, pv[k], xv[k]);  // X = alpha*P + X
+    }
+    io.store(z, zv);
+    io.store(q, qv);
+    io.store(p, pv);
+    io.store(x, xv);
+    if (flags & F_XONLY) return;
+    T vv[B];
+    io.load(v, vv);
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      uv[k] = fma(-alpha, qv[k], uv[k]);  // U = -alpha*Q + U
+      vv[k] = fma(-alpha, zv[k], vv[k]);  // V = -alpha*Z + V
+    }
+    io.store(u, uv);
+    io.store(v, vv);
+  }
+  __device__ void finalize(const double (&)[1]) const {}
+};
+
 // ---- GropCG (src/iterative/GropCG.f90) ---------------------------------------
 // init: u = M^-1 r; gamma = (r, u)
 template <typename T, int B, bool PREC>
@@ -1283,6 +1439,131 @@ int pipecg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W,
   return 0;
 }
 
+// PipeCR (monolis method 4, src/iterative/PipeCR.f90): three fused launches per
+// iteration -- m = M^-1 v with (v, u), (v, m), (u, u) and the test on the
+// preconditioned residual; l = A m; the five vector recurrences -- and the
+// true residual every 50 iterations (PipeCR.f90 iter_RR = 50).
+template <typename T, int B, bool PREC>
+int pipecr_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, SolveResult* out,
+                hipStream_t s) {
+  constexpr int kPipeRR = 50;
+  Context* c = nullptr;
+  MH_TRY(context(&c));
+  const size_t vec = (size_t)M.np * B;
+  MH_TRY(W.ensure(cg_bytes<T>(M, 8, P.max_iter)));
+  Carve cv{W.raw.p};
+  T* r = cv.take<T>(vec);
+  T* u = cv.take<T>(vec);
+  T* v = cv.take<T>(vec);
+  T* q = cv.take<T>(vec);
+  T* p = cv.take<T>(vec);
+  T* z = cv.take<T>(vec);
+  T* l = cv.take<T>(vec);
+  T* mm = cv.take<T>(vec);
+  const size_t nparts = partials_needed((long long)M.n) * 4 + partials_needed((long long)M.nslices * kWave) * 4;
+  double* parts = cv.take<double>(nparts);
+  KState* st = cv.take<KState>(1);
+  double* hist = cv.take<double>((size_t)P.max_iter + 2);
+  const T* D = nullptr;
+  if constexpr (PREC) D = (sizeof(T) == 8) ? (const T*)M.D64.p : (const T*)M.D32.p;
+  const int n = M.n;
+  const size_t m = (size_t)n * B;
+  const int var = sizeof(T) == 8 ? M.variant64 : M.variant32;
+  double* hp = P.keep_hist ? hist : nullptr;
+  const bool vprec = PREC && B == 1 && M.vb != nullptr;
+  auto vapply = [&](const T* src, T* dst) -> int {
+    if constexpr (PREC && B == 1) {
+      const VBlock& V = *M.vb;
+      const T* DV = sizeof(T) == 8 ? (const T*)V.D64.p : (const T*)V.D32.p;
+      OpVPrec<T, false> vp{src, dst, V.list.p, V.dof_index.p, V.doff.p, DV, st, 0};
+      return launch_rows(V.nb, vp, nullptr, s);
+    }
+    (void)src;
+    (void)dst;
+    return 0;
+  };
+  // u = M^-1 r
+  auto prec = [&]() -> int {
+    if (vprec) {
+      OpPrecCheck<T, B, false> op{r, u, nullptr, n, st, nullptr, 0, 0};
+      MH_TRY(launch_rows(n, op, parts, s));
+      return vapply(r, u);
+    }
+    OpPrecCheck<T, B, PREC> op{r, u, D, n, st, nullptr, 0, 0};
+    return launch_rows(n, op, parts, s);
+  };
+
+  k_state_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
+  MH_HIP(hipGetLastError());
+  if (P.init_x) MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+  for (T* w : {q, p, z, l, mm, u, v}) MH_HIP(hipMemsetAsync(w, 0, sizeof(T) * vec, s));
+  {
+    OpResidSolve<T, B> op{x, b, r, nullptr, &st->B2, st};
+    MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
+  }
+  double B2 = 0.0;
+  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  out->hist.clear();
+  if (B2 == 0.0) {  // as mat_converge.f90:40-48 for the other methods (PipeCR.f90 itself divides 0/0)
+    MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
+    MH_HIP(hipStreamSynchronize(s));
+    out->zero_rhs = 1;
+    out->converged = 1;
+    out->iter = -1;
+    out->resid = 0.0;
+    return 0;
+  }
+  // U = M^-1 R; phi = sqrt(R2/U2); V = A U
+  MH_TRY(prec());
+  {
+    OpCrPhi<T, B> ph{u, st};
+    MH_TRY(launch_rows(n, ph, parts, s));
+    OpSpmvSolve<T, B> sp{u, v, st};
+    MH_TRY((launch_sell<T, B>(M, sp, parts, var, s)));
+  }
+  Poller poll{c, &st->done};
+  int done = 0;
+  for (int it = 1; it <= P.max_iter && !done;) {
+    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    for (; it <= end; ++it) {
+      if (vprec) {
+        MH_TRY(vapply(v, mm));
+        OpCrA<T, B, false, true> a{u, v, mm, nullptr, n, st, hp, it};
+        MH_TRY(launch_rows(n, a, parts, s));
+      } else {
+        OpCrA<T, B, PREC, false> a{u, v, mm, D, n, st, hp, it};
+        MH_TRY(launch_rows(n, a, parts, s));
+      }
+      OpSpmvSolve<T, B> sl{mm, l, st};
+      MH_TRY((launch_sell<T, B>(M, sl, parts, var, s)));
+      const bool rr = it % kPipeRR == 0;
+      OpCrB<T, B> ub{z, q, p, u, v, x, l, mm, st, rr ? F_XONLY : 0, (T)0, (T)0};
+      MH_TRY(launch_rows(n, ub, parts, s));
+      if (rr) {
+        OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
+        MH_TRY((launch_sell<T, B>(M, rs, parts, var, s)));
+        MH_TRY(prec());
+        OpSpmvSolve<T, B> sp{u, v, st};
+        MH_TRY((launch_sell<T, B>(M, sp, parts, var, s)));
+      }
+    }
+    MH_TRY(poll.issue(s));
+    MH_TRY(poll.check(1, &done));
+  }
+  MH_HIP(hipStreamSynchronize(s));
+  KState hs;
+  MH_HIP(hipMemcpy(&hs, st, sizeof(KState), hipMemcpyDeviceToHost));
+  out->iter = hs.iter;
+  out->resid = hs.resid;
+  out->converged = hs.converged;
+  if (P.keep_hist && hs.iter > 0) {
+    out->hist.resize((size_t)hs.iter + 1);
+    MH_HIP(hipMemcpy(out->hist.data(), hist, sizeof(double) * ((size_t)hs.iter + 1), hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
 // PipeBiCGSTAB (monolis method 6; method 8 = no preconditioner): per iteration
 // the recurrence pass with (Q, Y), (Y, Y) and ZT = M^-1 Z; V = A ZT; the update
 // pass with five dots, WT = M^-1 W0 and the test; T = A WT.  True residual
@@ -1593,8 +1874,8 @@ int krylov_solve(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W
     set_error("preconditioner %d is not implemented on the GPU path (NONE=0, DIAG=1)", P.precond);
     return -3;
   }
-  if (P.method < 1 || P.method > 8 || P.method == 4) {
-    set_error("method %d is not implemented on the GPU path (CG=1, GropCG=2, PipeCG=3, BiCGSTAB=5, "
+  if (P.method < 1 || P.method > 8) {
+    set_error("method %d is not implemented on the GPU path (CG=1, GropCG=2, PipeCG=3, PipeCR=4, BiCGSTAB=5, "
               "PipeBiCGSTAB=6, BiCGSTAB_noprec=7, PipeBiCGSTAB_noprec=8)", P.method);
     return -4;
   }
@@ -1623,6 +1904,12 @@ int krylov_solve(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W
       MH_DISPATCH_B(M.b, rc = (pipecg_impl<T, B, true>(M, P, b, x, W, out, s)));
     } else {
       MH_DISPATCH_B(M.b, rc = (pipecg_impl<T, B, false>(M, P, b, x, W, out, s)));
+    }
+  } else if (P.method == 4) {
+    if (P.precond == 1) {
+      MH_DISPATCH_B(M.b, rc = (pipecr_impl<T, B, true>(M, P, b, x, W, out, s)));
+    } else {
+      MH_DISPATCH_B(M.b, rc = (pipecr_impl<T, B, false>(M, P, b, x, W, out, s)));
     }
   } else if (P.method == 6) {
     if (P.precond == 1) {

@@ -14,7 +14,13 @@ module mod_monolis_hip
   implicit none
 
   integer(c_int), parameter :: monolis_hip_iter_CG = 1
+  integer(c_int), parameter :: monolis_hip_iter_GropCG = 2
+  integer(c_int), parameter :: monolis_hip_iter_PipeCG = 3
+  integer(c_int), parameter :: monolis_hip_iter_PipeCR = 4
   integer(c_int), parameter :: monolis_hip_iter_BiCGSTAB = 5
+  integer(c_int), parameter :: monolis_hip_iter_PipeBiCGSTAB = 6
+  integer(c_int), parameter :: monolis_hip_iter_BiCGSTAB_noprec = 7
+  integer(c_int), parameter :: monolis_hip_iter_PipeBiCGSTAB_noprec = 8
   integer(c_int), parameter :: monolis_hip_iter_COCG = 13
   integer(c_int), parameter :: monolis_hip_prec_NONE = 0
   integer(c_int), parameter :: monolis_hip_prec_DIAG = 1

@@ -35,6 +35,7 @@ I_IS_INIT_X, I_IS_ERROR_ABORT = 10, 16
 R_TOL, R_CUR_RESID = 1, 2
 ITER_CG, ITER_GROPCG, ITER_PIPECG, ITER_BICGSTAB, ITER_PIPEBICGSTAB, ITER_BICGSTAB_NOPREC = 1, 2, 3, 5, 6, 7
 ITER_PIPEBICGSTAB_NOPREC = 8
+ITER_PIPECR = 4
 ITER_COCG = 13
 ZERO_DIAG = -6   # status of orc_solve_C when a diagonal block has a zero pivot
 PREC_NONE, PREC_DIAG = 0, 1

@@ -39,6 +39,7 @@
 #define ORC_ITER_CG 1
 #define ORC_ITER_GROPCG 2
 #define ORC_ITER_PIPECG 3
+#define ORC_ITER_PIPECR 4
 #define ORC_ITER_BICGSTAB 5
 #define ORC_ITER_PIPEBICGSTAB 6
 #define ORC_ITER_BICGSTAB_NOPREC 7
@@ -587,6 +588,83 @@ done:
   return status;
 }
 
+/* src/iterative/PipeCR.f90:18-146 (monolis method 4): pipelined preconditioned
+ * conjugate residuals, three dots per iteration -- gamma = (v, u),
+ * delta = (v, m), U2 = (u, u) -- and the true residual every 50 iterations.
+ * The test is on the preconditioned residual, before the updates of the
+ * iteration: sqrt(U2/B2) <= tol/phi with phi = sqrt(R2/U2_0) (PipeCR.f90:72-97).
+ * The reference writes neither cur_iter nor cur_resid (no
+ * monolis_check_converge call); here they hold the loop counter at the test
+ * and resid*phi, the quantity of the commented log line PipeCR.f90:96.
+ * Reaching max_iter is silent in the reference (ierr stays as it was); here
+ * it returns ORC_NOT_CONVERGED like the other methods.  A zero initial
+ * residual makes the reference divide 0/0 (phi, then alpha) and return NaN
+ * in x; here x = 0 as mat_converge.f90:40-48 does for the other methods. */
+static int orc_pipecr(const orc_ctx* c, const double* B, double* X, int* Iarray,
+                      double* Rarray) {
+  const int m = c->m;
+  const int iter_RR = 50;
+  if (Iarray[I_IS_INIT_X]) memset(X, 0, sizeof(double) * (size_t)m);
+  double* V8[8];
+  for (int k = 0; k < 8; ++k) V8[k] = (double*)calloc((size_t)m + 1, sizeof(double));
+  double *R = V8[0], *U = V8[1], *V = V8[2], *Q = V8[3], *P = V8[4], *Z = V8[5], *L = V8[6],
+         *M = V8[7];
+  int status = ORC_NOT_CONVERGED;
+  ctx_residual(c, X, B, R);
+  const double B2 = orc_inner_product(m, R, R);
+  const double R2 = orc_inner_product(m, R, R);
+  if (B2 == 0.0) {
+    memset(X, 0, sizeof(double) * (size_t)m);
+    status = ORC_OK;
+    goto done;
+  }
+  ctx_prec_apply(c, R, U);
+  double U2 = orc_inner_product(m, U, U);
+  ctx_matvec(c, U, V);
+  const double phi = sqrt(R2 / U2);
+  const double utol = Rarray[R_TOL] / phi;
+  double gamma1 = 0.0, alpha1 = 0.0;
+  for (int iter = 1; iter <= Iarray[I_MAX_ITER]; ++iter) {
+    ctx_prec_apply(c, V, M);
+    const double gamma = orc_inner_product(m, V, U);
+    const double delta = orc_inner_product(m, V, M);
+    U2 = orc_inner_product(m, U, U);
+    ctx_matvec(c, M, L);
+    const double resid = sqrt(U2 / B2);
+    Iarray[I_CUR_ITER] = iter;
+    Rarray[R_CUR_RESID] = resid * phi;
+    if (resid <= utol) {
+      status = ORC_OK;
+      break;
+    }
+    double alpha, beta;
+    if (1 < iter) {
+      beta = gamma * gamma1;
+      alpha = gamma / (delta - beta * gamma * alpha1);
+    } else {
+      beta = 0.0;
+      alpha = gamma / delta;
+    }
+    orc_vec_axpby(m, beta, Z, 1.0, L, Z);
+    orc_vec_axpby(m, beta, Q, 1.0, M, Q);
+    orc_vec_axpby(m, beta, P, 1.0, U, P);
+    orc_vec_axpby(m, alpha, P, 1.0, X, X);
+    if (iter % iter_RR == 0) {
+      ctx_residual(c, X, B, R);
+      ctx_prec_apply(c, R, U);
+      ctx_matvec(c, U, V);
+    } else {
+      orc_vec_axpby(m, -alpha, Q, 1.0, U, U);
+      orc_vec_axpby(m, -alpha, Z, 1.0, V, V);
+    }
+    gamma1 = 1.0 / gamma;
+    alpha1 = 1.0 / alpha;
+  }
+done:
+  for (int k = 0; k < 8; ++k) free(V8[k]);
+  return status;
+}
+
 /* src/iterative/BiCGSTAB.f90:18-192 (right preconditioned). */
 /* noprec = 1: src/iterative/BiCGSTAB_noprec.f90 (method 7): no preconditioner
  * (whatever Iarray[2] says), no sorted-sum fallback, omega = (t,s)/(t,t) as is,
@@ -819,6 +897,8 @@ int orc_solve(int n, int np, int nz, int n_dof, const int* n_dof_list,
     st = orc_gropcg(&c, b, x, Iarray, Rarray);
   else if (Iarray[I_METHOD] == ORC_ITER_PIPECG)
     st = orc_pipecg(&c, b, x, Iarray, Rarray);
+  else if (Iarray[I_METHOD] == ORC_ITER_PIPECR)
+    st = orc_pipecr(&c, b, x, Iarray, Rarray);
   else if (Iarray[I_METHOD] == ORC_ITER_BICGSTAB)
     st = orc_bicgstab(&c, b, x, Iarray, Rarray, 0);
   else if (Iarray[I_METHOD] == ORC_ITER_BICGSTAB_NOPREC)
