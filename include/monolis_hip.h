@@ -313,6 +313,17 @@ int monolis_hip_set_Dirichlet_bc_R(monolis_hip_matrix* M, double* b, int n_bc, c
 typedef struct monolis_hip_matrix_C monolis_hip_matrix_C;
 int monolis_hip_matrix_create_C(monolis_hip_matrix_C** out, int n, int np, int nz, int n_dof, const int* index,
                                 const int* item, const double _Complex* A);
+/* Variable block sizes (n_dof = -1) for complex values: the arrays as
+ * monolis_hip_matrix_create_V takes them, A of n_dof_index2[nz] complex
+ * entries.  Replaces the complex variable-block path of
+ * monolis_matvec_product_main_C (src/linalg/matvec.f90:218-220, matvec_V_C
+ * :293-351) and of the DIAG preconditioner (src/prec/diag.f90:53-57,
+ * diag_V.f90:88-158 setup, :208-250 apply).  Blocks of up to 16 DOFs.  A zero
+ * pivot in a diagonal block is reported (status -6) by the first solve with
+ * the DIAG preconditioner, where the reference stops. */
+int monolis_hip_matrix_create_V_C(monolis_hip_matrix_C** out, int n, int np, int nz, const int* n_dof_list,
+                                  const int* n_dof_index, const int* n_dof_index2, const int* index, const int* item,
+                                  const double _Complex* A);
 int monolis_hip_matrix_destroy_C(monolis_hip_matrix_C* M);
 int monolis_hip_matrix_set_values_C(monolis_hip_matrix_C* M, const double _Complex* A);
 /* info[0..7] as monolis_hip_matrix_info, value bytes at 16 per entry (info[7] = 0). */

@@ -31,6 +31,8 @@ SIGNATURES = [
     ("monolis_inner_product_C_c_main", None, [c_int, c_void_p, c_void_p, c_void_p, c_int]),
     ("monolis_hip_matrix_create_C", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int,
                                             c_void_p, c_void_p, c_void_p]),
+    ("monolis_hip_matrix_create_V_C", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_void_p, c_void_p]),
     ("monolis_hip_matrix_destroy_C", c_int, [c_void_p]),
     ("monolis_hip_matrix_set_values_C", c_int, [c_void_p, c_void_p]),
     ("monolis_hip_matrix_info_C", c_int, [c_void_p, c_ll_p]),

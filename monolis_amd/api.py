@@ -461,9 +461,16 @@ class DeviceMatrixC:
         h = ctypes.c_void_p()
         mat.ensure_dof_index()
         A = np.ascontiguousarray(mat.A, np.complex128)
-        _lib.check(L.monolis_hip_matrix_create_C(ctypes.byref(h), mat.n, mat.np_, mat.nz, mat.ndof,
-                                                 _p(mat.index), _p(mat.item), _p(A)),
-                   "monolis_hip_matrix_create_C")
+        if mat.ndof == -1:
+            _lib.check(L.monolis_hip_matrix_create_V_C(
+                ctypes.byref(h), mat.n, mat.np_, mat.nz, _p(np.ascontiguousarray(mat.n_dof_list, np.int32)),
+                _p(np.ascontiguousarray(mat.n_dof_index, np.int32)),
+                _p(np.ascontiguousarray(mat.n_dof_index2, np.int32)), _p(mat.index), _p(mat.item), _p(A)),
+                "monolis_hip_matrix_create_V_C")
+        else:
+            _lib.check(L.monolis_hip_matrix_create_C(ctypes.byref(h), mat.n, mat.np_, mat.nz, mat.ndof,
+                                                     _p(mat.index), _p(mat.item), _p(A)),
+                       "monolis_hip_matrix_create_C")
         self.h = h.value
         self.mat = mat
         self.m = mat.nndof

@@ -1072,12 +1072,31 @@ int monolis_hip_matrix_create_C(monolis_hip_matrix_C** out, int n, int np, int n
   }
   *out = nullptr;
   if (n_dof == -1) {
-    mh::set_error("variable block sizes (n_dof = -1) are not supported on the complex GPU path");
+    mh::set_error("variable block sizes (n_dof = -1) need monolis_hip_matrix_create_V_C");
     return ST_ARGS;
   }
   Matrix* M = nullptr;
   MH_TRY(mh::matrix_create_complex(&M, n, np, nz, n_dof, index, item, reinterpret_cast<const double*>(A),
                                    c->stream));
+  auto* H = new monolis_hip_matrix_C();
+  H->M = M;
+  *out = H;
+  return 0;
+}
+
+int monolis_hip_matrix_create_V_C(monolis_hip_matrix_C** out, int n, int np, int nz, const int* n_dof_list,
+                                  const int* n_dof_index, const int* n_dof_index2, const int* index, const int* item,
+                                  const double _Complex* A) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (!out) {
+    mh::set_error("null output handle");
+    return ST_ARGS;
+  }
+  *out = nullptr;
+  Matrix* M = nullptr;
+  MH_TRY(mh::matrix_create_vblock_complex(&M, n, np, nz, n_dof_list, n_dof_index, n_dof_index2, index, item,
+                                          reinterpret_cast<const mh::zd*>(A), c->stream));
   auto* H = new monolis_hip_matrix_C();
   H->M = M;
   *out = H;
@@ -1095,7 +1114,10 @@ int monolis_hip_matrix_destroy_C(monolis_hip_matrix_C* H) {
 int monolis_hip_matrix_set_values_C(monolis_hip_matrix_C* H, const double _Complex* A) {
   Context* c = nullptr;
   MH_TRY(mh::context(&c));
-  MH_TRY(mh::matrix_set_values(H->M, reinterpret_cast<const double*>(A), c->stream));
+  if (H->M->vb)
+    MH_TRY(mh::matrix_set_values_vblock_complex(H->M, reinterpret_cast<const mh::zd*>(A), c->stream));
+  else
+    MH_TRY(mh::matrix_set_values(H->M, reinterpret_cast<const double*>(A), c->stream));
   MH_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1194,7 +1216,7 @@ void monolis_solve_C_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZ
                             int recv_n_neib, int recv_nitem, int* recv_neib_pe, int* recv_index, int* recv_item,
                             int send_n_neib, int send_nitem, int* send_neib_pe, int* send_index, int* send_item,
                             int* Iarray, double* Rarray) {
-  (void)NZNDOF2; (void)comm; (void)n_dof_list; (void)n_dof_index; (void)n_dof_index2;
+  (void)NZNDOF2; (void)comm;
   (void)recv_n_neib; (void)recv_nitem; (void)recv_neib_pe; (void)recv_index; (void)recv_item;
   (void)send_n_neib; (void)send_nitem; (void)send_neib_pe; (void)send_index; (void)send_item;
   const double t0 = now();
@@ -1232,7 +1254,9 @@ void monolis_solve_C_c_main(int n, int np, int nz, int n_dof, int NPNDOF, int NZ
   }
   const double tp0 = now();
   monolis_hip_matrix_C* H = nullptr;
-  int rc = monolis_hip_matrix_create_C(&H, n, np, nz, n_dof, index, item, A);
+  int rc = n_dof == -1 ? monolis_hip_matrix_create_V_C(&H, n, np, nz, n_dof_list, n_dof_index, n_dof_index2, index,
+                                                       item, A)
+                       : monolis_hip_matrix_create_C(&H, n, np, nz, n_dof, index, item, A);
   if (rc) {
     fail(rc == -2 ? ST_ARGS : ST_HIP);
     return;
@@ -1297,14 +1321,17 @@ void monolis_matvec_product_C_c_main(int n, int np, int nz, int n_dof, int NPNDO
                                      int comm_size, int recv_n_neib, int recv_nitem, int* recv_neib_pe,
                                      int* recv_index, int* recv_item, int send_n_neib, int send_nitem,
                                      int* send_neib_pe, int* send_index, int* send_item) {
-  (void)NZNDOF2; (void)my_rank; (void)comm; (void)n_dof_list; (void)n_dof_index; (void)n_dof_index2;
+  (void)NZNDOF2; (void)my_rank; (void)comm;
   (void)recv_n_neib; (void)recv_nitem; (void)recv_neib_pe; (void)recv_index; (void)recv_item;
   (void)send_n_neib; (void)send_nitem; (void)send_neib_pe; (void)send_index; (void)send_item;
   if (comm_size > 1) error_stop("monolis_matvec_product_C (GPU path): comm_size > 1 is not supported");
   Context* c = nullptr;
   if (mh::context(&c)) error_stop(mh::last_error());
   monolis_hip_matrix_C* H = nullptr;
-  if (monolis_hip_matrix_create_C(&H, n, np, nz, n_dof, index, item, A)) error_stop(mh::last_error());
+  const int rc = n_dof == -1 ? monolis_hip_matrix_create_V_C(&H, n, np, nz, n_dof_list, n_dof_index, n_dof_index2,
+                                                             index, item, A)
+                             : monolis_hip_matrix_create_C(&H, n, np, nz, n_dof, index, item, A);
+  if (rc) error_stop(mh::last_error());
   std::unique_ptr<monolis_hip_matrix_C> guard(H);
   const size_t vec = (size_t)NPNDOF;
   const size_t m = (size_t)H->M->n * H->M->b;

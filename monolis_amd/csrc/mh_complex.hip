@@ -21,7 +21,7 @@ namespace mh {
 namespace {
 
 constexpr int kIterRR = 200;  // COCG.f90: iter_RR = 200
-enum : int { F_UPD = 1, F_CHECK = 2, F_FIRST = 4, F_XONLY = 8 };
+enum : int { F_UPD = 1, F_CHECK = 2, F_FIRST = 4, F_XONLY = 8, F_EXTPREC = 16 };
 
 // part[k] + i part[k+1] += a * b
 template <int NV>
@@ -240,6 +240,7 @@ struct OpZSpmvPQ {
 };
 
 // ---- COCG: x += alpha p; r -= alpha q; test; z = M^-1 r; rho = (r, z); beta ----
+// F_EXTPREC: z, rho and beta come from OpZVPrec (variable blocks) instead.
 template <int B, bool PREC>
 struct OpZUpdate {
   using VT = zd;
@@ -278,6 +279,13 @@ struct OpZUpdate {
       for (int k = 0; k < B; ++k) v[k] = (-alpha) * qv[k] + v[k];  // R = -alpha*Q + R  (COCG.f90:92)
       io.store(r, v);
     }
+    if (flags & F_EXTPREC) {
+      if (valid) {
+#pragma unroll
+        for (int k = 0; k < B; ++k) zacc(part, 0, v[k], v[k]);
+      }
+      return;
+    }
     zd w[B];
 #pragma unroll
     for (int k = 0; k < B; ++k) w[k] = v[k];
@@ -296,8 +304,50 @@ struct OpZUpdate {
     if (flags & F_CHECK) {
       if (zfinalize_check(st, hist, it, zd(tot[0], tot[1]))) return;
     }
+    if (flags & F_EXTPREC) return;
     const zd rho(tot[2], tot[3]);
     st->beta = (flags & F_FIRST) ? zd(0.0) : zdiv(rho, st->rho);
+    st->rho = rho;
+  }
+};
+
+// ---- variable blocks: z = M_V^-1 r per block row (diag_V.f90:208-250), rho = (r, z),
+// beta = rho / rho_old (first: 0) ------------------------------------------------
+struct OpZVPrec {
+  using VT = zd;
+  static constexpr int BS = 1;
+  static constexpr int NV = 2;
+  const zd* __restrict__ r;
+  zd* __restrict__ z;
+  const int* __restrict__ list;
+  const int* __restrict__ dof_index;
+  const long long* __restrict__ doff;
+  const zd* __restrict__ D;
+  ZState* st;
+  int first;
+  __device__ bool skip() const { return st->done != 0; }
+  __device__ void begin() {}
+  __device__ void row(int i, bool valid, double (&part)[2], const WaveRows<zd, 1>&) const {
+    if (!valid) return;
+    const int nd = list[i];
+    const int o = dof_index[i];
+    const zd* L = D + doff[i];
+    zd t[kMaxVBlock];
+    for (int j = 0; j < nd; ++j) t[j] = r[o + j];
+    for (int j = 1; j < nd; ++j)
+      for (int k = 0; k < j; ++k) t[j] = t[j] - L[nd * j + k] * t[k];
+    for (int j = nd - 1; j >= 0; --j) {
+      for (int k = nd - 1; k > j; --k) t[j] = t[j] - L[nd * j + k] * t[k];
+      t[j] = L[(nd + 1) * j] * t[j];
+    }
+    for (int j = 0; j < nd; ++j) {
+      z[o + j] = t[j];
+      zacc(part, 0, r[o + j], t[j]);
+    }
+  }
+  __device__ void finalize(const double (&tot)[2]) const {
+    const zd rho(tot[0], tot[1]);
+    st->beta = first ? zd(0.0) : zdiv(rho, st->rho);
     st->rho = rho;
   }
 };
@@ -368,6 +418,7 @@ int cocg_impl(Matrix& M, const SolveParams& P, const zd* b, zd* x, Workspace& W,
   const zd* D = PREC ? M.Dz.p : nullptr;
   const int n = M.n;
   const size_t m = (size_t)n * B;
+  const bool vprec = PREC && B == 1 && M.vb != nullptr;  // variable blocks: diag_V_C per block row
 
   k_zstate_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
   MH_HIP(hipGetLastError());
@@ -391,7 +442,20 @@ int cocg_impl(Matrix& M, const SolveParams& P, const zd* b, zd* x, Workspace& W,
     out->iter = -1;
     return 0;
   }
+  auto vprec_apply = [&](int first) -> int {
+    const VBlock& V = *M.vb;
+    OpZVPrec op{r, z, V.list.p, V.dof_index.p, V.doff.p, V.Dz.p, st, first};
+    return launch_rows(V.nb, op, parts, s);
+  };
   auto upd = [&](int it, int flags) -> int {
+    if (vprec) {
+      if (flags & (F_UPD | F_XONLY | F_CHECK)) {
+        OpZUpdate<B, false> op{x, r, z, p, q, nullptr, n, st, P.keep_hist ? hist : nullptr, it, flags | F_EXTPREC,
+                               zd(0.0)};
+        MH_TRY(launch_rows(n, op, parts, s));
+      }
+      return (flags & F_XONLY) ? 0 : vprec_apply((flags & F_FIRST) != 0);
+    }
     OpZUpdate<B, PREC> op{x, r, z, p, q, D, n, st, P.keep_hist ? hist : nullptr, it, flags, zd(0.0)};
     return launch_rows(n, op, parts, s);
   };
@@ -461,6 +525,15 @@ int zdot(long long m, const zd* x, const zd* y, double* out, double* partials, h
 
 int zdiag_setup(Matrix& M, hipStream_t s) {
   MH_TRY(require_complex(M));
+  if (M.vb) {  // the diag_V_C factors are built with the values
+    if (M.vb->zbad >= 0) {
+      set_error("zero diag in monolis_precond_diag_V_setup_C: zero pivot in the diagonal block of block row %d "
+                "(0-based)", M.vb->zbad);
+      return -6;
+    }
+    M.diagz_ready = true;
+    return 0;
+  }
   if (M.diagz_ready) return 0;
   const size_t need = (size_t)M.n * M.b * M.b;
   if (M.Dz.n < need) MH_TRY(M.Dz.alloc(need));

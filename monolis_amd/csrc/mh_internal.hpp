@@ -89,6 +89,8 @@ struct VBlock {
   DevArray<double> D64;                    // diag_V factors (no-pivot LU, reciprocal pivots)
   DevArray<float> D32;
   bool d32_ready = false;
+  DevArray<zd> Dz;                         // complex matrices: diag_V_C factors
+  int zbad = -1;                           // first block row whose complex factor hit a zero pivot, or -1
 };
 
 // ---------------------------------------------------------------------------
@@ -143,6 +145,12 @@ int matrix_create_vblock(Matrix** out, int n, int np, int nz, const int* n_dof_l
                          const int* n_dof_index2, const int* index, const int* item, const double* A,
                          hipStream_t s);
 int matrix_set_values_vblock(Matrix* M, const double* A, hipStream_t s);
+// The same for complex values (monolis_matvec_V_C / diag_V_*_C); the diag_V_C
+// factors are built with the values, a zero pivot recorded in VBlock::zbad.
+int matrix_create_vblock_complex(Matrix** out, int n, int np, int nz, const int* n_dof_list, const int* n_dof_index,
+                                 const int* n_dof_index2, const int* index, const int* item, const zd* A,
+                                 hipStream_t s);
+int matrix_set_values_vblock_complex(Matrix* M, const zd* A, hipStream_t s);
 // FP32 copy of the diag_V factors (the FP64 ones are built with the values).
 int vdiag_make_fp32(Matrix& M, hipStream_t s);
 // y = M_V^-1 x (diag_V apply) over all block rows.

@@ -14,7 +14,11 @@
 // The block-Jacobi preconditioner keeps the blocks: diag_V
 // (src/prec/diag/diag_V.f90:13-86 setup, :177-204 apply) factors each
 // diagonal block with the no-pivot LU of the fixed-size paths (reciprocal
-// pivots, zero pivot -> 1) and applies it per block row.
+// pivots, zero pivot -> 1) and applies it per block row.  Complex values
+// (monolis_matvec_V_C, matvec.f90:293-351; diag_V.f90:88-158, :208-250) take
+// the same expansion with 16-byte entries; their factorisation stops on a zero
+// pivot instead (diag_V.f90:123), recorded here and reported when a solve asks
+// for the preconditioner.
 #include <algorithm>
 #include <memory>
 #include <vector>
@@ -27,10 +31,11 @@ namespace mh {
 namespace {
 
 // Scalar expansion of the values (and the column ids when fill_col).
+template <typename V>
 __global__ void k_v_fill(int nb, const int* __restrict__ list, const int* __restrict__ dof_index,
                          const int* __restrict__ index2, const int* __restrict__ index, const int* __restrict__ item,
-                         const int* __restrict__ srowptr, const double* __restrict__ A, int* __restrict__ item_s,
-                         double* __restrict__ A_s, int fill_col) {
+                         const int* __restrict__ srowptr, const V* __restrict__ A, int* __restrict__ item_s,
+                         V* __restrict__ A_s, int fill_col) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
   const int n1 = list[i];
@@ -77,6 +82,38 @@ __global__ void k_vdiag_setup(int nb, const int* __restrict__ list, const int* _
   }
 }
 
+// diag_V_C setup (diag_V.f90:88-158): as k_vdiag_setup in complex arithmetic;
+// a zero pivot stops the factorisation of that row and records the smallest
+// such row in *bad.
+__global__ void k_zvdiag_setup(int nb, const int* __restrict__ list, const int* __restrict__ index2,
+                               const int* __restrict__ index, const int* __restrict__ item,
+                               const long long* __restrict__ doff, const zd* __restrict__ A, zd* __restrict__ D,
+                               int* __restrict__ bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const int nd = list[i];
+  zd* LU = D + doff[i];
+  for (int e = 0; e < nd * nd; ++e) LU[e] = zd(0.0);
+  int found = -1;
+  for (int e = index[i]; e < index[i + 1]; ++e)
+    if (item[e] - 1 == i) found = e;
+  if (found < 0) return;
+  const zd* src = A + index2[found];
+  for (int e = 0; e < nd * nd; ++e) LU[e] = src[e];
+  for (int k = 0; k < nd; ++k) {
+    const zd piv = LU[k * nd + k];
+    if (piv.re == 0.0 && piv.im == 0.0) {
+      atomicMin(bad, i);
+      return;
+    }
+    LU[k * nd + k] = zdiv(zd(1.0), piv);
+    for (int l = k + 1; l < nd; ++l) {
+      LU[l * nd + k] = LU[l * nd + k] * LU[k * nd + k];
+      for (int j = k + 1; j < nd; ++j) LU[l * nd + j] = LU[l * nd + j] - LU[l * nd + k] * LU[k * nd + j];
+    }
+  }
+}
+
 __global__ void k_to_fp32_v(long long m, const double* __restrict__ a, float* __restrict__ b) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
     b[i] = (float)a[i];
@@ -104,34 +141,58 @@ int to_host(std::vector<U>& dst, const U* src, size_t count, hipStream_t s) {
 
 // Values of the expansion (device) from monolis's A (host or device), and the
 // diag_V factors from the same A.
-int fill_vblock_values(Matrix* M, const double* A, hipStream_t s, DevArray<int>* item_s, DevArray<double>* A_s) {
-  VBlock& V = *M->vb;
-  const double* dA = A;
-  DevArray<double> tmp;
+template <typename V>
+int fill_vblock_values(Matrix* M, const V* A, hipStream_t s, DevArray<int>* item_s, DevArray<V>* A_s) {
+  constexpr bool cplx = sizeof(V) == sizeof(zd);
+  VBlock& Vb = *M->vb;
+  const V* dA = A;
+  DevArray<V> tmp;
   if (!is_device_pointer(A)) {
-    MH_TRY(tmp.alloc((size_t)V.nzs + 1));
-    MH_TRY(upload(tmp.p, A, sizeof(double) * (size_t)V.nzs, s));
+    MH_TRY(tmp.alloc((size_t)Vb.nzs + 1));
+    MH_TRY(upload(tmp.p, A, sizeof(V) * (size_t)Vb.nzs, s));
     dA = tmp.p;
   }
-  if (V.nb > 0) {
-    k_v_fill<<<(V.nb + kBlock - 1) / kBlock, kBlock, 0, s>>>(V.nb, V.list.p, V.dof_index.p, V.index2.p, V.index.p,
-                                                             V.item.p, V.srowptr.p, dA, item_s ? item_s->p : nullptr,
-                                                             A_s->p, item_s != nullptr);
+  DevArray<int> bad;
+  if constexpr (cplx) {
+    if (Vb.Dz.n < Vb.D64.n) MH_TRY(Vb.Dz.alloc(Vb.D64.n));
+    MH_TRY(bad.alloc(1));
+    MH_HIP(hipMemsetAsync(bad.p, 0x7f, sizeof(int), s));
+  }
+  if (Vb.nb > 0) {
+    const unsigned grid = (unsigned)((Vb.nb + kBlock - 1) / kBlock);
+    k_v_fill<V><<<grid, kBlock, 0, s>>>(Vb.nb, Vb.list.p, Vb.dof_index.p, Vb.index2.p, Vb.index.p, Vb.item.p,
+                                        Vb.srowptr.p, dA, item_s ? item_s->p : nullptr, A_s->p, item_s != nullptr);
     MH_HIP(hipGetLastError());
-    k_vdiag_setup<<<(V.nb + kBlock - 1) / kBlock, kBlock, 0, s>>>(V.nb, V.list.p, V.index2.p, V.index.p, V.item.p,
-                                                                  V.doff.p, dA, V.D64.p);
+    if constexpr (cplx) {
+      k_zvdiag_setup<<<grid, kBlock, 0, s>>>(Vb.nb, Vb.list.p, Vb.index2.p, Vb.index.p, Vb.item.p, Vb.doff.p, dA,
+                                             Vb.Dz.p, bad.p);
+    } else {
+      k_vdiag_setup<<<grid, kBlock, 0, s>>>(Vb.nb, Vb.list.p, Vb.index2.p, Vb.index.p, Vb.item.p, Vb.doff.p, dA,
+                                            Vb.D64.p);
+    }
     MH_HIP(hipGetLastError());
   }
-  V.d32_ready = false;
+  Vb.d32_ready = false;
+  if constexpr (cplx) {
+    int hbad = 0x7f7f7f7f;
+    MH_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    MH_HIP(hipStreamSynchronize(s));
+    Vb.zbad = hbad == 0x7f7f7f7f ? -1 : hbad;
+  }
   MH_HIP(hipStreamSynchronize(s));  // tmp is released on return
   return 0;
 }
 
-}  // namespace
+// Shared by the real and complex builds: validates monolis's block arrays,
+// uploads them, expands the values and builds the scalar SELL-64 matrix.
+template <typename VT>
+int vblock_build(Matrix** out, int n, int np, int nz, const int* n_dof_list, const int* n_dof_index,
+                 const int* n_dof_index2, const int* index, const int* item, const VT* A, hipStream_t s);
 
-int matrix_create_vblock(Matrix** out, int n, int np, int nz, const int* n_dof_list, const int* n_dof_index,
-                         const int* n_dof_index2, const int* index, const int* item, const double* A,
-                         hipStream_t s) {
+template <typename VT>
+int vblock_build(Matrix** out, int n, int np, int nz, const int* n_dof_list, const int* n_dof_index,
+                 const int* n_dof_index2, const int* index, const int* item, const VT* A, hipStream_t s) {
+  constexpr bool cplx = sizeof(VT) == sizeof(zd);
   *out = nullptr;
   if (n < 0 || np < n || nz < 0) {
     set_error("invalid matrix sizes n=%d np=%d nz=%d", n, np, nz);
@@ -235,14 +296,17 @@ int matrix_create_vblock(Matrix** out, int n, int np, int nz, const int* n_dof_l
   tmpM.vb = V;
   vguard.release();
   DevArray<int> item_s;
-  DevArray<double> A_s;
+  DevArray<VT> A_s;
   int rc = 0;
   if ((rc = item_s.alloc((size_t)acc + 1)) || (rc = A_s.alloc((size_t)acc + 1)) ||
-      (rc = fill_vblock_values(&tmpM, A, s, &item_s, &A_s))) {
+      (rc = fill_vblock_values<VT>(&tmpM, A, s, &item_s, &A_s))) {
     return rc;  // tmpM owns V and frees it
   }
   Matrix* M = nullptr;
-  rc = matrix_create(&M, ns, nps, (int)acc, 1, V->srowptr.p, item_s.p, A_s.p, s);
+  if constexpr (cplx)
+    rc = matrix_create_complex(&M, ns, nps, (int)acc, 1, V->srowptr.p, item_s.p, (const double*)A_s.p, s);
+  else
+    rc = matrix_create(&M, ns, nps, (int)acc, 1, V->srowptr.p, item_s.p, A_s.p, s);
   if (rc) return rc;
   MH_HIP(hipStreamSynchronize(s));
   M->vb = V;
@@ -251,11 +315,34 @@ int matrix_create_vblock(Matrix** out, int n, int np, int nz, const int* n_dof_l
   return 0;
 }
 
+}  // namespace
+
+int matrix_create_vblock(Matrix** out, int n, int np, int nz, const int* n_dof_list, const int* n_dof_index,
+                         const int* n_dof_index2, const int* index, const int* item, const double* A,
+                         hipStream_t s) {
+  return vblock_build<double>(out, n, np, nz, n_dof_list, n_dof_index, n_dof_index2, index, item, A, s);
+}
+
+int matrix_create_vblock_complex(Matrix** out, int n, int np, int nz, const int* n_dof_list, const int* n_dof_index,
+                                 const int* n_dof_index2, const int* index, const int* item, const zd* A,
+                                 hipStream_t s) {
+  return vblock_build<zd>(out, n, np, nz, n_dof_list, n_dof_index, n_dof_index2, index, item, A, s);
+}
+
 int matrix_set_values_vblock(Matrix* M, const double* A, hipStream_t s) {
   DevArray<double> A_s;
   MH_TRY(A_s.alloc((size_t)M->vb->nzs + 1));
-  MH_TRY(fill_vblock_values(M, A, s, nullptr, &A_s));
+  MH_TRY(fill_vblock_values<double>(M, A, s, nullptr, &A_s));
   MH_TRY(matrix_set_values(M, A_s.p, s));
+  MH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int matrix_set_values_vblock_complex(Matrix* M, const zd* A, hipStream_t s) {
+  DevArray<zd> A_s;
+  MH_TRY(A_s.alloc((size_t)M->vb->nzs + 1));
+  MH_TRY(fill_vblock_values<zd>(M, A, s, nullptr, &A_s));
+  MH_TRY(matrix_set_values(M, (const double*)A_s.p, s));
   MH_HIP(hipStreamSynchronize(s));
   return 0;
 }

@@ -94,7 +94,7 @@ class Matrix:
         """Dense copy (tiny matrices only: brute-force checks)."""
         if self.ndof == -1:
             m = int(self.n_dof_index[self.np_])
-            D = np.zeros((int(self.n_dof_index[self.n]), m))
+            D = np.zeros((int(self.n_dof_index[self.n]), m), self.A.dtype)
             for i in range(self.n):
                 n1 = self.n_dof_list[i]
                 r0 = self.n_dof_index[i]

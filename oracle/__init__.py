@@ -86,6 +86,13 @@ def lib():
         L.orc_diag_nn_apply_C.argtypes = [c_int, c_int, _c128p, _c128p, _c128p]
         L.orc_solve_C.argtypes = [c_int, c_int, _i32p, _i32p, _c128p, _c128p, _c128p, _i32p, _f64p]
         L.orc_solve_C.restype = c_int
+        L.orc_matvec_V_C.argtypes = [c_int, _i32p, _i32p, _i32p, _i32p, _i32p, _c128p, _c128p, _c128p]
+        L.orc_diag_V_setup_C.argtypes = [c_int, _i32p, _i32p, _i32p, _i32p, _c128p, _i64p, _c128p]
+        L.orc_diag_V_setup_C.restype = c_int
+        L.orc_diag_V_apply_C.argtypes = [c_int, _i32p, _i32p, _i64p, _c128p, _c128p, _c128p]
+        L.orc_solve_V_C.argtypes = [c_int, _i32p, _i32p, _i32p, _i32p, _i32p, _c128p, _c128p, _c128p, _i32p,
+                                    _f64p]
+        L.orc_solve_V_C.restype = c_int
         L.orc_pattern_simple_mesh.argtypes = [c_int, c_int, c_int, _i32p, _i32p, _i32p, ctypes.c_longlong]
         L.orc_pattern_simple_mesh.restype = c_int
         L.orc_add_matrix.argtypes = [c_int, _i32p, _i32p, _f64p, c_int, _i32p, _f64p]
@@ -270,11 +277,26 @@ def _z(a):
     return np.ascontiguousarray(a, np.complex128)
 
 
+def _vl(mat):
+    return (np.ascontiguousarray(mat.n_dof_list, np.int32), np.ascontiguousarray(mat.n_dof_index, np.int32),
+            np.ascontiguousarray(mat.n_dof_index2, np.int32))
+
+
+def _mC(mat):
+    return int(mat.n_dof_index[mat.n]) if mat.ndof == -1 else mat.n * mat.ndof
+
+
 def matvec_C(mat, x):
-    """y = A x in complex arithmetic (src/linalg/matvec.f90:192-227, fixed block sizes)."""
-    y = np.zeros(max(mat.n * mat.ndof, 1), np.complex128)
-    lib().orc_matvec_C(mat.n, mat.ndof, mat.index, mat.item, _z(mat.A), _z(x), y)
-    return y[: mat.n * mat.ndof]
+    """y = A x in complex arithmetic (src/linalg/matvec.f90:192-227; n_dof = -1:
+    monolis_matvec_V_C, :293-351)."""
+    m = _mC(mat)
+    y = np.zeros(max(m, 1), np.complex128)
+    if mat.ndof == -1:
+        lst, dofi, dofi2 = _vl(mat)
+        lib().orc_matvec_V_C(mat.n, lst, dofi, dofi2, mat.index, mat.item, _z(mat.A), _z(x), y)
+    else:
+        lib().orc_matvec_C(mat.n, mat.ndof, mat.index, mat.item, _z(mat.A), _z(x), y)
+    return y[:m]
 
 
 def inner_product_C(x, y):
@@ -296,8 +318,16 @@ def check_converge_C(r, B2, it, tol):
 
 
 def diag_setup_C(mat):
-    """Complex block-Jacobi factors (diag_nn.f90:90-159).  Returns (D, status);
-    status ZERO_DIAG where the reference stops on a zero pivot."""
+    """Complex block-Jacobi factors (diag_nn.f90:90-159; n_dof = -1: diag_V.f90:88-158,
+    returned as (doff, D)).  Returns (D, status); status ZERO_DIAG where the
+    reference stops on a zero pivot."""
+    if mat.ndof == -1:
+        lst, dofi, dofi2 = _vl(mat)
+        tot = int(np.sum(lst[:mat.n].astype(np.int64) ** 2))
+        doff = np.zeros(mat.n + 1, np.int64)
+        D = np.zeros(max(tot, 1), np.complex128)
+        st = lib().orc_diag_V_setup_C(mat.n, lst, dofi2, mat.index, mat.item, _z(mat.A), doff, D)
+        return (doff, D), int(st)
     D = np.zeros(max(mat.ndof * mat.ndof * mat.n, 1), np.complex128)
     st = lib().orc_diag_nn_setup_C(mat.n, mat.ndof, mat.index, mat.item, _z(mat.A), D)
     return D, int(st)
@@ -306,6 +336,11 @@ def diag_setup_C(mat):
 def diag_apply_C(mat, D, x):
     x = _z(x)
     y = np.zeros_like(x)
+    if mat.ndof == -1:
+        doff, DD = D
+        lst, dofi, _ = _vl(mat)
+        lib().orc_diag_V_apply_C(mat.n, lst, dofi, doff, _z(DD), x, y)
+        return y
     lib().orc_diag_nn_apply_C(mat.n, mat.ndof, _z(D), x, y)
     return y
 
@@ -315,9 +350,13 @@ def solve_C(mat, b, x0=None, Iarray=None, Rarray=None, **kw):
     if Iarray is None:
         kw.setdefault("method", ITER_COCG)
         Iarray, Rarray = default_arrays(**kw)
-    m = mat.n * mat.ndof
+    m = _mC(mat)
     x = np.zeros(max(m, 1), np.complex128) if x0 is None else np.array(x0, np.complex128, copy=True)
-    st = lib().orc_solve_C(mat.n, mat.ndof, mat.index, mat.item, _z(mat.A), x, _z(b), Iarray, Rarray)
+    if mat.ndof == -1:
+        lst, dofi, dofi2 = _vl(mat)
+        st = lib().orc_solve_V_C(mat.n, lst, dofi, dofi2, mat.index, mat.item, _z(mat.A), x, _z(b), Iarray, Rarray)
+    else:
+        st = lib().orc_solve_C(mat.n, mat.ndof, mat.index, mat.item, _z(mat.A), x, _z(b), Iarray, Rarray)
     return x[:m], int(st), Iarray, Rarray
 
 

@@ -20,6 +20,11 @@
  *                         R2/B2, whose principal root is purely imaginary.
  *   orc_solve_C (COCG)    the reference's COCG_test chain (x = 1 + i), dense
  *                         numpy solves, and the real-valued case equal to CG.
+ *   orc_matvec_V_C, orc_diag_V_setup_C / apply_C, orc_solve_V_C (variable
+ *                         blocks, n_dof = -1): dense brute force; every block
+ *                         of one size b reproduces the fixed-b routines bit for
+ *                         bit (same sums in the same order); real-valued data
+ *                         reproduces the real variable-block oracle.
  */
 #include <complex.h>
 #include <math.h>
@@ -157,22 +162,107 @@ void orc_diag_nn_apply_C(int N, int ndof, const zc* D, const zc* X, zc* Y) {
   }
 }
 
+/* src/linalg/matvec.f90:293-351 (monolis_matvec_V_C): block (i, j) of size
+ * n_i x n_j row-major at A(n_dof_index2(e)); Y(n_dof_index(i) + k) = sum over
+ * the row's blocks, then l = 1..n_j, of A * X. */
+void orc_matvec_V_C(int N, const int* n_dof_list, const int* n_dof_index, const int* n_dof_index2,
+                    const int* index, const int* item, const zc* A, const zc* X, zc* Y) {
+  int i;
+#pragma omp parallel for schedule(static)
+  for (i = 0; i < N; ++i) {
+    zc yt[64];
+    const int n1 = n_dof_list[i];
+    for (int k = 0; k < n1; ++k) yt[k] = 0.0;
+    for (int j = index[i]; j < index[i + 1]; ++j) {
+      const int in = item[j] - 1;
+      const int n2 = n_dof_list[in];
+      const int kn = n_dof_index[in];
+      const long long an = n_dof_index2[j];
+      for (int k = 0; k < n1; ++k)
+        for (int l = 0; l < n2; ++l) yt[k] = yt[k] + A[an + (long long)n2 * k + l] * X[kn + l];
+    }
+    for (int k = 0; k < n1; ++k) Y[n_dof_index[i] + k] = yt[k];
+  }
+}
+
+/* src/prec/diag/diag_V.f90:88-158 (monolis_precond_diag_V_setup_C): the
+ * no-pivot LU of each diagonal block with reciprocal pivots, stored at
+ * doff[i] (doff = prefix sums of n_i^2).  A zero pivot stops the reference
+ * (:123); here it returns ORC_ZERO_DIAG.  A row without a diagonal block keeps
+ * zeros (the reference's packed offset does not advance for such a row, which
+ * misaligns every later row's factors; such matrices are outside its contract). */
+int orc_diag_V_setup_C(int N, const int* n_dof_list, const int* n_dof_index2, const int* index,
+                       const int* item, const zc* A, long long* doff, zc* D) {
+  doff[0] = 0;
+  for (int i = 0; i < N; ++i) doff[i + 1] = doff[i] + (long long)n_dof_list[i] * n_dof_list[i];
+  memset(D, 0, sizeof(zc) * (size_t)(doff[N] > 0 ? doff[N] : 1));
+  for (int i = 0; i < N; ++i) {
+    for (int jj = index[i]; jj < index[i + 1]; ++jj) {
+      if (item[jj] - 1 != i) continue;
+      const int nd = n_dof_list[i];
+      zc* LU = D + doff[i];
+      for (int e = 0; e < nd * nd; ++e) LU[e] = A[(long long)n_dof_index2[jj] + e];
+      for (int k = 0; k < nd; ++k) {
+        if (LU[nd * k + k] == 0.0) return ORC_ZERO_DIAG;
+        LU[nd * k + k] = 1.0 / LU[nd * k + k];
+        for (int l = k + 1; l < nd; ++l) {
+          LU[nd * l + k] = LU[nd * l + k] * LU[nd * k + k];
+          for (int j = k + 1; j < nd; ++j) LU[nd * l + j] = LU[nd * l + j] - LU[nd * l + k] * LU[nd * k + j];
+        }
+      }
+    }
+  }
+  return ORC_OK;
+}
+
+/* src/prec/diag/diag_V.f90:208-250 (monolis_precond_diag_V_apply_C). */
+void orc_diag_V_apply_C(int N, const int* n_dof_list, const int* n_dof_index, const long long* doff, const zc* D,
+                        const zc* X, zc* Y) {
+  int i;
+#pragma omp parallel for schedule(static)
+  for (i = 0; i < N; ++i) {
+    zc t[64];
+    const int nd = n_dof_list[i];
+    const int in = n_dof_index[i];
+    const zc* LU = D + doff[i];
+    for (int j = 0; j < nd; ++j) t[j] = X[in + j];
+    for (int j = 1; j < nd; ++j)
+      for (int k = 0; k < j; ++k) t[j] = t[j] - LU[nd * j + k] * t[k];
+    for (int j = nd - 1; j >= 0; --j) {
+      for (int k = nd - 1; k > j; --k) t[j] = t[j] - LU[nd * j + k] * t[k];
+      t[j] = LU[(nd + 1) * j] * t[j];
+    }
+    for (int j = 0; j < nd; ++j) Y[in + j] = t[j];
+  }
+}
+
 typedef struct {
   int N, ndof, m, precond;
   const int *index, *item;
+  const int *n_dof_list, *n_dof_index, *n_dof_index2;  /* n_dof = -1 only */
   const zc* A;
   zc* D;
+  long long* doff;                                      /* n_dof = -1 only */
 } orc_zctx;
+
+static void zctx_matvec(const orc_zctx* c, const zc* X, zc* Y) {
+  if (c->ndof == -1)
+    orc_matvec_V_C(c->N, c->n_dof_list, c->n_dof_index, c->n_dof_index2, c->index, c->item, c->A, X, Y);
+  else
+    orc_matvec_C(c->N, c->ndof, c->index, c->item, c->A, X, Y);
+}
 
 static void zctx_residual(const orc_zctx* c, const zc* X, const zc* B, zc* R) {
   /* src/linalg/matvec.f90:907-933: R = B - A X */
-  orc_matvec_C(c->N, c->ndof, c->index, c->item, c->A, X, R);
+  zctx_matvec(c, X, R);
   axpby_c(c->m, 1.0, B, -1.0, R, R);
 }
 
 static void zctx_prec(const orc_zctx* c, const zc* X, zc* Y) {
   /* src/prec/precond.f90:235-286 (DIAG or NONE) */
-  if (c->precond == ORC_PREC_DIAG)
+  if (c->precond == ORC_PREC_DIAG && c->ndof == -1)
+    orc_diag_V_apply_C(c->N, c->n_dof_list, c->n_dof_index, c->doff, c->D, X, Y);
+  else if (c->precond == ORC_PREC_DIAG)
     orc_diag_nn_apply_C(c->N, c->ndof, c->D, X, Y);
   else
     memcpy(Y, X, sizeof(zc) * (size_t)c->m);
@@ -208,7 +298,7 @@ static int orc_cocg(const orc_zctx* c, const zc* B, zc* X, int* Iarray, double* 
     } else {
       memcpy(P, Z, sizeof(zc) * (size_t)m);
     }
-    orc_matvec_C(c->N, c->ndof, c->index, c->item, c->A, P, Q);
+    zctx_matvec(c, P, Q);
     zc omega;
     orc_inner_product_C(m, P, Q, &omega);
     const zc alpha = rho / omega;
@@ -232,10 +322,10 @@ done:
 }
 
 /* monolis_solve_main_C (src/solver/solver.f90:189-213) on one rank: the
- * preconditioner setup of precond.f90:105-143, then monolis_solver_select_C,
- * which accepts COCG only. */
-int orc_solve_C(int n, int n_dof, const int* index, const int* item, const zc* A, zc* x, const zc* b,
-                int* Iarray, double* Rarray) {
+ * preconditioner setup of precond.f90:105-143 (diag.f90:40-58: diag_V for
+ * n_dof = -1, diag_nn otherwise), then monolis_solver_select_C, which accepts
+ * COCG only. */
+static int solve_C_ctx(orc_zctx* c, zc* x, const zc* b, int* Iarray, double* Rarray) {
   const double t0 = orc_now_c();
   for (int k = R_TIME_SOL; k <= 9; ++k) Rarray[k] = 0.0;
   if (Iarray[I_METHOD] != ORC_ITER_COCG) {
@@ -246,6 +336,32 @@ int orc_solve_C(int n, int n_dof, const int* index, const int* item, const zc* A
     Iarray[I_IERR] = ORC_BAD_PRECOND;
     return ORC_BAD_PRECOND;
   }
+  c->precond = Iarray[I_PRECOND];
+  int st = ORC_OK;
+  const double tp = orc_now_c();
+  if (c->precond == ORC_PREC_DIAG) {
+    if (c->ndof == -1) {
+      long long tot = 0;
+      for (int i = 0; i < c->N; ++i) tot += (long long)c->n_dof_list[i] * c->n_dof_list[i];
+      c->doff = (long long*)malloc(sizeof(long long) * ((size_t)c->N + 1));
+      c->D = (zc*)malloc(sizeof(zc) * (size_t)(tot > 0 ? tot : 1));
+      st = orc_diag_V_setup_C(c->N, c->n_dof_list, c->n_dof_index2, c->index, c->item, c->A, c->doff, c->D);
+    } else {
+      c->D = (zc*)malloc(sizeof(zc) * (size_t)c->ndof * c->ndof * (size_t)(c->N > 0 ? c->N : 1));
+      st = orc_diag_nn_setup_C(c->N, c->ndof, c->index, c->item, c->A, c->D);
+    }
+  }
+  Rarray[R_TIME_PREP] = orc_now_c() - tp;
+  if (st == ORC_OK) st = orc_cocg(c, b, x, Iarray, Rarray);
+  free(c->D);
+  free(c->doff);
+  Iarray[I_IERR] = st;
+  Rarray[R_TIME_SOL] = orc_now_c() - t0;
+  return st;
+}
+
+int orc_solve_C(int n, int n_dof, const int* index, const int* item, const zc* A, zc* x, const zc* b,
+                int* Iarray, double* Rarray) {
   orc_zctx c;
   memset(&c, 0, sizeof(c));
   c.N = n;
@@ -254,17 +370,22 @@ int orc_solve_C(int n, int n_dof, const int* index, const int* item, const zc* A
   c.index = index;
   c.item = item;
   c.A = A;
-  c.precond = Iarray[I_PRECOND];
-  int st = ORC_OK;
-  const double tp = orc_now_c();
-  if (c.precond == ORC_PREC_DIAG) {
-    c.D = (zc*)malloc(sizeof(zc) * (size_t)n_dof * n_dof * (size_t)(n > 0 ? n : 1));
-    st = orc_diag_nn_setup_C(n, n_dof, index, item, A, c.D);
-  }
-  Rarray[R_TIME_PREP] = orc_now_c() - tp;
-  if (st == ORC_OK) st = orc_cocg(&c, b, x, Iarray, Rarray);
-  free(c.D);
-  Iarray[I_IERR] = st;
-  Rarray[R_TIME_SOL] = orc_now_c() - t0;
-  return st;
+  return solve_C_ctx(&c, x, b, Iarray, Rarray);
+}
+
+/* The same for variable block sizes (monolis n_dof = -1). */
+int orc_solve_V_C(int n, const int* n_dof_list, const int* n_dof_index, const int* n_dof_index2, const int* index,
+                  const int* item, const zc* A, zc* x, const zc* b, int* Iarray, double* Rarray) {
+  orc_zctx c;
+  memset(&c, 0, sizeof(c));
+  c.N = n;
+  c.ndof = -1;
+  c.m = n_dof_index[n];
+  c.index = index;
+  c.item = item;
+  c.n_dof_list = n_dof_list;
+  c.n_dof_index = n_dof_index;
+  c.n_dof_index2 = n_dof_index2;
+  c.A = A;
+  return solve_C_ctx(&c, x, b, Iarray, Rarray);
 }

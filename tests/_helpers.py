@@ -201,6 +201,42 @@ def dense_C(m):
     return D
 
 
+def random_vblock_C(n, density, seed, maxb=4, sizes=None):
+    """Complex symmetric variable-block matrix (A = A^T, no conjugation; monolis
+    n_dof = -1): the pattern of random_vblock, complex values, diagonally
+    dominant with a positive imaginary shift on the diagonal."""
+    m = random_vblock(n, density, seed, maxb=maxb, spd=False, sizes=sizes)
+    rng = np.random.default_rng(seed + 2000)
+    m.A = m.A + 1j * rng.uniform(-1, 1, len(m.A))
+    D = m.dense()
+    D = 0.5 * (D + D.T)
+    D += np.eye(len(D)) * (np.abs(D).sum(1).max() + 1.0) * (1.0 + 0.5j)
+    _store_vblock_dense(m, D)
+    return m
+
+
+def _store_vblock_dense(m, D):
+    for i in range(m.n):
+        r0, n1 = int(m.n_dof_index[i]), int(m.n_dof_list[i])
+        for e in range(m.index[i], m.index[i + 1]):
+            j = int(m.item[e]) - 1
+            c0, n2 = int(m.n_dof_index[j]), int(m.n_dof_list[j])
+            a0 = int(m.n_dof_index2[e])
+            m.A[a0:a0 + n1 * n2] = D[r0:r0 + n1, c0:c0 + n2].reshape(-1)
+
+
+def as_vblock(m):
+    """The fixed-block matrix m restated with n_dof = -1 (every block m.ndof):
+    same index / item / values, n_dof_list = ndof, n_dof_index = i*ndof,
+    n_dof_index2 = e*ndof^2."""
+    b = m.ndof
+    v = mm.Matrix(m.n, m.np_, -1, m.index.copy(), m.item.copy(), m.A.copy())
+    v.n_dof_list = np.full(m.np_, b, np.int32)
+    v.n_dof_index = (np.arange(m.np_ + 1, dtype=np.int64) * b).astype(np.int32)
+    v.n_dof_index2 = (np.arange(len(m.item) + 1, dtype=np.int64) * b * b).astype(np.int32)
+    return v
+
+
 def hex_mesh(nx, ny=None, nz=None):
     """Structured hexahedral mesh: (nx+1)(ny+1)(nz+1) nodes, nx*ny*nz 8-node
     elements, 0-based node ids, element-major (n_elem, 8)."""

@@ -12,7 +12,8 @@ import oracle
 from monolis_amd import _lib, api, workloads as W
 from monolis_amd import matrix as mm
 
-from _helpers import chain_matrix, chain_matrix_C, chain_matrix_C_wrapper, random_bcsr_C
+from _helpers import (as_vblock, chain_matrix, chain_matrix_C, chain_matrix_C_wrapper, random_bcsr_C, random_vblock,
+                      random_vblock_C)
 
 pytestmark = pytest.mark.gpu
 
@@ -183,7 +184,7 @@ def test_cocg_zero_rhs_and_errors():
     np.testing.assert_allclose(x, 1 + 1j, rtol=1e-8, atol=1e-8)
 
 
-def test_cocg_not_converged_status_and_variable_blocks_rejected():
+def test_cocg_not_converged_status_and_fixed_handle_rejects_variable_blocks():
     m = W.helmholtz7(16, shift=2.0, damping=0.05)
     b = _crand(m.n, 1)
     x, prm = _flat_solve_C(m, b, 1, tol=1e-12, max_iter=5, abort=False)
@@ -203,3 +204,82 @@ def test_cocg_bcsr3_complex_workload():
     xo, st, I, R = oracle.solve_C(m, b, precond=1, tol=1e-10)
     assert prm.Iarray[5] == st == 0 and _iter_ok(prm.Iarray[4], I[4])
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+# ---- variable blocks (n_dof = -1): monolis_matvec_V_C, diag_V_*_C ------------------
+@pytest.mark.parametrize("maxb", [3, 6])
+def test_matvec_V_C_matches_oracle(maxb):
+    m = random_vblock_C(300, 0.02, seed=maxb, maxb=maxb)
+    x = _crand(m.npndof, maxb)
+    y = api.monolis_matvec_product_C(api.Monolis(m), x)[: m.nndof]
+    yo = oracle.matvec_C(m, x)
+    assert np.linalg.norm(y - yo) <= 1e-12 * np.linalg.norm(yo)
+    dm = api.DeviceMatrixC(m)
+    dx, dy = api.DeviceBuffer.from_numpy(x), api.DeviceBuffer(m.npndof, np.complex128)
+    dm.matvec(dx, dy)
+    assert np.linalg.norm(dy.numpy()[: m.nndof] - yo) <= 1e-12 * np.linalg.norm(yo)
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_cocg_V_matches_oracle(precond):
+    m = random_vblock_C(400, 0.012, seed=17, maxb=5)
+    b = _crand(m.nndof, 18)
+    x, prm = _flat_solve_C(m, b, precond, tol=1e-10)
+    xo, st, I, R = oracle.solve_C(m, b, precond=precond, tol=1e-10)
+    assert prm.Iarray[5] == st == 0 and _iter_ok(prm.Iarray[4], I[4]), (prm.Iarray[4], I[4])
+    assert np.linalg.norm(x[: m.nndof] - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_cocg_V_with_one_block_size_is_the_fixed_block_path():
+    m = random_bcsr_C(500, 3, 0.01, seed=19)
+    b = _crand(m.n * 3, 20)
+    xf, pf = _flat_solve_C(m, b, 1, tol=1e-10)
+    xv, pv = _flat_solve_C(as_vblock(m), b, 1, tol=1e-10)
+    assert pf.Iarray[5] == pv.Iarray[5] == 0 and abs(int(pf.Iarray[4]) - int(pv.Iarray[4])) <= 1
+    assert np.linalg.norm(xv - xf) <= 1e-9 * np.linalg.norm(xf)
+
+
+def test_cocg_V_real_data_matches_real_V_cg():
+    mr = random_vblock(300, 0.015, seed=21, maxb=4, spd=True)
+    mc = random_vblock(300, 0.015, seed=21, maxb=4, spd=True)
+    mc.A = mr.A.astype(np.complex128)
+    b = oracle.matvec(mr, np.ones(mr.npndof))
+    xc, pc = _flat_solve_C(mc, b.astype(np.complex128), 1, tol=1e-10)
+    mo = api.Monolis(mr)
+    mo.prm.quiet()
+    api.monolis_set_tolerance(mo, 1e-10)
+    xr = np.zeros(mr.npndof)
+    api.monolis_solve_R(mo, b, xr)
+    assert pc.Iarray[5] == 0 and _iter_ok(pc.Iarray[4], mo.prm.Iarray[4])
+    assert np.abs(xc.imag).max() == 0.0
+    np.testing.assert_allclose(xc.real, xr, rtol=1e-10, atol=1e-10)
+
+
+def test_cocg_V_zero_pivot_reported_then_cleared_by_set_values():
+    m = random_vblock_C(120, 0.03, seed=23, maxb=4)
+    b = _crand(m.nndof, 24)
+    big = int(np.argmax(m.n_dof_list[: m.n]))
+    e = next(e for e in range(m.index[big], m.index[big + 1]) if m.item[e] - 1 == big)
+    good = m.A.copy()
+    m.A[int(m.n_dof_index2[e])] = 0.0
+    # the reference stops in diag_V_setup_C (diag_V.f90:123); here status -6
+    _, prm = _flat_solve_C(m, b, 1, abort=False)
+    assert prm.Iarray[5] == -6
+    dm = api.DeviceMatrixC(m)
+    db, dx = api.DeviceBuffer.from_numpy(b), api.DeviceBuffer(m.npndof, np.complex128)
+    prm = api.Prm().quiet()
+    prm.Iarray[1], prm.Iarray[2] = 13, 1
+    with pytest.raises(_lib.MonolisHipError, match="zero diag"):
+        dm.solve(db, dx, prm)
+    prm.Iarray[2] = 0                                   # without the preconditioner it solves
+    dm.solve(db, dx, prm)
+    assert prm.Iarray[5] == 0
+    dm.set_values(good)                                  # new values: the factors are rebuilt
+    prm = api.Prm().quiet()
+    prm.Iarray[1], prm.Iarray[2] = 13, 1
+    prm.Rarray[1] = 1e-10
+    dm.solve(db, dx, prm)
+    m.A = good
+    xo, st, I, R = oracle.solve_C(m, b, precond=1, tol=1e-10)
+    assert prm.Iarray[5] == st == 0 and _iter_ok(prm.Iarray[4], I[4])
+    assert np.linalg.norm(dx.numpy()[: m.nndof] - xo) <= 1e-6 * np.linalg.norm(xo)

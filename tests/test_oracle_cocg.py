@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 import oracle
-from _helpers import chain_matrix, chain_matrix_C, chain_matrix_C_wrapper, dense_C, random_bcsr, random_bcsr_C
+from _helpers import (as_vblock, chain_matrix, chain_matrix_C, chain_matrix_C_wrapper, dense_C, random_bcsr,
+                      random_bcsr_C, random_vblock, random_vblock_C)
 from monolis_amd import matrix as mm
 
 
@@ -143,3 +144,76 @@ def test_cocg_zero_rhs_and_bad_arguments():
     mm.set_scalar(m, 3, 3, 1, 1, 0.0)
     _, st, I, _ = oracle.solve_C(m, np.ones(20, np.complex128), precond=oracle.PREC_DIAG)
     assert st == oracle.ZERO_DIAG and I[oracle.I_IERR] == oracle.ZERO_DIAG
+
+
+# ---- variable blocks (n_dof = -1): monolis_matvec_V_C, diag_V_*_C -------------------
+def _dense_V(m):
+    return m.dense()
+
+
+def test_matvec_V_C_matches_dense_brute_force():
+    m = random_vblock_C(40, 0.12, seed=3, maxb=5)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(m.npndof) + 1j * rng.standard_normal(m.npndof)
+    np.testing.assert_allclose(oracle.matvec_C(m, x), _dense_V(m) @ x, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("ndof", [1, 2, 3])
+def test_V_C_with_one_block_size_is_the_fixed_block_path(ndof):
+    """Every block of size b: the same sums in the same order as matvec_nn_C /
+    diag_nn_*_C / COCG on the fixed-b matrix, so bit for bit on one thread."""
+    oracle.set_threads(1)
+    m = random_bcsr_C(30, ndof, 0.15, seed=30 + ndof)
+    v = as_vblock(m)
+    rng = np.random.default_rng(ndof)
+    x = rng.standard_normal(m.n * ndof) + 1j * rng.standard_normal(m.n * ndof)
+    assert np.array_equal(oracle.matvec_C(v, x), oracle.matvec_C(m, x))
+    (doff, Dv), stv = oracle.diag_setup_C(v)
+    Dm, stm = oracle.diag_setup_C(m)
+    assert stv == stm == 0 and np.array_equal(Dv, Dm)
+    assert np.array_equal(oracle.diag_apply_C(v, (doff, Dv), x), oracle.diag_apply_C(m, Dm, x))
+    for precond in (oracle.PREC_NONE, oracle.PREC_DIAG):
+        xv, sv, Iv, Rv = oracle.solve_C(v, x, precond=precond, tol=1e-12)
+        xm, sm, Im, Rm = oracle.solve_C(m, x, precond=precond, tol=1e-12)
+        assert sv == sm == 0 and Iv[oracle.I_CUR_ITER] == Im[oracle.I_CUR_ITER]
+        assert np.array_equal(xv, xm)
+
+
+def test_diag_V_C_inverts_each_diagonal_block_and_reports_zero_pivot():
+    m = random_vblock_C(30, 0.15, seed=6, maxb=6)
+    (doff, D), st = oracle.diag_setup_C(m)
+    assert st == 0
+    full = _dense_V(m)
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal(m.npndof) + 1j * rng.standard_normal(m.npndof)
+    y = oracle.diag_apply_C(m, (doff, D), v)
+    for i in range(m.n):
+        r0, n1 = int(m.n_dof_index[i]), int(m.n_dof_list[i])
+        blk = full[r0:r0 + n1, r0:r0 + n1]
+        np.testing.assert_allclose(y[r0:r0 + n1], np.linalg.solve(blk, v[r0:r0 + n1]), rtol=1e-12, atol=1e-12)
+    # a zero pivot stops the reference (diag_V.f90:123); the oracle reports it
+    big = int(np.argmax(m.n_dof_list[:m.n]))
+    e = next(e for e in range(m.index[big], m.index[big + 1]) if m.item[e] - 1 == big)
+    m.A[int(m.n_dof_index2[e])] = 0.0
+    _, st = oracle.diag_setup_C(m)
+    assert st == oracle.ZERO_DIAG
+
+
+def test_cocg_V_matches_dense_solve_and_real_V_cg():
+    m = random_vblock_C(80, 0.06, seed=9, maxb=4)
+    rng = np.random.default_rng(10)
+    b = rng.standard_normal(m.nndof) + 1j * rng.standard_normal(m.nndof)
+    for precond in (oracle.PREC_NONE, oracle.PREC_DIAG):
+        x, st, I, R = oracle.solve_C(m, b, precond=precond, tol=1e-12)
+        assert st == 0
+        np.testing.assert_allclose(x, np.linalg.solve(_dense_V(m), b), rtol=1e-9, atol=1e-9)
+    # real-valued data: COCG on the complex V matrix is the real CG on the V matrix
+    oracle.set_threads(1)
+    mr = random_vblock(50, 0.08, seed=12, maxb=4, spd=True)
+    mc = random_vblock(50, 0.08, seed=12, maxb=4, spd=True)
+    mc.A = mr.A.astype(np.complex128)
+    br = oracle.matvec(mr, np.ones(mr.npndof))
+    xr, sr, Ir, Rr = oracle.solve(mr, br, method=oracle.ITER_CG, precond=oracle.PREC_DIAG, tol=1e-12)
+    xc, sc, Ic, Rc = oracle.solve_C(mc, br.astype(np.complex128), precond=oracle.PREC_DIAG, tol=1e-12)
+    assert sr == sc == 0 and Ir[oracle.I_CUR_ITER] == Ic[oracle.I_CUR_ITER]
+    assert np.array_equal(xc.real, xr) and not xc.imag.any()
