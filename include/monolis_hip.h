@@ -119,7 +119,9 @@ int monolis_hip_memset(void* dst, int value, size_t bytes);
 
 /* Uploads a monolis BCSR matrix (host or device arrays, same layout as part 1)
  * and builds the GPU layout.  The counterpart of monolis's device staging
- * (src/solver/solver.f90:102-156) done once instead of per solve. */
+ * (src/solver/solver.f90:102-156) done once instead of per solve.  n_dof 1..8
+ * takes the fixed-block kernels; 9..16 is stored as the variable-block
+ * expansion (the same sums in the same order as matvec_nn / diag_nn). */
 int monolis_hip_matrix_create(monolis_hip_matrix** out, int n, int np, int nz, int n_dof, const int* index,
                               const int* item, const double* A);
 /* Variable block sizes (monolis n_dof = -1): n_dof_list[NP], n_dof_index[NP+1]

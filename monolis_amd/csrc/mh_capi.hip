@@ -119,6 +119,31 @@ void print_after(const int* Iarray, const double* Rarray, const std::vector<doub
   fflush(stdout);
 }
 
+// Fixed blocks larger than the fixed-block kernels take (kMaxB) are stored as
+// the variable-block expansion with every block n_dof: matvec_nn and diag_nn
+// (src/linalg/matvec.f90:123-188 nn branch, src/prec/diag/diag_nn.f90) sum and
+// factor in the same order as matvec_V and diag_V, and the zero-pivot rules
+// agree (real: pivot 0 -> 1; complex: stop).
+struct UniformBlocks {
+  std::vector<int> list, dofi, dofi2;
+  int make(int np, int nz, int b) {
+    if (np < 0 || nz < 0) {
+      mh::set_error("invalid matrix sizes np=%d nz=%d", np, nz);
+      return ST_ARGS;
+    }
+    if ((long long)nz * b * b >= (1LL << 31) || (long long)np * b >= (1LL << 31)) {
+      mh::set_error("block size %d with %d blocks exceeds 32-bit offsets", b, nz);
+      return ST_ARGS;
+    }
+    list.assign((size_t)np + 1, b);
+    dofi.resize((size_t)np + 1);
+    dofi2.resize((size_t)nz + 1);
+    for (int i = 0; i <= np; ++i) dofi[i] = i * b;
+    for (int e = 0; e <= nz; ++e) dofi2[e] = e * b * b;
+    return 0;
+  }
+};
+
 [[noreturn]] void error_stop(const char* msg) {
   fprintf(stderr, "** monolis error: %s\n", msg);
   fflush(stderr);
@@ -236,6 +261,11 @@ int monolis_hip_matrix_create(monolis_hip_matrix** out, int n, int np, int nz, i
   if (n_dof == -1) {
     mh::set_error("variable block sizes (n_dof = -1) need monolis_hip_matrix_create_V");
     return ST_ARGS;
+  }
+  if (n_dof > mh::kMaxB) {
+    UniformBlocks u;
+    MH_TRY(u.make(np, nz, n_dof));
+    return monolis_hip_matrix_create_V(out, n, np, nz, u.list.data(), u.dofi.data(), u.dofi2.data(), index, item, A);
   }
   Matrix* M = nullptr;
   MH_TRY(mh::matrix_create(&M, n, np, nz, n_dof, index, item, A, c->stream));
@@ -1074,6 +1104,12 @@ int monolis_hip_matrix_create_C(monolis_hip_matrix_C** out, int n, int np, int n
   if (n_dof == -1) {
     mh::set_error("variable block sizes (n_dof = -1) need monolis_hip_matrix_create_V_C");
     return ST_ARGS;
+  }
+  if (n_dof > mh::kMaxB) {
+    UniformBlocks u;
+    MH_TRY(u.make(np, nz, n_dof));
+    return monolis_hip_matrix_create_V_C(out, n, np, nz, u.list.data(), u.dofi.data(), u.dofi2.data(), index, item,
+                                         A);
   }
   Matrix* M = nullptr;
   MH_TRY(mh::matrix_create_complex(&M, n, np, nz, n_dof, index, item, reinterpret_cast<const double*>(A),

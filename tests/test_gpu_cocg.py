@@ -283,3 +283,17 @@ def test_cocg_V_zero_pivot_reported_then_cleared_by_set_values():
     xo, st, I, R = oracle.solve_C(m, b, precond=1, tol=1e-10)
     assert prm.Iarray[5] == st == 0 and _iter_ok(prm.Iarray[4], I[4])
     assert np.linalg.norm(dx.numpy()[: m.nndof] - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("ndof", [10, 16])
+def test_cocg_fixed_blocks_above_8(ndof):
+    m = random_bcsr_C(80, ndof, 0.04, seed=ndof)
+    x = _crand(m.n * ndof, ndof)
+    y = api.monolis_matvec_product_C(api.Monolis(m), x)[: m.n * ndof]
+    yo = oracle.matvec_C(m, x)
+    assert np.linalg.norm(y - yo) <= 1e-12 * np.linalg.norm(yo)
+    b = _crand(m.n * ndof, ndof + 1)
+    xs, prm = _flat_solve_C(m, b, 1, tol=1e-10)
+    xo, st, I, R = oracle.solve_C(m, b, precond=1, tol=1e-10)
+    assert prm.Iarray[5] == st == 0 and _iter_ok(prm.Iarray[4], I[4])
+    assert np.linalg.norm(xs[: m.nndof] - xo) <= 1e-6 * np.linalg.norm(xo)

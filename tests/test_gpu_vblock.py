@@ -13,7 +13,7 @@ from monolis_amd import api
 from monolis_amd import matrix as mm
 from monolis_amd._lib import MonolisHipError
 
-from _helpers import random_vblock
+from _helpers import random_bcsr, random_vblock
 from test_oracle_vblock import reference_matvec_v_case
 
 pytestmark = pytest.mark.gpu
@@ -146,4 +146,34 @@ def test_v_rejects_oversized_and_inconsistent_blocks():
     m.n_dof_index = m.n_dof_index.copy()
     m.n_dof_index[2] += 1                      # disagrees with n_dof_list
     with pytest.raises(MonolisHipError):
+        api.DeviceMatrix(m)
+
+
+@pytest.mark.parametrize("ndof", [9, 12, 16])
+def test_fixed_blocks_above_8_go_through_the_expansion(ndof):
+    """monolis takes any fixed n_dof (matvec_nn, diag_nn); blocks above the
+    fixed-block kernels' 8 are stored as the variable-block expansion."""
+    m = random_bcsr(120, ndof, 0.03, seed=ndof, spd=True)
+    x = np.random.default_rng(ndof).uniform(-1, 1, m.npndof)
+    y = api.monolis_matvec_product_R(api.Monolis(m), x)[: m.nndof]
+    yo = oracle.matvec(m, x)
+    assert np.max(np.abs(y - yo)) <= 1e-12 * np.max(np.abs(yo))
+    b = np.random.default_rng(ndof + 1).uniform(-1, 1, m.nndof)
+    for method in (1, 5):
+        mo = api.Monolis(m)
+        mo.prm.quiet()
+        api.monolis_set_method(mo, method)
+        api.monolis_set_precond(mo, 1)
+        api.monolis_set_tolerance(mo, 1e-10)
+        xs = np.zeros(m.npndof)
+        api.monolis_solve_R(mo, b, xs)
+        xo, st, I, R = oracle.solve(m, b, method=method, precond=1, tol=1e-10)
+        assert mo.prm.Iarray[5] == 0 and st == 0
+        assert _iter_ok(mo.prm.Iarray[4], I[4]), (method, mo.prm.Iarray[4], I[4])
+        assert np.linalg.norm(xs[: m.nndof] - xo) / np.linalg.norm(xo) <= 1e-6
+
+
+def test_fixed_block_above_16_is_rejected():
+    m = random_bcsr(10, 17, 0.3, seed=1, spd=True)
+    with pytest.raises(MonolisHipError, match="16"):
         api.DeviceMatrix(m)
