@@ -63,7 +63,7 @@ __global__ void k_transpose_values(int n, int b, const int* __restrict__ slice_p
     const size_t slot = (size_t)(k0 + k);
     double* dst = AT + (size_t)iperm[e0 + k] * bb;
     for (int r = 0; r < b; ++r)
-      for (int c = 0; c < b; ++c) dst[c * b + r] = val[(slot * bb + r * b + c) * kWave + lane];
+      for (int c = 0; c < b; ++c) dst[c * b + r] = val[sell_vpos<double>(slot, r * b + c, bb, lane)];
   }
 }
 
@@ -111,13 +111,13 @@ __global__ void k_scalar_access(int ci, int cj, int si, int sj, int b, double v,
   if (found < 0) return;
   const int bb = b * b;
   const size_t slot = (size_t)(slice_ptr[ci / kWave] + (found - rowptr[ci]));
-  const size_t p = (slot * bb + si * b + sj) * kWave + ci % kWave;
+  const size_t p = sell_vpos<double>(slot, si * b + sj, bb, ci % kWave);
   double a = val64[p];
   if (mode == 1) a = v;
   else if (mode == 2) a = a + v;
   if (mode != 0) {
     val64[p] = a;
-    if (val32) val32[p] = (float)a;
+    if (val32) val32[sell_vpos<float>(slot, si * b + sj, bb, ci % kWave)] = (float)a;
   }
   out[1] = a;
 }

@@ -142,7 +142,7 @@ __device__ __forceinline__ size_t sell_pos(int blk, int e, int bb, int n, const 
   }
   const int r = lo;
   const size_t slot = (size_t)slice_ptr[r / kWave] + (size_t)(blk - rowptr[r]);
-  return (slot * bb + e) * kWave + (r % kWave);
+  return sell_vpos<double>(slot, e, bb, r % kWave);
 }
 
 // Thread (i, component): at the head i of each run of equal block keys, adds
@@ -211,7 +211,7 @@ __global__ void k_bc_apply(int n, int b, const int* __restrict__ slice_ptr, cons
       const int c = col[slot * kWave + lane];
       for (int k2 = 0; k2 < b; ++k2) {
         const bool colbc = isbc[c * b + k2] != 0;
-        const size_t pos = (slot * bb + k1 * b + k2) * kWave + lane;
+        const size_t pos = sell_vpos<double>(slot, k1 * b + k2, bb, lane);
         if (colbc && !rowbc) {
           const double t = bcval[c * b + k2] * val[pos];
           acc = acc - t;  // B(n) = B(n) - val*A(m)  (spmat_handler_util.f90:636)
@@ -233,7 +233,7 @@ __global__ void k_sell_gather(int n, int b, const int* __restrict__ slice_ptr, c
   const int bb = b * b;
   for (int k = 0; k < len; ++k)
     for (int e = 0; e < bb; ++e)
-      A[((size_t)j0 + k) * bb + e] = val[(((size_t)k0 + k) * bb + e) * kWave + lane];
+      A[((size_t)j0 + k) * bb + e] = val[sell_vpos<double>((size_t)k0 + k, e, bb, lane)];
 }
 
 int require_assemblable(const Matrix& M, const char* what) {

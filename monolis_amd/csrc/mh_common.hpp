@@ -209,6 +209,59 @@ __device__ __forceinline__ zd ld_val(const zd* p) {
   return zd(v.x, v.y);
 }
 
+// SELL value layout inside one slot (B*B block entries x 64 lanes).  FP64
+// values are stored in pairs: entries (2q, 2q+1) of a lane side by side, so one
+// 16-byte load per lane fetches both and a wavefront's pair load is one
+// contiguous 1 KiB request; for an odd B*B the last entry has a plane of its
+// own after the pairs.  Measured on the 100^3 BCSR3 SpMV: 325 -> 319 us (FP64).
+// FP32 values keep one plane per entry: pairs of floats measured 1 % slower
+// (162.5 vs 160.8 us); MH_PAIR_F32=1 selects them.  16-byte complex values keep
+// one plane per entry.
+#ifndef MH_PAIR_F32
+#define MH_PAIR_F32 0
+#endif
+template <typename TV>
+__host__ __device__ constexpr bool sell_paired() {
+  return sizeof(TV) == 8 || (sizeof(TV) == 4 && MH_PAIR_F32 != 0);
+}
+// Offset of entry e of lane `lane` from the start of its slot.
+template <typename TV>
+__host__ __device__ __forceinline__ size_t sell_vofs(int e, int bb, int lane) {
+  if constexpr (!sell_paired<TV>()) {
+    return (size_t)e * kWave + lane;
+  } else {
+    const int np2 = bb & ~1;  // entries held in pairs
+    return e < np2 ? (size_t)(e >> 1) * (2 * kWave) + 2 * lane + (e & 1) : (size_t)np2 * kWave + lane;
+  }
+}
+// Position of entry e of slot `slot`, lane `lane`, in the value array.
+template <typename TV>
+__host__ __device__ __forceinline__ size_t sell_vpos(size_t slot, int e, int bb, int lane) {
+  return slot * (size_t)bb * kWave + sell_vofs<TV>(e, bb, lane);
+}
+
+// Loads the B*B entries of one slot for lane `lane` (slot: start of the slot).
+template <int BB, bool NT, typename TV>
+__device__ __forceinline__ void ld_slot(const TV* __restrict__ slot, int lane, TV (&a)[BB]) {
+  if constexpr (sell_paired<TV>()) {
+    using P2 = TV __attribute__((ext_vector_type(2)));
+    constexpr int NP = BB / 2;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const P2* p = (const P2*)(slot + (size_t)q * (2 * kWave)) + lane;
+      P2 v;
+      if constexpr (NT) v = __builtin_nontemporal_load(p);
+      else v = *p;
+      a[2 * q] = v.x;
+      a[2 * q + 1] = v.y;
+    }
+    if constexpr (BB & 1) a[BB - 1] = ld_val<NT>(slot + (size_t)(2 * NP) * kWave + lane);
+  } else {
+#pragma unroll
+    for (int e = 0; e < BB; ++e) a[e] = ld_val<NT>(slot + (size_t)e * kWave + lane);
+  }
+}
+
 // Non-temporal store of one element (16-byte complex values as one vector store).
 template <typename T>
 __device__ __forceinline__ void st_stream(T* p, T v) {

@@ -372,7 +372,7 @@ __global__ void k_zdiag_setup(int n, int b, const int* __restrict__ slice_ptr, c
     const size_t slot = (size_t)(k0 + k);
     if (col[slot * kWave + lane] == i) {
       found = 1;
-      for (int e = 0; e < bb; ++e) D[(size_t)e * N + i] = val[(slot * bb + e) * kWave + lane];
+      for (int e = 0; e < bb; ++e) D[(size_t)e * N + i] = val[sell_vpos<zd>(slot, e, bb, lane)];
     }
   }
   if (!found) return;

@@ -7,9 +7,11 @@
 //   slot k of lane l stores the k-th block of row 64*s+l (CSR order, so columns
 //   stay ascending).  For slot t = slice_ptr[s] + k:
 //     col[t*64 + l]                 0-based block column
-//     val[(t*B*B + e)*64 + l]       block entry e = r*B + c (row-major block)
-//   so every load of a slot is one contiguous 256 B (int) or 512 B (double)
-//   line-aligned request for the whole wavefront.  Slots past a row's length
+//     val[t*B*B*64 + sell_vofs(e)]  block entry e = r*B + c (row-major block);
+//                                   entries 2q, 2q+1 of a lane side by side
+//                                   (real values), the odd last one alone
+//   so every load of a slot is one contiguous 256 B (int), 512 B (double) or
+//   1 KiB (double pair) line-aligned request for the whole wavefront.  Slots past a row's length
 //   are padding (col = own row, val = 0); a padded slot is loaded but masked
 //   off by the row length taken from monolis's `index` array.  When every
 //   |col - row| fits in 16 bits the SpMV reads the columns from a 16-bit delta
@@ -87,7 +89,7 @@ __device__ __forceinline__ void sell_row_impl(const SellView<TV>& M, int s, int 
   const int w = M.slice_ptr[s + 1] - k0;
   const int row = s * kWave + lane;
   const bool valid = row < M.nrows;
-  const TV* valp = M.val + (size_t)k0 * kWave * (B * B) + lane;
+  const TV* vals = M.val + (size_t)k0 * kWave * (B * B);
   int k = 0;
   for (; k + U <= w; k += U) {
     int c[U];
@@ -96,9 +98,7 @@ __device__ __forceinline__ void sell_row_impl(const SellView<TV>& M, int s, int 
 #pragma unroll
     for (int u = 0; u < U; ++u) c[u] = sell_col<NT, C16>(M, (size_t)(k0 + k + u), lane, row, valid);
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int e = 0; e < B * B; ++e) a[u][e] = ld_val<NT>(valp + ((size_t)(k + u) * (B * B) + e) * kWave);
+    for (int u = 0; u < U; ++u) ld_slot<B * B, NT>(vals + (size_t)(k + u) * (B * B) * kWave, lane, a[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) xg(c[u], xv[u]);
     const int lim = row_len(len, k);
@@ -117,8 +117,7 @@ __device__ __forceinline__ void sell_row_impl(const SellView<TV>& M, int s, int 
   for (; k < w; ++k) {
     const int c = sell_col<NT, C16>(M, (size_t)(k0 + k), lane, row, valid);
     TV a[B * B];
-#pragma unroll
-    for (int e = 0; e < B * B; ++e) a[e] = ld_val<NT>(valp + ((size_t)k * (B * B) + e) * kWave);
+    ld_slot<B * B, NT>(vals + (size_t)k * (B * B) * kWave, lane, a);
     TA xv[B];
     xg(c, xv);
     const bool on = k < row_len(len, k);
@@ -162,16 +161,14 @@ __device__ __forceinline__ void sell_row_pipe_impl(const SellView<TV>& M, int s,
   const int w = M.slice_ptr[s + 1] - k0;
   const int row = s * kWave + lane;
   const bool valid = row < M.nrows;
-  const TV* valp = M.val + (size_t)k0 * kWave * (B * B) + lane;
+  const TV* vals = M.val + (size_t)k0 * kWave * (B * B);
   const int ns = w / U;  // full steps
   auto fetch = [&](SellStep<TV, B, U>& st, int j) {
     const int k = j * U;
 #pragma unroll
     for (int u = 0; u < U; ++u) st.c[u] = sell_col<NT, C16>(M, (size_t)(k0 + k + u), lane, row, valid);
 #pragma unroll
-    for (int u = 0; u < U; ++u)
-#pragma unroll
-      for (int e = 0; e < B * B; ++e) st.a[u][e] = ld_val<NT>(valp + ((size_t)(k + u) * (B * B) + e) * kWave);
+    for (int u = 0; u < U; ++u) ld_slot<B * B, NT>(vals + (size_t)(k + u) * (B * B) * kWave, lane, st.a[u]);
   };
   // step j from buffer `cur`; if nxt != nullptr, prefetch step jn into it
   auto step = [&](const SellStep<TV, B, U>& cur, int j, SellStep<TV, B, U>* nxt, int jn) {
@@ -213,8 +210,7 @@ __device__ __forceinline__ void sell_row_pipe_impl(const SellView<TV>& M, int s,
   for (int k = ns * U; k < w; ++k) {
     const int c = sell_col<NT, C16>(M, (size_t)(k0 + k), lane, row, valid);
     TV a[B * B];
-#pragma unroll
-    for (int e = 0; e < B * B; ++e) a[e] = ld_val<NT>(valp + ((size_t)k * (B * B) + e) * kWave);
+    ld_slot<B * B, NT>(vals + (size_t)k * (B * B) * kWave, lane, a);
     TA xv[B];
     xg(c, xv);
     const bool on = k < row_len(len, k);

@@ -159,11 +159,10 @@ __global__ void k_sell_fill(int n, int nslices, int b, const int* __restrict__ r
     const size_t slot = (size_t)(k0 + k);
     if (k < len) {
       if (fill_col) col[slot * kWave + lane] = item[j0 + k] - 1;
-      for (int e = 0; e < bb; ++e)
-        val[(slot * bb + e) * kWave + lane] = (TV)A[(size_t)(j0 + k) * bb + e];
+      for (int e = 0; e < bb; ++e) val[sell_vpos<TV>(slot, e, bb, lane)] = (TV)A[(size_t)(j0 + k) * bb + e];
     } else {
       if (fill_col) col[slot * kWave + lane] = row < n ? row : 0;
-      for (int e = 0; e < bb; ++e) val[(slot * bb + e) * kWave + lane] = (TV)0;
+      for (int e = 0; e < bb; ++e) val[sell_vpos<TV>(slot, e, bb, lane)] = (TV)0;
     }
   }
 }
@@ -203,6 +202,22 @@ __global__ void k_col16_fill(int n, int nslices, const int* __restrict__ slice_p
 __global__ void k_to_fp32(size_t m, const double* __restrict__ a, float* __restrict__ b) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
     b[i] = (float)a[i];
+}
+
+// FP32 copy of the SELL values; position i of the FP32 array holds the entry
+// (slot, e, lane) that its own layout puts there (the FP64 and FP32 value
+// layouts differ when only 8-byte values are stored in pairs).
+__global__ void k_vals_to_fp32(size_t m, int bb, const double* __restrict__ a, float* __restrict__ b) {
+  const size_t per = (size_t)bb * kWave;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
+    if constexpr (sell_paired<float>() == sell_paired<double>()) {
+      b[i] = (float)a[i];
+    } else {  // FP32 planes one per entry: i = slot*per + e*64 + lane
+      const size_t slot = i / per;
+      const int r = (int)(i % per);
+      b[i] = (float)a[sell_vpos<double>(slot, r / kWave, bb, r % kWave)];
+    }
+  }
 }
 
 static int grid_1d(size_t m) {
@@ -412,7 +427,7 @@ int matrix_make_fp32(Matrix* M, hipStream_t s) {
   }
   if (M->val32_ready) return 0;
   if (M->val32.n < M->val64.n) MH_TRY(M->val32.alloc(M->val64.n));
-  k_to_fp32<<<grid_1d(M->val64.n), kBlock, 0, s>>>(M->val64.n, M->val64.p, M->val32.p);
+  k_vals_to_fp32<<<grid_1d(M->val64.n), kBlock, 0, s>>>(M->val64.n, M->b * M->b, M->val64.p, M->val32.p);
   MH_HIP(hipGetLastError());
   M->val32_ready = true;
   return 0;
@@ -574,7 +589,7 @@ __global__ void k_diag_setup(int n, int b, const int* __restrict__ slice_ptr, co
     const size_t slot = (size_t)(k0 + k);
     if (col[slot * kWave + lane] == i) {
       found = 1;
-      for (int e = 0; e < bb; ++e) D[(size_t)e * N + i] = val[(slot * bb + e) * kWave + lane];
+      for (int e = 0; e < bb; ++e) D[(size_t)e * N + i] = val[sell_vpos<double>(slot, e, bb, lane)];
     }
   }
   if (!found && b != 3) return;
