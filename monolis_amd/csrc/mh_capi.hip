@@ -397,7 +397,7 @@ int monolis_hip_solve_refined_R(monolis_hip_matrix* H, const double* b, double* 
   P.precond = Iarray[I_PRECOND];
   P.init_x = Iarray[I_IS_INIT_X] != 0;
   mh::RefineResult R;
-  const int rc = mh::refine_solve(*H->M, P, b, x, H->W, &R, c->stream);
+  const int rc = mh::refine_solve(*H->M, P, b, x, H->W, H->W_outer, &R, c->stream);
   if (rc) {
     Iarray[I_IERR] = ST_HIP;
     return rc;

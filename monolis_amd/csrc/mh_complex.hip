@@ -62,9 +62,14 @@ __device__ __forceinline__ void zdiag_apply(const zd* __restrict__ D, int nrows,
 // Solver scalars in device memory.
 struct ZState {
   zd rho, alpha, beta, B2;
-  double resid_abs, resid_re, tol;
-  int iter, max_iter, converged, done;
+  double resid_abs;  // resid_abs, iter, done: the PollView the host polls
+  int iter, done;
+  double resid_re, tol;
+  int max_iter, converged;
 };
+static_assert(offsetof(ZState, iter) == offsetof(ZState, resid_abs) + 8 &&
+                  offsetof(ZState, done) == offsetof(ZState, resid_abs) + 12,
+              "ZState::resid_abs, iter, done must form a PollView");
 
 __global__ void k_zstate_init(ZState* st, double tol, int max_iter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -460,10 +465,10 @@ int cocg_impl(Matrix& M, const SolveParams& P, const zd* b, zd* x, Workspace& W,
     return launch_rows(n, op, parts, s);
   };
   MH_TRY(upd(0, F_FIRST));  // z = M^-1 r, rho of iteration 1
-  Poller poll{c, &st->done};
+  Poller poll{c, &st->resid_abs, P.tol};
   int done = 0;
   for (int it = 1; it <= P.max_iter && !done;) {
-    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    const int end = std::min(it + poll.next_chunk(it, P.chunk) - 1, P.max_iter);
     for (; it <= end; ++it) {
       OpZP pu{z, p, st, zd(0.0)};
       MH_TRY(launch_rows((int)m, pu, parts, s));

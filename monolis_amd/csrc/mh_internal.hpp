@@ -21,10 +21,19 @@ constexpr int kMaxB = 8;
 // Per-process device context: one HIP stream at default priority, pinned host
 // slots for polling solver flags, and a reusable pool of reduction scratch.
 // ---------------------------------------------------------------------------
+// What the host reads of a running solve: the last tested residual, its
+// iteration and the done flag (laid out as in KState / ZState).
+struct PollView {
+  double resid;
+  int iter;
+  int done;
+};
+
 struct Context {
   int device = -1;
   hipStream_t stream = nullptr;
   int* h_flags = nullptr;            // pinned host, kFlagSlots ints
+  PollView* h_poll = nullptr;        // pinned host, kFlagSlots solver progress views
   hipEvent_t ev[16] = {};
   bool ok = false;
 };
@@ -59,20 +68,25 @@ struct KState {
   double pq;        // (p, q) / (rt, v)
   double rr;        // (r, r)
   double B2;        // (r0, r0), denominator of the relative residual
+  // resid, iter and done are adjacent: the host polls them as one 16-byte PollView
   double resid;     // sqrt(rr / B2)
+  int iter;         // last iteration whose convergence test ran
+  int done;
   double tol;
   double ts, tt;    // BiCGSTAB (t, s), (t, t)
   double gamma, delta;  // PipeCG (r, u), (v, u)
   double aux[4];
-  int iter;         // last iteration whose convergence test ran
   int max_iter;
   int converged;
-  int done;
   int breakdown;    // bit mask of exact-zero dot products seen (BiCGSTAB)
   int bd_stage;     // != 0: the loop stopped for the sorted-sum fallback at this stage
   int bd_iter;      // ... in this iteration
   int pad;
 };
+
+static_assert(offsetof(KState, iter) == offsetof(KState, resid) + 8 &&
+                  offsetof(KState, done) == offsetof(KState, resid) + 12,
+              "KState::resid, iter, done must form a PollView");
 
 // ---------------------------------------------------------------------------
 // Variable block sizes (monolis n_dof = -1): the matrix is stored as the scalar
@@ -201,6 +215,7 @@ struct SolveParams {
   double tol = 1e-8;
   int chunk = 8;         // iterations enqueued between convergence polls
   int keep_hist = 1;     // record per-iteration residuals
+  int rhs_nonzero = 0;   // caller guarantees b - A x0 != 0: skip the zero right-hand-side read-back
 };
 
 struct SolveResult {
@@ -243,7 +258,7 @@ struct RefineResult {
   std::vector<double> outer_resid;
   std::vector<int> inner_iters;
 };
-int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, Workspace& W,
+int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, Workspace& W, Workspace& Wo,
                  RefineResult* out, hipStream_t s);
 
 }  // namespace mh

@@ -31,6 +31,17 @@ constexpr int kIterRR = 200;
 enum : int { F_UPD = 1, F_CHECK = 2, F_FIRST = 4, F_XONLY = 8, F_EXTPREC = 16 };
 
 // ---- shared finalize: monolis_check_converge_R --------------------------------
+// B2 = (r0, r0) read back for the zero right-hand-side test (mat_converge.f90:40-48).
+// Callers that know r0 != 0 (the inner solves of the refinement, whose right-hand
+// side is a normalised nonzero residual) skip the host round trip.
+int read_B2(const SolveParams& P, const KState* st, double* B2, hipStream_t s) {
+  *B2 = 1.0;
+  if (P.rhs_nonzero) return 0;
+  MH_HIP(hipMemcpyAsync(B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 __device__ __forceinline__ bool finalize_check(KState* st, double* hist, int it, double rr) {
   const double resid = sqrt(rr / st->B2);
   st->rr = rr;
@@ -1140,8 +1151,7 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
     MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
   }
   double B2 = 0.0;
-  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
-  MH_HIP(hipStreamSynchronize(s));
+  MH_TRY(read_B2(P, st, &B2, s));
   out->hist.clear();
   if (B2 == 0.0) {  // mat_converge.f90:40-48: X = 0, converged, no iteration
     MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
@@ -1172,10 +1182,10 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
     return launch_rows(n, op, parts, s);
   };
   MH_TRY(upd(0, F_FIRST));
-  Poller poll{c, &st->done};
+  Poller poll{c, &st->resid, P.tol};
   int done = 0;
   for (int it = 1; it <= P.max_iter && !done;) {
-    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    const int end = std::min(it + poll.next_chunk(it, P.chunk) - 1, P.max_iter);
     for (; it <= end; ++it) {
       OpCgP<T> pu{z, p, st, (T)0};
       MH_TRY(launch_rows((int)m, pu, parts, s));
@@ -1270,8 +1280,7 @@ int gropcg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W,
     MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
   }
   double B2 = 0.0;
-  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
-  MH_HIP(hipStreamSynchronize(s));
+  MH_TRY(read_B2(P, st, &B2, s));
   out->hist.clear();
   if (B2 == 0.0) {  // mat_converge.f90:40-48
     MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
@@ -1290,10 +1299,10 @@ int gropcg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W,
     MH_TRY(launch_rows(n, op, parts, s));
   }
   MH_TRY(spmv_step());  // beta = 0 from the state init, p = s = 0
-  Poller poll{c, &st->done};
+  Poller poll{c, &st->resid, P.tol};
   int done = 0;
   for (int it = 1; it <= P.max_iter && !done;) {
-    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    const int end = std::min(it + poll.next_chunk(it, P.chunk) - 1, P.max_iter);
     for (; it <= end; ++it) {
       OpGropUpd<T, B> up{x, r, u, p, sv, q, st, hp, it, (T)0};
       MH_TRY(launch_rows(n, up, parts, s));
@@ -1379,8 +1388,7 @@ int pipecg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W,
     MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
   }
   double B2 = 0.0;
-  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
-  MH_HIP(hipStreamSynchronize(s));
+  MH_TRY(read_B2(P, st, &B2, s));
   out->hist.clear();
   if (B2 == 0.0) {  // mat_converge.f90:40-48
     MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
@@ -1397,10 +1405,10 @@ int pipecg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W,
     OpSpmvSolve<T, B> sp{u, v, st};
     MH_TRY((launch_sell<T, B>(M, sp, parts, var, s)));
   }
-  Poller poll{c, &st->done};
+  Poller poll{c, &st->resid, P.tol};
   int done = 0;
   for (int it = 1; it <= P.max_iter && !done;) {
-    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    const int end = std::min(it + poll.next_chunk(it, P.chunk) - 1, P.max_iter);
     for (; it <= end; ++it) {
       if (vprec) {
         OpPipeA<T, B, false> a{r, u, v, mm, nullptr, n, st, it == 1};
@@ -1502,8 +1510,7 @@ int pipecr_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W,
     MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
   }
   double B2 = 0.0;
-  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
-  MH_HIP(hipStreamSynchronize(s));
+  MH_TRY(read_B2(P, st, &B2, s));
   out->hist.clear();
   if (B2 == 0.0) {  // as mat_converge.f90:40-48 for the other methods (PipeCR.f90 itself divides 0/0)
     MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
@@ -1522,10 +1529,10 @@ int pipecr_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W,
     OpSpmvSolve<T, B> sp{u, v, st};
     MH_TRY((launch_sell<T, B>(M, sp, parts, var, s)));
   }
-  Poller poll{c, &st->done};
+  Poller poll{c, &st->resid, P.tol};
   int done = 0;
   for (int it = 1; it <= P.max_iter && !done;) {
-    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    const int end = std::min(it + poll.next_chunk(it, P.chunk) - 1, P.max_iter);
     for (; it <= end; ++it) {
       if (vprec) {
         MH_TRY(vapply(v, mm));
@@ -1639,8 +1646,7 @@ int pipebicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspa
     MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
   }
   double B2 = 0.0;
-  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
-  MH_HIP(hipStreamSynchronize(s));
+  MH_TRY(read_B2(P, st, &B2, s));
   out->hist.clear();
   if (B2 == 0.0) {  // mat_converge.f90:40-48
     MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
@@ -1655,10 +1661,10 @@ int pipebicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspa
   MH_TRY(spmv(rt, w0));     // W0 = A RT
   MH_TRY(rows_init(1));     // WT = M^-1 W0, RR, RW, alpha
   MH_TRY(spmv(wt, t));      // T = A WT
-  Poller poll{c, &st->done};
+  Poller poll{c, &st->resid, P.tol};
   int done = 0;
   for (int it = 1; it <= P.max_iter && !done;) {
-    const int end = std::min(it + P.chunk - 1, P.max_iter);
+    const int end = std::min(it + poll.next_chunk(it, P.chunk) - 1, P.max_iter);
     for (; it <= end; ++it) {
       if (vprec) {
         OpPbA<T, B, false> a{pt, sv, stv, z, zt, q, qt, y, rt, w0, wt, t, v, r, nullptr, n, st, (T)0, (T)0, (T)0};
@@ -1739,8 +1745,7 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
     MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
   }
   double B2 = 0.0;
-  MH_HIP(hipMemcpyAsync(&B2, &st->B2, sizeof(double), hipMemcpyDeviceToHost, s));
-  MH_HIP(hipStreamSynchronize(s));
+  MH_TRY(read_B2(P, st, &B2, s));
   out->hist.clear();
   if (B2 == 0.0) {
     MH_HIP(hipMemsetAsync(x, 0, sizeof(T) * m, s));
@@ -1776,10 +1781,10 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   KState hs;
   out->fallbacks = 0;
   for (;;) {
-    Poller poll{c, &st->done};
+    Poller poll{c, &st->resid, P.tol};
     int done = 0;
     while (it <= P.max_iter && !done) {
-      const int end = std::min(it + P.chunk - 1, P.max_iter);
+      const int end = std::min(it + poll.next_chunk(it, P.chunk) - 1, P.max_iter);
       for (; it <= end; ++it, stage = 0) {
         if (stage <= 0) {
           if (vprec) {
@@ -1954,20 +1959,19 @@ static int grid_1d(size_t m) {
   return (int)(g > 0 ? g : 1);
 }
 
-int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, Workspace& W, RefineResult* out,
-                 hipStream_t s) {
+int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, Workspace& W, Workspace& Wo,
+                 RefineResult* out, hipStream_t s) {
   const size_t vec = (size_t)M.np * M.b;
   const size_t m = (size_t)M.n * M.b;
-  // outer buffers live in their own allocation; W serves the inner solve
-  DevArray<double> r64, scal;
-  DevArray<float> r32, d32;
-  MH_TRY(r64.alloc(vec));
-  MH_TRY(r32.alloc(vec));
-  MH_TRY(d32.alloc(vec));
+  // outer buffers live in their own workspace (kept across calls); W serves the inner solve
   const size_t nparts = partials_needed((long long)M.nslices * kWave) * 2;
-  MH_TRY(scal.alloc(nparts + 64));
-  double* parts = scal.p;
-  double* rr_dev = scal.p + nparts;
+  MH_TRY(Wo.ensure(vec * (sizeof(double) + 2 * sizeof(float)) + (nparts + 64) * sizeof(double) + 4 * 256));
+  Carve cv{Wo.raw.p};
+  double* r64 = cv.take<double>(vec);
+  float* r32 = cv.take<float>(vec);
+  float* d32 = cv.take<float>(vec);
+  double* parts = cv.take<double>(nparts + 64);
+  double* rr_dev = parts + nparts;
   MH_TRY(matrix_make_fp32(&M, s));
   if (P.precond == 1) {
     MH_TRY(diag_setup<double>(M, s));
@@ -1975,10 +1979,10 @@ int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, W
   }
   if (P.init_x) MH_HIP(hipMemsetAsync(x, 0, sizeof(double) * m, s));
   if (vec > m) {
-    MH_HIP(hipMemsetAsync(r32.p + m, 0, sizeof(float) * (vec - m), s));
-    MH_HIP(hipMemsetAsync(d32.p + m, 0, sizeof(float) * (vec - m), s));
+    MH_HIP(hipMemsetAsync(r32 + m, 0, sizeof(float) * (vec - m), s));
+    MH_HIP(hipMemsetAsync(d32 + m, 0, sizeof(float) * (vec - m), s));
   }
-  MH_TRY(residual<double>(M, x, b, r64.p, parts, rr_dev, s));
+  MH_TRY(residual<double>(M, x, b, r64, parts, rr_dev, s));
   double rr = 0.0;
   MH_HIP(hipMemcpyAsync(&rr, rr_dev, sizeof(double), hipMemcpyDeviceToHost, s));
   MH_HIP(hipStreamSynchronize(s));
@@ -2003,17 +2007,18 @@ int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, W
   ip.init_x = 1;
   ip.tol = P.inner_tol;
   ip.keep_hist = 0;
+  ip.rhs_nonzero = 1;  // r / ||r|| with ||r|| > 0
   for (int k = 1; k <= P.max_outer; ++k) {
     const double nr = std::sqrt(rr);
-    k_scale_to_fp32<<<grid_1d(m), kBlock, 0, s>>>(m, r64.p, 1.0 / nr, r32.p);
+    k_scale_to_fp32<<<grid_1d(m), kBlock, 0, s>>>(m, r64, 1.0 / nr, r32);
     MH_HIP(hipGetLastError());
     SolveResult ir;
-    MH_TRY(krylov_solve<float>(M, ip, r32.p, d32.p, W, &ir, s));
+    MH_TRY(krylov_solve<float>(M, ip, r32, d32, W, &ir, s));
     out->inner_iters.push_back(ir.iter);
     out->inner_total += ir.iter > 0 ? ir.iter : 0;
-    k_add_scaled<<<grid_1d(m), kBlock, 0, s>>>(m, x, d32.p, nr);
+    k_add_scaled<<<grid_1d(m), kBlock, 0, s>>>(m, x, d32, nr);
     MH_HIP(hipGetLastError());
-    MH_TRY(residual<double>(M, x, b, r64.p, parts, rr_dev, s));
+    MH_TRY(residual<double>(M, x, b, r64, parts, rr_dev, s));
     MH_HIP(hipMemcpyAsync(&rr, rr_dev, sizeof(double), hipMemcpyDeviceToHost, s));
     MH_HIP(hipStreamSynchronize(s));
     resid = std::sqrt(rr / B2);

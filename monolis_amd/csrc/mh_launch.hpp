@@ -2,6 +2,7 @@
 // row-parallel kernels, each followed by its reduction finalize kernel.
 #pragma once
 
+#include <cmath>
 #include <deque>
 
 #include "mh_internal.hpp"
@@ -81,18 +82,26 @@ struct Carve {
   }
 };
 
-// Polls a device `done` flag one chunk behind the enqueue front: each issue()
-// copies the flag into a pinned host slot and records an event after it.
+// Convergence polling one chunk behind the device: after each chunk of
+// iterations the solver enqueues a 16-byte copy of its PollView and waits for
+// the previous chunk's.  The chunk length adapts: from the last two polled
+// residuals the geometric rate predicts the iterations still needed for `tol`,
+// and the next chunk covers no more than those (at least kMinChunk), so few
+// iterations of no-op launches follow convergence.
 struct Poller {
+  static constexpr int kMinChunk = 2;
   Context* c;
-  const int* done_flag;
+  const void* view;  // device PollView (KState::resid / ZState::resid_abs)
+  double tol;
   std::deque<int> pending;
   int next = 0;
+  int n_seen = 0;
+  PollView seen[2] = {};  // the last two polled views, oldest first
   int issue(hipStream_t s) {
     const int slot = next;
     next = (next + 1) % kFlagSlots;
-    c->h_flags[slot] = 0;
-    MH_HIP(hipMemcpyAsync(&c->h_flags[slot], done_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    c->h_poll[slot] = PollView{0.0, 0, 0};
+    MH_HIP(hipMemcpyAsync(&c->h_poll[slot], view, sizeof(PollView), hipMemcpyDeviceToHost, s));
     MH_HIP(hipEventRecord(c->ev[slot], s));
     pending.push_back(slot);
     return 0;
@@ -104,13 +113,32 @@ struct Poller {
       const int slot = pending.front();
       pending.pop_front();
       MH_HIP(hipEventSynchronize(c->ev[slot]));
-      if (c->h_flags[slot]) {
+      const PollView v = c->h_poll[slot];
+      if (v.done) {
         *done = 1;
         pending.clear();
         return 0;
       }
+      if (v.iter > 0 && (n_seen == 0 || v.iter > seen[1].iter)) {
+        seen[0] = seen[1];
+        seen[1] = v;
+        ++n_seen;
+      }
     }
     return 0;
+  }
+  // Iterations to enqueue next, the first of them being `it`.
+  int next_chunk(int it, int max_chunk) const {
+    if (max_chunk <= kMinChunk || n_seen < 2 || !(tol > 0.0)) return max_chunk;
+    const PollView& a = seen[0];
+    const PollView& b = seen[1];
+    if (!(b.resid > 0.0) || !(a.resid > b.resid) || b.iter <= a.iter) return max_chunk;
+    if (b.resid <= tol) return kMinChunk;
+    const double rate = std::log(b.resid / a.resid) / (double)(b.iter - a.iter);  // < 0
+    const double need = std::log(tol / b.resid) / rate;                           // iterations after b
+    const double left = std::ceil(need) - (double)(it - 1 - b.iter);
+    if (!(left < (double)max_chunk)) return max_chunk;
+    return left < (double)kMinChunk ? kMinChunk : (int)left;
   }
 };
 

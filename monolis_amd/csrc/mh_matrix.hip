@@ -43,6 +43,7 @@ int context(Context** out) {
     MH_HIP(hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, g_ctx.device));
     MH_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     MH_HIP(hipHostMalloc((void**)&g_ctx.h_flags, sizeof(int) * kFlagSlots, hipHostMallocDefault));
+    MH_HIP(hipHostMalloc((void**)&g_ctx.h_poll, sizeof(PollView) * kFlagSlots, hipHostMallocDefault));
     for (int i = 0; i < kFlagSlots; ++i) MH_HIP(hipEventCreateWithFlags(&g_ctx.ev[i], hipEventDisableTiming));
     g_ctx.ok = true;
   }
