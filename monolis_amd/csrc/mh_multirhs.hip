@@ -58,6 +58,31 @@ __device__ __forceinline__ void unit_range(long long units, long long* u0, long 
   *u1 = units * (t + 1) / g;
 }
 
+// Unit order of the SpMM; each XCD owns a contiguous part of the units.
+//   0: every workgroup a contiguous sub-range, its wavefronts interleaved;
+//   1: the XCD's workgroups interleaved unit by unit, so the resident workgroups
+//      of an XCD sweep one narrow band of rows and share the x rows they gather
+//      in that XCD's L2.
+#ifndef MH_SPMM_ORDER
+#define MH_SPMM_ORDER 0
+#endif
+__device__ __forceinline__ void spmm_units(long long units, int w, long long* first, long long* stride,
+                                           long long* end) {
+  if constexpr (MH_SPMM_ORDER == 0) {
+    long long u0, u1;
+    unit_range(units, &u0, &u1);
+    *first = u0 + w;
+    *stride = kWpb;
+    *end = u1;
+  } else {
+    const int per = gridDim.x / kXcd;
+    const int xcd = blockIdx.x % kXcd, l = blockIdx.x / kXcd;
+    *first = units * xcd / kXcd + (long long)l * kWpb + w;
+    *stride = (long long)per * kWpb;
+    *end = units * (xcd + 1) / kXcd;
+  }
+}
+
 // Per right-hand side sum of the lanes' partials over the workgroup (fixed
 // order: wavefronts, then row sub-slots); wave 0 writes 64 slots per value.
 template <int NV>
@@ -131,9 +156,9 @@ __global__ void __launch_bounds__(kBlock) k_spmm(int n, int k, int nslices, cons
   const int xj = on ? j : 0;
   const int lt = lane >> 3, lr = lane & 7;  // slot / row of the loaded tile entry
   double part[1] = {0.0};
-  long long u0, u1;
-  unit_range((long long)nslices * 8, &u0, &u1);
-  for (long long u = u0 + w; u < u1; u += kWpb) {
+  long long ufirst, ustride, uend;
+  spmm_units((long long)nslices * 8, w, &ufirst, &ustride, &uend);
+  for (long long u = ufirst; u < uend; u += ustride) {
     const int s = (int)(u >> 3);
     const int rbase = (int)(u & 7) * 8;
     const int row0 = s * kWave + rbase;
