@@ -18,6 +18,11 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libmonolis_hip.so")
+# monolis's user-facing C API (include/monolis.h) over the flat entry points:
+# host C++ in its own library, so that programs keeping monolis's own C
+# front-ends can interpose libmonolis_hip.so alone
+FRONT_SRC = "mh_front.cpp"
+FRONT_LIB = os.path.join(LIBDIR, "libmonolis_front.so")
 
 SOURCES = ["mh_matrix.hip", "mh_krylov.hip", "mh_capi.hip", "mh_sorted.hip", "mh_batch.hip", "mh_multirhs.hip", "mh_adjoint.hip", "mh_vblock.hip", "mh_complex.hip", "mh_assembly.hip"]
 
@@ -48,6 +53,23 @@ def _compile(src: str) -> str:
 BINDIR = os.path.join(HERE, "bin")
 AMDFLANG = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
 CC = shutil.which("gcc") or "cc"
+CXX = shutil.which("g++") or "c++"
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_front(verbose: bool = True) -> str:
+    """libmonolis_front.so: monolis's C API (MONOLIS, monolis_initialize, the
+    pattern / assembly calls, monolis_solve_R ...) over libmonolis_hip.so."""
+    tmp = FRONT_LIB + ".tmp"
+    cmd = [CXX, "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-I", INCLUDE, os.path.join(CSRC, FRONT_SRC),
+           "-L", LIBDIR, "-lmonolis_hip", "-Wl,-rpath,$ORIGIN", "-Wl,--no-undefined", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"front-end build failed:\n{r.stderr}")
+    os.replace(tmp, FRONT_LIB)
+    if verbose:
+        print(f"[monolis_amd] built {FRONT_LIB}")
+    return FRONT_LIB
 
 
 def build_callers(verbose: bool = True) -> list:
@@ -64,6 +86,14 @@ def build_callers(verbose: bool = True) -> list:
     if r.returncode != 0:
         raise RuntimeError(f"C caller build failed:\n{r.stderr}")
     built.append(c_exe)
+    # a C program written against monolis's C API, linked against the front-end library only
+    f_exe = os.path.join(BINDIR, "test_front")
+    cmd = [CC, "-O2", "-std=c11", "-I", INCLUDE, os.path.join(HERE, "c", "test_front.c"), "-L", LIBDIR,
+           "-lmonolis_front", rpath, "-lm", "-o", f_exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C front-end caller build failed:\n{r.stderr}")
+    built.append(f_exe)
     if os.path.exists(AMDFLANG):
         mod_obj = os.path.join(BUILD, "mod_monolis_hip.o")
         r = subprocess.run([AMDFLANG, "-O2", "-c", os.path.join(HERE, "fortran", "mod_monolis_hip.f90"),
@@ -99,6 +129,7 @@ def build(verbose: bool = True, out: str = None) -> str:
     if verbose:
         print(f"[monolis_amd] built {target}")
     if out is None:
+        build_front(verbose)
         build_callers(verbose)
     return target
 

@@ -1,6 +1,7 @@
-"""CPU checks of the C-ABI boundary: the library loads (no GPU call is made)
-and exports every function declared in include/*.h, with the ctypes binding
-covering each of them."""
+"""CPU checks of the C-ABI boundary: the libraries load (no GPU call is made)
+and export every function their headers declare -- include/monolis_hip.h by
+libmonolis_hip.so, with the ctypes binding covering each of them, and
+include/monolis.h (monolis's C API) by libmonolis_front.so."""
 import ctypes
 import glob
 import os
@@ -11,9 +12,9 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_functions():
+def declared_functions(header="monolis_hip.h"):
     names = []
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+    for h in glob.glob(os.path.join(ROOT, "include", header)):
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
@@ -44,6 +45,30 @@ def test_ctypes_binding_covers_every_declared_symbol():
     bound = {s[0] for s in _lib.SIGNATURES}
     missing = [n for n in declared_functions() if n not in bound]
     assert not missing, missing
+
+
+def test_front_library_exports_monolis_c_api():
+    names = declared_functions("monolis.h")
+    for ref in ["monolis_initialize", "monolis_finalize", "monolis_get_nonzero_pattern_by_simple_mesh_R",
+                "monolis_add_scalar_to_sparse_matrix_R", "monolis_set_Dirichlet_bc_R", "monolis_solve_R",
+                "monolis_matvec_product_R", "monolis_inner_product_R", "monolis_solve_C", "monolis_set_method"]:
+        assert ref in names
+    from monolis_amd import _lib
+    front = ctypes.CDLL(os.path.join(os.path.dirname(_lib.LIB_PATH), "libmonolis_front.so"))
+    missing = [n for n in names if not hasattr(front, n)]
+    assert not missing, missing
+
+
+def test_front_c_program_host_checks():
+    """monolis_amd/c/test_front.c, a C program written against include/monolis.h
+    and linked against libmonolis_front.so: pattern, assembly, Dirichlet and
+    parameter checks (the reference's unit-test answers); no GPU call."""
+    import subprocess
+    exe = os.path.join(ROOT, "monolis_amd", "bin", "test_front")
+    assert os.path.exists(exe), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    r = subprocess.run([exe, "host"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ok (0 failed)" in r.stdout
 
 
 def test_library_is_gfx950_code():

@@ -400,6 +400,23 @@ def test_compiled_callers(exe, token):
     assert token in r.stdout
 
 
+def test_monolis_c_api_program_on_gpu():
+    """monolis_amd/c/test_front.c: a C program written against monolis's C API
+    (include/monolis.h), linked against libmonolis_front.so only -- SpMV against
+    a host loop, every real method x NONE / DIAG on the chain of
+    wrapper_test/solver/monolis_solver_c_test.c (x = 1), COCG (x = 1 + i),
+    variable blocks."""
+    import os
+    import subprocess
+    from monolis_amd import build as mb
+    path = os.path.join(mb.BINDIR, "test_front")
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (run __graft_entry__.build())")
+    r = subprocess.run([path, "gpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ok (0 failed)" in r.stdout
+
+
 # --- BiCGSTAB exact-zero dots: sorted-sum fallback and resume ---------------------------
 def test_bicgstab_breakdown_stage_d_diagonal_system():
     """A = 2I with DIAG: s = 0 exactly, so (t, t) = 0 -> sorted sum -> omega = 0
