@@ -107,6 +107,19 @@ def build_callers(verbose: bool = True) -> list:
             if r.returncode != 0:
                 raise RuntimeError(f"Fortran caller build failed:\n{r.stderr}")
             built.append(f_exe)
+        # monolis's Fortran API (mod_monolis over libmonolis_front.so) and a program using it
+        api_obj = os.path.join(BUILD, "mod_monolis.o")
+        r = subprocess.run([AMDFLANG, "-O2", "-c", os.path.join(HERE, "fortran", "mod_monolis.f90"), "-J", BUILD,
+                            "-o", api_obj], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran API module build failed:\n{r.stderr}")
+        f_exe = os.path.join(BINDIR, "test_fortran_api")
+        r = subprocess.run([AMDFLANG, "-O2", "-J", BUILD, os.path.join(HERE, "fortran", "test_fortran_api.f90"),
+                            api_obj, "-L", LIBDIR, "-lmonolis_front", rpath, "-o", f_exe],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran API caller build failed:\n{r.stderr}")
+        built.append(f_exe)
     if verbose:
         print(f"[monolis_amd] built callers {', '.join(os.path.basename(b) for b in built)}")
     return built

@@ -385,10 +385,12 @@ def test_invalid_csr_is_rejected_without_launching():
 
 # --- C and Fortran callers of the flat ABI ----------------------------------------------
 @pytest.mark.parametrize("exe,token", [("test_c_abi", "C_ABI_OK"), ("test_mtx_dat", "FORTRAN_OK"),
-                                       ("test_cocg_chain", "FORTRAN_COCG_OK")])
+                                       ("test_cocg_chain", "FORTRAN_COCG_OK"),
+                                       ("test_fortran_api", "FORTRAN_API_OK")])
 def test_compiled_callers(exe, token):
-    """Programs compiled against include/monolis_hip.h (C) and mod_monolis_hip
-    (amdflang, ISO_C_BINDING) run the reference's own test cases on the GPU."""
+    """Programs compiled against include/monolis_hip.h (C), mod_monolis_hip
+    (amdflang, ISO_C_BINDING) and mod_monolis (monolis's Fortran API over
+    libmonolis_front.so) run the reference's own test cases on the GPU."""
     import os
     import subprocess
     from monolis_amd import build as mb
