@@ -173,6 +173,13 @@ def main() -> int:
     B64 = metrics.spmv_bytes(m.n, m.nz, 3, 8)
     gbs = B64 / (ms * 1e-3) / 1e9
     log(f"SpMV FP64: {ms * 1e3:.1f} us/launch, {gbs:.0f} GB/s ({100 * gbs / metrics.HBM_PEAK_GBS:.1f} % of peak)")
+    # per-launch event pairs (outside the timed batch above): median / spread over >= 50 launches
+    each = dm.time_matvec_each(dx, dy, warmup=2, reps=max(50, min(args.steps, 200)))
+    med = float(np.median(each))
+    per_launch = {"reps": int(len(each)), "median_ms": round(med, 5), "min_ms": round(float(each.min()), 5),
+                  "max_ms": round(float(each.max()), 5), "median_gbs": round(B64 / (med * 1e-3) / 1e9, 1)}
+    log(f"SpMV FP64 per launch: median {med * 1e3:.1f} us over {len(each)} launches "
+        f"(min {each.min() * 1e3:.1f}, max {each.max() * 1e3:.1f})")
 
     dx32 = api.DeviceBuffer.from_numpy(W.uniform_vector(m.nndof, seed=args.seed + 2).astype(np.float32))
     dy32 = api.DeviceBuffer(m.nndof, np.float32)
@@ -203,6 +210,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": metrics.HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / metrics.HBM_PEAK_GBS, 4), "traffic": measured_traffic(args.grid),
                      "kernel": "k_sell<double,double,3,OpSpmv>", "algorithmic_bytes": B64},
+        "spmv_per_launch": per_launch,
         "spmv_fp32": {"ms": round(ms32, 5), "gbs": round(gbs32, 1), "algorithmic_bytes": B32},
         "stream_copy": cp,
         "autotune": {"variant_fp64": tune["best64"], "variant_fp32": tune["best32"],
@@ -282,7 +290,7 @@ def main() -> int:
         cpu_gbs = B64 / tc / 1e9
         log(f"CPU SpMV ({cores} threads): {tc * 1e3:.1f} ms, {cpu_gbs:.1f} GB/s")
         out["cpu_baseline"] = {"value": round(cpu_gbs, 2), "unit": "GB/s", "cores": oracle.get_threads(),
-                               "kind": "port",
+                               "kind": "port", "cpu_model": cpu_model(),
                                "sample": f"{reps} FP64 SpMVs of the same 100^3 BCSR3 matrix "
                                          f"({tc * 1e3:.1f} ms each)"}
         if not args.no_solve:
@@ -305,6 +313,17 @@ def main() -> int:
 
     print(json.dumps(out), flush=True)
     return 0
+
+
+def cpu_model() -> str:
+    """Host CPU model name (/proc/cpuinfo), for the CPU baseline."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _prm(method=1, precond=1, tol=1e-8, max_iter=5000):

@@ -165,6 +165,11 @@ int monolis_hip_solve_refined_R(monolis_hip_matrix* M, const double* b, double* 
  * FP32 layout (x, y are float* then). */
 int monolis_hip_time_matvec(monolis_hip_matrix* M, int fp32, const void* x, void* y, int warmup, int reps,
                             double* ms);
+/* The same with a pair of events around each of the `reps` SpMVs:
+ * ms_each[k] = milliseconds of SpMV k (for medians; the event pairs add a
+ * little per-launch overhead that the batch average above does not have). */
+int monolis_hip_time_matvec_each(monolis_hip_matrix* M, int fp32, const void* x, void* y, int warmup, int reps,
+                                 double* ms_each);
 
 /* Autotuner (no monolis counterpart): times each of the SELL SpMV kernel
  * variants (slots per step x temporal / non-temporal matrix streams) on this

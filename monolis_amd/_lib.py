@@ -66,6 +66,7 @@ SIGNATURES = [
     ("monolis_hip_solve_refined_R", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
                                             c_int, c_void_p]),
     ("monolis_hip_time_matvec", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_double_p]),
+    ("monolis_hip_time_matvec_each", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     ("monolis_hip_stream_copy", c_int, [c_void_p, c_void_p, ctypes.c_longlong, c_int, c_double_p]),
     ("monolis_hip_num_variants", c_int, []),
     ("monolis_hip_matrix_autotune", c_int, [c_void_p, c_int, c_int, c_int_p, c_double_p]),

@@ -430,6 +430,15 @@ class DeviceMatrix:
                    "time_matvec")
         return ms.value
 
+    def time_matvec_each(self, x: DeviceBuffer, y: DeviceBuffer, warmup: int = 5, reps: int = 50) -> np.ndarray:
+        """Milliseconds of each of `reps` SpMVs (one event pair around each)."""
+        out = np.zeros(max(reps, 1), np.float64)
+        fp32 = 1 if x.dtype == np.float32 else 0
+        _lib.check(_lib.load().monolis_hip_time_matvec_each(ctypes.c_void_p(self.h), fp32, ctypes.c_void_p(x.ptr),
+                                                            ctypes.c_void_p(y.ptr), warmup, reps, _p(out)),
+                   "time_matvec_each")
+        return out[:reps]
+
 
 def device_matrix_by_simple_mesh(n_node: int, n_dof: int, elems) -> "DeviceMatrix":
     """monolis_get_nonzero_pattern_by_simple_mesh_R on the GPU: a DeviceMatrix

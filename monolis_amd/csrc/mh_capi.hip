@@ -440,6 +440,39 @@ int monolis_hip_time_matvec(monolis_hip_matrix* H, int fp32, const void* x, void
   return 0;
 }
 
+int monolis_hip_time_matvec_each(monolis_hip_matrix* H, int fp32, const void* x, void* y, int warmup, int reps,
+                                 double* ms_each) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (reps < 0) {
+    mh::set_error("reps < 0");
+    return ST_ARGS;
+  }
+  if (fp32) MH_TRY(mh::matrix_make_fp32(H->M, c->stream));
+  auto one = [&]() -> int {
+    if (fp32) return mh::spmv<float>(*H->M, (const float*)x, (float*)y, c->stream);
+    return mh::spmv<double>(*H->M, (const double*)x, (double*)y, c->stream);
+  };
+  for (int i = 0; i < warmup; ++i) MH_TRY(one());
+  std::vector<hipEvent_t> ev(2 * (size_t)reps);
+  for (auto& e : ev) MH_HIP(hipEventCreate(&e));
+  int rc = 0;
+  for (int i = 0; i < reps && !rc; ++i) {
+    if (hipEventRecord(ev[2 * i], c->stream) != hipSuccess) rc = ST_HIP;
+    if (!rc) rc = one();
+    if (!rc && hipEventRecord(ev[2 * i + 1], c->stream) != hipSuccess) rc = ST_HIP;
+  }
+  if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = ST_HIP;
+  for (int i = 0; i < reps && !rc; ++i) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]) != hipSuccess) rc = ST_HIP;
+    ms_each[i] = (double)t;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc == ST_HIP) mh::set_error("event timing failed");
+  return rc;
+}
+
 int monolis_hip_num_variants(void) { return mh::kNumVariants; }
 
 int monolis_hip_matrix_set_variant(monolis_hip_matrix* H, int fp32, int variant) {
