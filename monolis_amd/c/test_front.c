@@ -10,6 +10,10 @@
  *   test_front gpu    matrix-vector products against a host loop, the chain
  *                     solves of wrapper_test/solver/monolis_solver_c_test.c
  *                     (x = 1, x = 1 + i) for every method, variable blocks
+ *   test_front err_*  one error monolis stops on (the process exits 1 with a
+ *                     message): err_pattern (entry outside the pattern),
+ *                     err_dof (DOF outside its block), err_dbc (Dirichlet
+ *                     condition on a row without its diagonal block)
  *
  * Prints one line per failed check and exits 1 if any failed.
  */
@@ -325,9 +329,33 @@ static void gpu_checks(void) {
   monolis_com_finalize(&com);
 }
 
+static void error_case(const char* which) {
+  MONOLIS mat;
+  monolis_initialize(&mat);
+  int** e = chain_elems(4);
+  monolis_get_nonzero_pattern_by_simple_mesh_R(&mat, 4, 2, 2, 3, e);
+  free_elems(e, 3);
+  if (!strcmp(which, "err_pattern")) {
+    monolis_add_scalar_to_sparse_matrix_R(&mat, 0, 3, 0, 0, 1.0);  /* nodes 0 and 3 share no element */
+  } else if (!strcmp(which, "err_dof")) {
+    monolis_set_scalar_to_sparse_matrix_R(&mat, 1, 1, 2, 0, 1.0);  /* n_dof is 2 */
+  } else if (!strcmp(which, "err_dbc")) {
+    MONOLIS mb;
+    monolis_initialize(&mb);
+    int idx[3] = {0, 1, 2}, itm[2] = {1, 0};                         /* no diagonal blocks */
+    double A[2] = {1.0, 1.0}, B[2] = {0.0, 0.0};
+    monolis_set_matrix_BCSR_R(&mb, 2, 2, 1, 2, A, idx, itm);
+    monolis_set_Dirichlet_bc_R(&mb, B, 0, 0, 1.0);
+  }
+  printf("no error raised\n");
+  exit(0);
+}
+
 int main(int argc, char** argv) {
   const char* mode = argc > 1 ? argv[1] : "host";
   monolis_global_initialize();
+  if (!strncmp(mode, "err_", 4))
+    error_case(mode);
   if (!strcmp(mode, "host"))
     host_checks();
   else if (!strcmp(mode, "gpu"))

@@ -71,6 +71,21 @@ def test_front_c_program_host_checks():
     assert "ok (0 failed)" in r.stdout
 
 
+@pytest.mark.parametrize("case,msg", [("err_pattern", "not in the nonzero pattern"),
+                                      ("err_dof", "outside its block"),
+                                      ("err_dbc", "without its diagonal block")])
+def test_front_c_program_stops_like_monolis(case, msg):
+    """monolis stops the program on these (monolis_stop_by_matrix_assemble,
+    _by_submatrix_access, _by_set_DBC); the C API does the same: exit status 1
+    and a message on stderr."""
+    import subprocess
+    exe = os.path.join(ROOT, "monolis_amd", "bin", "test_front")
+    assert os.path.exists(exe), "build first: python -c 'import __graft_entry__ as g; g.build()'"
+    r = subprocess.run([exe, case], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1, r.stdout + r.stderr
+    assert msg in r.stderr and "no error raised" not in r.stdout
+
+
 def test_library_is_gfx950_code():
     """The shared object carries a gfx950 code object (hipcc --offload-arch=gfx950)."""
     from monolis_amd import _lib
