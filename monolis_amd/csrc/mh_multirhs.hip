@@ -222,6 +222,33 @@ __global__ void __launch_bounds__(kBlock) k_spmm(int n, int k, int nslices, cons
 //   MODE 2: X += alpha P                                          (replacement iteration)
 //   MODE 3: (R,R), (R, M^-1 R)
 // Updates of X, R and P touch only right-hand sides that are not done.
+// Streams of the elementwise passes: every vector entry is read and written
+// once per pass, so FP64 loads and stores bypass the caches (non-temporal).
+// Measured on the 128^3 x 64-RHS CG: FP64 1460 -> 1409 ms; the same hint on
+// the FP32 passes of the refined solve cost 4 %, so they stay plain.
+// MH_MROWS_NT = 0 turns the hint off, 2 applies it to FP32 too; MH_MROWS_UR
+// row groups per wavefront step are loaded together (8 measured no faster).
+#ifndef MH_MROWS_NT
+#define MH_MROWS_NT 1
+#endif
+#ifndef MH_MROWS_UR
+#define MH_MROWS_UR 4
+#endif
+template <typename T>
+constexpr bool mrows_nt() {
+  return MH_MROWS_NT == 2 || (MH_MROWS_NT == 1 && sizeof(T) == 8);
+}
+template <typename T>
+__device__ __forceinline__ T ld_vec(const T* p) {
+  if constexpr (mrows_nt<T>()) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <typename T>
+__device__ __forceinline__ void st_vec(T* p, T v) {
+  if constexpr (mrows_nt<T>()) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kBlock) k_mrows(int n, int k, int ks, T* __restrict__ x, T* __restrict__ r,
                                                   T* __restrict__ p, const T* __restrict__ q,
@@ -241,7 +268,7 @@ __global__ void __launch_bounds__(kBlock) k_mrows(int n, int k, int ks, T* __res
   const long long groups = (n + L.rpw - 1) / L.rpw;
   long long u0, u1;
   unit_range(groups, &u0, &u1);
-  constexpr int UR = 4;  // row groups per wavefront step, loads issued together
+  constexpr int UR = MH_MROWS_UR;  // row groups per wavefront step, loads issued together
   if (L.on) {
     for (long long g0 = u0 + (long long)w * UR; g0 < u1; g0 += (long long)kWpb * UR) {
       size_t e[UR];
@@ -258,35 +285,35 @@ __global__ void __launch_bounds__(kBlock) k_mrows(int n, int k, int ks, T* __res
         T pv[UR], rv[UR], dv[UR];
 #pragma unroll
         for (int uu = 0; uu < UR; ++uu) {
-          pv[uu] = p[e[uu]];
-          rv[uu] = r[e[uu]];
+          pv[uu] = ld_vec(p + e[uu]);
+          rv[uu] = ld_vec(r + e[uu]);
           dv[uu] = dinv ? dinv[row[uu]] : (T)1;
         }
 #pragma unroll
         for (int uu = 0; uu < UR; ++uu) {
           const T zz = dinv ? dv[uu] * rv[uu] : rv[uu];  // diag_nn apply with B = 1 (diag_nn.f90:207-228)
-          if (active && ok[uu]) p[e[uu]] = fma(beta, pv[uu], zz);
+          if (active && ok[uu]) st_vec(p + e[uu], fma(beta, pv[uu], zz));
         }
       } else if constexpr (MODE == 2) {
         T pv[UR], xv[UR];
 #pragma unroll
         for (int uu = 0; uu < UR; ++uu) {
-          pv[uu] = p[e[uu]];
-          xv[uu] = x[e[uu]];
+          pv[uu] = ld_vec(p + e[uu]);
+          xv[uu] = ld_vec(x + e[uu]);
         }
 #pragma unroll
         for (int uu = 0; uu < UR; ++uu)
-          if (active && ok[uu]) x[e[uu]] = fma(alpha, pv[uu], xv[uu]);
+          if (active && ok[uu]) st_vec(x + e[uu], fma(alpha, pv[uu], xv[uu]));
       } else {
         T rv[UR], dv[UR], xv[UR], pv[UR], qv[UR];
 #pragma unroll
         for (int uu = 0; uu < UR; ++uu) {
-          rv[uu] = r[e[uu]];
+          rv[uu] = ld_vec(r + e[uu]);
           dv[uu] = dinv ? dinv[row[uu]] : (T)1;
           if constexpr (MODE == 1) {
-            xv[uu] = x[e[uu]];
-            pv[uu] = p[e[uu]];
-            qv[uu] = q[e[uu]];
+            xv[uu] = ld_vec(x + e[uu]);
+            pv[uu] = ld_vec(p + e[uu]);
+            qv[uu] = ld_vec(q + e[uu]);
           }
         }
 #pragma unroll
@@ -294,9 +321,9 @@ __global__ void __launch_bounds__(kBlock) k_mrows(int n, int k, int ks, T* __res
           if (!ok[uu]) continue;
           if constexpr (MODE == 1) {
             if (active) {
-              x[e[uu]] = fma(alpha, pv[uu], xv[uu]);
+              st_vec(x + e[uu], fma(alpha, pv[uu], xv[uu]));
               rv[uu] = fma(-alpha, qv[uu], rv[uu]);
-              r[e[uu]] = rv[uu];
+              st_vec(r + e[uu], rv[uu]);
             }
           }
           const T zz = dinv ? dv[uu] * rv[uu] : rv[uu];
