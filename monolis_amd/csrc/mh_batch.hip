@@ -165,31 +165,43 @@ __global__ void __launch_bounds__(NT) k_batch_cg(BatchView V, const double* __re
       });
       wg_sum<NT, 2>(a, red);
       const double alpha = rho / a[0];
-      // x += alpha p ; r -= alpha q (or the true residual every 200 iterations)
+      // x += alpha p ; r -= alpha q (or the true residual every 200 iterations);
+      // ||r||, z = M^-1 r, (r, z).  Outside the replacement iterations both are
+      // one pass over the rows (the same operations per row, the same per-thread
+      // order of the sums).
       const bool rr = (it % kIterRR) == 0;
-      for (int l = threadIdx.x; l < nrow; l += NT) {
-        const int g = row0 + l;
-        x[g] = fma(alpha, p[g], x[g]);
-        if (!rr) r[g] = fma(-alpha, q[g], r[g]);
-      }
-      __syncthreads();
-      if (rr) {
+      double c[2] = {0.0, 0.0};
+      if (!rr) {
+        for (int l = threadIdx.x; l < nrow; l += NT) {
+          const int g = row0 + l;
+          x[g] = fma(alpha, p[g], x[g]);
+          const double rv = fma(-alpha, q[g], r[g]);
+          r[g] = rv;
+          const double zv = precond ? V.dinv[g] * rv : rv;
+          z[g] = zv;
+          c[0] += rv * rv;
+          c[1] += rv * zv;
+        }
+      } else {
+        for (int l = threadIdx.x; l < nrow; l += NT) {
+          const int g = row0 + l;
+          x[g] = fma(alpha, p[g], x[g]);
+        }
+        __syncthreads();
         for_rows([&](int s, int l, bool valid) {
           const int len = valid ? V.rowlen[s * kWave + lane] : 0;
           const double ax = sell_row_scalar<U>(V, s, lane, len, x);
           if (valid) r[row0 + l] = b[row0 + l] - ax;
         });
         __syncthreads();
-      }
-      // ||r||, z = M^-1 r, (r, z)
-      double c[2] = {0.0, 0.0};
-      for (int l = threadIdx.x; l < nrow; l += NT) {
-        const int g = row0 + l;
-        const double rv = r[g];
-        const double zv = precond ? V.dinv[g] * rv : rv;
-        z[g] = zv;
-        c[0] += rv * rv;
-        c[1] += rv * zv;
+        for (int l = threadIdx.x; l < nrow; l += NT) {
+          const int g = row0 + l;
+          const double rv = r[g];
+          const double zv = precond ? V.dinv[g] * rv : rv;
+          z[g] = zv;
+          c[0] += rv * rv;
+          c[1] += rv * zv;
+        }
       }
       wg_sum<NT, 2>(c, red);
       resid = sqrt(c[0] / B2);
