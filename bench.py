@@ -304,7 +304,7 @@ def main() -> int:
 
     if not args.no_extras and not args.no_solve:
         dm = dx = dy = dx32 = dy32 = None             # release the headline buffers first
-        out["precision"] = bench_precision(m, xt, args)
+        out["precision"] = bench_precision(m, xt, args, tune)
         m = None
         out["batching"] = bench_batching(args)
         out["differentiation"] = bench_differentiation(args)
@@ -344,9 +344,10 @@ def _timed(fn, reps=1):
     return t, r
 
 
-def bench_precision(m, xt, args):
+def bench_precision(m, xt, args, tune):
     """FP32-inner / FP64-refined solves on an ill-conditioned twin of the headline
-    system and on the nonsymmetric convection-diffusion problem."""
+    system (the headline's tuned SpMV variants: same pattern) and on the
+    nonsymmetric convection-diffusion problem (tuned for its own pattern)."""
     import numpy as np
     from monolis_amd import api, workloads as W
     res = {}
@@ -355,6 +356,8 @@ def bench_precision(m, xt, args):
     b = W.rhs_from_solution(ms_, xt)
     log(f"built D A D twin in {time.time() - t0:.1f}s")
     dm = api.DeviceMatrix(ms_)
+    dm.set_variant(tune["best64"])
+    dm.set_variant(tune["best32"], fp32=True)
     db = api.DeviceBuffer.from_numpy(b)
     x64, xr = api.DeviceBuffer(ms_.npndof), api.DeviceBuffer(ms_.npndof)
     p64 = _prm(tol=1e-10)
@@ -381,6 +384,12 @@ def bench_precision(m, xt, args):
     cd = W.convdiff7(128, peclet=0.5)
     b = W.uniform_vector(cd.nndof, seed=args.seed + 6)
     dm = api.DeviceMatrix(cd)
+    if args.variant >= 0:
+        dm.set_variant(args.variant)
+        dm.set_variant(args.variant, fp32=True)
+    else:
+        ct = dm.autotune(fp32=True, reps=10)
+        log(f"precision/convdiff autotune: FP64 variant {ct['best64']}, FP32 variant {ct['best32']}")
     db = api.DeviceBuffer.from_numpy(b)
     x64, xr = api.DeviceBuffer(cd.npndof), api.DeviceBuffer(cd.npndof)
     p64 = _prm(method=5, tol=1e-8)
