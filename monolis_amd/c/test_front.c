@@ -13,7 +13,10 @@
  *   test_front err_*  one error monolis stops on (the process exits 1 with a
  *                     message): err_pattern (entry outside the pattern),
  *                     err_dof (DOF outside its block), err_dbc (Dirichlet
- *                     condition on a row without its diagonal block)
+ *                     condition on a row without its diagonal block), and two
+ *                     that would write outside monolis's arrays: err_kind
+ *                     (complex entry into a real pattern), err_matval
+ *                     (set_matrix_BCSR_mat_val with the wrong nz)
  *
  * Prints one line per failed check and exits 1 if any failed.
  */
@@ -346,6 +349,11 @@ static void error_case(const char* which) {
     double A[2] = {1.0, 1.0}, B[2] = {0.0, 0.0};
     monolis_set_matrix_BCSR_R(&mb, 2, 2, 1, 2, A, idx, itm);
     monolis_set_Dirichlet_bc_R(&mb, B, 0, 0, 1.0);
+  } else if (!strcmp(which, "err_kind")) {
+    monolis_add_scalar_to_sparse_matrix_C(&mat, 0, 0, 0, 0, 1.0 + 1.0 * I);
+  } else if (!strcmp(which, "err_matval")) {
+    double A[64] = {0.0};
+    monolis_set_matrix_BCSR_mat_val_R(&mat, 2, 16, A);  /* the pattern has 10 blocks */
   }
   printf("no error raised\n");
   exit(0);

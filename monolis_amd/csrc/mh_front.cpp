@@ -220,6 +220,25 @@ long long entry(const MONOLIS_MAT* M, int ci, int cj, int si, int sj) {
   return (long long)M->n_dof_index2[e] + (long long)nj * si + sj;
 }
 
+// Value arrays of the pattern: a pattern is built for real or for complex
+// values, and the other kind is absent.
+double* vals_R(const MONOLIS_MAT* M) {
+  if (!M->R.A) stop("the matrix holds no real values (its pattern was built for complex values, or none was built)");
+  return M->R.A;
+}
+zc* vals_C(const MONOLIS_MAT* M) {
+  if (!M->C.A) stop("the matrix holds no complex values (its pattern was built for real values, or none was built)");
+  return reinterpret_cast<zc*>(M->C.A);
+}
+
+// monolis_set_matrix_BCSR_mat_val_* copies n_dof^2 nz values into the value
+// array (spmat_handler.f90:337-352); sizes other than the pattern's would
+// write outside it, so they stop the program.
+void check_mat_val(const MONOLIS_MAT* M, int n_dof, int nz) {
+  if (M->NDOF != n_dof || M->CSR.index == nullptr || M->CSR.index[M->NP] != nz)
+    stop("set_matrix_BCSR_mat_val: n_dof / nz differ from the matrix's pattern");
+}
+
 template <typename T>
 void add_matrix_main(MONOLIS_MAT* M, T* A, int n1, int n2, const int* c1, const int* c2, T** val) {
   // monolis_add_matrix_to_sparse_matrix_main_R (spmat_handler_util.f90:398-497):
@@ -527,10 +546,10 @@ void monolis_get_nonzero_pattern_by_nodal_graph_V_C(MONOLIS* mat, int n_node, in
 
 // ---- assembly -------------------------------------------------------------
 void monolis_set_scalar_to_sparse_matrix_R(MONOLIS* mat, int i, int j, int si, int sj, double val) {
-  mat->mat.R.A[entry(&mat->mat, i, j, si, sj)] = val;
+  vals_R(&mat->mat)[entry(&mat->mat, i, j, si, sj)] = val;
 }
 void monolis_add_scalar_to_sparse_matrix_R(MONOLIS* mat, int i, int j, int si, int sj, double val) {
-  double& a = mat->mat.R.A[entry(&mat->mat, i, j, si, sj)];
+  double& a = vals_R(&mat->mat)[entry(&mat->mat, i, j, si, sj)];
   a = a + val;
 }
 void monolis_get_scalar_from_sparse_matrix_R(MONOLIS* mat, int i, int j, int si, int sj, double* val,
@@ -545,31 +564,32 @@ void monolis_get_scalar_from_sparse_matrix_R(MONOLIS* mat, int i, int j, int si,
     stop("DOF index outside its block (monolis_stop_by_submatrix_access)");
   const int e = find_block(M, i, j);
   if (e < 0) return;
-  *val = M->R.A[(long long)M->n_dof_index2[e] + (long long)block_ndof(M, j) * si + sj];
+  *val = vals_R(M)[(long long)M->n_dof_index2[e] + (long long)block_ndof(M, j) * si + sj];
   *is_find = true;
 }
 void monolis_add_matrix_to_sparse_matrix_R(MONOLIS* mat, int n_base, int* connectivity, double** val) {
-  add_matrix_main<double>(&mat->mat, mat->mat.R.A, n_base, n_base, connectivity, connectivity, val);
+  add_matrix_main<double>(&mat->mat, vals_R(&mat->mat), n_base, n_base, connectivity, connectivity, val);
 }
 void monolis_add_matrix_to_sparse_matrix_offdiag_R(MONOLIS* mat, int n_base1, int n_base2, int* connectivity1,
                                                    int* connectivity2, double** val) {
-  add_matrix_main<double>(&mat->mat, mat->mat.R.A, n_base1, n_base2, connectivity1, connectivity2, val);
+  add_matrix_main<double>(&mat->mat, vals_R(&mat->mat), n_base1, n_base2, connectivity1, connectivity2, val);
 }
 void monolis_set_matrix_BCSR_R(MONOLIS* mat, int n, int np, int n_dof, int nz, double* A, int* index, int* item) {
   set_bcsr<double>(mat, n, np, n_dof, nz, A, index, item, false);
 }
 void monolis_set_matrix_BCSR_mat_val_R(MONOLIS* mat, int n_dof, int nz, double* A) {
-  std::copy(A, A + (size_t)nz * n_dof * n_dof, mat->mat.R.A);
+  check_mat_val(&mat->mat, n_dof, nz);
+  std::copy(A, A + (size_t)nz * n_dof * n_dof, vals_R(&mat->mat));
 }
 void monolis_set_Dirichlet_bc_R(MONOLIS* mat, double* b, int node_id, int n_dof_bc, double val) {
-  dirichlet_main<double>(&mat->mat, mat->mat.R.A, b, node_id, n_dof_bc, val);
+  dirichlet_main<double>(&mat->mat, vals_R(&mat->mat), b, node_id, n_dof_bc, val);
 }
 
 void monolis_set_scalar_to_sparse_matrix_C(MONOLIS* mat, int i, int j, int si, int sj, double _Complex val) {
-  zptr(mat->mat.C.A)[entry(&mat->mat, i, j, si, sj)] = reinterpret_cast<const zc&>(val);
+  vals_C(&mat->mat)[entry(&mat->mat, i, j, si, sj)] = reinterpret_cast<const zc&>(val);
 }
 void monolis_add_scalar_to_sparse_matrix_C(MONOLIS* mat, int i, int j, int si, int sj, double _Complex val) {
-  zc& a = zptr(mat->mat.C.A)[entry(&mat->mat, i, j, si, sj)];
+  zc& a = vals_C(&mat->mat)[entry(&mat->mat, i, j, si, sj)];
   a = a + reinterpret_cast<const zc&>(val);
 }
 void monolis_get_scalar_from_sparse_matrix_C(MONOLIS* mat, int i, int j, int si, int sj, double _Complex* val,
@@ -584,16 +604,16 @@ void monolis_get_scalar_from_sparse_matrix_C(MONOLIS* mat, int i, int j, int si,
     stop("DOF index outside its block (monolis_stop_by_submatrix_access)");
   const int e = find_block(M, i, j);
   if (e < 0) return;
-  *out = zptr(M->C.A)[(long long)M->n_dof_index2[e] + (long long)block_ndof(M, j) * si + sj];
+  *out = vals_C(M)[(long long)M->n_dof_index2[e] + (long long)block_ndof(M, j) * si + sj];
   *is_find = true;
 }
 void monolis_add_matrix_to_sparse_matrix_C(MONOLIS* mat, int n_base, int* connectivity, double _Complex** val) {
-  add_matrix_main<zc>(&mat->mat, zptr(mat->mat.C.A), n_base, n_base, connectivity, connectivity,
+  add_matrix_main<zc>(&mat->mat, vals_C(&mat->mat), n_base, n_base, connectivity, connectivity,
                       reinterpret_cast<zc**>(val));
 }
 void monolis_add_matrix_to_sparse_matrix_offdiag_C(MONOLIS* mat, int n_base1, int n_base2, int* connectivity1,
                                                    int* connectivity2, double _Complex** val) {
-  add_matrix_main<zc>(&mat->mat, zptr(mat->mat.C.A), n_base1, n_base2, connectivity1, connectivity2,
+  add_matrix_main<zc>(&mat->mat, vals_C(&mat->mat), n_base1, n_base2, connectivity1, connectivity2,
                       reinterpret_cast<zc**>(val));
 }
 void monolis_set_matrix_BCSR_C(MONOLIS* mat, int n, int np, int n_dof, int nz, double _Complex* A, int* index,
@@ -601,10 +621,11 @@ void monolis_set_matrix_BCSR_C(MONOLIS* mat, int n, int np, int n_dof, int nz, d
   set_bcsr<zc>(mat, n, np, n_dof, nz, zptr(A), index, item, true);
 }
 void monolis_set_matrix_BCSR_mat_val_C(MONOLIS* mat, int n_dof, int nz, double _Complex* A) {
-  std::copy(zptr(A), zptr(A) + (size_t)nz * n_dof * n_dof, zptr(mat->mat.C.A));
+  check_mat_val(&mat->mat, n_dof, nz);
+  std::copy(zptr(A), zptr(A) + (size_t)nz * n_dof * n_dof, vals_C(&mat->mat));
 }
 void monolis_set_Dirichlet_bc_C(MONOLIS* mat, double _Complex* b, int node_id, int n_dof_bc, double _Complex val) {
-  dirichlet_main<zc>(&mat->mat, zptr(mat->mat.C.A), zptr(b), node_id, n_dof_bc, reinterpret_cast<const zc&>(val));
+  dirichlet_main<zc>(&mat->mat, vals_C(&mat->mat), zptr(b), node_id, n_dof_bc, reinterpret_cast<const zc&>(val));
 }
 
 // ---- solver and linear algebra (GPU path) ---------------------------------
@@ -613,7 +634,7 @@ void monolis_solve_R(MONOLIS* mat, MONOLIS_COM* com, double* b, double* x) {
   sizes(mat, com, &n, &nz, &nndof, &npndof, &nzndof2);
   MONOLIS_MAT* M = &mat->mat;
   monolis_solve_R_c_main(n, M->NP, nz, M->NDOF, npndof, nzndof2, M->n_dof_list, M->n_dof_index, M->n_dof_index2,
-                         M->R.A, x, b, M->CSR.index, M->CSR.item, com->my_rank, com->comm, com->comm_size,
+                         vals_R(M), x, b, M->CSR.index, M->CSR.item, com->my_rank, com->comm, com->comm_size,
                          com->recv_n_neib, com->recv_index[com->recv_n_neib], com->recv_neib_pe, com->recv_index,
                          com->recv_item, com->send_n_neib, com->send_index[com->send_n_neib], com->send_neib_pe,
                          com->send_index, com->send_item, mat->prm.Iarray, mat->prm.Rarray);
@@ -624,7 +645,7 @@ void monolis_solve_C(MONOLIS* mat, MONOLIS_COM* com, double _Complex* b, double 
   sizes(mat, com, &n, &nz, &nndof, &npndof, &nzndof2);
   MONOLIS_MAT* M = &mat->mat;
   monolis_solve_C_c_main(n, M->NP, nz, M->NDOF, npndof, nzndof2, M->n_dof_list, M->n_dof_index, M->n_dof_index2,
-                         M->C.A, x, b, M->CSR.index, M->CSR.item, com->my_rank, com->comm, com->comm_size,
+                         reinterpret_cast<double _Complex*>(vals_C(M)), x, b, M->CSR.index, M->CSR.item, com->my_rank, com->comm, com->comm_size,
                          com->recv_n_neib, com->recv_index[com->recv_n_neib], com->recv_neib_pe, com->recv_index,
                          com->recv_item, com->send_n_neib, com->send_index[com->send_n_neib], com->send_neib_pe,
                          com->send_index, com->send_item, mat->prm.Iarray, mat->prm.Rarray);
@@ -635,7 +656,7 @@ void monolis_matvec_product_R(MONOLIS* mat, MONOLIS_COM* com, double* x, double*
   sizes(mat, com, &n, &nz, &nndof, &npndof, &nzndof2);
   MONOLIS_MAT* M = &mat->mat;
   monolis_matvec_product_R_c_main(n, M->NP, nz, M->NDOF, npndof, nzndof2, M->n_dof_list, M->n_dof_index,
-                                  M->n_dof_index2, M->R.A, x, y, M->CSR.index, M->CSR.item, com->my_rank, com->comm,
+                                  M->n_dof_index2, vals_R(M), x, y, M->CSR.index, M->CSR.item, com->my_rank, com->comm,
                                   com->comm_size, com->recv_n_neib, com->recv_index[com->recv_n_neib],
                                   com->recv_neib_pe, com->recv_index, com->recv_item, com->send_n_neib,
                                   com->send_index[com->send_n_neib], com->send_neib_pe, com->send_index,
@@ -647,7 +668,7 @@ void monolis_matvec_product_C(MONOLIS* mat, MONOLIS_COM* com, double _Complex* x
   sizes(mat, com, &n, &nz, &nndof, &npndof, &nzndof2);
   MONOLIS_MAT* M = &mat->mat;
   monolis_matvec_product_C_c_main(n, M->NP, nz, M->NDOF, npndof, nzndof2, M->n_dof_list, M->n_dof_index,
-                                  M->n_dof_index2, M->C.A, x, y, M->CSR.index, M->CSR.item, com->my_rank, com->comm,
+                                  M->n_dof_index2, reinterpret_cast<double _Complex*>(vals_C(M)), x, y, M->CSR.index, M->CSR.item, com->my_rank, com->comm,
                                   com->comm_size, com->recv_n_neib, com->recv_index[com->recv_n_neib],
                                   com->recv_neib_pe, com->recv_index, com->recv_item, com->send_n_neib,
                                   com->send_index[com->send_n_neib], com->send_neib_pe, com->send_index,

@@ -73,11 +73,15 @@ def test_front_c_program_host_checks():
 
 @pytest.mark.parametrize("case,msg", [("err_pattern", "not in the nonzero pattern"),
                                       ("err_dof", "outside its block"),
-                                      ("err_dbc", "without its diagonal block")])
+                                      ("err_dbc", "without its diagonal block"),
+                                      ("err_kind", "holds no complex values"),
+                                      ("err_matval", "differ from the matrix's pattern")])
 def test_front_c_program_stops_like_monolis(case, msg):
     """monolis stops the program on these (monolis_stop_by_matrix_assemble,
     _by_submatrix_access, _by_set_DBC); the C API does the same: exit status 1
-    and a message on stderr."""
+    and a message on stderr.  The last two would write outside monolis's own
+    arrays (a value kind the pattern was not built for, a value copy of the
+    wrong size); they stop the same way."""
     import subprocess
     exe = os.path.join(ROOT, "monolis_amd", "bin", "test_front")
     assert os.path.exists(exe), "build first: python -c 'import __graft_entry__ as g; g.build()'"
