@@ -172,7 +172,8 @@ int monolis_hip_time_matvec_each(monolis_hip_matrix* M, int fp32, const void* x,
                                  double* ms_each);
 
 /* Autotuner (no monolis counterpart): times each of the SELL SpMV kernel
- * variants (slots per step x temporal / non-temporal matrix streams) on this
+ * variants (slots per step, plain or software-pipelined, x launch shape: one
+ * wavefront per slice or a persistent grid of 4..24 waves per CU) on this
  * matrix with HIP events, `reps` launches each, and keeps the fastest one per
  * precision for every later SpMV and solve on the handle.  best[0] / best[1]
  * receive the FP64 / FP32 choice; ms (if not NULL) receives 2 x
@@ -180,7 +181,8 @@ int monolis_hip_time_matvec_each(monolis_hip_matrix* M, int fp32, const void* x,
  * only. */
 int monolis_hip_num_variants(void);
 int monolis_hip_matrix_autotune(monolis_hip_matrix* M, int fp32, int reps, int* best, double* ms);
-/* Forces a variant (0 = default) for one precision (fp32 = 0 or 1). */
+/* Forces a variant for one precision (fp32 = 0 or 1); an untuned handle uses
+ * variant 4 (one wavefront per slice, 3 slots per step). */
 int monolis_hip_matrix_set_variant(monolis_hip_matrix* M, int fp32, int variant);
 
 /* Batched solves (no monolis counterpart; equivalent to one monolis_solve_R

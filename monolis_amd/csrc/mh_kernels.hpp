@@ -239,9 +239,8 @@ __device__ __forceinline__ void sell_row_pipe(const SellView<TV>& M, int s, int 
 //           w > 0 = a persistent grid of w wavefronts per CU, each wavefront
 //           walking a contiguous range of slices (fewer, longer HBM streams).
 //   pipe:   1 = software-pipelined slot loop (sell_row_pipe), 0 = sell_row.
-// Matrix streams are always non-temporal.  Variant 0 (blocks) or 4 (scalar
-// rows) is the default before tuning; monolis_hip_matrix_autotune times all of
-// them per matrix.
+// Matrix streams are always non-temporal.  Variant 4 is the default before
+// tuning; monolis_hip_matrix_autotune times all of them per matrix.
 struct SellVariant {
   int unroll;  // slots per step, plus kSellPipe when pipelined
   int wpc;

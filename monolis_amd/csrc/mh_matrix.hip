@@ -321,11 +321,12 @@ static int matrix_build(Matrix** out, int n, int np, int nz, int b, const int* i
   M->b = b;
   M->is_complex = cplx;
   M->nslices = (n + kWave - 1) / kWave;
-  // Untuned default: a persistent grid for blocks (3 slots per step), one
-  // wavefront per slice for scalar rows, whose short rows gain nothing from
-  // longer streams (7-point 128^3 convection-diffusion: 30.8 us against 41.7 us
-  // for FP64, 21.3 against 33.3 for FP32).
-  if (b == 1) M->variant64 = M->variant32 = 4;
+  // Untuned default (the flat entry points never tune): one wavefront per
+  // slice, 3 slots per step.  Against the persistent 8-waves-per-CU grid it
+  // takes 30.8 / 21.3 us instead of 41.7 / 33.3 (FP64 / FP32, 7-point 128^3
+  // convection-diffusion) and 328 / 161 us instead of 351 / 171 (27-point
+  // BCSR3 on 100^3 nodes, where it is the fastest FP32 variant).
+  M->variant64 = M->variant32 = 4;
   int rc = 0;
   do {
     if ((rc = M->rowptr.alloc((size_t)np + 1))) break;
