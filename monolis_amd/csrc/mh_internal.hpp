@@ -35,9 +35,11 @@ struct Context {
   int* h_flags = nullptr;            // pinned host, kFlagSlots ints
   PollView* h_poll = nullptr;        // pinned host, kFlagSlots solver progress views
   hipEvent_t ev[16] = {};
+  hipStream_t up[8] = {};            // streams of the threaded pageable uploads
   bool ok = false;
 };
 constexpr int kFlagSlots = 16;
+constexpr int kUploadThreads = 8;
 
 // Returns 0 and fills *c on success.  Initialises HIP on first use.
 int context(Context** c);
