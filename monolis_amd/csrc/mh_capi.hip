@@ -235,7 +235,11 @@ int monolis_hip_memcpy(void* dst, const void* src, size_t bytes) {
   Context* c = nullptr;
   MH_TRY(mh::context(&c));
   if (bytes == 0) return 0;
-  MH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c->stream));
+  if (mh::is_device_pointer(dst) && !mh::is_device_pointer(src)) {
+    MH_TRY(mh::upload(dst, src, bytes, c->stream));  // large pageable sources: threaded
+  } else {
+    MH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c->stream));
+  }
   MH_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }

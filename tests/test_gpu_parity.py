@@ -96,7 +96,8 @@ def test_matvec_matrix_staged_by_threaded_upload():
     """Host arrays of 64 MB and more are copied by several threads, each taking
     a 64 KB-aligned share (the last one the ragged tail).  The 40^3-node BCSR3
     value array is 118 MB: the device values read back must equal the host's
-    bit for bit, and the product must match the oracle."""
+    bit for bit, and the product must match the oracle.  The same holds for a
+    device vector filled from a 72 MB host array."""
     m = W.stencil27_bcsr3(40, seed=9)
     assert m.A.nbytes >= 64 << 20 and m.A.nbytes % (1 << 16) != 0
     dm = api.DeviceMatrix(m)
@@ -106,6 +107,9 @@ def test_matvec_matrix_staged_by_threaded_upload():
     dy = api.DeviceBuffer(m.nndof)
     dm.matvec(dx, dy)
     assert _rel(dy.numpy(), oracle.matvec(m, x)) <= 1e-12
+    # a device vector filled from a 72 MB host array (monolis_hip_memcpy)
+    v = np.random.default_rng(5).uniform(-1, 1, 9_000_001)
+    assert np.array_equal(api.DeviceBuffer.from_numpy(v).numpy(), v)
 
 
 @pytest.mark.parametrize("gen", ["stencil27", "poisson7"])
