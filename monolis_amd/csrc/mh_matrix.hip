@@ -112,20 +112,27 @@ int upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
     MH_TRY(context(&c));
     MH_HIP(hipStreamSynchronize(s));  // earlier work on s may still use dst
     hipError_t err[kUploadThreads];
-    std::vector<std::thread> th;
     constexpr size_t kAlign = (size_t)1 << 16;
-    for (int k = 0; k < kUploadThreads; ++k) {
+    // share k: bytes [b0, b1), 64 KB-aligned starts, the last share takes the tail
+    auto job = [=, &err](int k) {
       const size_t b0 = std::min(bytes, (bytes * k / kUploadThreads) / kAlign * kAlign);
-      const size_t b1 = k + 1 == kUploadThreads ? bytes : std::min(bytes, (bytes * (k + 1) / kUploadThreads) / kAlign * kAlign);
-      err[k] = hipSuccess;
-      th.emplace_back([=, &err]() {
-        hipError_t e = hipSetDevice(c->device);
-        if (e == hipSuccess && b1 > b0)
-          e = hipMemcpyAsync(static_cast<char*>(dst) + b0, static_cast<const char*>(src) + b0, b1 - b0,
-                             hipMemcpyHostToDevice, c->up[k]);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->up[k]);
-        err[k] = e;
-      });
+      const size_t b1 =
+          k + 1 == kUploadThreads ? bytes : std::min(bytes, (bytes * (k + 1) / kUploadThreads) / kAlign * kAlign);
+      hipError_t e = hipSetDevice(c->device);
+      if (e == hipSuccess && b1 > b0)
+        e = hipMemcpyAsync(static_cast<char*>(dst) + b0, static_cast<const char*>(src) + b0, b1 - b0,
+                           hipMemcpyHostToDevice, c->up[k]);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->up[k]);
+      err[k] = e;
+    };
+    std::vector<std::thread> th;
+    th.reserve(kUploadThreads);
+    for (int k = 0; k < kUploadThreads; ++k) {
+      try {
+        th.emplace_back(job, k);
+      } catch (...) {
+        job(k);  // no thread available: this share is copied by the caller
+      }
     }
     for (auto& t : th) t.join();
     for (int k = 0; k < kUploadThreads; ++k)
