@@ -2,13 +2,21 @@
  * C caller of the GPU path through monolis's flat C-ABI, in the shape of
  * monolis's own C tests:
  *   - the R33 matvec known answer (wrapper_test/linalg/monolis_matvec_c_test.c:161-252),
- *   - the 2-DOF chain solve with diagonal rand()%1001+5000 and symmetric
- *     off-diagonal blocks rand()%1001, b = A 1, tol 1e-10, x = 1 expected
+ *   - the 2-DOF chain solve with diagonal rand()%1001+5000 and off-diagonal
+ *     blocks rand()%1001, each written untransposed at (i, i+1) and (i+1, i),
+ *     so the matrix is nonsymmetric; b = A 1, tol 1e-10, x = 1 expected
  *     (wrapper_test/solver/monolis_solver_c_test.c:8-81),
  *   - its complex twin solved by COCG: diagonal v + 0.1 v i with v = rand()%1001+5000,
  *     off-diagonal blocks v + 0.1 v i with v = rand()%1001 set on both sides,
  *     b = A (1 + i), tol 1e-10, x = 1 + i expected
  *     (wrapper_test/solver/monolis_solver_c_test.c:83-158).
+ *
+ * Every value is drawn from glibc rand() after srand(1), real case first, and
+ * before the first library call, so the matrices are fixed whatever the
+ * runtime does to the generator later.  "test_c_abi draws" prints the 112 draws
+ * and exits without calling the library.  "test_c_abi IT_R IT_C" runs the
+ * checks and requires status 0 and the expected CG / COCG iteration counts
+ * (+-1) for the two solves.
  * Prints "C_ABI_OK" when all pass, exits 1 otherwise.
  */
 #include <complex.h>
@@ -52,7 +60,44 @@ static void prm_default(int* Ip, double* Rp) {
   Rp[1] = 1.0e-8;
 }
 
-int main(void) {
+/* glibc rand() draws of both chain cases, in the order monolis's C test makes them */
+enum { N_NODE = 10, N_DOF = 2, N_DIAG = N_NODE * N_DOF, N_OFF = (N_NODE - 1) * N_DOF * N_DOF };
+typedef struct {
+  int diag[N_DIAG];  /* rand() % 1001 */
+  int off[N_OFF];    /* rand() % 1001 */
+} chain_draws;
+
+static void draw_chain(chain_draws* d) {
+  for (int k = 0; k < N_DIAG; ++k) d->diag[k] = rand() % 1001;
+  for (int k = 0; k < N_OFF; ++k) d->off[k] = rand() % 1001;
+}
+
+static int iters_close(int it, int expect) { return it >= expect - 1 && it <= expect + 1; }
+
+int main(int argc, char** argv) {
+  chain_draws dr, dc;
+  srand(1);
+  draw_chain(&dr);
+  draw_chain(&dc);
+  long long checksum = 0;
+  for (int k = 0; k < N_DIAG; ++k) checksum += (long long)(k + 1) * (dr.diag[k] + 7 * dc.diag[k]);
+  for (int k = 0; k < N_OFF; ++k) checksum += (long long)(k + 1 + N_DIAG) * (dr.off[k] + 7 * dc.off[k]);
+  printf("draws checksum %lld\n", checksum);
+  if (argc == 2 && strcmp(argv[1], "draws") == 0) {
+    /* the values only, no library call */
+    for (int k = 0; k < N_DIAG; ++k) printf("%d ", dr.diag[k]);
+    for (int k = 0; k < N_OFF; ++k) printf("%d ", dr.off[k]);
+    for (int k = 0; k < N_DIAG; ++k) printf("%d ", dc.diag[k]);
+    for (int k = 0; k < N_OFF; ++k) printf("%d ", dc.off[k]);
+    printf("\n");
+    return 0;
+  }
+  if (argc != 3) {
+    fprintf(stderr, "usage: %s draws | %s IT_R IT_C\n", argv[0], argv[0]);
+    return 2;
+  }
+  const int expect_it_r = atoi(argv[1]), expect_it_c = atoi(argv[2]);
+
   int ok = 1;
   int empty[1] = {0};
   /* --- R33 known answer --- */
@@ -78,77 +123,80 @@ int main(void) {
   }
   /* --- 2-DOF chain solve, x = 1 --- */
   {
-    const int n = 10, nd = 2;
-    int index[11], item[30];
+    const int n = N_NODE, nd = N_DOF;
+    int index[N_NODE + 1], item[3 * N_NODE];
     const int nz = chain_pattern(n, index, item);
     double* A = calloc((size_t)nz * 4, sizeof(double));
     for (int i = 0; i < n; ++i)
-      for (int j = 0; j < nd; ++j) set_entry(A, index, item, nd, i, i, j, j, rand() % 1001 + 5000.0, 0);
-    for (int i = 0; i < n - 1; ++i)
+      for (int j = 0; j < nd; ++j) set_entry(A, index, item, nd, i, i, j, j, dr.diag[nd * i + j] + 5000.0, 0);
+    for (int i = 0, t = 0; i < n - 1; ++i)
       for (int j = 0; j < nd; ++j)
-        for (int k = 0; k < nd; ++k) {
-          const double val = rand() % 1001;
+        for (int k = 0; k < nd; ++k, ++t) {
+          const double val = dr.off[t];
           set_entry(A, index, item, nd, i, i + 1, j, k, val, 0);
           set_entry(A, index, item, nd, i + 1, i, j, k, val, 0);
         }
-    double a[20], b[20];
-    for (int k = 0; k < 20; ++k) a[k] = 1.0;
+    double a[N_DIAG], b[N_DIAG];
+    for (int k = 0; k < N_DIAG; ++k) a[k] = 1.0;
     monolis_matvec_product_R_c_main(n, n, nz, nd, n * nd, nz * 4, empty, empty, empty, A, a, b, index, item, 0, 0,
                                     1, 0, 0, empty, empty, empty, 0, 0, empty, empty, empty);
-    for (int k = 0; k < 20; ++k) a[k] = 0.0;
+    for (int k = 0; k < N_DIAG; ++k) a[k] = 0.0;
     int Ir[100];
     double Rr[100];
     prm_default(Ir, Rr);
     Rr[1] = 1.0e-10;
+    Ir[16] = 0;  /* report non-convergence through the status instead of stopping */
     monolis_solve_R_c_main(n, n, nz, nd, n * nd, nz * 4, empty, empty, empty, A, a, b, index, item, 0, 0, 1, 0, 0,
                            empty, empty, empty, 0, 0, empty, empty, empty, Ir, Rr);
     double err = 0.0;
-    for (int k = 0; k < 20; ++k) err = fmax(err, fabs(a[k] - 1.0));
-    printf("chain solve: iter %d resid %.3e max|x-1| %.3e status %d\n", Ir[4], Rr[2], err, Ir[5]);
-    if (err > 1e-8 || Ir[5] != 0) ok = 0;
+    for (int k = 0; k < N_DIAG; ++k) err = fmax(err, fabs(a[k] - 1.0));
+    printf("chain solve: iter %d (expected %d) resid %.3e max|x-1| %.3e status %d\n", Ir[4], expect_it_r, Rr[2],
+           err, Ir[5]);
+    if (err > 1e-8 || Ir[5] != 0 || !iters_close(Ir[4], expect_it_r) || !(Rr[2] < 1.0e-10)) ok = 0;
     double s = 0.0;
-    monolis_inner_product_R_c_main(20, a, a, &s, 0);
+    monolis_inner_product_R_c_main(N_DIAG, a, a, &s, 0);
     if (fabs(s - 20.0) > 1e-6) ok = 0;
     free(A);
   }
   /* --- complex 2-DOF chain, COCG (method 13), x = 1 + i --- */
   {
-    const int n = 10, nd = 2;
-    int index[11], item[30];
+    const int n = N_NODE, nd = N_DOF;
+    int index[N_NODE + 1], item[3 * N_NODE];
     const int nz = chain_pattern(n, index, item);
     double _Complex* A = calloc((size_t)nz * 4, sizeof(double _Complex));
     for (int i = 0; i < n; ++i)
       for (int j = 0; j < nd; ++j) {
-        const double v = rand() % 1001 + 5000.0;
+        const double v = dc.diag[nd * i + j] + 5000.0;
         A[4 * find_block(index, item, i, i) + 2 * j + j] = v + 0.1 * v * I;
       }
-    for (int i = 0; i < n - 1; ++i)
+    for (int i = 0, t = 0; i < n - 1; ++i)
       for (int j = 0; j < nd; ++j)
-        for (int k = 0; k < nd; ++k) {
-          const double v = rand() % 1001;
+        for (int k = 0; k < nd; ++k, ++t) {
+          const double v = dc.off[t];
           A[4 * find_block(index, item, i, i + 1) + 2 * j + k] = v + 0.1 * v * I;
           A[4 * find_block(index, item, i + 1, i) + 2 * j + k] = v + 0.1 * v * I;
         }
-    double _Complex a[20], b[20];
-    for (int k = 0; k < 20; ++k) a[k] = 1.0 + 1.0 * I;
+    double _Complex a[N_DIAG], b[N_DIAG];
+    for (int k = 0; k < N_DIAG; ++k) a[k] = 1.0 + 1.0 * I;
     monolis_matvec_product_C_c_main(n, n, nz, nd, n * nd, nz * 4, empty, empty, empty, A, a, b, index, item, 0, 0,
                                     1, 0, 0, empty, empty, empty, 0, 0, empty, empty, empty);
-    for (int k = 0; k < 20; ++k) a[k] = 0.0;
+    for (int k = 0; k < N_DIAG; ++k) a[k] = 0.0;
     int Ia[100];
     double Ra[100];
     prm_default(Ia, Ra);
     Ia[1] = 13;  /* MONOLIS_ITER_COCG */
     Ra[1] = 1.0e-10;
+    Ia[16] = 0;
     monolis_solve_C_c_main(n, n, nz, nd, n * nd, nz * 4, empty, empty, empty, A, a, b, index, item, 0, 0, 1, 0, 0,
                            empty, empty, empty, 0, 0, empty, empty, empty, Ia, Ra);
     double err = 0.0;
-    for (int k = 0; k < 20; ++k) err = fmax(err, cabs(a[k] - (1.0 + 1.0 * I)));
-    printf("complex chain solve (COCG): iter %d resid %.3e max|x-(1+i)| %.3e status %d\n", Ia[4], Ra[2], err,
-           Ia[5]);
-    if (err > 1e-8 || Ia[5] != 0) ok = 0;
+    for (int k = 0; k < N_DIAG; ++k) err = fmax(err, cabs(a[k] - (1.0 + 1.0 * I)));
+    printf("complex chain solve (COCG): iter %d (expected %d) resid %.3e max|x-(1+i)| %.3e status %d\n", Ia[4],
+           expect_it_c, Ra[2], err, Ia[5]);
+    if (err > 1e-8 || Ia[5] != 0 || !iters_close(Ia[4], expect_it_c) || !(Ra[2] < 1.0e-10)) ok = 0;
     /* unconjugated: (x, x) with x = 1 + i is 2i per entry */
     double _Complex s = 0.0;
-    monolis_inner_product_C_c_main(20, a, a, &s, 0);
+    monolis_inner_product_C_c_main(N_DIAG, a, a, &s, 0);
     if (cabs(s - 40.0 * I) > 1e-6) ok = 0;
     free(A);
   }

@@ -23,6 +23,14 @@ def from_mesh_case(case):
     return m
 
 
+def from_mesh_case_complex(case):
+    """Complex matrix of a fixture case whose "set_complex" rows are (i, j, si, sj, re, im)."""
+    m = mm.nonzero_pattern_by_simple_mesh(case["n_node"], case["ndof"], case["elems"], dtype=np.complex128)
+    for i, j, si, sj, re, im in case["set_complex"]:
+        mm.set_scalar(m, i, j, si, sj, complex(re, im))
+    return m
+
+
 def mtx_matrix(case):
     """Symmetric matrix of mtx.dat as main.f90:176-190 assembles it."""
     n = case["n_node"]

@@ -45,28 +45,44 @@ def glibc_rand_sequence(count: int):
     return [int(libc.rand()) for _ in range(count)]
 
 
-def solver_c_case():
-    """wrapper_test/solver/monolis_solver_c_test.c:8-81: 10-node chain, 2 DOF,
-    diagonal rand()%1001+5000, symmetric off-diagonal blocks rand()%1001, b = A 1,
-    tol 1e-10, expected x = 1.  (The reference binary's rand() state at this point
-    depends on the tests that run before it; any draw is a valid instance of the
-    test, which is pinned by its closed form x = 1.)"""
+def solver_c_cases():
+    """wrapper_test/solver/monolis_solver_c_test.c:8-158: 10-node chain, 2 DOF.
+    Real case: diagonal rand()%1001+5000 and off-diagonal blocks rand()%1001,
+    each written untransposed at (i, i+1) and (i+1, i) (a nonsymmetric matrix),
+    b = A 1, tol 1e-10, x = 1 expected.  Complex case, drawn next from the same
+    stream: the same values v as v + 0.1 v i, solved by COCG to x = 1 + i.
+    Both draws start from srand(1); monolis_amd/c/test_c_abi.c draws the same
+    values before its first library call.  ("oracle_iter" is the iteration
+    count of this repository's CPU oracle on the drawn matrix, one thread; the
+    reference test itself checks only x.)"""
     n_node, ndof = 10, 2
     elems = [[i, i + 1] for i in range(9)]
     r = iter(glibc_rand_sequence(1000))
-    sets = []
-    for i in range(n_node):
-        for j in range(ndof):
-            sets.append([i, i, j, j, float(next(r) % 1001 + 5000)])
-    for e in elems:
-        for j in range(ndof):
-            for k in range(ndof):
-                v = float(next(r) % 1001)
-                sets.append([e[0], e[1], j, k, v])
-                sets.append([e[1], e[0], j, k, v])
-    return {"name": "solve_c_R", "source": "wrapper_test/solver/monolis_solver_c_test.c:8-81",
-            "n_node": n_node, "ndof": ndof, "elems": elems, "set": sets, "tol": 1e-10,
-            "x_expected": [1.0] * (n_node * ndof)}
+    out = []
+    for name, src, scale_im, x_re, x_im, iters in (
+            ("solve_c_R", "wrapper_test/solver/monolis_solver_c_test.c:8-81", 0.0, 1.0, 0.0, 14),
+            ("solve_c_C", "wrapper_test/solver/monolis_solver_c_test.c:83-158", 0.1, 1.0, 1.0, 35)):
+        sets = []
+        for i in range(n_node):
+            for j in range(ndof):
+                v = float(next(r) % 1001 + 5000)
+                sets.append([i, i, j, j, v, scale_im * v])
+        for e in elems:
+            for j in range(ndof):
+                for k in range(ndof):
+                    v = float(next(r) % 1001)
+                    sets.append([e[0], e[1], j, k, v, scale_im * v])
+                    sets.append([e[1], e[0], j, k, v, scale_im * v])
+        case = {"name": name, "source": src, "n_node": n_node, "ndof": ndof, "elems": elems, "tol": 1e-10,
+                "oracle_iter": iters}
+        if scale_im == 0.0:
+            case["set"] = [s[:5] for s in sets]
+            case["x_expected"] = [x_re] * (n_node * ndof)
+        else:
+            case["set_complex"] = sets
+            case["x_expected_re_im"] = [x_re, x_im]
+        out.append(case)
+    return out
 
 
 def mtx_case():
@@ -90,7 +106,9 @@ def blas_kats():
 
 
 def main():
-    data = {"matvec": matvec_kats(), "solve_c": solver_c_case(), "mtx": mtx_case(), "blas": blas_kats()}
+    solve_r, solve_c = solver_c_cases()
+    data = {"matvec": matvec_kats(), "solve_c": solve_r, "solve_c_complex": solve_c, "mtx": mtx_case(),
+            "blas": blas_kats()}
     path = os.path.join(HERE, "monolis_kat.json")
     with open(path, "w") as f:
         json.dump(data, f, indent=1)

@@ -13,7 +13,7 @@ import pytest
 import oracle
 from monolis_amd import api, workloads as W
 
-from _helpers import chain_matrix, from_mesh_case, kat, mtx_matrix, random_bcsr
+from _helpers import chain_matrix, from_mesh_case, from_mesh_case_complex, kat, mtx_matrix, random_bcsr
 
 pytestmark = pytest.mark.gpu
 
@@ -263,11 +263,36 @@ def test_solve_chain_matches_oracle(ndof, precond, method):
 
 
 def test_solve_c_reference_case():
+    """The fixture's draw of wrapper_test/solver/monolis_solver_c_test.c:8-81 (the
+    matrix test_c_abi.c solves): status 0 and the oracle's 14 CG iterations."""
     case = kat()["solve_c"]
     m = from_mesh_case(case)
     b = oracle.matvec(m, np.ones(20))
     x, mon = _solve_flat(m, b, tol=case["tol"])
+    xo, st, I, R = oracle.solve(m, b, tol=case["tol"])
+    assert st == 0 and api.monolis_get_error_tag(mon) == 0
+    assert api.monolis_get_converge_iter(mon) == int(I[4]) == case["oracle_iter"]
+    assert api.monolis_get_converge_residual(mon) < case["tol"]
     assert np.allclose(x, case["x_expected"], rtol=0, atol=1e-8)
+
+
+def test_solve_c_complex_reference_case():
+    """Complex twin (monolis_solver_c_test.c:83-158), COCG: status 0 and the oracle's
+    35 iterations +-1.  The oracle's residual at 34 iterations is 1.0022e-10
+    against the 1e-10 tolerance, so another summation order may stop one
+    iteration earlier."""
+    case = kat()["solve_c_complex"]
+    m = from_mesh_case_complex(case)
+    b = oracle.matvec_C(m, np.full(20, 1 + 1j))
+    mon = api.Monolis(m, api.Prm().quiet())
+    api.monolis_set_method(mon, api.MONOLIS_ITER_COCG)
+    api.monolis_set_tolerance(mon, case["tol"])
+    x = np.zeros(20, np.complex128)
+    api.monolis_solve_C(mon, b, x)
+    xo, st, I, R = oracle.solve_C(m, b, tol=case["tol"])
+    assert st == 0 and api.monolis_get_error_tag(mon) == 0
+    assert abs(api.monolis_get_converge_iter(mon) - int(I[4])) <= 1 and int(I[4]) == case["oracle_iter"]
+    np.testing.assert_allclose(x, 1 + 1j, rtol=0, atol=1e-8)
 
 
 def test_solve_poisson32_cg_diag_matches_oracle():
@@ -417,7 +442,11 @@ def test_compiled_callers(exe, token):
     path = os.path.join(mb.BINDIR, exe)
     if not os.path.exists(path):
         pytest.fail(f"{path} not built (run __graft_entry__.build())")
-    r = subprocess.run([path], capture_output=True, text=True, timeout=120)
+    args = [path]
+    if exe == "test_c_abi":   # expected CG / COCG iteration counts of the fixture's draws
+        args += [str(kat()["solve_c"]["oracle_iter"]), str(kat()["solve_c_complex"]["oracle_iter"])]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert token in r.stdout
 

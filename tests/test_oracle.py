@@ -11,7 +11,7 @@ import scipy.sparse.linalg as spla
 import oracle
 from monolis_amd import workloads as W
 
-from _helpers import chain_matrix, from_mesh_case, kat, mtx_matrix, random_bcsr
+from _helpers import chain_matrix, from_mesh_case, from_mesh_case_complex, kat, mtx_matrix, random_bcsr
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -159,12 +159,50 @@ def test_solve_chain_x_is_one(ndof, precond, method):
     assert np.allclose(x, 1.0, rtol=0, atol=1e-8)
 
 
-def test_solve_c_test_case():
+@pytest.mark.parametrize("threads", [1, 2, 4])
+def test_solve_c_test_case(threads):
+    """The fixture's draw of the reference C test converges in the recorded
+    number of CG iterations under every summation order tried (its residual
+    passes 1e-10 at 7.6e-11, after 3.1e-10 one iteration earlier)."""
     case = kat()["solve_c"]
     m = from_mesh_case(case)
     b = oracle.matvec(m, np.ones(20))
-    x, st, I, R = oracle.solve(m, b, tol=case["tol"])
+    oracle.set_threads(threads)
+    try:
+        x, st, I, R = oracle.solve(m, b, tol=case["tol"])
+    finally:
+        oracle.set_threads(1)
     assert st == 0 and np.allclose(x, case["x_expected"], rtol=0, atol=1e-8)
+    assert int(I[4]) == case["oracle_iter"] == 14
+
+
+def test_solve_c_complex_test_case():
+    """The complex twin (wrapper_test/solver/monolis_solver_c_test.c:83-158) drawn
+    next from the same glibc stream: COCG reaches x = 1 + i in the recorded count."""
+    case = kat()["solve_c_complex"]
+    m = from_mesh_case_complex(case)
+    b = oracle.matvec_C(m, np.full(20, 1 + 1j))
+    x, st, I, R = oracle.solve_C(m, b, tol=case["tol"])
+    assert st == 0 and int(I[4]) == case["oracle_iter"] == 35
+    np.testing.assert_allclose(x, 1 + 1j, rtol=0, atol=1e-8)
+
+
+def test_c_caller_draws_the_fixture_values():
+    """monolis_amd/c/test_c_abi.c draws the chain values from srand(1) before its
+    first library call; "draws" prints them without calling the library."""
+    import os
+    import subprocess
+    from monolis_amd import build as mb
+    path = os.path.join(mb.BINDIR, "test_c_abi")
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (run __graft_entry__.build())")
+    out = subprocess.run([path, "draws"], capture_output=True, text=True, timeout=60, check=True).stdout
+    vals = [float(v) for v in out.splitlines()[-1].split()]
+    assert len(vals) == 112
+    for off, case, col in ((0, kat()["solve_c"]["set"], 4), (56, kat()["solve_c_complex"]["set_complex"], 4)):
+        diag = [s[col] - 5000.0 for s in case[:20]]
+        offd = [s[col] for s in case[20::2]]
+        assert vals[off:off + 20] == diag and vals[off + 20:off + 56] == offd
 
 
 def _jacobi(m):
