@@ -37,7 +37,7 @@ CFLAGS = [
 ]
 
 
-EXTRA = os.environ.get("MONOLIS_HIP_CFLAGS", "").split()   # e.g. -DMH_STORE_CPOL=2 for kernel-option builds
+EXTRA = os.environ.get("MONOLIS_HIP_CFLAGS", "").split()   # extra hipcc flags, e.g. -g
 
 
 def _compile(src: str) -> str:

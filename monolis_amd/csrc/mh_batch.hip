@@ -418,8 +418,9 @@ int batch_solve(Batch& B, const BatchParams& P, const double* b, double* x, int 
     MH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue();
     hipError_t e = hipStreamEndCapture(s, &g);
-    if (rc) return rc;
-    if (e != hipSuccess) {
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);  // a partial capture still owns its graph
+      if (rc) return rc;
       set_error("graph capture failed: %s", hipGetErrorString(e));
       return -1;
     }

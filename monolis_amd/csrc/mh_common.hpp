@@ -61,12 +61,6 @@ __device__ __forceinline__ int xcd_tile(int nwg_logical) {
   const int t = (b % kXcd) * per + b / kXcd;
   return t < nwg_logical ? t : -1;
 }
-#ifndef MH_XCD_SELL
-#define MH_XCD_SELL 0
-#endif
-#ifndef MH_XCD_ROWS
-#define MH_XCD_ROWS 1
-#endif
 
 // ---------------------------------------------------------------------------
 // Reductions.  All sums are deterministic: fixed butterfly order inside a
@@ -215,14 +209,10 @@ __device__ __forceinline__ zd ld_val(const zd* p) {
 // contiguous 1 KiB request; for an odd B*B the last entry has a plane of its
 // own after the pairs.  Measured on the 100^3 BCSR3 SpMV: 325 -> 319 us (FP64).
 // FP32 values keep one plane per entry: pairs of floats measured 1 % slower
-// (162.5 vs 160.8 us); MH_PAIR_F32=1 selects them.  16-byte complex values keep
-// one plane per entry.
-#ifndef MH_PAIR_F32
-#define MH_PAIR_F32 0
-#endif
+// (162.5 vs 160.8 us).  16-byte complex values keep one plane per entry.
 template <typename TV>
 __host__ __device__ constexpr bool sell_paired() {
-  return sizeof(TV) == 8 || (sizeof(TV) == 4 && MH_PAIR_F32 != 0);
+  return sizeof(TV) == 8;
 }
 // Offset of entry e of lane `lane` from the start of its slot.
 template <typename TV>

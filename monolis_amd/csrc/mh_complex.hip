@@ -103,28 +103,22 @@ __device__ __forceinline__ bool zfinalize_check(ZState* st, double* hist, int it
 }
 
 // SELL launch for complex values: a slot of a B x B complex block holds 4*B*B
-// VGPRs, so the unroll is MH_ZSELL_U slots up to B = 3 (twice that for B = 1)
-// and 1 beyond; the geometry is that of real-path variant MH_ZSELL_GEOM.
-// Measured on the 100^3 BCSR3 pattern (26.5 M complex 3x3 blocks): one slot
-// per step with contiguous slice ranges at 8 waves per CU is the fastest of
-// unroll 1..3, the pipelined loop and wpc 0..24 (664-689 us, 73-76 % of HBM
-// peak; unroll 2 705-735 us; the pipelined loop 1.2-2.1 ms).
-#ifndef MH_ZSELL_U
-#define MH_ZSELL_U 1
-#endif
-#ifndef MH_ZSELL_GEOM
-#define MH_ZSELL_GEOM 0
-#endif
+// VGPRs, so one slot per step (twice that for B = 1, through SellCfg) with the
+// geometry of real-path variant 0: contiguous slice ranges at 8 waves per CU.
+// Measured on the 100^3 BCSR3 pattern (26.5 M complex 3x3 blocks) it is the
+// fastest of unroll 1..3, the pipelined loop and wpc 0..24 (664-689 us,
+// 73-76 % of HBM peak; unroll 2 705-735 us; the pipelined loop 1.2-2.1 ms).
+constexpr int kZSellGeom = 0;
 template <int B>
 struct ZUnroll {
-  static constexpr int U = B <= 3 ? MH_ZSELL_U : 1;
+  static constexpr int U = 1;
 };
 
 template <int B, class Op>
 int launch_zsell(const Matrix& M, const Op& op, double* partials, hipStream_t s) {
   int nwg = 0, spw = 0;
   if (M.nslices > 0) {
-    sell_geometry(M, MH_ZSELL_GEOM, &nwg, &spw);
+    sell_geometry(M, kZSellGeom, &nwg, &spw);
     k_sell<zd, zd, B, ZUnroll<B>::U, Op><<<padded_grid(nwg), kBlock, 0, s>>>(view<zd>(M), op, partials, nwg, spw);
     MH_HIP(hipGetLastError());
   }

@@ -275,12 +275,9 @@ __device__ __forceinline__ void wave_sync() {
 // partial cache lines.  For B > 1 the values go through a 64*B LDS tile and
 // are read / written by the wavefront as B contiguous 64-entry runs.  Every
 // lane of the wavefront must call load/store (rows >= nrows are skipped).
-// Vector loads of the row kernels: plain (0) or non-temporal (1).  Measured on
-// the 100^3 BCSR3 solves: CG 20.8 vs 21.0 ms, BiCGSTAB 32.7 vs 31.9 ms,
-// refined CG 12.7 vs 12.9 ms (plain vs non-temporal): within run-to-run noise.
-#ifndef MH_ROWS_NT_LOAD
-#define MH_ROWS_NT_LOAD 0
-#endif
+// Vector loads of the row kernels are plain: non-temporal loads measured the
+// same on the 100^3 BCSR3 solves (CG 20.8 vs 21.0 ms, BiCGSTAB 32.7 vs 31.9 ms,
+// refined CG 12.7 vs 12.9 ms, plain vs non-temporal).
 template <typename T, int B>
 struct WaveRows {
   T* lds;     // 64*B entries private to this wavefront (unused for B == 1)
@@ -294,14 +291,14 @@ struct WaveRows {
   __device__ __forceinline__ void load(const T* __restrict__ src, T (&v)[B]) const {
     if constexpr (B == 1) {
       v[0] = (T)0;
-      if (row0 + lane < nrows) v[0] = ld_val<MH_ROWS_NT_LOAD != 0>(src + (size_t)row0 + lane);
+      if (row0 + lane < nrows) v[0] = ld_val<false>(src + (size_t)row0 + lane);
     } else {
       const int cnt = count();
       const T* base = src + (size_t)row0 * B;
 #pragma unroll
       for (int j = 0; j < B; ++j) {
         const int e = j * kWave + lane;
-        if (e < cnt) lds[e] = ld_val<MH_ROWS_NT_LOAD != 0>(base + e);
+        if (e < cnt) lds[e] = ld_val<false>(base + e);
       }
       wave_sync();
 #pragma unroll
@@ -312,22 +309,7 @@ struct WaveRows {
   // Outputs are written non-temporally: they are consumed by a later kernel,
   // and streaming them out during the kernel avoids leaving megabytes of dirty
   // L2 lines for the end-of-kernel write-back.
-  __device__ __forceinline__ void put(T* __restrict__ dst, size_t i, T v) const {
-#if defined(MH_STORE_CPOL) && MH_STORE_CPOL >= 0
-    static_assert(sizeof(T) <= 8, "cache-policy store covers 4- and 8-byte values");
-    // buffer store with explicit cache-policy bits (bit 0 sc0, bit 1 nt, bit 4 sc1)
-    using U2 = unsigned int __attribute__((ext_vector_type(2)));
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc(dst, 0, (int)(sizeof(T) * (size_t)nrows * B), 0x00020000);
-    if constexpr (sizeof(T) == 8)
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), rs, (int)(i * sizeof(T)), 0, MH_STORE_CPOL);
-    else
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), rs, (int)(i * sizeof(T)), 0,
-                                            MH_STORE_CPOL);
-#else
-    st_stream(dst + i, v);
-#endif
-  }
+  __device__ __forceinline__ void put(T* __restrict__ dst, size_t i, T v) const { st_stream(dst + i, v); }
   __device__ __forceinline__ void store(T* __restrict__ dst, const T (&v)[B]) const {
     if constexpr (B == 1) {
       if (row0 + lane < nrows) put(dst, (size_t)row0 + lane, v[0]);
@@ -395,13 +377,9 @@ __device__ __forceinline__ void diag_apply(const TD* __restrict__ D, int nrows, 
 //                 const WaveRows<TA, B>& io)     (every lane calls it; vectors via io)
 //   void finalize(const double (&tot)[NV])       run once by k_finalize (thread 0)
 // and NV >= 0 partial sums per workgroup, stored for the finalize kernel.
-#ifndef MH_SELL_WAVES_PER_EU
-#define MH_SELL_WAVES_PER_EU 1
-#endif
 template <typename TV, typename TA, int B, int U, class Op>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MH_SELL_WAVES_PER_EU)))
-k_sell(SellView<TV> M, Op op0, double* partials, int nwg, int spw) {
-  const int wg = xcd_tile<MH_XCD_SELL != 0>(nwg);
+__global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op0, double* partials, int nwg, int spw) {
+  const int wg = xcd_tile<false>(nwg);
   if (wg < 0) return;
   Op op = op0;
   if (op.skip()) return;
@@ -464,7 +442,7 @@ k_sell(SellView<TV> M, Op op0, double* partials, int nwg, int spw) {
 //   void finalize(...)
 template <class Op>
 __global__ void __launch_bounds__(kBlock) k_rows(int nrows, Op op0, double* partials, int ntiles) {
-  const int tile = xcd_tile<MH_XCD_ROWS != 0>(ntiles);
+  const int tile = xcd_tile<true>(ntiles);
   if (tile < 0) return;
   Op op = op0;
   if (op.skip()) return;

@@ -58,29 +58,17 @@ __device__ __forceinline__ void unit_range(long long units, long long* u0, long 
   *u1 = units * (t + 1) / g;
 }
 
-// Unit order of the SpMM; each XCD owns a contiguous part of the units.
-//   0: every workgroup a contiguous sub-range, its wavefronts interleaved;
-//   1: the XCD's workgroups interleaved unit by unit, so the resident workgroups
-//      of an XCD sweep one narrow band of rows and share the x rows they gather
-//      in that XCD's L2.
-#ifndef MH_SPMM_ORDER
-#define MH_SPMM_ORDER 0
-#endif
+// Unit order of the SpMM: each XCD owns a contiguous part of the units, every
+// workgroup a contiguous sub-range of it, its wavefronts interleaved.  (An
+// order that interleaves an XCD's workgroups unit by unit over one narrow band
+// fetched 8 % fewer bytes from beyond L2 and ran in the same time.)
 __device__ __forceinline__ void spmm_units(long long units, int w, long long* first, long long* stride,
                                            long long* end) {
-  if constexpr (MH_SPMM_ORDER == 0) {
-    long long u0, u1;
-    unit_range(units, &u0, &u1);
-    *first = u0 + w;
-    *stride = kWpb;
-    *end = u1;
-  } else {
-    const int per = gridDim.x / kXcd;
-    const int xcd = blockIdx.x % kXcd, l = blockIdx.x / kXcd;
-    *first = units * xcd / kXcd + (long long)l * kWpb + w;
-    *stride = (long long)per * kWpb;
-    *end = units * (xcd + 1) / kXcd;
-  }
+  long long u0, u1;
+  unit_range(units, &u0, &u1);
+  *first = u0 + w;
+  *stride = kWpb;
+  *end = u1;
 }
 
 // Per right-hand side sum of the lanes' partials over the workgroup (fixed
@@ -225,18 +213,12 @@ __global__ void __launch_bounds__(kBlock) k_spmm(int n, int k, int nslices, cons
 // Streams of the elementwise passes: every vector entry is read and written
 // once per pass, so FP64 loads and stores bypass the caches (non-temporal).
 // Measured on the 128^3 x 64-RHS CG: FP64 1460 -> 1409 ms; the same hint on
-// the FP32 passes of the refined solve cost 4 %, so they stay plain.
-// MH_MROWS_NT = 0 turns the hint off, 2 applies it to FP32 too; MH_MROWS_UR
-// row groups per wavefront step are loaded together (8 measured no faster).
-#ifndef MH_MROWS_NT
-#define MH_MROWS_NT 1
-#endif
-#ifndef MH_MROWS_UR
-#define MH_MROWS_UR 4
-#endif
+// the FP32 passes of the refined solve cost 4 %, so they stay plain.  Four row
+// groups per wavefront step are loaded together (eight measured no faster).
+constexpr int kMrowsUnroll = 4;
 template <typename T>
 constexpr bool mrows_nt() {
-  return MH_MROWS_NT == 2 || (MH_MROWS_NT == 1 && sizeof(T) == 8);
+  return sizeof(T) == 8;
 }
 template <typename T>
 __device__ __forceinline__ T ld_vec(const T* p) {
@@ -268,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_mrows(int n, int k, int ks, T* __res
   const long long groups = (n + L.rpw - 1) / L.rpw;
   long long u0, u1;
   unit_range(groups, &u0, &u1);
-  constexpr int UR = MH_MROWS_UR;  // row groups per wavefront step, loads issued together
+  constexpr int UR = kMrowsUnroll;  // row groups per wavefront step, loads issued together
   if (L.on) {
     for (long long g0 = u0 + (long long)w * UR; g0 < u1; g0 += (long long)kWpb * UR) {
       size_t e[UR];

@@ -413,6 +413,33 @@ def test_refined_solve_reaches_fp64_accuracy(scaled):
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
 
 
+@pytest.mark.parametrize("n", [16, 32])
+def test_refined_bicgstab_convdiff_matches_fp64_oracle(n):
+    """FP32-inner BiCGSTAB (monolis method 5, BiCGSTAB.f90:18-192) with point-Jacobi
+    scaling inside FP64 refinement, on the nonsymmetric 7-point convection-diffusion
+    operator (cell Peclet 0.5) -- the refined BiCGSTAB workload at test size.
+    Bar: status 0, true relative residual < 1e-10 in FP64, and the solution within
+    1e-6 (relative 2-norm) of the oracle's FP64 BiCGSTAB solution at the same tolerance."""
+    m = W.convdiff7(n, peclet=0.5)
+    b = W.uniform_vector(m.nndof, seed=40 + n)
+    dm = api.DeviceMatrix(m)
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(m.npndof)
+    prm = api.Prm().quiet()
+    prm.Iarray[1] = 5           # BiCGSTAB
+    prm.Iarray[2] = 1           # DIAG
+    prm.Rarray[1] = 1e-10
+    st = dm.solve_refined(db, dx, prm, inner_tol=1e-4, max_outer=30)
+    assert st["converged"] and st["resid"] < 1e-10 and int(prm.Iarray[5]) == 0
+    assert st["outer"] >= 2          # the FP32 inner solve alone cannot reach 1e-10
+    x = dx.numpy()[:m.nndof]
+    true = np.linalg.norm(b - oracle.matvec(m, x)) / np.linalg.norm(b)
+    assert true < 1e-10
+    xo, s0, I, R = oracle.solve(m, b, method=5, precond=1, tol=1e-10, max_iter=5000)
+    assert s0 == 0
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) <= 1e-6
+
+
 def test_invalid_csr_is_rejected_without_launching():
     m = chain_matrix(10, 2, seed=1)
     bad = m.item.copy()
