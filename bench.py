@@ -87,7 +87,13 @@ def main() -> int:
                     help="only time the complex SpMV of the headline pattern (profiling runs)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the streaming-copy probe (1 GiB read + 1 GiB write per launch)")
+    ap.add_argument("--trace-summary", metavar="CSV",
+                    help="summarise a rocprofv3 kernel_trace.csv per kernel and grid size (no GPU), then exit")
     args = ap.parse_args()
+
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary))
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -112,6 +118,17 @@ def main() -> int:
                                              ctypes.byref(ms_c)), "stream_copy")
         return {"bytes_per_launch": 16 * n, "ms": round(ms_c.value, 5),
                 "gbs": round(16 * n / (ms_c.value * 1e-3) / 1e9, 1)}
+
+    def read_probe(reps):
+        import ctypes
+        n = 1 << 28                                  # 2 GiB of doubles read per launch
+        src = api.DeviceBuffer(n)
+        src.zero()
+        ms_c = ctypes.c_double()
+        _lib.check(L.monolis_hip_stream_read(ctypes.c_void_p(src.ptr), n, reps, ctypes.byref(ms_c)),
+                   "stream_read")
+        return {"bytes_per_launch": 8 * n, "ms": round(ms_c.value, 5),
+                "gbs": round(8 * n / (ms_c.value * 1e-3) / 1e9, 1)}
 
     if args.multi_spmm:
         pm = W.poisson7(128)
@@ -190,6 +207,8 @@ def main() -> int:
 
     cp = copy_probe(20)
     log(f"stream copy probe: {cp['gbs']:.0f} GB/s")
+    rp = read_probe(20)
+    log(f"stream read probe: {rp['gbs']:.0f} GB/s; the FP64 SpMV reaches {gbs / rp['gbs']:.3f} of it")
 
     out = {
         "metric": METRIC,
@@ -213,6 +232,8 @@ def main() -> int:
         "spmv_per_launch": per_launch,
         "spmv_fp32": {"ms": round(ms32, 5), "gbs": round(gbs32, 1), "algorithmic_bytes": B32},
         "stream_copy": cp,
+        "stream_read": rp,
+        "spmv_vs_read_probe": round(gbs / rp["gbs"], 4),
         "autotune": {"variant_fp64": tune["best64"], "variant_fp32": tune["best32"],
                      "us_fp64": [round(v * 1e3, 1) for v in tune["ms64"]],
                      "us_fp32": [round(v * 1e3, 1) for v in tune["ms32"]]},
@@ -245,6 +266,16 @@ def main() -> int:
         errt = float(np.linalg.norm(xrh - xt) / np.linalg.norm(xt))
         log(f"CG refined: {st['outer']} outer / {st['inner_total']} inner, {tr * 1e3:.1f} ms, "
             f"speedup {t64 / tr:.2f}x, rel. err vs FP64 {err:.2e}")
+        # monolis's time split (Rarray time_spmv / time_dotp / time_prec) of one more FP64 solve with
+        # is_measurement on: an event after every kernel charges each interval to its kernel class
+        prm_m = api.Prm().quiet()
+        prm_m.Iarray[api.MONOLIS_PRM_I_IS_MEASUREMENT] = 1
+        dm.solve(db, api.DeviceBuffer(m.npndof), prm_m)
+        split = {"solve_ms": round(prm_m.Rarray[3] * 1e3, 3), "spmv_ms": round(prm_m.Rarray[5] * 1e3, 3),
+                 "dotp_ms": round(prm_m.Rarray[6] * 1e3, 3), "prec_ms": round(prm_m.Rarray[7] * 1e3, 3),
+                 "iters": int(prm_m.Iarray[4])}
+        log(f"CG FP64 time split (is_measurement): SpMV {split['spmv_ms']:.2f} ms, vector/dot "
+            f"{split['dotp_ms']:.2f} ms, preconditioner step {split['prec_ms']:.2f} ms of {split['solve_ms']:.2f} ms")
         out["cg"] = {
             "tol": 1e-8, "precond": "block-Jacobi 3x3",
             "fp64_ms": round(t64 * 1e3, 3), "fp64_iters": it64, "fp64_true_resid": res64,
@@ -254,6 +285,7 @@ def main() -> int:
             "refined_inner_tol": args.inner_tol,
             "speedup_refined_vs_fp64": round(t64 / tr, 3),
             "refined_rel_err_vs_fp64": err, "refined_rel_err_vs_xtrue": errt,
+            "time_split_fp64": split,
             "cg_iter_bytes_fp64": metrics.cg_iter_bytes(m.n, m.nz, 3, 8),
             "cg_fp64_gbs": round(metrics.cg_iter_bytes(m.n, m.nz, 3, 8) * it64 / t64 / 1e9, 1),
         }
@@ -337,6 +369,28 @@ def main() -> int:
 
     print(json.dumps(out), flush=True)
     return 0
+
+
+def trace_summary(path: str) -> str:
+    """Markdown table of a rocprofv3 --kernel-trace CSV: per (kernel, grid size)
+    the dispatch count and the mean / median / min / max duration in us."""
+    import csv
+    import statistics
+    groups = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+            name = r["Kernel_Name"].replace("(anonymous namespace)::", "").split("(")[0].strip()
+            if name.startswith("void "):
+                name = name[5:]
+            key = (name[:120], int(r["Grid_Size_X"]) // max(int(r["Workgroup_Size_X"]), 1))
+            groups.setdefault(key, []).append(d)
+    lines = ["| kernel | workgroups | calls | mean us | median us | min us | max us |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    for (name, wg), ds in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"| `{name}` | {wg} | {len(ds)} | {statistics.mean(ds):.1f} | {statistics.median(ds):.1f} | "
+                     f"{min(ds):.1f} | {max(ds):.1f} |")
+    return "\n".join(lines)
 
 
 def cpu_model() -> str:

@@ -238,12 +238,13 @@ int monolis_hip_matvec_multi_R(monolis_hip_matrix* M, int nrhs, const double* x,
 int monolis_hip_matvec_multi_S(monolis_hip_matrix* M, int nrhs, const float* x, float* y);
 
 /* Autotuner of the multi-RHS SpMM: row batches whose gathers are issued
- * together x workgroups per CU.  Times `reps` SpMMs with nrhs columns per
+ * together x workgroups per CU x right-hand sides per pass over the matrix
+ * (chunk; 0 = all in one pass).  Times `reps` SpMMs with nrhs columns per
  * configuration (fp32 = 1: the FP32 copy used by the refined solve's inner CG)
  * and keeps the fastest on the handle; ms (if not NULL) gets
  * monolis_hip_multi_num_configs() averages. */
 int monolis_hip_multi_num_configs(void);
-int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu);
+int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk);
 int monolis_hip_multi_autotune(monolis_hip_matrix* M, int nrhs, int fp32, int reps, int* best, double* ms);
 /* Forces a multi-RHS SpMM configuration for one precision (fp32 = 0 or 1). */
 int monolis_hip_multi_set_config(monolis_hip_matrix* M, int fp32, int cfg);
@@ -354,6 +355,13 @@ int monolis_hip_time_matvec_C(monolis_hip_matrix_C* M, const double _Complex* x,
  * *ms = average per launch.  Used to calibrate HBM byte counters and to state
  * the copy bandwidth the SpMV is compared with. */
 int monolis_hip_stream_copy(double* dst, const double* src, long long n, int reps, double* ms);
+
+/* Read-bandwidth probe (no monolis counterpart): every one of the n doubles of
+ * src is read once per launch (summed per wavefront so none is dropped; one
+ * double per wavefront written), `reps` timed launches on the library stream
+ * for each of six load shapes and grid sizes; *ms = the fastest average per
+ * launch.  States the read ceiling that the read-dominated SpMV is compared with. */
+int monolis_hip_stream_read(const double* src, long long n, int reps, double* ms);
 
 #ifdef __cplusplus
 }

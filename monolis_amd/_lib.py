@@ -68,6 +68,7 @@ SIGNATURES = [
     ("monolis_hip_time_matvec", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_double_p]),
     ("monolis_hip_time_matvec_each", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     ("monolis_hip_stream_copy", c_int, [c_void_p, c_void_p, ctypes.c_longlong, c_int, c_double_p]),
+    ("monolis_hip_stream_read", c_int, [c_void_p, ctypes.c_longlong, c_int, c_double_p]),
     ("monolis_hip_num_variants", c_int, []),
     ("monolis_hip_matrix_autotune", c_int, [c_void_p, c_int, c_int, c_int_p, c_double_p]),
     ("monolis_hip_matrix_set_variant", c_int, [c_void_p, c_int, c_int]),
@@ -95,7 +96,7 @@ SIGNATURES = [
     ("monolis_hip_matvec_multi_R", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("monolis_hip_matvec_multi_S", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("monolis_hip_multi_num_configs", c_int, []),
-    ("monolis_hip_multi_config", c_int, [c_int, c_int_p, c_int_p]),
+    ("monolis_hip_multi_config", c_int, [c_int, c_int_p, c_int_p, c_int_p]),
     ("monolis_hip_multi_autotune", c_int, [c_void_p, c_int, c_int, c_int, c_int_p, c_double_p]),
     ("monolis_hip_multi_set_config", c_int, [c_void_p, c_int, c_int]),
     ("monolis_hip_solve_multi_refined_R", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,

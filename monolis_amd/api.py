@@ -327,9 +327,9 @@ class DeviceMatrix:
                                                 ctypes.byref(best), ms), "multi_autotune")
         cfgs = []
         for c in range(nc):
-            sb, wpc = ctypes.c_int(), ctypes.c_int()
-            L.monolis_hip_multi_config(c, ctypes.byref(sb), ctypes.byref(wpc))
-            cfgs.append((sb.value, wpc.value))
+            sb, wpc, kc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            L.monolis_hip_multi_config(c, ctypes.byref(sb), ctypes.byref(wpc), ctypes.byref(kc))
+            cfgs.append((sb.value, wpc.value, kc.value))
         return {"best": best.value, "ms": list(ms), "configs": cfgs}
 
     # device-side scalar access (monolis_{set,add,get}_scalar_*_sparse_matrix_R, 0-based)

@@ -11,6 +11,7 @@
 #include "../../include/monolis_hip.h"
 #include "mh_internal.hpp"
 #include "mh_kernels.hpp"
+#include "mh_launch.hpp"
 #include "mh_batch.hpp"
 #include "mh_multirhs.hpp"
 #include "mh_adjoint.hpp"
@@ -43,7 +44,8 @@ namespace {
 // Iarray / Rarray slots (src/define/def_solver_prm.f90:90-163)
 enum {
   I_METHOD = 1, I_PRECOND = 2, I_MAX_ITER = 3, I_CUR_ITER = 4, I_IERR = 5, I_IS_INIT_X = 10,
-  I_IS_MEASUREMENT = 13, I_IS_ERROR_ABORT = 16, I_SHOW_ITERLOG = 20, I_SHOW_TIME = 21, I_SHOW_SUMMARY = 22
+  I_IS_MEASUREMENT = 13, I_IS_ERROR_ABORT = 16, I_SHOW_ITERLOG = 20, I_SHOW_TIME = 21, I_SHOW_SUMMARY = 22,
+  I_SHOW_TIME_STATISTICS = 23
 };
 enum { R_TOL = 1, R_CUR_RESID = 2, R_TIME_SOL = 3, R_TIME_PREP = 4, R_TIME_SPMV = 5, R_TIME_DOTP = 6,
        R_TIME_PREC = 7, R_TIME_COMM_DOTP = 8, R_TIME_COMM_SPMV = 9 };
@@ -150,6 +152,38 @@ struct UniformBlocks {
   exit(EXIT_FAILURE);
 }
 
+// Solver time split of monolis's Rarray (time_spmv 5, time_dotp 6,
+// time_prec 7, time_comm_dotp 8, time_comm_spmv 9; src/define/
+// def_solver_prm.f90:151-163, zeroed per solve as in monolis_timer_initialize
+// :237-251).  The split is measured when the caller asks for timing
+// (Iarray is_measurement 13, show_time 21 or show_time_statistics 23): an event
+// after every solver kernel charges the interval to the kernel's class (SpMV,
+// vector update / dot product, preconditioner apply).  Without those flags the
+// slots stay 0, so no events sit between the kernels of the hot loop.  One
+// GPU exchanges no halo and reduces without MPI: the communication slots are 0.
+struct SplitTimer {
+  std::unique_ptr<mh::KTimer> t;
+  SplitTimer(const int* Iarray, double* Rarray, hipStream_t s) {
+    for (int k = R_TIME_SPMV; k <= R_TIME_COMM_SPMV; ++k) Rarray[k] = 0.0;
+    if (Iarray[I_IS_MEASUREMENT] || Iarray[I_SHOW_TIME] || Iarray[I_SHOW_TIME_STATISTICS]) {
+      t.reset(new mh::KTimer(s));
+      mh::active_ktimer() = t.get();
+    }
+  }
+  void finish(double* Rarray) {
+    if (!t) return;
+    mh::active_ktimer() = nullptr;
+    t->harvest(true);
+    if (t->err) return;
+    Rarray[R_TIME_SPMV] = t->ms[mh::kClsSpmv] * 1e-3;
+    Rarray[R_TIME_DOTP] = t->ms[mh::kClsDot] * 1e-3;
+    Rarray[R_TIME_PREC] = t->ms[mh::kClsPrec] * 1e-3;
+  }
+  ~SplitTimer() {
+    if (t) mh::active_ktimer() = nullptr;
+  }
+};
+
 // Common device-resident solve with matrix A (the handle's matrix or its
 // transpose).  Fills Iarray/Rarray; returns a status code.
 int solve_device(Matrix& A, mh::Workspace& W, const double* b, double* x, int* Iarray, double* Rarray,
@@ -160,8 +194,10 @@ int solve_device(Matrix& A, mh::Workspace& W, const double* b, double* x, int* I
   int rc = check_method(P.method, P.precond);
   if (rc) return rc;
   mh::SolveResult res;
+  SplitTimer timer(Iarray, Rarray, c->stream);
   rc = mh::krylov_solve<double>(A, P, b, x, W, &res, c->stream);
   if (rc) return rc == -3 ? ST_PRECOND : rc == -4 ? ST_METHOD : rc == -2 ? ST_ARGS : ST_HIP;
+  timer.finish(Rarray);
   if (!res.zero_rhs) {
     Iarray[I_CUR_ITER] = res.iter;
     Rarray[R_CUR_RESID] = res.resid;
@@ -179,8 +215,10 @@ int solve_device_C(Matrix& A, mh::Workspace& W, const mh::zd* b, mh::zd* x, int*
   if (mh::context(&c)) return ST_HIP;
   const mh::SolveParams P = params_from(Iarray, Rarray);
   mh::ZSolveResult res;
+  SplitTimer timer(Iarray, Rarray, c->stream);
   const int rc = mh::cocg_solve(A, P, b, x, W, &res, c->stream);
   if (rc) return rc == -3 ? ST_PRECOND : rc == -4 ? ST_METHOD : rc == -2 ? ST_ARGS : rc == -6 ? ST_ZERO_PIVOT : ST_HIP;
+  timer.finish(Rarray);
   if (!res.zero_rhs) {
     Iarray[I_CUR_ITER] = res.iter;
     Rarray[R_CUR_RESID] = res.resid_re;  // mat_converge.f90:178: complex resid stored in a real slot
@@ -189,6 +227,34 @@ int solve_device_C(Matrix& A, mh::Workspace& W, const mh::zd* b, mh::zd* x, int*
   }
   if (hist_out) *hist_out = std::move(res.hist);
   return res.converged ? ST_OK : ST_NOT_CONVERGED;
+}
+
+// Read-bandwidth probe kernel: every element of src is read once per launch,
+// UNR non-temporal loads of type V in flight per thread over a grid-stride
+// loop, summed so that no load can be dropped; each wavefront writes one double.
+template <typename V, int UNR>
+__global__ void __launch_bounds__(256) k_stream_read(long long nv, const V* __restrict__ src,
+                                                     double* __restrict__ out) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  for (; i + (UNR - 1) * stride < nv; i += UNR * stride) {
+    V v[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      if constexpr (sizeof(V) == 16) acc += v[u].x + v[u].y;
+      else acc += v[u];
+    }
+  }
+  for (; i < nv; i += stride) {
+    const V v = src[i];
+    if constexpr (sizeof(V) == 16) acc += v.x + v.y;
+    else acc += v;
+  }
+  acc = mh::wave_sum(acc);
+  if ((threadIdx.x & (mh::kWave - 1)) == 0) out[(size_t)blockIdx.x * (256 / mh::kWave) + threadIdx.x / mh::kWave] = acc;
 }
 
 }  // namespace
@@ -347,9 +413,9 @@ int monolis_hip_matvec_S(monolis_hip_matrix* H, const float* x, float* y) {
 int monolis_hip_inner_product_R(int m, const double* x, const double* y, double* sum) {
   Context* c = nullptr;
   MH_TRY(mh::context(&c));
-  mh::DevArray<double> scratch;
+  mh::DevArray<double> parts;
   const size_t np = mh::partials_needed(m);
-  MH_TRY(scratch.alloc(np + 16));
+  MH_TRY(parts.alloc(np + 16));
   const double* dx = x;
   const double* dy = y;
   mh::DevArray<double> hx, hy;
@@ -363,8 +429,8 @@ int monolis_hip_inner_product_R(int m, const double* x, const double* y, double*
     MH_TRY(mh::upload(hy.p, y, sizeof(double) * (size_t)m, c->stream));
     dy = hy.p;
   }
-  MH_TRY(mh::dot<double>(m, dx, dy, scratch.p + np, scratch.p, c->stream));
-  MH_HIP(hipMemcpyAsync(sum, scratch.p + np, sizeof(double), hipMemcpyDefault, c->stream));
+  MH_TRY(mh::dot<double>(m, dx, dy, parts.p + np, parts.p, c->stream));
+  MH_HIP(hipMemcpyAsync(sum, parts.p + np, sizeof(double), hipMemcpyDefault, c->stream));
   MH_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -401,11 +467,13 @@ int monolis_hip_solve_refined_R(monolis_hip_matrix* H, const double* b, double* 
   P.precond = Iarray[I_PRECOND];
   P.init_x = Iarray[I_IS_INIT_X] != 0;
   mh::RefineResult R;
+  SplitTimer timer(Iarray, Rarray, c->stream);
   const int rc = mh::refine_solve(*H->M, P, b, x, H->W, H->W_outer, &R, c->stream);
   if (rc) {
     Iarray[I_IERR] = ST_HIP;
     return rc;
   }
+  timer.finish(Rarray);
   Iarray[I_CUR_ITER] = R.inner_total;
   Rarray[R_CUR_RESID] = R.resid;
   Iarray[I_IERR] = R.converged ? ST_OK : ST_NOT_CONVERGED;
@@ -856,12 +924,12 @@ int monolis_hip_adjoint_R(monolis_hip_matrix* H, const double* x, const double* 
 
 int monolis_hip_multi_num_configs(void) { return mh::multi_num_configs(); }
 
-int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu) {
+int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk) {
   if (cfg < 0 || cfg >= mh::multi_num_configs()) {
     mh::set_error("multi-RHS config %d out of range", cfg);
     return ST_ARGS;
   }
-  mh::multi_config_info(cfg, steps_per_batch, wg_per_cu);
+  mh::multi_config_info(cfg, steps_per_batch, wg_per_cu, chunk);
   return 0;
 }
 
@@ -919,6 +987,48 @@ int monolis_hip_stream_copy(double* dst, const double* src, long long n, int rep
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *ms = reps > 0 ? (double)t / reps : 0.0;
+  return 0;
+}
+
+int monolis_hip_stream_read(const double* src, long long n, int reps, double* ms) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  *ms = 0.0;
+  if (n < 2 || reps <= 0) return 0;
+  // shapes tried: 8-byte loads x 8 in flight and 16-byte loads x 4 in flight,
+  // each at 2, 4 and 8 workgroups of 256 threads per CU; the fastest is reported
+  const int cus = mh::cu_count();
+  mh::DevArray<double> out;
+  MH_TRY(out.alloc((size_t)8 * cus * (256 / mh::kWave)));
+  hipEvent_t e0, e1;
+  MH_HIP(hipEventCreate(&e0));
+  MH_HIP(hipEventCreate(&e1));
+  double best = 0.0;
+  int rc = 0;
+  for (int shape = 0; shape < 6 && !rc; ++shape) {
+    const int grid = (2 << (shape % 3)) * cus;
+    const bool wide = shape >= 3;
+    auto launch = [&]() {
+      if (wide) k_stream_read<mh::d2v, 4><<<grid, 256, 0, c->stream>>>(n / 2, (const mh::d2v*)src, out.p);
+      else k_stream_read<double, 8><<<grid, 256, 0, c->stream>>>(n, src, out.p);
+    };
+    launch();  // warm-up
+    if (hipGetLastError() != hipSuccess || hipEventRecord(e0, c->stream) != hipSuccess) rc = ST_HIP;
+    for (int r = 0; r < reps && !rc; ++r) launch();
+    float t = 0.f;
+    if (!rc && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+      rc = ST_HIP;
+    const double per = (double)t / reps;
+    if (!rc && (best == 0.0 || per < best)) best = per;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    mh::set_error("stream_read probe failed");
+    return rc;
+  }
+  *ms = best;
   return 0;
 }
 

@@ -82,10 +82,11 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// Sum NV values over the workgroup; result valid in thread 0.
-template <int NV>
+// Sum NV values over a workgroup of NT threads; result valid in thread 0.
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-  __shared__ double red[kWpb][NV];
+  constexpr int NW = NT / kWave;
+  __shared__ double red[NW][NV];
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
 #pragma unroll
@@ -99,7 +100,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = red[0][k];
-      for (int i = 1; i < kWpb; ++i) s += red[i][k];
+      for (int i = 1; i < NW; ++i) s += red[i][k];
       v[k] = s;
     }
   }
@@ -120,17 +121,36 @@ __device__ __forceinline__ void store_partials(const double (&v)[NV], double* pa
   }
 }
 
-// Fixed-order sum of ntiles x NV partials by one workgroup; valid in every thread.
-template <int NV>
+// Fixed-order sum of ntiles x NV partials by one workgroup of NT threads;
+// valid in every thread.  A pass covers NT * PER tiles (4096 for NV <= 2) and
+// issues all of its loads before the first add (out-of-range slots read the
+// last tile and add 0), so a sum of a few thousand tiles costs about one
+// memory latency: kernels that take the previous kernel's partials in their
+// prologue run it in every workgroup.  The order depends on NT, so every
+// derivation of one scalar uses the same NT.
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void sum_partials(const double* partials, int ntiles, double (&total)[NV]) {
+  constexpr int PER = NV <= 2 ? 16 * kBlock / NT : 8 * kBlock / NT;
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-  for (int i = threadIdx.x; i < ntiles; i += kBlock) {
+  for (int base = 0; base < ntiles; base += NT * PER) {
+    double v[PER][NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] += partials[(size_t)i * NV + k];
+    for (int u = 0; u < PER; ++u) {
+      const int t = base + u * NT + (int)threadIdx.x;
+      const int tc = t < ntiles ? t : ntiles - 1;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) v[u][k] = partials[(size_t)tc * NV + k];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int t = base + u * NT + (int)threadIdx.x;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) acc[k] += t < ntiles ? v[u][k] : 0.0;
+    }
   }
-  block_sum<NV>(acc);
+  block_sum<NV, NT>(acc);
   __shared__ double tot[NV];
   if (threadIdx.x == 0) {
 #pragma unroll

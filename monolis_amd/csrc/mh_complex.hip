@@ -121,10 +121,12 @@ int launch_zsell(const Matrix& M, const Op& op, double* partials, hipStream_t s)
     sell_geometry(M, kZSellGeom, &nwg, &spw);
     k_sell<zd, zd, B, ZUnroll<B>::U, Op><<<padded_grid(nwg), kBlock, 0, s>>>(view<zd>(M), op, partials, nwg, spw);
     MH_HIP(hipGetLastError());
+    ktimer_mark(kClsSpmv);
   }
   if constexpr (Op::NV > 0) {
     k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, nwg);
     MH_HIP(hipGetLastError());
+    ktimer_mark(kClsDot);
   }
   return 0;
 }
@@ -242,6 +244,7 @@ struct OpZSpmvPQ {
 // F_EXTPREC: z, rho and beta come from OpZVPrec (variable blocks) instead.
 template <int B, bool PREC>
 struct OpZUpdate {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = zd;
   static constexpr int BS = B;
   static constexpr int NV = 4;
@@ -313,6 +316,7 @@ struct OpZUpdate {
 // ---- variable blocks: z = M_V^-1 r per block row (diag_V.f90:208-250), rho = (r, z),
 // beta = rho / rho_old (first: 0) ------------------------------------------------
 struct OpZVPrec {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = zd;
   static constexpr int BS = 1;
   static constexpr int NV = 2;

@@ -84,6 +84,9 @@ struct KState {
   int bd_stage;     // != 0: the loop stopped for the sorted-sum fallback at this stage
   int bd_iter;      // ... in this iteration
   int pad;
+  // rho of iteration k in rhok[k & 1]: a kernel that derives rho_k stores it
+  // while its other workgroups still read rho_{k-1} from the other slot
+  double rhok[2];
 };
 
 static_assert(offsetof(KState, iter) == offsetof(KState, resid) + 8 &&

@@ -18,6 +18,8 @@
 //   copy (SellView::col16), which halves the index stream.
 #pragma once
 
+#include <type_traits>
+
 #include "mh_common.hpp"
 
 namespace mh {
@@ -365,6 +367,46 @@ __device__ __forceinline__ void diag_apply(const TD* __restrict__ D, int nrows, 
   }
 }
 
+// Deferred reduction finalize.  An Op may take the partial sums that the
+// previous kernel on the stream stored, instead of a one-workgroup finalize
+// kernel launched between the two.  Such an Op has `static constexpr int
+// NPRE > 0` and members `pre`, `npre` (NPRE values per tile) and `pre2`,
+// `npre2` (optional: one value per tile that replaces the last of the NPRE
+// sums).  With a non-null `pre`, every workgroup sums them in the fixed order
+// of sum_partials (so every workgroup derives the same bits) and calls
+// `bool prologue(const double (&tot)[NPRE], bool writer)`.  The writer
+// (logical workgroup 0, thread 0) records the derived scalars in the solver
+// state; false ends the launch (the solve is done).  The kernel boundary makes
+// the previous kernel's partials visible, so no device-scope fence is needed.
+template <class Op, class = void>
+struct op_npre {
+  static constexpr int value = 0;
+};
+template <class Op>
+struct op_npre<Op, std::void_t<decltype(Op::NPRE)>> {
+  static constexpr int value = Op::NPRE;
+};
+
+template <int NT, class Op>
+__device__ __forceinline__ bool run_prologue(Op& op, int wg) {
+  constexpr int NP = op_npre<Op>::value;
+  if constexpr (NP > 0) {
+    if (op.pre == nullptr) return true;
+    double tot[NP];
+    sum_partials<NP, NT>(op.pre, op.npre, tot);
+    if (op.pre2) {
+      double t1[1];
+      sum_partials<1, NT>(op.pre2, op.npre2, t1);
+      tot[NP - 1] = t1[0];
+    }
+    return op.prologue(tot, wg == 0 && threadIdx.x == 0);
+  } else {
+    (void)op;
+    (void)wg;
+    return true;
+  }
+}
+
 // Generic fused SpMV kernel: one lane per block row, one wavefront per 64-row
 // slice at a time.  With spw == 0 wavefront w of the grid handles slice w; with
 // spw == 1 the nwg*4 wavefronts split the slices into contiguous ranges.
@@ -384,6 +426,7 @@ __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op0, double*
   Op op = op0;
   if (op.skip()) return;
   op.begin();
+  if (!run_prologue<kBlock>(op, wg)) return;
   const int lane = threadIdx.x & (kWave - 1);
   const int lw = __builtin_amdgcn_readfirstlane(wg * kWpb + (int)(threadIdx.x / kWave));
   int s0 = lw, s1 = lw + 1;
@@ -434,36 +477,72 @@ __global__ void __launch_bounds__(kBlock) k_sell(SellView<TV> M, Op op0, double*
     default: { constexpr int U = 3; __VA_ARGS__; } break;           \
   }
 
-// Row-parallel (non-SpMV) fused vector kernel: one thread per block row.
+// Row-parallel (non-SpMV) fused vector kernel over tiles of NT block rows: in
+// a tile a thread takes one block row (wavefront w the rows NT*t + 64w .. +63).
+// The grid is persistent -- nwg logical workgroups, workgroup g taking the
+// contiguous tiles [g*ntiles/nwg, (g+1)*ntiles/nwg) -- so a prologue runs once
+// per workgroup and a launch leaves nwg partial sets.  Rows of up to 24 bytes
+// (B <= 3 in FP64) run 1024-thread workgroups; wider rows keep 256 threads,
+// which bounds the LDS staging (64 * B entries per wavefront).
 //   using VT (vector type) and BS (block size);
 //   bool skip(); void begin();
 //   void row(int i, bool valid, double (&part)[NV], const WaveRows<VT, BS>& io)
 //       (every lane calls it; per-row vectors via io)
 //   void finalize(...)
+template <typename VT, int BS>
+__host__ __device__ constexpr int row_block() {
+  return BS * sizeof(VT) <= 24 ? 1024 : kBlock;
+}
+template <typename VT, int BS>
+__host__ __device__ constexpr int rows_tiles(long long nrows) {
+  return (int)((nrows + row_block<VT, BS>() - 1) / row_block<VT, BS>());
+}
+
 template <class Op>
-__global__ void __launch_bounds__(kBlock) k_rows(int nrows, Op op0, double* partials, int ntiles) {
-  const int tile = xcd_tile<true>(ntiles);
-  if (tile < 0) return;
+__host__ __device__ constexpr int op_row_block() {
+  return row_block<typename Op::VT, Op::BS>();
+}
+
+template <class Op>
+__global__ void __launch_bounds__(op_row_block<Op>()) k_rows(int nrows, Op op0, double* partials, int nwg,
+                                                             int ntiles) {
+  using VT = typename Op::VT;
+  constexpr int BS = Op::BS;
+  constexpr int NT = row_block<VT, BS>();
+  const int g = xcd_tile<true>(nwg);
+  if (g < 0) return;
   Op op = op0;
   if (op.skip()) return;
   op.begin();
-  using VT = typename Op::VT;
-  constexpr int BS = Op::BS;
-  __shared__ VT lds[kWpb * kWave * BS];
-  const int i = tile * kBlock + threadIdx.x;
+  if (!run_prologue<NT>(op, g)) return;
+  __shared__ VT lds[NT * BS];
   constexpr int NV = Op::NV;
   double part[NV > 0 ? NV : 1];
 #pragma unroll
   for (int k = 0; k < (NV > 0 ? NV : 1); ++k) part[k] = 0.0;
-  const int row0 = tile * kBlock + (int)(threadIdx.x / kWave) * kWave;
-  if (row0 < nrows) {
-    const WaveRows<VT, BS> io{lds + (threadIdx.x / kWave) * kWave * BS, row0, nrows, (int)(threadIdx.x & (kWave - 1))};
-    op.row(i, i < nrows, part, io);
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const int wofs = (int)(threadIdx.x / kWave) * kWave;
+  const int t0 = (int)((long long)g * ntiles / nwg), t1 = (int)((long long)(g + 1) * ntiles / nwg);
+  for (int tile = t0; tile < t1; ++tile) {
+    const int row0 = tile * NT + wofs;
+    if (row0 < nrows) {
+      const WaveRows<VT, BS> io{lds + wofs * BS, row0, nrows, lane};
+      op.row(row0 + lane, row0 + lane < nrows, part, io);
+    }
   }
   if constexpr (NV > 0) {
-    block_sum<NV>(part);
-    store_partials<NV>(part, partials, tile);
+    block_sum<NV, NT>(part);
+    store_partials<NV>(part, partials, g);
   }
+}
+
+// The prologue of an Op alone (one workgroup of kBlock threads): consumes the
+// partials of the last kernel of a loop when no further kernel follows.
+template <int NT, class Op>
+__global__ void __launch_bounds__(NT) k_prologue_only(Op op) {
+  if (op.skip()) return;
+  op.begin();
+  run_prologue<NT>(op, 0);
 }
 
 // Second phase of a reduction: one workgroup, fixed-order sum, op.finalize().

@@ -22,6 +22,11 @@
 
 namespace mh {
 
+KTimer*& active_ktimer() {
+  thread_local KTimer* t = nullptr;
+  return t;
+}
+
 namespace {
 
 constexpr int kIterRR = 200;
@@ -92,26 +97,69 @@ struct OpResidSolve {
   }
 };
 
-// ---- CG: p = z + beta p (CG.f90:80-84; first iteration beta = 0, p = z) -------
-// Elementwise: launched over n*B entries.
+// ---- deferred convergence test (mat_converge.f90:90-141) ---------------------
+// Evaluated by every workgroup of the kernel that consumes iteration it's
+// (r, r) partials; only the writer records it.  True when the solve is done.
+__device__ __forceinline__ bool check_deferred(KState* st, double* hist, int it, double rr, bool writer) {
+  const double resid = sqrt(rr / st->B2);
+  const bool conv = resid < st->tol;
+  const bool done = conv || it >= st->max_iter;
+  if (writer) {
+    st->rr = rr;
+    st->resid = resid;
+    st->iter = it;
+    if (hist) hist[it] = resid;
+    if (conv) st->converged = 1;
+    if (done) st->done = 1;
+  }
+  return done;
+}
+
+// ---- CG: test of iteration it-1; rho, beta; p = z + beta p ---------------------
+// (CG.f90:71-84, and mat_converge.f90:90-141 for the previous iteration.)
+// Elementwise over n*B entries.  The prologue takes the previous update's
+// partials (r, r), (r, z) -- with variable blocks (r, z) comes from the
+// preconditioner kernel's own partials -- and derives rho and
+// beta = rho / rho_1 (first iteration: beta = 0 with p = 0, so p = z).
+// (Forming p at every gather of the SpMV instead, from z and the previous p,
+// measured 361 us against 334 + 14 us for the SpMV and this pass.)
 template <typename T>
 struct OpCgP {
   using VT = T;
   static constexpr int BS = 1;
   static constexpr int NV = 0;
+  static constexpr int NPRE = 2;
   const T* __restrict__ z;
   T* __restrict__ p;
   KState* st;
+  double* hist;
+  const double* pre;   // update partials, (r, r) and (r, z) per tile
+  int npre;
+  const double* pre2;  // optional (r, z) partials, one per tile
+  int npre2;
+  int it;
   T beta;
   __device__ bool skip() const { return st->done != 0; }
-  __device__ void begin() { beta = (T)st->beta; }
+  __device__ void begin() {}
+  __device__ bool prologue(const double (&tot)[2], bool writer) {
+    if (it > 1 && check_deferred(st, hist, it - 1, tot[0], writer)) return false;
+    const double rho = tot[1];
+    const double bt = it > 1 ? rho / st->rhok[(it - 1) & 1] : 0.0;  // CG.f90:80-84
+    beta = (T)bt;
+    if (writer) {
+      st->rhok[it & 1] = rho;
+      st->rho = rho;
+      st->beta = bt;
+    }
+    return true;
+  }
   __device__ void row(int i, bool valid, double (&)[1], const WaveRows<T, 1>&) const {
     if (valid) p[i] = fma(beta, p[i], z[i]);  // P = beta*P + 1*Z
   }
   __device__ void finalize(const double (&)[1]) const {}
 };
 
-// ---- CG: q = A p; (p, q) -> alpha = rho / (p, q) (CG.f90:86-88) -----------------
+// ---- CG: q = A p; partials (p, q) for the update's alpha (CG.f90:86-88) ---------
 template <typename T, int B>
 struct OpCgSpmv {
   static constexpr int NV = 1;
@@ -133,18 +181,20 @@ struct OpCgSpmv {
       for (int k = 0; k < B; ++k) part[0] += (double)pv[k] * (double)acc[k];
     }
   }
-  __device__ void finalize(const double (&tot)[1]) const {
-    st->pq = tot[0];
-    st->alpha = st->rho / tot[0];
-  }
+  __device__ void finalize(const double (&)[1]) const {}
 };
 
-// ---- CG: x += alpha p; r -= alpha q; test; z = M^-1 r; rho, beta -------------
+// ---- CG: alpha = rho / (p, q); x += alpha p; r -= alpha q; z = M^-1 r ---------
+// (CG.f90:86-100, then the next iteration's preconditioner CG.f90:71.)  The
+// prologue (with F_UPD or F_XONLY) takes the SpMV's (p, q) partials; the
+// partials (r, r) and (r, z) are left for the next SpMV's prologue.
 template <typename T, int B, bool PREC>
 struct OpCgUpdate {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 2;
+  static constexpr int NPRE = 1;
   T* __restrict__ x;
   T* __restrict__ r;
   T* __restrict__ z;
@@ -153,12 +203,24 @@ struct OpCgUpdate {
   const T* __restrict__ D;
   int n;
   KState* st;
-  double* hist;
   int it;
   int flags;
+  const double* pre;  // SpMV partials (p, q), one per tile; null: no x / r update
+  int npre;
   T alpha;
+  const double* pre2 = nullptr;  // unused: every sum comes from `pre`
+  int npre2 = 0;
   __device__ bool skip() const { return st->done != 0; }
-  __device__ void begin() { alpha = (T)st->alpha; }
+  __device__ void begin() {}
+  __device__ bool prologue(const double (&tot)[1], bool writer) {
+    const double a = st->rhok[it & 1] / tot[0];  // CG.f90:87-88
+    alpha = (T)a;
+    if (writer) {
+      st->pq = tot[0];
+      st->alpha = a;
+    }
+    return true;
+  }
   __device__ void row(int i, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
     if (flags & (F_UPD | F_XONLY)) {
       T xv[B], pv[B];
@@ -198,25 +260,24 @@ struct OpCgUpdate {
     }
     io.store(z, w);
   }
-  __device__ void finalize(const double (&tot)[2]) const {
-    if (flags & F_XONLY) return;
-    if (flags & F_CHECK) {
-      if (finalize_check(st, hist, it, tot[0])) return;
-    }
-    if (flags & F_EXTPREC) return;  // rho and beta come from the preconditioner kernel
-    const double rho = tot[1];
-    st->beta = (flags & F_FIRST) ? 0.0 : rho / st->rho;
-    st->rho = rho;
-  }
+  __device__ void finalize(const double (&)[2]) const {}
 };
 
 // ---- BiCGSTAB ---------------------------------------------------------------
-// a: p = r + beta (p - omega v) (first: p = r); pt = M^-1 p
+// Five launches per iteration and no finalize kernels: each kernel's prologue
+// takes the partial sums the kernel before it stored.  A resumed stage (after
+// the host's sorted-sum fallback) runs without its prologue (pre = null) and
+// takes its scalars from the state the host wrote.
+//
+// a: test of iteration it-1 and rho = (r, rt), beta (prologue, from e's
+//    partials); p = r + beta (p - omega v) (first: p = r); pt = M^-1 p
 template <typename T, int B, bool PREC>
 struct OpBiA {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 0;
+  static constexpr int NPRE = 2;
   const T* __restrict__ r;
   T* __restrict__ p;
   const T* __restrict__ v;
@@ -224,17 +285,45 @@ struct OpBiA {
   const T* __restrict__ D;
   int n;
   KState* st;
-  int first;
+  double* hist;
+  const double* pre;  // e's partials (r, r), (r, rt)
+  int npre;
+  int it;
   T beta, omega;
+  const double* pre2 = nullptr;  // unused: every sum comes from `pre`
+  int npre2 = 0;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() {
     beta = (T)st->beta;
     omega = (T)st->omega;
   }
+  __device__ bool prologue(const double (&tot)[2], bool writer) {
+    if (check_deferred(st, hist, it - 1, tot[0], writer)) return false;
+    const double rho = tot[1];
+    if (rho == 0.0) {  // BiCGSTAB.f90:89-93: the host recomputes it with the sorted sum
+      if (writer) {
+        st->breakdown |= 1;
+        st->bd_stage = 5;
+        st->bd_iter = it - 1;
+        st->done = 1;
+      }
+      return false;
+    }
+    const double rho1 = st->rhok[(it - 1) & 1];
+    const double bt = (rho / rho1) * (st->alpha / st->omega);  // BiCGSTAB.f90:96
+    beta = (T)bt;
+    if (writer) {
+      st->beta = bt;
+      st->rho_prev = rho1;
+      st->rho = rho;
+      st->rhok[it & 1] = rho;
+    }
+    return true;
+  }
   __device__ void row(int i, bool valid, double (&)[1], const WaveRows<T, B>& io) const {
     T pv[B];
     io.load(r, pv);
-    if (!first) {
+    if (it > 1) {
       T po[B], vv[B];
       io.load(p, po);
       io.load(v, vv);
@@ -250,7 +339,7 @@ struct OpBiA {
   __device__ void finalize(const double (&)[1]) const {}
 };
 
-// b: v = A pt; (rt, v) -> alpha = rho / c2
+// b: v = A pt; partials (rt, v) for c
 template <typename T, int B>
 struct OpBiB {
   static constexpr int NV = 1;
@@ -258,7 +347,6 @@ struct OpBiB {
   T* __restrict__ v;
   const T* __restrict__ rt;
   KState* st;
-  int it;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() {}
   __device__ void gather(int c, T (&xv)[B]) const {
@@ -274,26 +362,17 @@ struct OpBiB {
       for (int k = 0; k < B; ++k) part[0] += (double)rtv[k] * (double)acc[k];
     }
   }
-  __device__ void finalize(const double (&tot)[1]) const {
-    const double c2 = tot[0];
-    st->pq = c2;
-    if (c2 == 0.0) {  // BiCGSTAB.f90:110-114: recompute with the sorted sum
-      st->breakdown |= 2;
-      st->bd_stage = 2;
-      st->bd_iter = it;
-      st->done = 1;
-      return;
-    }
-    st->alpha = st->rho / c2;
-  }
+  __device__ void finalize(const double (&)[1]) const {}
 };
 
-// c: s = r - alpha v; st = M^-1 s
+// c: alpha = rho / (rt, v) (prologue, from b's partials); s = r - alpha v; st = M^-1 s
 template <typename T, int B, bool PREC>
 struct OpBiC {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 0;
+  static constexpr int NPRE = 1;
   const T* __restrict__ r;
   const T* __restrict__ v;
   T* __restrict__ s;
@@ -301,9 +380,34 @@ struct OpBiC {
   const T* __restrict__ D;
   int n;
   KState* st;
+  const double* pre;  // b's partials (rt, v)
+  int npre;
+  int it;
   T alpha;
+  const double* pre2 = nullptr;  // unused: every sum comes from `pre`
+  int npre2 = 0;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() { alpha = (T)st->alpha; }
+  __device__ bool prologue(const double (&tot)[1], bool writer) {
+    const double c2 = tot[0];
+    if (c2 == 0.0) {  // BiCGSTAB.f90:110-114: the host recomputes it with the sorted sum
+      if (writer) {
+        st->pq = c2;
+        st->breakdown |= 2;
+        st->bd_stage = 2;
+        st->bd_iter = it;
+        st->done = 1;
+      }
+      return false;
+    }
+    const double a = st->rhok[it & 1] / c2;  // BiCGSTAB.f90:116
+    alpha = (T)a;
+    if (writer) {
+      st->pq = c2;
+      st->alpha = a;
+    }
+    return true;
+  }
   __device__ void row(int i, bool valid, double (&)[1], const WaveRows<T, B>& io) const {
     T rv[B], vv[B];
     io.load(r, rv);
@@ -319,7 +423,7 @@ struct OpBiC {
   __device__ void finalize(const double (&)[1]) const {}
 };
 
-// d: t = A st; (t, s), (t, t) -> omega
+// d: t = A st; partials (t, s), (t, t) for e
 template <typename T, int B>
 struct OpBiD {
   static constexpr int NV = 2;
@@ -327,7 +431,6 @@ struct OpBiD {
   T* __restrict__ t;
   const T* __restrict__ s;
   KState* st;
-  int it;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() {}
   __device__ void gather(int c, T (&xv)[B]) const {
@@ -346,26 +449,17 @@ struct OpBiD {
       }
     }
   }
-  __device__ void finalize(const double (&tot)[2]) const {
-    st->ts = tot[0];
-    st->tt = tot[1];
-    if (tot[1] == 0.0) {  // BiCGSTAB.f90:132-137: recompute (t, t) with the sorted sum
-      st->breakdown |= 4;
-      st->bd_stage = 4;
-      st->bd_iter = it;
-      st->done = 1;
-      return;
-    }
-    st->omega = tot[0] / tot[1];
-  }
+  __device__ void finalize(const double (&)[2]) const {}
 };
 
-// e: x += alpha pt + omega st; r = s - omega t; test; rho = (r, rt); beta
+// e: omega = (t, s) / (t, t) (prologue, from d's partials); x += alpha pt +
+//    omega st; r = s - omega t; partials (r, r), (r, rt) for the next a
 template <typename T, int B>
 struct OpBiE {
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 2;
+  static constexpr int NPRE = 2;
   T* __restrict__ x;
   T* __restrict__ r;
   const T* __restrict__ pt;
@@ -374,14 +468,38 @@ struct OpBiE {
   const T* __restrict__ t;
   const T* __restrict__ rt;
   KState* st;
-  double* hist;
   int it;
   int flags;
+  const double* pre;  // d's partials (t, s), (t, t); null after a true residual
+  int npre;
   T alpha, omega;
+  const double* pre2 = nullptr;  // unused: every sum comes from `pre`
+  int npre2 = 0;
   __device__ bool skip() const { return st->done != 0; }
   __device__ void begin() {
     alpha = (T)st->alpha;
     omega = (T)st->omega;
+  }
+  __device__ bool prologue(const double (&tot)[2], bool writer) {
+    if (tot[1] == 0.0) {  // BiCGSTAB.f90:132-137: the host recomputes (t, t) with the sorted sum
+      if (writer) {
+        st->ts = tot[0];
+        st->tt = tot[1];
+        st->breakdown |= 4;
+        st->bd_stage = 4;
+        st->bd_iter = it;
+        st->done = 1;
+      }
+      return false;
+    }
+    const double w = tot[0] / tot[1];  // BiCGSTAB.f90:139
+    omega = (T)w;
+    if (writer) {
+      st->ts = tot[0];
+      st->tt = tot[1];
+      st->omega = w;
+    }
+    return true;
   }
   __device__ void row(int, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
     if (flags & (F_UPD | F_XONLY)) {
@@ -415,21 +533,7 @@ struct OpBiE {
       }
     }
   }
-  __device__ void finalize(const double (&tot)[2]) const {
-    if (flags & F_XONLY) return;
-    if (finalize_check(st, hist, it, tot[0])) return;
-    const double rho = tot[1];
-    if (rho == 0.0) {  // BiCGSTAB.f90:89-93 of the next iteration: sorted sum
-      st->breakdown |= 1;
-      st->bd_stage = 5;
-      st->bd_iter = it;
-      st->done = 1;
-      return;
-    }
-    st->beta = (rho / st->rho) * (st->alpha / st->omega);  // BiCGSTAB.f90:96
-    st->rho_prev = st->rho;
-    st->rho = rho;
-  }
+  __device__ void finalize(const double (&)[2]) const {}
 };
 
 // ---- PipeCG (src/iterative/PipeCG.f90) ---------------------------------------
@@ -438,6 +542,7 @@ struct OpBiE {
 //    alpha = gamma / (delta - beta*gamma*alpha1) (first: alpha = gamma/delta, beta = 0).
 template <typename T, int B, bool PREC>
 struct OpPipeA {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 2;
@@ -576,6 +681,7 @@ struct OpPipeB {
 // u = M^-1 r, with the test on (r, r) when F_CHECK (after a true residual).
 template <typename T, int B, bool PREC>
 struct OpPrecCheck {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 1;
@@ -640,6 +746,7 @@ struct OpCrPhi {
 //    sqrt(U2/B2)*phi.
 template <typename T, int B, bool PREC, bool MGIVEN>
 struct OpCrA {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 3;
@@ -764,6 +871,7 @@ struct OpCrB {
 // init: u = M^-1 r; gamma = (r, u)
 template <typename T, int B, bool PREC>
 struct OpGropInit {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 1;
@@ -889,6 +997,7 @@ struct OpGropSpmv {
 //    omega1 = QY / YY (0 when YY = 0).
 template <typename T, int B, bool PREC>
 struct OpPbA {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 2;
@@ -959,6 +1068,7 @@ struct OpPbA {
 //    as the reference writes it).
 template <typename T, int B, bool PREC>
 struct OpPbB {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 5;
@@ -1049,6 +1159,7 @@ struct OpPbB {
 // true residual).
 template <typename T, int B, bool PREC>
 struct OpPbInit {
+  static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
   static constexpr int NV = 2;
@@ -1126,17 +1237,25 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
   Carve cv{W.raw.p};
   T* r = cv.take<T>(vec);
   T* z = cv.take<T>(vec);
-  T* q = cv.take<T>(vec);  // p follows
+  T* q = cv.take<T>(vec);
   T* p = cv.take<T>(vec);
-  const size_t nparts = partials_needed((long long)M.n) * 4 + partials_needed((long long)M.nslices * kWave) * 4;
+  const size_t pn = partials_needed((long long)M.n);
+  const size_t nparts = pn * 4 + partials_needed((long long)M.nslices * kWave) * 4;
   double* parts = cv.take<double>(nparts);
+  double* parts_u = parts;           // update: (r, r), (r, z) per tile
+  double* parts_v = parts + 2 * pn;  // variable-block preconditioner: (r, z) per tile
+  double* parts_s = parts + 4 * pn;  // SpMV: (p, q) per workgroup
   KState* st = cv.take<KState>(1);
   double* hist = cv.take<double>((size_t)P.max_iter + 2);
+  double* hp = P.keep_hist ? hist : nullptr;
   const T* D = nullptr;
   if constexpr (PREC) D = (sizeof(T) == 8) ? (const T*)M.D64.p : (const T*)M.D32.p;
   const int n = M.n;
   const size_t m = (size_t)n * B;
   const int var = sizeof(T) == 8 ? M.variant64 : M.variant32;
+  int nwg_s = 0, spw = 0;
+  if (M.nslices > 0) sell_geometry(M, var, &nwg_s, &spw);
+  const int ntile_u = rows_grid<T, B>(n);  // partial sets of the update
 
   k_state_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
   MH_HIP(hipGetLastError());
@@ -1148,7 +1267,7 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
   // R = B - A X; B2 = (R, R)   (CG.f90:61-63, mat_converge.f90:38)
   {
     OpResidSolve<T, B> op{x, b, r, nullptr, &st->B2, st};
-    MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
+    MH_TRY((launch_sell<T, B>(M, op, parts_s, var, s)));
   }
   double B2 = 0.0;
   MH_TRY(read_B2(P, st, &B2, s));
@@ -1162,47 +1281,58 @@ int cg_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& W, Sol
     out->resid = 0.0;
     return 0;
   }
-  // first z = M^-1 r, rho_1
-  // variable block sizes: diag_V runs after the update as its own kernel
+  // Three launches per iteration and no finalize kernels: the p step's
+  // prologue runs the previous iteration's test and derives rho and beta from
+  // the update's partials; the update's prologue derives alpha from the
+  // SpMV's partials.  Variable block sizes: diag_V runs after the update as
+  // its own kernel and leaves (r, z) in parts_v.
   const bool vprec = PREC && B == 1 && M.vb != nullptr;
+  const int ntile_v = vprec ? rows_grid<T, 1>(M.vb->nb) : 0;
   auto upd = [&](int it, int flags) -> int {
+    const double* pre = (flags & (F_UPD | F_XONLY)) ? parts_s : nullptr;
     if constexpr (PREC && B == 1) {
       if (vprec) {
-        OpCgUpdate<T, B, false> op{x, r, z, p, q, nullptr, n, st, P.keep_hist ? hist : nullptr, it,
-                                   flags | F_EXTPREC, (T)0};
-        MH_TRY(launch_rows(n, op, parts, s));
+        OpCgUpdate<T, B, false> op{x, r, z, p, q, nullptr, n, st, it, flags | F_EXTPREC, pre, nwg_s, (T)0};
+        MH_TRY((launch_rows<decltype(op), false>(n, op, parts_u, s)));
         if (flags & F_XONLY) return 0;
         const VBlock& V = *M.vb;
         const T* DV = sizeof(T) == 8 ? (const T*)V.D64.p : (const T*)V.D32.p;
-        OpVPrec<T, true> vp{r, z, V.list.p, V.dof_index.p, V.doff.p, DV, st, (flags & F_FIRST) ? 1 : 0};
-        return launch_rows(V.nb, vp, parts, s);
+        OpVPrec<T, true> vp{r, z, V.list.p, V.dof_index.p, V.doff.p, DV, st, 0};
+        return launch_rows<decltype(vp), false>(V.nb, vp, parts_v, s);
       }
     }
-    OpCgUpdate<T, B, PREC> op{x, r, z, p, q, D, n, st, P.keep_hist ? hist : nullptr, it, flags, (T)0};
-    return launch_rows(n, op, parts, s);
+    OpCgUpdate<T, B, PREC> op{x, r, z, p, q, D, n, st, it, flags, pre, nwg_s, (T)0};
+    return launch_rows<decltype(op), false>(n, op, parts_u, s);
   };
-  MH_TRY(upd(0, F_FIRST));
+  auto p_op = [&](int it) {
+    return OpCgP<T>{z, p, st, hp, parts_u, ntile_u, vprec ? parts_v : nullptr, ntile_v, it, (T)0};
+  };
+  MH_TRY(upd(0, 0));  // z = M^-1 r and the partials of the first rho
   Poller poll{c, &st->resid, P.tol};
   int done = 0;
-  for (int it = 1; it <= P.max_iter && !done;) {
+  int it = 1;
+  while (it <= P.max_iter && !done) {
     const int end = std::min(it + poll.next_chunk(it, P.chunk) - 1, P.max_iter);
     for (; it <= end; ++it) {
-      OpCgP<T> pu{z, p, st, (T)0};
-      MH_TRY(launch_rows((int)m, pu, parts, s));
+      MH_TRY(launch_rows((int)m, p_op(it), nullptr, s));
       OpCgSpmv<T, B> sp{p, q, st};
-      MH_TRY((launch_sell<T, B>(M, sp, parts, var, s)));
-      if (it % kIterRR == 0) {
+      MH_TRY((launch_sell<T, B, OpCgSpmv<T, B>, false>(M, sp, parts_s, var, s)));
+      if (it % kIterRR == 0) {  // true residual (CG.f90:91-93)
         MH_TRY(upd(it, F_XONLY));
         OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
-        MH_TRY((launch_sell<T, B>(M, rs, parts, var, s)));
-        MH_TRY(upd(it, F_CHECK));
+        MH_TRY((launch_sell<T, B, OpResidSolve<T, B>, false>(M, rs, parts_s, var, s)));
+        MH_TRY(upd(it, 0));
       } else {
-        MH_TRY(upd(it, F_UPD | F_CHECK));
+        MH_TRY(upd(it, F_UPD));
       }
     }
     MH_TRY(poll.issue(s));
     MH_TRY(poll.check(1, &done));
   }
+  // test of the last enqueued iteration (a no-op once the solve is done)
+  k_prologue_only<row_block<T, 1>()><<<1, row_block<T, 1>(), 0, s>>>(p_op(it));
+  MH_HIP(hipGetLastError());
+  ktimer_mark(kClsDot);
   MH_HIP(hipStreamSynchronize(s));
   KState hs;
   MH_HIP(hipMemcpy(&hs, st, sizeof(KState), hipMemcpyDeviceToHost));
@@ -1721,8 +1851,14 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   T* stv = cv.take<T>(vec);
   T* t = cv.take<T>(vec);
   T* v = cv.take<T>(vec);
-  const size_t nparts = partials_needed((long long)M.n) * 4 + partials_needed((long long)M.nslices * kWave) * 4;
+  const size_t pn = partials_needed((long long)M.n);
+  const size_t pn_s = partials_needed((long long)M.nslices * kWave);
+  const size_t nparts = pn * 4 + pn_s * 4;
   double* parts = cv.take<double>(nparts);
+  double* parts_e = parts;               // e: (r, r), (r, rt) per tile
+  double* parts_b = parts + 4 * pn;      // b: (rt, v) per workgroup
+  double* parts_d = parts_b + pn_s;      // d: (t, s), (t, t) per workgroup
+  double* dsum = parts_d + 2 * pn_s;     // sorted-sum result
   KState* st = cv.take<KState>(1);
   double* hist = cv.take<double>((size_t)P.max_iter + 2);
   const T* D = nullptr;
@@ -1730,6 +1866,9 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   const int n = M.n;
   const size_t m = (size_t)n * B;
   const int var = sizeof(T) == 8 ? M.variant64 : M.variant32;
+  int nwg_s = 0, spw = 0;
+  if (M.nslices > 0) sell_geometry(M, var, &nwg_s, &spw);
+  const int ntile_u = rows_grid<T, B>(n);  // partial sets of the update
 
   k_state_init<<<1, 64, 0, s>>>(st, P.tol, P.max_iter);
   MH_HIP(hipGetLastError());
@@ -1742,7 +1881,7 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   // R = B - A X, RT = R, B2 = (R, R); rho_1 = (R, RT) = B2
   {
     OpResidSolve<T, B> op{x, b, r, rt, &st->B2, st};
-    MH_TRY((launch_sell<T, B>(M, op, parts, var, s)));
+    MH_TRY((launch_sell<T, B>(M, op, parts_b, var, s)));
   }
   double B2 = 0.0;
   MH_TRY(read_B2(P, st, &B2, s));
@@ -1758,6 +1897,7 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
   }
   // rho of iteration 1 is (R, RT) with RT == R: the same sum as B2.
   MH_HIP(hipMemcpyAsync(&st->rho, &st->B2, sizeof(double), hipMemcpyDeviceToDevice, s));
+  MH_HIP(hipMemcpyAsync(&st->rhok[1], &st->B2, sizeof(double), hipMemcpyDeviceToDevice, s));
   // variable block sizes: diag_V runs as its own kernel after steps A and C
   const bool vprec = PREC && B == 1 && M.vb != nullptr;
   auto vapply = [&](const T* src, T* dst) -> int {
@@ -1772,12 +1912,16 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
     return 0;
   };
   double* hp = P.keep_hist ? hist : nullptr;
-  double* dsum = parts;  // sorted-sum result slot (partials are free between launches)
-  // Stage cursor: an iteration is A (p, pt), B (v, (rt, v) -> alpha), C (s, st),
-  // D (t, (t, s), (t, t) -> omega), E (x, r, test, (r, rt) -> beta).  A breakdown
-  // stops the device loop after B, D or E; the host then recomputes that dot with
-  // the sorted sum and resumes from the next stage.
+  auto a_op = [&](int it, bool with_pre) {
+    return OpBiA<T, B, PREC>{r, p, v, pt, D, n, st, hp, with_pre ? parts_e : nullptr, ntile_u, it, (T)0, (T)0};
+  };
+  // Stage cursor: an iteration is A (test of the previous iteration, rho,
+  // beta; p, pt), B (v, (rt, v)), C (alpha; s, st), D (t, (t, s), (t, t)),
+  // E (omega; x, r, (r, r), (r, rt)).  A breakdown stops the device loop in
+  // the prologue of A, C or E; the host then recomputes that dot with the
+  // sorted sum and resumes at that kernel without its prologue.
   int it = 1, stage = 0;
+  int resumed = -1;  // stage whose prologue the next pass skips
   KState hs;
   out->fallbacks = 0;
   for (;;) {
@@ -1785,45 +1929,53 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
     int done = 0;
     while (it <= P.max_iter && !done) {
       const int end = std::min(it + poll.next_chunk(it, P.chunk) - 1, P.max_iter);
-      for (; it <= end; ++it, stage = 0) {
+      for (; it <= end; ++it, stage = 0, resumed = -1) {
         if (stage <= 0) {
+          const bool pre_a = it > 1 && resumed != 0;
           if (vprec) {
-            OpBiA<T, B, false> a{r, p, v, pt, nullptr, n, st, it == 1, (T)0, (T)0};
-            MH_TRY(launch_rows(n, a, parts, s));
+            OpBiA<T, B, false> a{r, p, v, pt, nullptr, n, st, hp, pre_a ? parts_e : nullptr, ntile_u, it, (T)0, (T)0};
+            MH_TRY(launch_rows(n, a, nullptr, s));
             MH_TRY(vapply(p, pt));
           } else {
-            OpBiA<T, B, PREC> a{r, p, v, pt, D, n, st, it == 1, (T)0, (T)0};
-            MH_TRY(launch_rows(n, a, parts, s));
+            MH_TRY(launch_rows(n, a_op(it, pre_a), nullptr, s));
           }
-          OpBiB<T, B> bb{pt, v, rt, st, it};
-          MH_TRY((launch_sell<T, B>(M, bb, parts, var, s)));
+          OpBiB<T, B> bb{pt, v, rt, st};
+          MH_TRY((launch_sell<T, B, OpBiB<T, B>, false>(M, bb, parts_b, var, s)));
         }
         if (stage <= 2) {
+          const double* pre_c = resumed == 2 ? nullptr : parts_b;
           if (vprec) {
-            OpBiC<T, B, false> cc{r, v, sv, stv, nullptr, n, st, (T)0};
-            MH_TRY(launch_rows(n, cc, parts, s));
+            OpBiC<T, B, false> cc{r, v, sv, stv, nullptr, n, st, pre_c, nwg_s, it, (T)0};
+            MH_TRY(launch_rows(n, cc, nullptr, s));
             MH_TRY(vapply(sv, stv));
           } else {
-            OpBiC<T, B, PREC> cc{r, v, sv, stv, D, n, st, (T)0};
-            MH_TRY(launch_rows(n, cc, parts, s));
+            OpBiC<T, B, PREC> cc{r, v, sv, stv, D, n, st, pre_c, nwg_s, it, (T)0};
+            MH_TRY(launch_rows(n, cc, nullptr, s));
           }
-          OpBiD<T, B> dd{stv, t, sv, st, it};
-          MH_TRY((launch_sell<T, B>(M, dd, parts, var, s)));
+          OpBiD<T, B> dd{stv, t, sv, st};
+          MH_TRY((launch_sell<T, B, OpBiD<T, B>, false>(M, dd, parts_d, var, s)));
         }
-        if (it % rr_period == 0) {
-          OpBiE<T, B> e1{x, r, pt, stv, sv, t, rt, st, hp, it, F_XONLY, (T)0, (T)0};
-          MH_TRY(launch_rows(n, e1, parts, s));
+        const double* pre_e = resumed == 4 ? nullptr : parts_d;
+        if (it % rr_period == 0) {  // true residual (BiCGSTAB.f90:163-165)
+          OpBiE<T, B> e1{x, r, pt, stv, sv, t, rt, st, it, F_XONLY, pre_e, nwg_s, (T)0, (T)0};
+          MH_TRY((launch_rows<OpBiE<T, B>, false>(n, e1, parts_e, s)));
           OpResidSolve<T, B> rs{x, b, r, nullptr, nullptr, st};
-          MH_TRY((launch_sell<T, B>(M, rs, parts, var, s)));
-          OpBiE<T, B> e2{x, r, pt, stv, sv, t, rt, st, hp, it, F_CHECK, (T)0, (T)0};
-          MH_TRY(launch_rows(n, e2, parts, s));
+          MH_TRY((launch_sell<T, B, OpResidSolve<T, B>, false>(M, rs, parts_b, var, s)));
+          OpBiE<T, B> e2{x, r, pt, stv, sv, t, rt, st, it, 0, nullptr, 0, (T)0, (T)0};
+          MH_TRY((launch_rows<OpBiE<T, B>, false>(n, e2, parts_e, s)));
         } else {
-          OpBiE<T, B> e{x, r, pt, stv, sv, t, rt, st, hp, it, F_UPD | F_CHECK, (T)0, (T)0};
-          MH_TRY(launch_rows(n, e, parts, s));
+          OpBiE<T, B> e{x, r, pt, stv, sv, t, rt, st, it, F_UPD, pre_e, nwg_s, (T)0, (T)0};
+          MH_TRY((launch_rows<OpBiE<T, B>, false>(n, e, parts_e, s)));
         }
       }
       MH_TRY(poll.issue(s));
       MH_TRY(poll.check(1, &done));
+    }
+    // test of the last enqueued iteration (a no-op once the solve is done)
+    if (it > 1) {
+      k_prologue_only<row_block<T, B>()><<<1, row_block<T, B>(), 0, s>>>(a_op(it, true));
+      MH_HIP(hipGetLastError());
+      ktimer_mark(kClsDot);
     }
     MH_HIP(hipStreamSynchronize(s));
     MH_HIP(hipMemcpy(&hs, st, sizeof(KState), hipMemcpyDeviceToHost));
@@ -1852,8 +2004,10 @@ int bicgstab_impl(Matrix& M, const SolveParams& P, const T* b, T* x, Workspace& 
       hs.rho_prev = hs.rho;
       hs.rho = val;
       it = hs.bd_iter + 1;
+      hs.rhok[it & 1] = val;
       stage = 0;
     }
+    resumed = stage;
     hs.bd_stage = 0;
     hs.done = 0;
     MH_HIP(hipMemcpy(st, &hs, sizeof(KState), hipMemcpyHostToDevice));
@@ -2010,6 +2164,11 @@ int refine_solve(Matrix& M, const RefineParams& P, const double* b, double* x, W
   ip.rhs_nonzero = 1;  // r / ||r|| with ||r|| > 0
   for (int k = 1; k <= P.max_outer; ++k) {
     const double nr = std::sqrt(rr);
+    // A correction only needs to take the residual from `resid` to below tol:
+    // once that is a smaller reduction than inner_tol asks for, the inner
+    // solve stops at half of it (tol / resid / 2) instead of running to
+    // inner_tol.  A correction that falls short is followed by another.
+    ip.tol = std::max(P.inner_tol, 0.5 * P.tol / resid);
     k_scale_to_fp32<<<grid_1d(m), kBlock, 0, s>>>(m, r64, 1.0 / nr, r32);
     MH_HIP(hipGetLastError());
     SolveResult ir;

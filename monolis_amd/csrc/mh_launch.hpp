@@ -4,6 +4,8 @@
 
 #include <cmath>
 #include <deque>
+#include <type_traits>
+#include <vector>
 
 #include "mh_internal.hpp"
 #include "mh_kernels.hpp"
@@ -30,10 +32,88 @@ inline void sell_geometry(const Matrix& M, int variant, int* nwg, int* spw) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Solver time split by kernel class (monolis Rarray slots time_spmv,
+// time_dotp, time_prec; src/define/def_solver_prm.f90:151-163).  While a
+// KTimer is active on the calling thread, every launch helper records one
+// event after its kernel; the interval since the previous event is charged to
+// that kernel's class: SpMV kernels (with their fused dot products) to spmv,
+// row kernels that apply the preconditioner to prec, the other row kernels
+// and reduction kernels (vector updates and dot products) to dotp.  The
+// classes therefore cover the solve's device time without overlap.
+// ---------------------------------------------------------------------------
+enum : int { kClsSpmv = 0, kClsDot = 1, kClsPrec = 2 };
+
+template <class Op, int DEF, class = void>
+struct op_kclass {
+  static constexpr int value = DEF;
+};
+template <class Op, int DEF>
+struct op_kclass<Op, DEF, std::void_t<decltype(Op::KCLASS)>> {
+  static constexpr int value = Op::KCLASS;
+};
+
+struct KTimer {
+  struct Mark {
+    hipEvent_t ev;
+    int cls;
+  };
+  hipStream_t s = nullptr;
+  std::deque<Mark> marks;        // marks.front() is the last harvested event
+  std::vector<hipEvent_t> pool;  // recycled events
+  double ms[3] = {0.0, 0.0, 0.0};
+  int err = 0;
+  explicit KTimer(hipStream_t st) : s(st) { mark(kClsDot); }
+  ~KTimer() {
+    for (auto& m : marks) (void)hipEventDestroy(m.ev);
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+  KTimer(const KTimer&) = delete;
+  KTimer& operator=(const KTimer&) = delete;
+  void mark(int cls) {
+    hipEvent_t e = nullptr;
+    if (!pool.empty()) {
+      e = pool.back();
+      pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      err = 1;
+      return;
+    }
+    if (hipEventRecord(e, s) != hipSuccess) {
+      err = 1;
+      pool.push_back(e);
+      return;
+    }
+    marks.push_back(Mark{e, cls});
+    if (marks.size() > 256) harvest(false);
+  }
+  // Charges every completed interval; with `wait` first waits for the last event.
+  void harvest(bool wait) {
+    if (wait && !marks.empty() && hipEventSynchronize(marks.back().ev) != hipSuccess) err = 1;
+    while (marks.size() >= 2) {
+      if (!wait && hipEventQuery(marks[1].ev) != hipSuccess) break;
+      float t = 0.0f;
+      if (hipEventElapsedTime(&t, marks[0].ev, marks[1].ev) != hipSuccess) err = 1;
+      ms[marks[1].cls] += t;
+      pool.push_back(marks[0].ev);
+      marks.pop_front();
+    }
+  }
+};
+
+// The timer launches on this thread report to (null: none).
+KTimer*& active_ktimer();
+
+inline void ktimer_mark(int cls) {
+  if (KTimer* t = active_ktimer()) t->mark(cls);
+}
+
 // Partials a SELL launch of this matrix may write (per value of NV).
 inline size_t sell_partials(const Matrix& M) { return (size_t)(M.nslices + kWpb - 1) / kWpb + 8; }
 
-template <typename T, int B, class Op>
+// FIN = false leaves the partial sums for the next kernel's prologue (an Op
+// with NPRE consumes them there) instead of launching the finalize kernel.
+template <typename T, int B, class Op, bool FIN = true>
 int launch_sell(const Matrix& M, const Op& op, double* partials, int variant, hipStream_t s) {
   int nwg = 0, spw = 0;
   if (M.nslices > 0) {
@@ -47,24 +127,41 @@ int launch_sell(const Matrix& M, const Op& op, double* partials, int variant, hi
       k_sell<T, T, B, 3, Op><<<padded_grid(nwg), kBlock, 0, s>>>(view<T>(M), op, partials, nwg, spw);
     }
     MH_HIP(hipGetLastError());
+    ktimer_mark(op_kclass<Op, kClsSpmv>::value);
   }
-  if constexpr (Op::NV > 0) {
+  if constexpr (FIN && Op::NV > 0) {
     k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, nwg);
     MH_HIP(hipGetLastError());
+    ktimer_mark(kClsDot);
   }
   return 0;
 }
 
-template <class Op>
+// Logical workgroups of a row-kernel launch over nrows block rows: one per
+// tile of row_block rows, at most 32 wavefronts' worth per CU (the persistent
+// grid of k_rows); also the number of partial sets the launch leaves.
+template <typename VT, int BS>
+int rows_grid(long long nrows) {
+  constexpr int NT = row_block<VT, BS>();
+  const long long cap = (long long)cu_count() * (32 * kWave / NT);
+  const long long t = rows_tiles<VT, BS>(nrows);
+  return (int)(t < cap ? t : cap);
+}
+
+template <class Op, bool FIN = true>
 int launch_rows(int nrows, const Op& op, double* partials, hipStream_t s) {
-  const int ntiles = (nrows + kBlock - 1) / kBlock;
+  constexpr int NT = row_block<typename Op::VT, Op::BS>();
+  const int ntiles = rows_tiles<typename Op::VT, Op::BS>(nrows);
+  const int nwg = rows_grid<typename Op::VT, Op::BS>(nrows);
   if (ntiles > 0) {
-    k_rows<Op><<<padded_grid(ntiles), kBlock, 0, s>>>(nrows, op, partials, ntiles);
+    k_rows<Op><<<padded_grid(nwg), NT, 0, s>>>(nrows, op, partials, nwg, ntiles);
     MH_HIP(hipGetLastError());
+    ktimer_mark(op_kclass<Op, kClsDot>::value);
   }
-  if constexpr (Op::NV > 0) {
-    k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, ntiles);
+  if constexpr (FIN && Op::NV > 0) {
+    k_finalize<Op><<<1, kBlock, 0, s>>>(op, partials, nwg);
     MH_HIP(hipGetLastError());
+    ktimer_mark(kClsDot);
   }
   return 0;
 }

@@ -93,6 +93,29 @@ __device__ __forceinline__ void rhs_partials(int kk, double (&v)[NV], double* __
   }
 }
 
+// Per right-hand side sum over the workgroup of the SpMM's partials when the
+// right-hand sides are processed in NCH chunks of KK: v[ch] of lane
+// (row sub-slot r, j) belongs to right-hand side ch*KK + j.  Fixed order
+// (wavefronts, then row sub-slots); wave 0 writes 64 slots, lane L = right-hand
+// side L (0 past k).  With one chunk this is rhs_partials<1>.
+template <int NCH, int KK>
+__device__ __forceinline__ void rhs_partials_chunked(int k, const double (&v)[NCH], double* __restrict__ partials) {
+  __shared__ double red[NCH][kWpb][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) red[ch][w][lane] = v[ch];
+  __syncthreads();
+  if (w == 0) {
+    const int ch = lane / KK, j = lane % KK;
+    double a = 0.0;
+    if (lane < k)
+      for (int u = 0; u < kWpb; ++u)
+        for (int r = 0; r < kWave / KK; ++r) a += red[ch][u][r * KK + j];
+    partials[(size_t)blockIdx.x * kWave + lane] = a;
+  }
+}
+
 // Broadcast of lane `src` of v to the wavefront (uniform src: a readlane).
 template <bool UNIFORM>
 __device__ __forceinline__ int bcast(int v, int src) {
@@ -116,13 +139,17 @@ __device__ __forceinline__ float bcast(float v, int src) {
 }
 
 // SpMM over units of 8 rows of one SELL slice, lane = (row sub-slot, right-hand
-// side) with KK = 2^ks (>= 8) lanes per row.  For each chunk of 8 slots one
-// vector load brings the unit's 8 x 8 column ids and one the values (lane
-// q*8 + r holds slot q of row r); every row then broadcasts its entries from
-// those lanes and issues all its gathers x(col, :) back to back, so no gather
-// waits on a dependent index load.  Padded slots (col = own row, val = 0) and
-// slots past the slice width (clamped to the last slot) are masked by the
-// row's length with a select.
+// side) with KK = 2^ks (>= 8) lanes per row.  The right-hand sides are taken in
+// chunks of KK columns (one chunk when KK >= k): each workgroup walks its unit
+// range once per chunk, so the rows of X gathered while a chunk is in flight
+// are KK * sizeof(T) bytes wide and the neighbour planes of a chunk fit in the
+// XCD's L2, at the price of reading the matrix once per chunk.  For each chunk
+// of 8 slots one vector load brings the unit's 8 x 8 column ids and one the
+// values (lane q*8 + r holds slot q of row r); every row then broadcasts its
+// entries from those lanes and issues all its gathers x(col, :) back to back,
+// so no gather waits on a dependent index load.  Padded slots (col = own row,
+// val = 0) and slots past the slice width (clamped to the last slot) are
+// masked by the row's length with a select.
 //   MODE 0: y = A x
 //   MODE 1: y = A x and partial (x, y) per right-hand side      (Q = A P, (P, Q))
 //   MODE 2: y = b - A x and partial (y, y) per right-hand side  (R = B - A X, (R, R))
@@ -137,69 +164,122 @@ __global__ void __launch_bounds__(kBlock) k_spmm(int n, int k, int nslices, cons
   constexpr int STEPS = 8 / RPW;               // steps per unit of 8 rows
   constexpr int BS = SB < STEPS ? SB : STEPS;  // steps whose gathers are in flight together
   constexpr bool UNI = RPW == 1;
+  constexpr int NCH = kWave / KK;              // chunks of KK right-hand sides (k <= 64)
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
   const int rsub = lane / KK, j = lane % KK;
-  const bool on = j < k;
-  const int xj = on ? j : 0;
   const int lt = lane >> 3, lr = lane & 7;  // slot / row of the loaded tile entry
-  double part[1] = {0.0};
+  double part[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) part[ch] = 0.0;
   long long ufirst, ustride, uend;
   spmm_units((long long)nslices * 8, w, &ufirst, &ustride, &uend);
-  for (long long u = ufirst; u < uend; u += ustride) {
+  // Unit header: the slice's first slot and width, the row lengths (lanes
+  // 0..7) and the unit's first 8 x 8 tile.  The header of the next unit is
+  // loaded before the current unit's gathers are issued, so its latency
+  // overlaps theirs (vector loads complete in order: waiting for the gathers
+  // never waits for a load issued after them).  Two headers alternate by name,
+  // since copying one would wait for its loads.
+  struct Head {
+    int k0, wd, lenv, cv;
+    T av;
+  };
+  auto head = [&](long long u, Head& h) {
     const int s = (int)(u >> 3);
     const int rbase = (int)(u & 7) * 8;
-    const int row0 = s * kWave + rbase;
-    if (row0 >= n) continue;
-    const int k0 = slice_ptr[s];
-    const int wd = slice_ptr[s + 1] - k0;
-    const int li = row0 + lr;
-    const int lenv = (lane < 8 && li < n) ? rowptr[li + 1] - rowptr[li] : 0;
-    int len[STEPS];
+    h.k0 = slice_ptr[s];
+    h.wd = slice_ptr[s + 1] - h.k0;
+    const int li = s * kWave + rbase + lr;
+    const int lic = li < n ? li : n - 1;
+    const int l1 = rowptr[lic + 1], l0 = rowptr[lic];
+    h.lenv = (lane < 8 && li < n) ? l1 - l0 : 0;
+    const int t = min(lt, h.wd - 1);
+    const size_t slot = h.wd > 0 ? (size_t)(h.k0 + t) * kWave + rbase + lr : 0;
+    h.cv = col[slot];
+    h.av = val[slot];
+  };
 #pragma unroll
-    for (int st = 0; st < STEPS; ++st) len[st] = bcast<UNI>(lenv, st * RPW + rsub);
-    T acc[STEPS];
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int c0 = ch * KK;
+    if (c0 >= k) break;
+    const bool on = c0 + j < k;
+    const int xj = on ? c0 + j : c0;
+    auto unit = [&](long long u, const Head& h) {
+      const int s = (int)(u >> 3);
+      const int rbase = (int)(u & 7) * 8;
+      const int row0 = s * kWave + rbase;
+      if (row0 >= n) return;
+      int len[STEPS];
 #pragma unroll
-    for (int st = 0; st < STEPS; ++st) acc[st] = (T)0;
-    for (int t0 = 0; t0 < wd; t0 += 8) {
-      const int t = min(t0 + lt, wd - 1);
-      const size_t slot = (size_t)(k0 + t) * kWave + rbase + lr;
-      const int cv = col[slot];
-      const T av = val[slot];
+      for (int st = 0; st < STEPS; ++st) len[st] = bcast<UNI>(h.lenv, st * RPW + rsub);
+      T acc[STEPS];
 #pragma unroll
-      for (int sb = 0; sb < STEPS; sb += BS) {
-        T xv[BS][8];
-#pragma unroll
-        for (int st = 0; st < BS; ++st)
-#pragma unroll
-          for (int q = 0; q < 8; ++q)
-            xv[st][q] = x[(size_t)bcast<UNI>(cv, q * 8 + (sb + st) * RPW + rsub) * k + xj];
-#pragma unroll
-        for (int st = 0; st < BS; ++st)
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            const T a = bcast<UNI>(av, q * 8 + (sb + st) * RPW + rsub);
-            acc[sb + st] = (t0 + q < len[sb + st]) ? fma(a, xv[st][q], acc[sb + st]) : acc[sb + st];
-          }
-      }
-    }
-#pragma unroll
-    for (int st = 0; st < STEPS; ++st) {
-      const int i = row0 + st * RPW + rsub;
-      if (i < n && on) {
-        const size_t e = (size_t)i * k + j;
-        if constexpr (MODE == 2) {
-          const T rv = b[e] - acc[st];
-          y[e] = rv;
-          part[0] += (double)rv * (double)rv;
+      for (int st = 0; st < STEPS; ++st) acc[st] = (T)0;
+      for (int t0 = 0; t0 < h.wd; t0 += 8) {
+        int cv;
+        T av;
+        if (t0 == 0) {
+          cv = h.cv;
+          av = h.av;
         } else {
-          y[e] = acc[st];
-          if constexpr (MODE == 1) part[0] += (double)x[e] * (double)acc[st];
+          const int t = min(t0 + lt, h.wd - 1);
+          const size_t slot = (size_t)(h.k0 + t) * kWave + rbase + lr;
+          cv = col[slot];
+          av = val[slot];
+        }
+#pragma unroll
+        for (int sb = 0; sb < STEPS; sb += BS) {
+          T xv[BS][8];
+#pragma unroll
+          for (int st = 0; st < BS; ++st)
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+              xv[st][q] = x[(size_t)bcast<UNI>(cv, q * 8 + (sb + st) * RPW + rsub) * k + xj];
+#pragma unroll
+          for (int st = 0; st < BS; ++st)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              const T a = bcast<UNI>(av, q * 8 + (sb + st) * RPW + rsub);
+              acc[sb + st] = (t0 + q < len[sb + st]) ? fma(a, xv[st][q], acc[sb + st]) : acc[sb + st];
+            }
+          // keep the next batch's gathers below this one's products: SB bounds
+          // the gathers in flight (and the registers they hold)
+          __builtin_amdgcn_sched_barrier(0);
         }
       }
+#pragma unroll
+      for (int st = 0; st < STEPS; ++st) {
+        const int i = row0 + st * RPW + rsub;
+        if (i < n && on) {
+          const size_t e = (size_t)i * k + c0 + j;
+          if constexpr (MODE == 2) {
+            const T rv = b[e] - acc[st];
+            y[e] = rv;
+            part[ch] += (double)rv * (double)rv;
+          } else {
+            y[e] = acc[st];
+            if constexpr (MODE == 1) part[ch] += (double)x[e] * (double)acc[st];
+          }
+        }
+      }
+    };
+    Head ha, hb;
+    long long u = ufirst;
+    if (u < uend) head(u, ha);
+    for (; u < uend; u += 2 * ustride) {
+      const long long u1 = u + ustride, u2 = u + 2 * ustride;
+      if (u1 < uend) head(u1, hb);
+      __builtin_amdgcn_sched_barrier(0);
+      unit(u, ha);
+      __builtin_amdgcn_sched_barrier(0);
+      if (u1 >= uend) break;
+      if (u2 < uend) head(u2, ha);
+      __builtin_amdgcn_sched_barrier(0);
+      unit(u1, hb);
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
-  if constexpr (MODE != 0) rhs_partials<1>(KK, part, partials);
+  if constexpr (MODE != 0) rhs_partials_chunked<NCH, KK>(k, part, partials);
 }
 
 // Elementwise steps over row groups of 64/kk rows, lane = (row, right-hand side).
@@ -483,10 +563,14 @@ unsigned grid_for(size_t m) {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Launch configurations: steps whose gathers are in flight together, workgroups
+// per CU, and the chunk width in right-hand sides (0: all k in one chunk).
 struct SpmmCfg {
-  int sb, wpc;
+  int sb, wpc, kc;
 };
-constexpr SpmmCfg kSpmmCfg[] = {{4, 4}, {8, 4}, {2, 4}, {1, 4}, {4, 8}, {2, 8}, {1, 8}, {8, 2}, {2, 2}};
+constexpr SpmmCfg kSpmmCfg[] = {{4, 4, 0},  {8, 4, 0},  {2, 4, 0},  {1, 4, 0},  {4, 8, 0},  {2, 8, 0},
+                                {1, 8, 0},  {8, 2, 0},  {2, 2, 0},  {2, 4, 16}, {1, 4, 16}, {2, 2, 16},
+                                {2, 8, 16}, {2, 4, 32}, {4, 4, 32}, {2, 2, 32}, {2, 4, 8},  {1, 8, 8}};
 constexpr int kNumSpmmCfg = sizeof(kSpmmCfg) / sizeof(kSpmmCfg[0]);
 
 int spmm_cfg(const Matrix& M, int fp32) {
@@ -514,6 +598,8 @@ template <typename T, int MODE>
 void spmm(const Matrix& M, int k, int ks, int cfg, const T* val, const T* x, const T* b, T* y, double* parts,
           const MState* ms, int grid, hipStream_t s) {
   const int sb = kSpmmCfg[cfg].sb;
+  const int kc = kSpmmCfg[cfg].kc;
+  if (kc > 0) ks = std::min(ks, kc == 8 ? 3 : kc == 16 ? 4 : kc == 32 ? 5 : 6);  // lanes per row = chunk width
   switch (ks <= 3 ? 3 : ks) {
     case 3:
       spmm_kk<T, MODE, 8, 1>(M, k, val, x, b, y, parts, ms, grid, s);
@@ -756,9 +842,10 @@ template int multi_matvec<float>(Matrix&, int, const float*, float*, hipStream_t
 
 int multi_num_configs() { return kNumSpmmCfg; }
 
-void multi_config_info(int cfg, int* steps_per_batch, int* wg_per_cu) {
+void multi_config_info(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk) {
   *steps_per_batch = kSpmmCfg[cfg].sb;
   *wg_per_cu = kSpmmCfg[cfg].wpc;
+  if (chunk) *chunk = kSpmmCfg[cfg].kc;
 }
 
 int multi_autotune(Matrix& M, int k, int fp32, int reps, int* best, double* ms, hipStream_t s) {

@@ -14,6 +14,7 @@ namespace mh {
 // CG.f90:71-84, or with `first` = 2 GropCG's gamma = (r, u).
 template <typename T, bool DOT>
 struct OpVPrec {
+  static constexpr int KCLASS = 2;  // kClsPrec: applies the preconditioner
   using VT = T;
   static constexpr int BS = 1;
   static constexpr int NV = DOT ? 1 : 0;

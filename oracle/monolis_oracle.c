@@ -26,6 +26,10 @@
  *   orc_solve         x = 1 closed forms of the reference's solver tests and
  *                     scipy cg/bicgstab iteration counts.
  *   orc_sorted_dot    brute force: sum of products ordered by |x_i y_i|.
+ *                     The order among products of equal magnitude is parity
+ *                     unpinned: monolis takes it from monolis_qsort_R_2d
+ *                     (monolis_utils, absent); this restatement keeps index
+ *                     order, as the GPU's stable radix sort does.
  */
 #include <math.h>
 #include <stdio.h>
@@ -199,19 +203,32 @@ void orc_vec_axpby(int m, double alpha, const double* X, double beta,
   for (i = 0; i < m; ++i) Z[i] = alpha * X[i] + beta * Y[i];
 }
 
+typedef struct {
+  double v;
+  int i;
+} orc_prod;
+
+/* ascending |v|; equal magnitudes keep index order */
 static int cmp_abs(const void* a, const void* b) {
-  const double x = fabs(((const double*)a)[0]), y = fabs(((const double*)b)[0]);
-  return (x < y) ? -1 : (x > y) ? 1 : 0;
+  const orc_prod* p = (const orc_prod*)a;
+  const orc_prod* q = (const orc_prod*)b;
+  const double x = fabs(p->v), y = fabs(q->v);
+  if (x < y) return -1;
+  if (x > y) return 1;
+  return (p->i > q->i) - (p->i < q->i);
 }
 
 /* Breakdown fallback of BiCGSTAB: products sorted by |x_i y_i| ascending, then
  * summed.  src/linalg/inner_product.f90:262-331 (one rank). */
 double orc_sorted_inner_product(int n, const double* X, const double* Y) {
-  double* xy = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
-  for (int i = 0; i < n; ++i) xy[i] = X[i] * Y[i];
-  qsort(xy, (size_t)n, sizeof(double), cmp_abs);
+  orc_prod* xy = (orc_prod*)malloc(sizeof(orc_prod) * (size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < n; ++i) {
+    xy[i].v = X[i] * Y[i];
+    xy[i].i = i;
+  }
+  qsort(xy, (size_t)n, sizeof(orc_prod), cmp_abs);
   double s = 0.0;
-  for (int i = 0; i < n; ++i) s += xy[i];
+  for (int i = 0; i < n; ++i) s += xy[i].v;
   free(xy);
   return s;
 }

@@ -190,6 +190,35 @@ def test_multi_every_spmm_config(k):
             assert np.linalg.norm(X - ref[0]) / np.linalg.norm(ref[0]) <= 1e-6
 
 
+@pytest.mark.parametrize("k", [7, 20, 64])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_matvec_multi_every_config_bit_identical(k, fp32):
+    """Every SpMM configuration -- including the ones that take the right-hand
+    sides in chunks of 8, 16 or 32 per pass over the matrix, with a ragged last
+    chunk at k = 7 and 20 -- forms each entry with the same multiply-add chain,
+    so all give bit-identical Y; config 0 agrees with the column-wise oracle."""
+    import oracle
+    mat = random_bcsr(411, 1, 0.03, seed=k + 11, spd=False)
+    X = np.random.default_rng(k + 2).uniform(-1, 1, size=(mat.n, k))
+    dt = np.float32 if fp32 else np.float64
+    dm = api.DeviceMatrix(mat)
+    dx = api.DeviceBuffer.from_numpy(X.astype(dt))
+    dy = api.DeviceBuffer(mat.n * k, dt)
+    ncfg = len(dm.autotune_multi(k, fp32=fp32, reps=1)["configs"])
+    ref = None
+    for cfg in range(ncfg):
+        dm.set_multi_config(cfg, fp32=fp32)
+        dm.matvec_multi(dx, dy, k)
+        Y = dy.numpy().reshape(mat.n, k)
+        if ref is None:
+            ref = Y.copy()
+            Yo = np.stack([oracle.matvec(mat, X.astype(dt).astype(np.float64)[:, j]) for j in range(k)], axis=1)
+            tol = 1e-5 if fp32 else 1e-12
+            assert np.max(np.abs(Y - Yo)) <= tol * np.max(np.abs(Yo))
+        else:
+            assert np.array_equal(Y, ref), f"config {cfg} differs"
+
+
 @pytest.mark.parametrize("k", [1, 3, 8, 20, 64])
 @pytest.mark.parametrize("fp32", [False, True])
 def test_matvec_multi_matches_columnwise_oracle(k, fp32):

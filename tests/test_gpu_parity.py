@@ -295,6 +295,33 @@ def test_solve_c_complex_reference_case():
     np.testing.assert_allclose(x, 1 + 1j, rtol=0, atol=1e-8)
 
 
+@pytest.mark.parametrize("method", [1, 3, 5])
+def test_solver_time_split(method):
+    """Rarray time_spmv (5), time_dotp (6), time_prec (7) and the two
+    communication slots (8, 9) (src/define/def_solver_prm.f90:151-163).
+    With is_measurement set the three computation slots are positive and sum
+    to no more than time_sol (3), the one-GPU communication slots are 0, and
+    the solve itself is unchanged (same iterations, bit-identical x); without
+    the timing flags every slot of 5..9 is 0."""
+    m = W.stencil27_bcsr3(16, seed=3)
+    b = oracle.matvec(m, W.uniform_vector(m.nndof, seed=4))
+    x0, mon0 = _solve_flat(m, b, method=method)
+    mon = api.Monolis(m, api.Prm().quiet())
+    api.monolis_set_method(mon, method)
+    api.monolis_set_error_abort(mon, False)
+    api.monolis_set_performance_measurement(mon, True)
+    x = np.zeros(m.npndof)
+    api.monolis_solve_R(mon, b, x)
+    R = mon.prm.Rarray
+    assert api.monolis_get_error_tag(mon) == 0 and api.monolis_get_error_tag(mon0) == 0
+    assert api.monolis_get_converge_iter(mon) == api.monolis_get_converge_iter(mon0)
+    assert np.array_equal(x, x0)
+    assert R[5] > 0 and R[6] > 0 and R[7] > 0, R[3:10]
+    assert R[8] == 0 and R[9] == 0
+    assert R[5] + R[6] + R[7] <= R[3], R[3:10]
+    assert np.all(mon0.prm.Rarray[5:10] == 0), mon0.prm.Rarray[3:10]
+
+
 def test_solve_poisson32_cg_diag_matches_oracle():
     """The CPU-runnable workload: 7-point Poisson 32^3, CG + DIAG, tol 1e-8."""
     m = W.poisson7(32)

@@ -82,6 +82,23 @@ def test_sorted_inner_product_bruteforce():
     assert oracle.sorted_inner_product(x, y) == s
 
 
+def test_sorted_inner_product_ties_keep_index_order():
+    """Products of equal magnitude and opposite sign: the fallback adds them in
+    index order (Python's sort is stable), so the sum depends on that order in
+    the last bit -- the order monolis_qsort_R_2d would give is unpinned."""
+    # 1e-16 first, then +1 and -1: (1e-16 + 1) - 1 = 0 but (1e-16 - 1) + 1 = 2^-53
+    x = np.array([1.0, 0.5, -1.0, 1e-16, -0.5])
+    y = np.ones_like(x)
+    s = 0.0
+    for p in sorted(x, key=abs):                  # stable: ties stay in index order
+        s += p
+    assert oracle.sorted_inner_product(x, y) == s
+    s_rev = 0.0                                   # ties in reverse index order
+    for p in sorted(x[::-1], key=abs):
+        s_rev += p
+    assert s_rev != s                             # the tie order matters for this data
+
+
 # --- block-Jacobi ---------------------------------------------------------------
 @pytest.mark.parametrize("ndof", [1, 2, 3, 4])
 def test_diag_apply_inverts_diagonal_blocks(ndof):
