@@ -144,6 +144,19 @@ int monolis_hip_matvec_R(monolis_hip_matrix* M, const double* x, double* y);
 int monolis_hip_matvec_S(monolis_hip_matrix* M, const float* x, float* y);
 /* Deterministic device dot product; *sum is host memory. */
 int monolis_hip_inner_product_R(int m, const double* x, const double* y, double* sum);
+/* Allocation-free variant of monolis_hip_inner_product_R.  No counterpart in
+ * monolis: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc / hipFree and initialises nothing, so
+ * monolis_hip_init must have run first (it also sets the call up).  x and y
+ * must be device arrays (staging host data would allocate); sum may be host
+ * or device memory.  The reduction's partial sums live in a file-scope device
+ * array of 4096 doubles per GPU (up to 4095 partial sets; a launch needs at
+ * most 2 per CU, so sizes beyond that are rejected with an error only on
+ * GPUs of more than 2047 CUs).  Every call on a GPU shares that array:
+ * calls that overlap on one GPU race.  Same kernels and partial count as
+ * monolis_hip_inner_product_R, so the two agree bit for bit. */
+int monolis_hip_inner_product_R_noalloc(int m, const double* x, const double* y, double* sum);
 
 /* monolis_solve_main_R on device-resident b and x (FP64), Iarray/Rarray host,
  * same slots and semantics as monolis_solve_R_c_main except that failures are

@@ -62,6 +62,7 @@ SIGNATURES = [
     ("monolis_hip_matvec_R", c_int, [c_void_p, c_void_p, c_void_p]),
     ("monolis_hip_matvec_S", c_int, [c_void_p, c_void_p, c_void_p]),
     ("monolis_hip_inner_product_R", c_int, [c_int, c_void_p, c_void_p, c_double_p]),
+    ("monolis_hip_inner_product_R_noalloc", c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     ("monolis_hip_solve_R", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int]),
     ("monolis_hip_solve_refined_R", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double,
                                             c_int, c_void_p]),

@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/monolis_hip.h"
@@ -229,6 +230,43 @@ int solve_device_C(Matrix& A, mh::Workspace& W, const mh::zd* b, mh::zd* x, int*
   return res.converged ? ST_OK : ST_NOT_CONVERGED;
 }
 
+// Device memory of the allocation-free inner product
+// (monolis_hip_inner_product_R_noalloc): the partial sums of the reduction
+// and its result live in this file-scope array instead of memory the library
+// allocates.  Each GPU holds its own copy until the process ends.  In host
+// code the name is a host-side stand-in, never a device address: the address
+// hipGetSymbolAddress returns for a device is looked up once per device at
+// set-up (monolis_hip_init, with that device current) and only that address is
+// handed to kernels and copies.
+constexpr int kScratchSlots = 4096;  // doubles: up to 4095 partial sums + the result
+static __device__ __align__(256) double scratch[kScratchSlots];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[kMaxDevices] = {};
+
+// Looks up this device's address of `scratch` (current device, once).
+int scratch_setup() {
+  int dev = 0;
+  MH_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    mh::set_error("device %d: the allocation-free entry points support devices 0..%d", dev, kMaxDevices - 1);
+    return ST_ARGS;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* p = nullptr;
+    MH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_addr[dev] = static_cast<double*>(p);
+  }
+  return 0;
+}
+
+double* scratch_addr(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_addr[dev];
+}
+
 // Read-bandwidth probe kernel: every element of src is read once per launch,
 // UNR non-temporal loads of type V in flight per thread over a grid-stride
 // loop, summed so that no load can be dropped; each wavefront writes one double.
@@ -265,7 +303,8 @@ const char* monolis_hip_last_error(void) { return mh::last_error(); }
 
 int monolis_hip_init(void) {
   Context* c = nullptr;
-  return mh::context(&c);
+  MH_TRY(mh::context(&c));
+  return scratch_setup();  // the context's device is current here
 }
 
 void* monolis_hip_stream(void) {
@@ -431,6 +470,44 @@ int monolis_hip_inner_product_R(int m, const double* x, const double* y, double*
   }
   MH_TRY(mh::dot<double>(m, dx, dy, parts.p + np, parts.p, c->stream));
   MH_HIP(hipMemcpyAsync(sum, parts.p + np, sizeof(double), hipMemcpyDefault, c->stream));
+  MH_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int monolis_hip_inner_product_R_noalloc(int m, const double* x, const double* y, double* sum) {
+  // no allocation here: the context must already exist (monolis_hip_init),
+  // x and y must be device arrays, and the partials go to `scratch`
+  Context* c = mh::context_if_ready();
+  if (!c) {
+    mh::set_error("monolis_hip_inner_product_R_noalloc: call monolis_hip_init first");
+    return ST_ARGS;
+  }
+  double* sp = scratch_addr(c->device);
+  if (!sp) {
+    mh::set_error("monolis_hip_inner_product_R_noalloc: device %d was not set up by monolis_hip_init", c->device);
+    return ST_ARGS;
+  }
+  if (m < 0 || !sum || (m > 0 && (!x || !y))) {
+    mh::set_error("monolis_hip_inner_product_R_noalloc: invalid arguments (m = %d)", m);
+    return ST_ARGS;
+  }
+  if (m > 0 && (!mh::is_device_pointer(x) || !mh::is_device_pointer(y))) {
+    mh::set_error("monolis_hip_inner_product_R_noalloc: x and y must be device arrays (staging host data allocates)");
+    return ST_ARGS;
+  }
+  const int nwg = mh::rows_grid<double, 1>(m);
+  if (nwg + 1 > kScratchSlots) {
+    mh::set_error("monolis_hip_inner_product_R_noalloc: %d partial sums exceed the %d-slot scratch array", nwg,
+                  kScratchSlots - 1);
+    return ST_ARGS;
+  }
+  // partials in sp[0 .. nwg), every one written by the launch before it is
+  // read, so nothing needs clearing; the result in the last slot unless sum is
+  // itself device memory
+  const bool dev_sum = mh::is_device_pointer(sum) != 0;
+  double* out = dev_sum ? sum : sp + (kScratchSlots - 1);
+  MH_TRY(mh::dot<double>(m, x, y, out, sp, c->stream));
+  if (!dev_sum) MH_HIP(hipMemcpyAsync(sum, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   MH_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -43,6 +43,8 @@ constexpr int kUploadThreads = 8;
 
 // Returns 0 and fills *c on success.  Initialises HIP on first use.
 int context(Context** c);
+// The context if it is already initialised (never initialises: that allocates), else null.
+Context* context_if_ready();
 
 template <typename T>
 struct DevArray {

@@ -54,6 +54,11 @@ int context(Context** out) {
   return 0;
 }
 
+Context* context_if_ready() {
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  return g_ctx.ok ? &g_ctx : nullptr;
+}
+
 template <typename T>
 int DevArray<T>::alloc(size_t count) {
   release();
