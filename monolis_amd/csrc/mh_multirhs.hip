@@ -174,59 +174,32 @@ __global__ void __launch_bounds__(kBlock) k_spmm(int n, int k, int nslices, cons
   for (int ch = 0; ch < NCH; ++ch) part[ch] = 0.0;
   long long ufirst, ustride, uend;
   spmm_units((long long)nslices * 8, w, &ufirst, &ustride, &uend);
-  // Unit header: the slice's first slot and width, the row lengths (lanes
-  // 0..7) and the unit's first 8 x 8 tile.  The header of the next unit is
-  // loaded before the current unit's gathers are issued, so its latency
-  // overlaps theirs (vector loads complete in order: waiting for the gathers
-  // never waits for a load issued after them).  Two headers alternate by name,
-  // since copying one would wait for its loads.
-  struct Head {
-    int k0, wd, lenv, cv;
-    T av;
-  };
-  auto head = [&](long long u, Head& h) {
-    const int s = (int)(u >> 3);
-    const int rbase = (int)(u & 7) * 8;
-    h.k0 = slice_ptr[s];
-    h.wd = slice_ptr[s + 1] - h.k0;
-    const int li = s * kWave + rbase + lr;
-    const int lic = li < n ? li : n - 1;
-    const int l1 = rowptr[lic + 1], l0 = rowptr[lic];
-    h.lenv = (lane < 8 && li < n) ? l1 - l0 : 0;
-    const int t = min(lt, h.wd - 1);
-    const size_t slot = h.wd > 0 ? (size_t)(h.k0 + t) * kWave + rbase + lr : 0;
-    h.cv = col[slot];
-    h.av = val[slot];
-  };
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
     const int c0 = ch * KK;
     if (c0 >= k) break;
     const bool on = c0 + j < k;
     const int xj = on ? c0 + j : c0;
-    auto unit = [&](long long u, const Head& h) {
+    for (long long u = ufirst; u < uend; u += ustride) {
       const int s = (int)(u >> 3);
       const int rbase = (int)(u & 7) * 8;
       const int row0 = s * kWave + rbase;
-      if (row0 >= n) return;
+      if (row0 >= n) continue;
+      const int k0 = slice_ptr[s];
+      const int wd = slice_ptr[s + 1] - k0;
+      const int li = row0 + lr;
+      const int lenv = (lane < 8 && li < n) ? rowptr[li + 1] - rowptr[li] : 0;
       int len[STEPS];
 #pragma unroll
-      for (int st = 0; st < STEPS; ++st) len[st] = bcast<UNI>(h.lenv, st * RPW + rsub);
+      for (int st = 0; st < STEPS; ++st) len[st] = bcast<UNI>(lenv, st * RPW + rsub);
       T acc[STEPS];
 #pragma unroll
       for (int st = 0; st < STEPS; ++st) acc[st] = (T)0;
-      for (int t0 = 0; t0 < h.wd; t0 += 8) {
-        int cv;
-        T av;
-        if (t0 == 0) {
-          cv = h.cv;
-          av = h.av;
-        } else {
-          const int t = min(t0 + lt, h.wd - 1);
-          const size_t slot = (size_t)(h.k0 + t) * kWave + rbase + lr;
-          cv = col[slot];
-          av = val[slot];
-        }
+      for (int t0 = 0; t0 < wd; t0 += 8) {
+        const int t = min(t0 + lt, wd - 1);
+        const size_t slot = (size_t)(k0 + t) * kWave + rbase + lr;
+        const int cv = col[slot];
+        const T av = val[slot];
 #pragma unroll
         for (int sb = 0; sb < STEPS; sb += BS) {
           T xv[BS][8];
@@ -242,9 +215,6 @@ __global__ void __launch_bounds__(kBlock) k_spmm(int n, int k, int nslices, cons
               const T a = bcast<UNI>(av, q * 8 + (sb + st) * RPW + rsub);
               acc[sb + st] = (t0 + q < len[sb + st]) ? fma(a, xv[st][q], acc[sb + st]) : acc[sb + st];
             }
-          // keep the next batch's gathers below this one's products: SB bounds
-          // the gathers in flight (and the registers they hold)
-          __builtin_amdgcn_sched_barrier(0);
         }
       }
 #pragma unroll
@@ -262,21 +232,6 @@ __global__ void __launch_bounds__(kBlock) k_spmm(int n, int k, int nslices, cons
           }
         }
       }
-    };
-    Head ha, hb;
-    long long u = ufirst;
-    if (u < uend) head(u, ha);
-    for (; u < uend; u += 2 * ustride) {
-      const long long u1 = u + ustride, u2 = u + 2 * ustride;
-      if (u1 < uend) head(u1, hb);
-      __builtin_amdgcn_sched_barrier(0);
-      unit(u, ha);
-      __builtin_amdgcn_sched_barrier(0);
-      if (u1 >= uend) break;
-      if (u2 < uend) head(u2, ha);
-      __builtin_amdgcn_sched_barrier(0);
-      unit(u1, hb);
-      __builtin_amdgcn_sched_barrier(0);
     }
   }
   if constexpr (MODE != 0) rhs_partials_chunked<NCH, KK>(k, part, partials);
@@ -565,6 +520,9 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Launch configurations: steps whose gathers are in flight together, workgroups
 // per CU, and the chunk width in right-hand sides (0: all k in one chunk).
+// (A row form -- one wavefront per slice, rows taken R at a time with scalar
+// loads of their column ids and values -- measured 1.19-1.82 ms against
+// 1.12 ms for the best configuration here, FP64 at 128^3 x 64, and was dropped.)
 struct SpmmCfg {
   int sb, wpc, kc;
 };

@@ -295,6 +295,42 @@ def test_solve_c_complex_reference_case():
     np.testing.assert_allclose(x, 1 + 1j, rtol=0, atol=1e-8)
 
 
+@pytest.mark.parametrize("ndof", [1, 3])
+def test_cg_row_without_diagonal_or_padding(ndof):
+    """A row without a diagonal block that fills its slice's width (no padding
+    slot, which would hold the row's own column): row 5 has three off-diagonal
+    blocks in a slice of width 3.  Five CG iterations (max_iter 5, no abort)
+    match the oracle's x; the block-Jacobi factor of that row follows monolis's
+    rule for a missing diagonal block."""
+    from monolis_amd import matrix as mm
+    n = 64
+    rows = []
+    for i in range(n):
+        if i == 5:
+            rows.append([3, 4, 6])
+        else:
+            rows.append(sorted({max(i - 1, 0), i, min(i + 1, n - 1)}))
+    index = np.zeros(n + 1, np.int32)
+    index[1:] = np.cumsum([len(r) for r in rows])
+    item = np.array([c + 1 for r in rows for c in r], np.int32)
+    rng = np.random.default_rng(31)
+    A = rng.uniform(-1, 1, size=len(item) * ndof * ndof)
+    for i, r in enumerate(rows):                       # dominant diagonal blocks where present
+        for k, c in enumerate(r):
+            if c == i:
+                blk = A[(index[i] + k) * ndof * ndof:(index[i] + k + 1) * ndof * ndof].reshape(ndof, ndof)
+                blk += 8.0 * np.eye(ndof)
+    m = mm.Matrix(n, n, ndof, index, item, A).ensure_dof_index()
+    assert max(len(r) for r in rows) == len(rows[5]) == 3 and 5 not in rows[5]
+    b = rng.uniform(-1, 1, m.nndof)
+    x, mon = _solve_flat(m, b, method=1, precond=1, tol=1e-30, max_iter=5)
+    Io, Ro = oracle.default_arrays(method=oracle.ITER_CG, precond=oracle.PREC_DIAG, tol=1e-30, max_iter=5)
+    Io[16] = 0                                           # report non-convergence instead of stopping
+    xo, st, I, R = oracle.solve(m, b, Iarray=Io, Rarray=Ro)
+    assert api.monolis_get_converge_iter(mon) == int(I[4]) == 5
+    assert np.linalg.norm(x - xo) <= 1e-10 * np.linalg.norm(xo)
+
+
 @pytest.mark.parametrize("method", [1, 3, 5])
 def test_solver_time_split(method):
     """Rarray time_spmv (5), time_dotp (6), time_prec (7) and the two
