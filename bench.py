@@ -81,12 +81,20 @@ def main() -> int:
     ap.add_argument("--multi-spmm", action="store_true",
                     help="only time the 64-RHS SpMM of the 128^3 Poisson workload (profiling runs)")
     ap.add_argument("--fp32", action="store_true", help="with --multi-spmm: the FP32 SpMM")
+    ap.add_argument("--spmm-config", type=int, default=-1,
+                    help="with --multi-spmm: force this SpMM launch configuration (default: the untuned one)")
     ap.add_argument("--assembly", action="store_true",
                     help="only run the device-assembly workload (profiling runs)")
     ap.add_argument("--complex-spmv", action="store_true",
                     help="only time the complex SpMV of the headline pattern (profiling runs)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run the streaming-copy probe (1 GiB read + 1 GiB write per launch)")
+    ap.add_argument("--convdiff-only", action="store_true",
+                    help="only the 128^3 convection-diffusion FP64 / refined BiCGSTAB solves (profiling runs)")
+    ap.add_argument("--inner-tol-sweep", action="store_true",
+                    help="refined solves of the convection-diffusion and headline systems over inner tolerances")
+    ap.add_argument("--cg-variant-sweep", action="store_true",
+                    help="time the headline FP64 CG once per SpMV kernel variant (JSON line), then exit")
     ap.add_argument("--trace-summary", metavar="CSV",
                     help="summarise a rocprofv3 kernel_trace.csv per kernel and grid size (no GPU), then exit")
     args = ap.parse_args()
@@ -135,6 +143,8 @@ def main() -> int:
         k = 64
         dt = np.float32 if args.fp32 else np.float64
         dmm = api.DeviceMatrix(pm)
+        if args.spmm_config >= 0:
+            dmm.set_multi_config(args.spmm_config, fp32=args.fp32)
         X = np.random.default_rng(args.seed).uniform(-1, 1, size=(pm.n, k)).astype(dt)
         dxm, dym = api.DeviceBuffer.from_numpy(X), api.DeviceBuffer(pm.n * k, dt)
         for _ in range(args.warmup):
@@ -155,6 +165,10 @@ def main() -> int:
         print(json.dumps({"complex": bench_complex(args, solve=False)}), flush=True)
         return 0
 
+    if args.convdiff_only or args.inner_tol_sweep:
+        print(json.dumps(bench_refine_sweep(args)), flush=True)
+        return 0
+
     if args.copy_only:
         cp = copy_probe(args.steps)
         log(f"stream copy: {cp['ms']:.3f} ms, {cp['gbs']:.0f} GB/s")
@@ -166,6 +180,25 @@ def main() -> int:
     xt = W.uniform_vector(m.nndof, seed=args.seed + 1)
     b = W.rhs_from_solution(m, xt)
     log(f"built {m.meta['workload']}: N_b={m.n} nnzb={m.nz} in {time.time() - t0:.1f}s")
+
+    if args.cg_variant_sweep:
+        dm = api.DeviceMatrix(m)
+        db, xs = api.DeviceBuffer.from_numpy(b), api.DeviceBuffer(m.npndof)
+        sweep = []
+        for v in range(L.monolis_hip_num_variants()):
+            dm.set_variant(v)
+            dm.solve(db, xs, api.Prm().quiet())                   # warm-up
+            ts, it = [], 0
+            for _ in range(3):
+                prm = api.Prm().quiet()
+                t, _ = _timed(lambda: dm.solve(db, xs, prm), reps=1)
+                ts.append(t)
+                it = int(prm.Iarray[4])
+            sweep.append({"variant": v, "cg_ms": round(min(ts) * 1e3, 3), "iters": it,
+                          "ms_per_iter": round(min(ts) * 1e3 / max(it, 1), 4)})
+            log(f"variant {v}: CG {min(ts) * 1e3:.2f} ms, {it} iterations")
+        print(json.dumps({"cg_variant_sweep": sweep}), flush=True)
+        return 0
 
     dm = api.DeviceMatrix(m)
     info = dm.info()
@@ -487,6 +520,51 @@ def bench_precision(m, xt, args, tune):
     }
     log(f"precision/convdiff: FP64 {t64 * 1e3:.1f} ms, refined {tr * 1e3:.1f} ms")
     return res
+
+
+def bench_refine_sweep(args):
+    """FP64 vs FP32-refined solves at several inner tolerances: BiCGSTAB on the 128^3
+    convection-diffusion system and (with --inner-tol-sweep) CG on the headline system.
+    With --convdiff-only: one FP64 and one refined BiCGSTAB solve at --inner-tol (after a
+    warm-up of each), for kernel traces."""
+    from monolis_amd import api, workloads as W
+    out = {}
+    cd = W.convdiff7(128, peclet=0.5)
+    b = W.uniform_vector(cd.nndof, seed=args.seed + 6)
+    dm = api.DeviceMatrix(cd)
+    dm.set_variant(4)
+    dm.set_variant(4, fp32=True)
+    db, x = api.DeviceBuffer.from_numpy(b), api.DeviceBuffer(cd.npndof)
+    tols = [args.inner_tol] if args.convdiff_only else [1e-4, 1e-5, 1e-6, 1e-7]
+    p64 = _prm(method=5, tol=1e-8)
+    t64, _ = _timed(lambda: dm.solve(db, x, p64), reps=2)
+    rows = []
+    for it in tols:
+        pr = _prm(method=5, tol=1e-8)
+        tr, st = _timed(lambda: dm.solve_refined(db, x, pr, inner_tol=it), reps=2)
+        rows.append({"inner_tol": it, "ms": round(tr * 1e3, 3), "outer": st["outer"], "inner": st["inner_total"],
+                     "speedup": round(t64 / tr, 3)})
+        log(f"convdiff refined, inner tol {it:g}: {tr * 1e3:.2f} ms, {st['outer']} outer / {st['inner_total']} inner")
+    out["convdiff_128_bicgstab"] = {"fp64_ms": round(t64 * 1e3, 3), "fp64_iters": int(p64.Iarray[4]), "refined": rows}
+    if args.convdiff_only:
+        return out
+    del dm, db, x
+    m = W.stencil27_bcsr3(args.grid, seed=args.seed)
+    b = W.rhs_from_solution(m, W.uniform_vector(m.nndof, seed=args.seed + 1))
+    dm = api.DeviceMatrix(m)
+    dm.autotune(fp32=True, reps=5)
+    db, x = api.DeviceBuffer.from_numpy(b), api.DeviceBuffer(m.npndof)
+    p64 = _prm(method=1, tol=1e-8)
+    t64, _ = _timed(lambda: dm.solve(db, x, p64), reps=2)
+    rows = []
+    for it in (1e-3, 1e-4, 1e-5, 1e-6):
+        pr = _prm(method=1, tol=1e-8)
+        tr, st = _timed(lambda: dm.solve_refined(db, x, pr, inner_tol=it), reps=2)
+        rows.append({"inner_tol": it, "ms": round(tr * 1e3, 3), "outer": st["outer"], "inner": st["inner_total"],
+                     "speedup": round(t64 / tr, 3)})
+        log(f"headline refined CG, inner tol {it:g}: {tr * 1e3:.2f} ms, {st['outer']} outer / {st['inner_total']} inner")
+    out["headline_cg"] = {"fp64_ms": round(t64 * 1e3, 3), "fp64_iters": int(p64.Iarray[4]), "refined": rows}
+    return out
 
 
 def bench_batching(args):

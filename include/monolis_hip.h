@@ -252,12 +252,13 @@ int monolis_hip_matvec_multi_S(monolis_hip_matrix* M, int nrhs, const float* x, 
 
 /* Autotuner of the multi-RHS SpMM: row batches whose gathers are issued
  * together x workgroups per CU x right-hand sides per pass over the matrix
- * (chunk; 0 = all in one pass).  Times `reps` SpMMs with nrhs columns per
+ * (chunk; 0 = all in one pass) x threads per workgroup x lds (1: the tile's
+ * column ids and values staged through LDS, x read with 32-bit offsets).  Times `reps` SpMMs with nrhs columns per
  * configuration (fp32 = 1: the FP32 copy used by the refined solve's inner CG)
  * and keeps the fastest on the handle; ms (if not NULL) gets
  * monolis_hip_multi_num_configs() averages. */
 int monolis_hip_multi_num_configs(void);
-int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk);
+int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk, int* threads, int* lds);
 int monolis_hip_multi_autotune(monolis_hip_matrix* M, int nrhs, int fp32, int reps, int* best, double* ms);
 /* Forces a multi-RHS SpMM configuration for one precision (fp32 = 0 or 1). */
 int monolis_hip_multi_set_config(monolis_hip_matrix* M, int fp32, int cfg);

@@ -97,7 +97,7 @@ SIGNATURES = [
     ("monolis_hip_matvec_multi_R", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("monolis_hip_matvec_multi_S", c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     ("monolis_hip_multi_num_configs", c_int, []),
-    ("monolis_hip_multi_config", c_int, [c_int, c_int_p, c_int_p, c_int_p]),
+    ("monolis_hip_multi_config", c_int, [c_int, c_int_p, c_int_p, c_int_p, c_int_p, c_int_p]),
     ("monolis_hip_multi_autotune", c_int, [c_void_p, c_int, c_int, c_int, c_int_p, c_double_p]),
     ("monolis_hip_multi_set_config", c_int, [c_void_p, c_int, c_int]),
     ("monolis_hip_solve_multi_refined_R", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,

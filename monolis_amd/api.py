@@ -327,9 +327,9 @@ class DeviceMatrix:
                                                 ctypes.byref(best), ms), "multi_autotune")
         cfgs = []
         for c in range(nc):
-            sb, wpc, kc = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-            L.monolis_hip_multi_config(c, ctypes.byref(sb), ctypes.byref(wpc), ctypes.byref(kc))
-            cfgs.append((sb.value, wpc.value, kc.value))
+            v = [ctypes.c_int() for _ in range(5)]  # steps per batch, wg/CU, chunk, threads, lds
+            L.monolis_hip_multi_config(c, *[ctypes.byref(a) for a in v])
+            cfgs.append(tuple(a.value for a in v))
         return {"best": best.value, "ms": list(ms), "configs": cfgs}
 
     # device-side scalar access (monolis_{set,add,get}_scalar_*_sparse_matrix_R, 0-based)

@@ -1001,12 +1001,12 @@ int monolis_hip_adjoint_R(monolis_hip_matrix* H, const double* x, const double* 
 
 int monolis_hip_multi_num_configs(void) { return mh::multi_num_configs(); }
 
-int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk) {
+int monolis_hip_multi_config(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk, int* threads, int* lds) {
   if (cfg < 0 || cfg >= mh::multi_num_configs()) {
     mh::set_error("multi-RHS config %d out of range", cfg);
     return ST_ARGS;
   }
-  mh::multi_config_info(cfg, steps_per_batch, wg_per_cu, chunk);
+  mh::multi_config_info(cfg, steps_per_batch, wg_per_cu, chunk, threads, lds);
   return 0;
 }
 

@@ -62,12 +62,13 @@ __device__ __forceinline__ void unit_range(long long units, long long* u0, long 
 // workgroup a contiguous sub-range of it, its wavefronts interleaved.  (An
 // order that interleaves an XCD's workgroups unit by unit over one narrow band
 // fetched 8 % fewer bytes from beyond L2 and ran in the same time.)
+template <int NW = kWpb>
 __device__ __forceinline__ void spmm_units(long long units, int w, long long* first, long long* stride,
                                            long long* end) {
   long long u0, u1;
   unit_range(units, &u0, &u1);
   *first = u0 + w;
-  *stride = kWpb;
+  *stride = NW;
   *end = u1;
 }
 
@@ -98,9 +99,9 @@ __device__ __forceinline__ void rhs_partials(int kk, double (&v)[NV], double* __
 // (row sub-slot r, j) belongs to right-hand side ch*KK + j.  Fixed order
 // (wavefronts, then row sub-slots); wave 0 writes 64 slots, lane L = right-hand
 // side L (0 past k).  With one chunk this is rhs_partials<1>.
-template <int NCH, int KK>
+template <int NCH, int KK, int NW = kWpb>
 __device__ __forceinline__ void rhs_partials_chunked(int k, const double (&v)[NCH], double* __restrict__ partials) {
-  __shared__ double red[NCH][kWpb][kWave];
+  __shared__ double red[NCH][NW][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
 #pragma unroll
@@ -110,7 +111,7 @@ __device__ __forceinline__ void rhs_partials_chunked(int k, const double (&v)[NC
     const int ch = lane / KK, j = lane % KK;
     double a = 0.0;
     if (lane < k)
-      for (int u = 0; u < kWpb; ++u)
+      for (int u = 0; u < NW; ++u)
         for (int r = 0; r < kWave / KK; ++r) a += red[ch][u][r * KK + j];
     partials[(size_t)blockIdx.x * kWave + lane] = a;
   }
@@ -153,8 +154,8 @@ __device__ __forceinline__ float bcast(float v, int src) {
 //   MODE 0: y = A x
 //   MODE 1: y = A x and partial (x, y) per right-hand side      (Q = A P, (P, Q))
 //   MODE 2: y = b - A x and partial (y, y) per right-hand side  (R = B - A X, (R, R))
-template <typename T, int MODE, int KK, int SB>
-__global__ void __launch_bounds__(kBlock) k_spmm(int n, int k, int nslices, const int* __restrict__ slice_ptr,
+template <typename T, int MODE, int KK, int SB, int NT = kBlock>
+__global__ void __launch_bounds__(NT) k_spmm(int n, int k, int nslices, const int* __restrict__ slice_ptr,
                                                  const int* __restrict__ rowptr, const int* __restrict__ col,
                                                  const T* __restrict__ val, const T* __restrict__ x,
                                                  const T* __restrict__ b, T* __restrict__ y,
@@ -173,7 +174,7 @@ __global__ void __launch_bounds__(kBlock) k_spmm(int n, int k, int nslices, cons
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) part[ch] = 0.0;
   long long ufirst, ustride, uend;
-  spmm_units((long long)nslices * 8, w, &ufirst, &ustride, &uend);
+  spmm_units<NT / kWave>((long long)nslices * 8, w, &ufirst, &ustride, &uend);
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
     const int c0 = ch * KK;
@@ -234,7 +235,138 @@ __global__ void __launch_bounds__(kBlock) k_spmm(int n, int k, int nslices, cons
       }
     }
   }
-  if constexpr (MODE != 0) rhs_partials_chunked<NCH, KK>(k, part, partials);
+  if constexpr (MODE != 0) rhs_partials_chunked<NCH, KK, NT / kWave>(k, part, partials);
+}
+
+// One (column id, value) entry of a wavefront's 8 x 8 tile in LDS.
+template <typename T>
+struct TileEnt;
+template <>
+struct alignas(16) TileEnt<double> {
+  int c, pad;
+  double v;
+};
+template <>
+struct alignas(8) TileEnt<float> {
+  int c;
+  float v;
+};
+
+// Buffer resource over x (word 3 = 0x00020000: 32-bit data format, gfx9):
+// lanes whose byte offset is at or past `bytes` load 0 without a memory access.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t x_rsrc(const void* x, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(x), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, unsigned off, double) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, unsigned off, float) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+// k_spmm with the tile's column ids and values staged through LDS (two LDS reads
+// per gathered entry instead of three cross-lane shuffles) and x read through a
+// buffer resource with 32-bit byte offsets (x must be under 4 GiB).  Entries past
+// a row's length get an offset past the end of x: they load 0 without touching
+// memory and their FMA is skipped, so every row sums exactly its own entries in
+// slot order, as k_spmm does.
+template <typename T, int MODE, int KK, int SB, int NT>
+__global__ void __launch_bounds__(NT) k_spmm_b(int n, int k, int nslices, const int* __restrict__ slice_ptr,
+                                                   const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                   const T* __restrict__ val, const T* __restrict__ x,
+                                                   const T* __restrict__ b, T* __restrict__ y,
+                                                   double* __restrict__ partials, const MState* ms) {
+  if (ms && ms->all_done) return;
+  constexpr int RPW = kWave / KK;
+  constexpr int STEPS = 8 / RPW;
+  constexpr int BS = SB < STEPS ? SB : STEPS;
+  constexpr int NCH = kWave / KK;
+  constexpr int NW = NT / kWave;
+  __shared__ TileEnt<T> tile[NW][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const int rsub = lane / KK, j = lane % KK;
+  const int lr = lane & 7;
+  const unsigned xbytes = (unsigned)((size_t)n * k * sizeof(T));
+  const __amdgpu_buffer_rsrc_t xr = x_rsrc(x, xbytes);
+  const unsigned rowb = (unsigned)(k * sizeof(T));
+  double part[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) part[ch] = 0.0;
+  long long ufirst, ustride, uend;
+  spmm_units<NW>((long long)nslices * 8, w, &ufirst, &ustride, &uend);
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    const int c0 = ch * KK;
+    if (c0 >= k) break;
+    const bool on = c0 + j < k;
+    const unsigned jb = (unsigned)((on ? c0 + j : c0) * sizeof(T));
+    for (long long u = ufirst; u < uend; u += ustride) {
+      const int s = (int)(u >> 3);
+      const int rbase = (int)(u & 7) * 8;
+      const int row0 = s * kWave + rbase;
+      if (row0 >= n) continue;
+      const int k0 = slice_ptr[s];
+      const int wd = slice_ptr[s + 1] - k0;
+      const int li = row0 + lr;
+      const int lenv = (lane < 8 && li < n) ? rowptr[li + 1] - rowptr[li] : 0;
+      int len[STEPS];
+#pragma unroll
+      for (int st = 0; st < STEPS; ++st) len[st] = __shfl(lenv, st * RPW + rsub, kWave);
+      T acc[STEPS];
+#pragma unroll
+      for (int st = 0; st < STEPS; ++st) acc[st] = (T)0;
+      for (int t0 = 0; t0 < wd; t0 += 8) {
+        {  // stage the 8 slots x 8 rows tile (slot-major, as stored)
+          const int t = min(t0 + (lane >> 3), wd - 1);
+          const size_t slot = (size_t)(k0 + t) * kWave + rbase + lr;
+          TileEnt<T> e;
+          e.c = col[slot];
+          e.v = val[slot];
+          __builtin_amdgcn_wave_barrier();
+          tile[w][lane] = e;
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+#pragma unroll
+        for (int sb = 0; sb < STEPS; sb += BS) {
+          T xv[BS][8];
+#pragma unroll
+          for (int st = 0; st < BS; ++st)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              const int c = tile[w][q * 8 + (sb + st) * RPW + rsub].c;
+              const unsigned off = (t0 + q < len[sb + st]) ? (unsigned)c * rowb + jb : 0xffffffffu;
+              xv[st][q] = buf_ld(xr, off, T());
+            }
+#pragma unroll
+          for (int st = 0; st < BS; ++st)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              const T a = tile[w][q * 8 + (sb + st) * RPW + rsub].v;
+              acc[sb + st] = (t0 + q < len[sb + st]) ? fma(a, xv[st][q], acc[sb + st]) : acc[sb + st];
+            }
+        }
+      }
+#pragma unroll
+      for (int st = 0; st < STEPS; ++st) {
+        const int i = row0 + st * RPW + rsub;
+        if (i < n && on) {
+          const size_t e = (size_t)i * k + c0 + j;
+          if constexpr (MODE == 2) {
+            const T rv = b[e] - acc[st];
+            y[e] = rv;
+            part[ch] += (double)rv * (double)rv;
+          } else {
+            y[e] = acc[st];
+            if constexpr (MODE == 1) part[ch] += (double)x[e] * (double)acc[st];
+          }
+        }
+      }
+    }
+  }
+  if constexpr (MODE != 0) rhs_partials_chunked<NCH, KK, NW>(k, part, partials);
 }
 
 // Elementwise steps over row groups of 64/kk rows, lane = (row, right-hand side).
@@ -519,21 +651,29 @@ unsigned grid_for(size_t m) {
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Launch configurations: steps whose gathers are in flight together, workgroups
-// per CU, and the chunk width in right-hand sides (0: all k in one chunk).
+// per CU, the chunk width in right-hand sides (0: all k in one chunk), the
+// workgroup size and the kernel form (1: k_spmm_b).  Each workgroup walks one
+// contiguous range of rows, so the rows of x an XCD gathers from at a time
+// scale with its workgroups times the chunk width.  FP64 at 128^3 x 64: four
+// 256-thread workgroups per CU fetched x about 4.5 times from beyond L2, one
+// 1024-thread workgroup with 32-column chunks about twice.  Entries 0 and 1
+// are the FP64 and FP32 defaults before autotuning (the fastest there).
 // (A row form -- one wavefront per slice, rows taken R at a time with scalar
 // loads of their column ids and values -- measured 1.19-1.82 ms against
-// 1.12 ms for the best configuration here, FP64 at 128^3 x 64, and was dropped.)
+// 1.12 ms for the best 256-thread configuration, FP64 at 128^3 x 64, and was dropped.)
 struct SpmmCfg {
-  int sb, wpc, kc;
+  int sb, wpc, kc, nt, lds;
 };
-constexpr SpmmCfg kSpmmCfg[] = {{4, 4, 0},  {8, 4, 0},  {2, 4, 0},  {1, 4, 0},  {4, 8, 0},  {2, 8, 0},
-                                {1, 8, 0},  {8, 2, 0},  {2, 2, 0},  {2, 4, 16}, {1, 4, 16}, {2, 2, 16},
-                                {2, 8, 16}, {2, 4, 32}, {4, 4, 32}, {2, 2, 32}, {2, 4, 8},  {1, 8, 8}};
+constexpr SpmmCfg kSpmmCfg[] = {
+    {2, 1, 32, 1024, 1}, {2, 2, 32, 1024, 1}, {4, 4, 32, 256, 1},  {8, 4, 32, 256, 1},
+    {2, 4, 32, 256, 1},  {4, 4, 0, 256, 1},   {4, 8, 0, 256, 1},   {8, 4, 0, 256, 1},
+    {2, 1, 0, 1024, 1},  {4, 4, 0, 256, 0},   {8, 4, 0, 256, 0},   {4, 8, 0, 256, 0},
+    {2, 8, 0, 256, 0},   {2, 4, 32, 256, 0},  {2, 1, 32, 1024, 0}, {1, 1, 32, 1024, 0}};
 constexpr int kNumSpmmCfg = sizeof(kSpmmCfg) / sizeof(kSpmmCfg[0]);
 
 int spmm_cfg(const Matrix& M, int fp32) {
   const int c = fp32 ? M.mrhs_cfg32 : M.mrhs_cfg64;
-  return (c >= 0 && c < kNumSpmmCfg) ? c : 0;
+  return (c >= 0 && c < kNumSpmmCfg) ? c : (fp32 ? 1 : 0);
 }
 
 // Persistent grid of the SpMM for configuration cfg.
@@ -543,40 +683,63 @@ int spmm_grid(const Matrix& M, int cfg) {
   return (int)((g + kXcd - 1) / kXcd * kXcd);
 }
 
-template <typename T, int MODE, int KK, int SB>
+template <typename T, int MODE, int KK, int SB, int NT, bool LDS>
 void spmm_kk(const Matrix& M, int k, const T* val, const T* x, const T* b, T* y, double* parts, const MState* ms,
              int grid, hipStream_t s) {
-  k_spmm<T, MODE, KK, SB><<<grid, kBlock, 0, s>>>(M.n, k, M.nslices, M.slice_ptr.p, M.rowptr.p, M.col.p, val, x, b,
-                                                  y, parts, ms);
+  if constexpr (LDS)
+    k_spmm_b<T, MODE, KK, SB, NT><<<grid, NT, 0, s>>>(M.n, k, M.nslices, M.slice_ptr.p, M.rowptr.p, M.col.p, val, x,
+                                                      b, y, parts, ms);
+  else
+    k_spmm<T, MODE, KK, SB, NT><<<grid, NT, 0, s>>>(M.n, k, M.nslices, M.slice_ptr.p, M.rowptr.p, M.col.p, val, x, b,
+                                                    y, parts, ms);
 }
 
 // SpMM launch: KK = max(8, 2^ks) lanes per row; steps per batch from the
-// configuration, clamped to the steps a unit has (8 rows / rows per step).
+// configuration, clamped to the steps a unit has (8 rows / rows per step) and,
+// for 1024-thread workgroups, to 2 (1 with 64 lanes per row): the VGPR budget
+// of 16 wavefronts per CU.
+template <typename T, int MODE, int NT, bool LDS>
+void spmm_nt(const Matrix& M, int k, int ks, int sb, const T* val, const T* x, const T* b, T* y, double* parts,
+             const MState* ms, int grid, hipStream_t s) {
+  constexpr bool BIG = NT > kBlock;
+  switch (ks <= 3 ? 3 : ks) {
+    case 3:
+      spmm_kk<T, MODE, 8, 1, NT, LDS>(M, k, val, x, b, y, parts, ms, grid, s);
+      break;
+    case 4:
+      if (sb >= 2) spmm_kk<T, MODE, 16, 2, NT, LDS>(M, k, val, x, b, y, parts, ms, grid, s);
+      else spmm_kk<T, MODE, 16, 1, NT, LDS>(M, k, val, x, b, y, parts, ms, grid, s);
+      break;
+    case 5:
+      if constexpr (!BIG)
+        if (sb >= 4) return spmm_kk<T, MODE, 32, 4, NT, LDS>(M, k, val, x, b, y, parts, ms, grid, s);
+      if (sb >= 2) spmm_kk<T, MODE, 32, 2, NT, LDS>(M, k, val, x, b, y, parts, ms, grid, s);
+      else spmm_kk<T, MODE, 32, 1, NT, LDS>(M, k, val, x, b, y, parts, ms, grid, s);
+      break;
+    default:
+      if constexpr (!BIG) {
+        if (sb >= 8) return spmm_kk<T, MODE, 64, 8, NT, LDS>(M, k, val, x, b, y, parts, ms, grid, s);
+        if (sb >= 4) return spmm_kk<T, MODE, 64, 4, NT, LDS>(M, k, val, x, b, y, parts, ms, grid, s);
+        if (sb >= 2) return spmm_kk<T, MODE, 64, 2, NT, LDS>(M, k, val, x, b, y, parts, ms, grid, s);
+      }
+      spmm_kk<T, MODE, 64, 1, NT, LDS>(M, k, val, x, b, y, parts, ms, grid, s);
+      break;
+  }
+}
+
 template <typename T, int MODE>
 void spmm(const Matrix& M, int k, int ks, int cfg, const T* val, const T* x, const T* b, T* y, double* parts,
           const MState* ms, int grid, hipStream_t s) {
-  const int sb = kSpmmCfg[cfg].sb;
-  const int kc = kSpmmCfg[cfg].kc;
-  if (kc > 0) ks = std::min(ks, kc == 8 ? 3 : kc == 16 ? 4 : kc == 32 ? 5 : 6);  // lanes per row = chunk width
-  switch (ks <= 3 ? 3 : ks) {
-    case 3:
-      spmm_kk<T, MODE, 8, 1>(M, k, val, x, b, y, parts, ms, grid, s);
-      break;
-    case 4:
-      if (sb >= 2) spmm_kk<T, MODE, 16, 2>(M, k, val, x, b, y, parts, ms, grid, s);
-      else spmm_kk<T, MODE, 16, 1>(M, k, val, x, b, y, parts, ms, grid, s);
-      break;
-    case 5:
-      if (sb >= 4) spmm_kk<T, MODE, 32, 4>(M, k, val, x, b, y, parts, ms, grid, s);
-      else if (sb == 2) spmm_kk<T, MODE, 32, 2>(M, k, val, x, b, y, parts, ms, grid, s);
-      else spmm_kk<T, MODE, 32, 1>(M, k, val, x, b, y, parts, ms, grid, s);
-      break;
-    default:
-      if (sb >= 8) spmm_kk<T, MODE, 64, 8>(M, k, val, x, b, y, parts, ms, grid, s);
-      else if (sb == 4) spmm_kk<T, MODE, 64, 4>(M, k, val, x, b, y, parts, ms, grid, s);
-      else if (sb == 2) spmm_kk<T, MODE, 64, 2>(M, k, val, x, b, y, parts, ms, grid, s);
-      else spmm_kk<T, MODE, 64, 1>(M, k, val, x, b, y, parts, ms, grid, s);
-      break;
+  const SpmmCfg& C = kSpmmCfg[cfg];
+  if (C.kc > 0) ks = std::min(ks, C.kc == 8 ? 3 : C.kc == 16 ? 4 : C.kc == 32 ? 5 : 6);  // lanes per row = chunk
+  // the LDS form addresses x with 32-bit byte offsets and keeps 0xffffffff out of range
+  const bool lds = C.lds && (size_t)M.n * k * sizeof(T) < 0xfff00000ull;
+  if (C.nt == 1024) {
+    if (lds) spmm_nt<T, MODE, 1024, true>(M, k, ks, C.sb, val, x, b, y, parts, ms, grid, s);
+    else spmm_nt<T, MODE, 1024, false>(M, k, ks, C.sb, val, x, b, y, parts, ms, grid, s);
+  } else {
+    if (lds) spmm_nt<T, MODE, kBlock, true>(M, k, ks, C.sb, val, x, b, y, parts, ms, grid, s);
+    else spmm_nt<T, MODE, kBlock, false>(M, k, ks, C.sb, val, x, b, y, parts, ms, grid, s);
   }
 }
 
@@ -800,10 +963,12 @@ template int multi_matvec<float>(Matrix&, int, const float*, float*, hipStream_t
 
 int multi_num_configs() { return kNumSpmmCfg; }
 
-void multi_config_info(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk) {
+void multi_config_info(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk, int* threads, int* lds) {
   *steps_per_batch = kSpmmCfg[cfg].sb;
   *wg_per_cu = kSpmmCfg[cfg].wpc;
   if (chunk) *chunk = kSpmmCfg[cfg].kc;
+  if (threads) *threads = kSpmmCfg[cfg].nt;
+  if (lds) *lds = kSpmmCfg[cfg].lds;
 }
 
 int multi_autotune(Matrix& M, int k, int fp32, int reps, int* best, double* ms, hipStream_t s) {

@@ -11,7 +11,8 @@ constexpr int kMaxRhs = 64;
 // SpMM launch configurations: steps (row batches) whose gathers are issued
 // together x workgroups per CU of the persistent grid.
 int multi_num_configs();
-void multi_config_info(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk = nullptr);
+void multi_config_info(int cfg, int* steps_per_batch, int* wg_per_cu, int* chunk = nullptr,
+                       int* threads = nullptr, int* lds = nullptr);
 // Times `reps` SpMM launches (Q = A P with the (P, Q) partials, as inside CG)
 // per configuration for k right-hand sides and keeps the fastest on M for that
 // precision.  ms (optional) receives multi_num_configs() averages.
