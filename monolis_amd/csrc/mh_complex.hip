@@ -247,6 +247,7 @@ struct OpZUpdate {
   static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = zd;
   static constexpr int BS = B;
+  static constexpr int MIN_WAVES = 8;  // two 1024-thread workgroups per CU
   static constexpr int NV = 4;
   zd* __restrict__ x;
   zd* __restrict__ r;

@@ -308,6 +308,42 @@ struct WaveRows {
       wave_sync();
     }
   }
+  // The wavefront's 64*B entries of src in coalesced order (lane takes entries
+  // j*64 + lane).  A row kernel fetches every vector it reads this way before
+  // its first store, so all of its loads are in flight together; elementwise
+  // updates work in this order directly (put_flat), block operations take
+  // transpose() to row order.  Entries past the last row re-read the last entry
+  // (no branch around the load, which would make the compiler wait for the
+  // loads before it): they are never stored and masked out of every sum.
+  __device__ __forceinline__ void fetch(const T* __restrict__ src, T (&c)[B]) const {
+    const int last = count() - 1;
+    const T* base = src + (size_t)row0 * B;
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int e = j * kWave + lane;
+      c[j] = ld_val<false>(base + (e < last ? e : last));
+    }
+  }
+  __device__ __forceinline__ void transpose(const T (&c)[B], T (&v)[B]) const {
+    if constexpr (B == 1) {
+      v[0] = c[0];
+    } else {
+#pragma unroll
+      for (int j = 0; j < B; ++j) lds[j * kWave + lane] = c[j];
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < B; ++k) v[k] = lds[lane * B + k];
+      wave_sync();
+    }
+  }
+  __device__ __forceinline__ void put_flat(T* __restrict__ dst, const T (&c)[B]) const {
+    const int cnt = count();
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const int e = j * kWave + lane;
+      if (e < cnt) put(dst, (size_t)row0 * B + e, c[j]);
+    }
+  }
   // Outputs are written non-temporally: they are consumed by a later kernel,
   // and streaming them out during the kernel avoids leaving megabytes of dirty
   // L2 lines for the end-of-kernel write-back.
@@ -366,6 +402,51 @@ __device__ __forceinline__ void diag_apply(const TD* __restrict__ D, int nrows, 
     }
   }
 }
+
+// Block-Jacobi apply with the block's B*B entries loaded up front (so their
+// loads can be issued with a row kernel's other loads), for B <= 4; larger
+// blocks stream them in diag_apply.  Same arithmetic order as diag_apply.
+template <typename T, typename TD, int B, bool ON>
+struct DiagRegs {
+  static constexpr bool PRE = ON && B <= 4;
+  TD d[PRE ? B * B : 1];
+  __device__ __forceinline__ void load(const TD* __restrict__ D, int nrows, int i) {
+    if constexpr (PRE) {
+#pragma unroll
+      for (int e = 0; e < B * B; ++e) d[e] = D[(size_t)e * nrows + i];
+    }
+  }
+  __device__ __forceinline__ void apply(const TD* __restrict__ D, int nrows, int i, T (&x)[B]) const {
+    if constexpr (!ON) {
+      return;
+    } else if constexpr (!PRE) {
+      diag_apply<T, TD, B>(D, nrows, i, x);
+    } else if constexpr (B == 1) {
+      x[0] = (T)d[0] * x[0];
+    } else if constexpr (B == 3) {
+      T x1 = x[0], x2 = x[1], x3 = x[2];
+      x2 = x2 - (T)d[3] * x1;
+      x3 = x3 - (T)d[6] * x1 - (T)d[7] * x2;
+      x3 = (T)d[8] * x3;
+      x2 = (T)d[4] * (x2 - (T)d[5] * x3);
+      x1 = (T)d[0] * (x1 - (T)d[2] * x3 - (T)d[1] * x2);
+      x[0] = x1;
+      x[1] = x2;
+      x[2] = x3;
+    } else {
+#pragma unroll
+      for (int j = 1; j < B; ++j)
+#pragma unroll
+        for (int k = 0; k < j; ++k) x[j] = x[j] - (T)d[B * j + k] * x[k];
+#pragma unroll
+      for (int j = B - 1; j >= 0; --j) {
+#pragma unroll
+        for (int k = B - 1; k > j; --k) x[j] = x[j] - (T)d[B * j + k] * x[k];
+        x[j] = (T)d[(B + 1) * j] * x[j];
+      }
+    }
+  }
+};
 
 // Deferred reduction finalize.  An Op may take the partial sums that the
 // previous kernel on the stream stored, instead of a one-workgroup finalize
@@ -503,9 +584,23 @@ __host__ __device__ constexpr int op_row_block() {
   return row_block<typename Op::VT, Op::BS>();
 }
 
+// An Op may ask for `static constexpr int MIN_WAVES` wavefronts per SIMD: 8 (at
+// most 64 VGPRs) keeps two 1024-thread workgroups on a CU where a row body
+// with all of its loads in flight first would otherwise take a few more VGPRs
+// and halve the resident waves.
+template <class Op, class = void>
+struct op_min_waves {
+  static constexpr int value = 1;
+};
 template <class Op>
-__global__ void __launch_bounds__(op_row_block<Op>()) k_rows(int nrows, Op op0, double* partials, int nwg,
-                                                             int ntiles) {
+struct op_min_waves<Op, std::void_t<decltype(Op::MIN_WAVES)>> {
+  static constexpr int value = Op::MIN_WAVES;
+};
+
+template <class Op>
+__global__ void __launch_bounds__(op_row_block<Op>())
+    __attribute__((amdgpu_waves_per_eu(op_row_block<Op>() == 1024 ? op_min_waves<Op>::value : 1, 8)))
+    k_rows(int nrows, Op op0, double* partials, int nwg, int ntiles) {
   using VT = typename Op::VT;
   constexpr int BS = Op::BS;
   constexpr int NT = row_block<VT, BS>();

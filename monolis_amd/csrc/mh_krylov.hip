@@ -127,6 +127,7 @@ template <typename T>
 struct OpCgP {
   using VT = T;
   static constexpr int BS = 1;
+  static constexpr int MIN_WAVES = 8;  // two 1024-thread workgroups per CU
   static constexpr int NV = 0;
   static constexpr int NPRE = 2;
   const T* __restrict__ z;
@@ -193,6 +194,7 @@ struct OpCgUpdate {
   static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
+  static constexpr int MIN_WAVES = B <= 3 ? 8 : 1;  // two 1024-thread workgroups per CU
   static constexpr int NV = 2;
   static constexpr int NPRE = 1;
   T* __restrict__ x;
@@ -221,26 +223,44 @@ struct OpCgUpdate {
     }
     return true;
   }
+  // The loads of x, p, r and q are issued before the first store; the
+  // elementwise updates run in the coalesced order of the loads.
+  // The usual flags (F_UPD) get a copy of the body with them as constants, so
+  // its loads are straight-line code.
   __device__ void row(int i, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
-    if (flags & (F_UPD | F_XONLY)) {
-      T xv[B], pv[B];
-      io.load(x, xv);
-      io.load(p, pv);
-#pragma unroll
-      for (int k = 0; k < B; ++k) xv[k] = fma(alpha, pv[k], xv[k]);  // X = alpha*P + X  (CG.f90:89)
-      io.store(x, xv);
+    if (flags == F_UPD) row_f<F_UPD>(i, valid, part, io);
+    else row_f<-1>(i, valid, part, io);
+  }
+  template <int FL>
+  __device__ __forceinline__ void row_f(int i, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
+    const int f = FL >= 0 ? FL : flags;
+    const bool upd = f & F_UPD, xupd = f & (F_UPD | F_XONLY), xonly = f & F_XONLY;
+    const bool prec = !xonly && !(f & F_EXTPREC);
+    T xc[B], pc[B], rc[B], qc[B];
+    if (xupd) {
+      io.fetch(x, xc);
+      io.fetch(p, pc);
     }
-    if (flags & F_XONLY) return;
+    if (!xonly) io.fetch(r, rc);
+    if (upd) io.fetch(q, qc);
+    if (xupd) {
+#pragma unroll
+      for (int j = 0; j < B; ++j) xc[j] = fma(alpha, pc[j], xc[j]);  // X = alpha*P + X  (CG.f90:89)
+      io.put_flat(x, xc);
+    }
+    // the diagonal block's loads follow the x store, in the registers x and p
+    // leave (loading it with the vectors would cost two workgroups per CU)
+    DiagRegs<T, T, B, PREC> dg;
+    if (prec && valid) dg.load(D, n, i);
+    if (xonly) return;
+    if (upd) {
+#pragma unroll
+      for (int j = 0; j < B; ++j) rc[j] = fma(-alpha, qc[j], rc[j]);  // R = -alpha*Q + R  (CG.f90:94)
+      io.put_flat(r, rc);
+    }
     T v[B];
-    io.load(r, v);
-    if (flags & F_UPD) {
-      T qv[B];
-      io.load(q, qv);
-#pragma unroll
-      for (int k = 0; k < B; ++k) v[k] = fma(-alpha, qv[k], v[k]);  // R = -alpha*Q + R  (CG.f90:94)
-      io.store(r, v);
-    }
-    if (flags & F_EXTPREC) {
+    io.transpose(rc, v);
+    if (!prec) {
       if (valid) {
 #pragma unroll
         for (int k = 0; k < B; ++k) part[0] += (double)v[k] * (double)v[k];
@@ -251,7 +271,7 @@ struct OpCgUpdate {
 #pragma unroll
     for (int k = 0; k < B; ++k) w[k] = v[k];
     if (valid) {
-      if constexpr (PREC) diag_apply<T, T, B>(D, n, i, w);
+      dg.apply(D, n, i, w);
 #pragma unroll
       for (int k = 0; k < B; ++k) {
         part[0] += (double)v[k] * (double)v[k];
@@ -276,6 +296,7 @@ struct OpBiA {
   static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
+  static constexpr int MIN_WAVES = B <= 3 ? 8 : 1;  // two 1024-thread workgroups per CU
   static constexpr int NV = 0;
   static constexpr int NPRE = 2;
   const T* __restrict__ r;
@@ -320,21 +341,33 @@ struct OpBiA {
     }
     return true;
   }
-  __device__ void row(int i, bool valid, double (&)[1], const WaveRows<T, B>& io) const {
-    T pv[B];
-    io.load(r, pv);
-    if (it > 1) {
-      T po[B], vv[B];
-      io.load(p, po);
-      io.load(v, vv);
+  __device__ void row(int i, bool valid, double (&part)[1], const WaveRows<T, B>& io) const {
+    if (it > 1) row_f<true>(i, valid, part, io);
+    else row_f<false>(i, valid, part, io);
+  }
+  template <bool REC>
+  __device__ __forceinline__ void row_f(int i, bool valid, double (&)[1], const WaveRows<T, B>& io) const {
+    T pc[B], po[B], vc[B];
+    io.fetch(r, pc);
+    if (REC) {
+      io.fetch(p, po);
+      io.fetch(v, vc);
+    }
+    DiagRegs<T, T, B, PREC> dg;
+    if (valid) dg.load(D, n, i);
+    if (REC) {
 #pragma unroll
-      for (int k = 0; k < B; ++k) pv[k] = pv[k] + beta * (po[k] - omega * vv[k]);  // BiCGSTAB.f90:103
+      for (int j = 0; j < B; ++j) pc[j] = pc[j] + beta * (po[j] - omega * vc[j]);  // BiCGSTAB.f90:103
     }
-    io.store(p, pv);
+    io.put_flat(p, pc);
     if constexpr (PREC) {
-      if (valid) diag_apply<T, T, B>(D, n, i, pv);
+      T pv[B];
+      io.transpose(pc, pv);
+      if (valid) dg.apply(D, n, i, pv);
+      io.store(pt, pv);
+    } else {
+      io.put_flat(pt, pc);
     }
-    io.store(pt, pv);
   }
   __device__ void finalize(const double (&)[1]) const {}
 };
@@ -371,6 +404,7 @@ struct OpBiC {
   static constexpr int KCLASS = kClsPrec;  // applies the preconditioner
   using VT = T;
   static constexpr int BS = B;
+  static constexpr int MIN_WAVES = B <= 3 ? 8 : 1;  // two 1024-thread workgroups per CU
   static constexpr int NV = 0;
   static constexpr int NPRE = 1;
   const T* __restrict__ r;
@@ -409,16 +443,22 @@ struct OpBiC {
     return true;
   }
   __device__ void row(int i, bool valid, double (&)[1], const WaveRows<T, B>& io) const {
-    T rv[B], vv[B];
-    io.load(r, rv);
-    io.load(v, vv);
+    T rc[B], vc[B];
+    io.fetch(r, rc);
+    io.fetch(v, vc);
+    DiagRegs<T, T, B, PREC> dg;
+    if (valid) dg.load(D, n, i);
 #pragma unroll
-    for (int k = 0; k < B; ++k) rv[k] = fma(-alpha, vv[k], rv[k]);  // S = -alpha*V + R  (BiCGSTAB.f90:118)
-    io.store(s, rv);
+    for (int j = 0; j < B; ++j) rc[j] = fma(-alpha, vc[j], rc[j]);  // S = -alpha*V + R  (BiCGSTAB.f90:118)
+    io.put_flat(s, rc);
     if constexpr (PREC) {
-      if (valid) diag_apply<T, T, B>(D, n, i, rv);
+      T rv[B];
+      io.transpose(rc, rv);
+      if (valid) dg.apply(D, n, i, rv);
+      io.store(st_, rv);
+    } else {
+      io.put_flat(st_, rc);
     }
-    io.store(st_, rv);
   }
   __device__ void finalize(const double (&)[1]) const {}
 };
@@ -458,6 +498,7 @@ template <typename T, int B>
 struct OpBiE {
   using VT = T;
   static constexpr int BS = B;
+  static constexpr int MIN_WAVES = B <= 3 ? 8 : 1;  // two 1024-thread workgroups per CU
   static constexpr int NV = 2;
   static constexpr int NPRE = 2;
   T* __restrict__ x;
@@ -501,30 +542,44 @@ struct OpBiE {
     }
     return true;
   }
-  __device__ void row(int, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
-    if (flags & (F_UPD | F_XONLY)) {
-      T xv[B], pv[B], sv[B];
-      io.load(x, xv);
-      io.load(pt, pv);
-      io.load(stv, sv);
-#pragma unroll
-      for (int k = 0; k < B; ++k) xv[k] = xv[k] + alpha * pv[k] + omega * sv[k];  // BiCGSTAB.f90:153
-      io.store(x, xv);
+  // Every load is issued before the first store; the elementwise updates run in
+  // the coalesced order of the loads, the dots in row order.  The usual flags
+  // (F_UPD) get a copy of the body with them as constants.
+  __device__ void row(int i, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
+    if (flags == F_UPD) row_f<F_UPD>(i, valid, part, io);
+    else row_f<-1>(i, valid, part, io);
+  }
+  template <int FL>
+  __device__ __forceinline__ void row_f(int, bool valid, double (&part)[2], const WaveRows<T, B>& io) const {
+    const int f = FL >= 0 ? FL : flags;
+    const bool upd = f & F_UPD, xupd = f & (F_UPD | F_XONLY), xonly = f & F_XONLY;
+    T xc[B], pc[B], sc[B], rc[B], tc[B], rtc[B];
+    if (xupd) {
+      io.fetch(x, xc);
+      io.fetch(pt, pc);
+      io.fetch(stv, sc);
     }
-    if (flags & F_XONLY) return;
-    T rv[B];
-    if (flags & F_UPD) {
-      T sv[B], tv[B];
-      io.load(s, sv);
-      io.load(t, tv);
-#pragma unroll
-      for (int k = 0; k < B; ++k) rv[k] = fma(-omega, tv[k], sv[k]);  // R = -omega*T + S  (BiCGSTAB.f90:166)
-      io.store(r, rv);
-    } else {
-      io.load(r, rv);
+    if (upd) {
+      io.fetch(s, rc);
+      io.fetch(t, tc);
+    } else if (!xonly) {
+      io.fetch(r, rc);
     }
-    T rtv[B];
-    io.load(rt, rtv);
+    if (!xonly) io.fetch(rt, rtc);
+    if (xupd) {
+#pragma unroll
+      for (int j = 0; j < B; ++j) xc[j] = xc[j] + alpha * pc[j] + omega * sc[j];  // BiCGSTAB.f90:153
+      io.put_flat(x, xc);
+    }
+    if (xonly) return;
+    if (upd) {
+#pragma unroll
+      for (int j = 0; j < B; ++j) rc[j] = fma(-omega, tc[j], rc[j]);  // R = -omega*T + S  (BiCGSTAB.f90:166)
+      io.put_flat(r, rc);
+    }
+    T rv[B], rtv[B];
+    io.transpose(rc, rv);
+    io.transpose(rtc, rtv);
     if (valid) {
 #pragma unroll
       for (int k = 0; k < B; ++k) {
