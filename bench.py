@@ -614,6 +614,15 @@ def bench_batching(args):
         "spmm_ms": round(best_ms, 4), "spmm_algorithmic_gbs": round(spmm_bytes / (best_ms * 1e-3) / 1e9, 1),
         "spmm_config_fp64": tune64["configs"][tune64["best"]], "spmm_config_fp32": tune32["configs"][tune32["best"]],
     }
+    # FP64 iteration: p = z + beta p (3 N k + N), SpMM q = A p with the (p, q) re-read
+    # (nnz*12 + (N+1)*4 + 3 N k), x / r / z pass (6 N k + N), 8-byte values
+    it_bytes = 12 * pm.n * k * 8 + 2 * pm.n * 8 + pm.nz * 12 + (pm.n + 1) * 4
+    it_max = int(r64["iters"].max())
+    res["multi_rhs_poisson_128"]["roofline"] = {
+        "bound": "hbm", "achieved": round(it_bytes * it_max / t64 / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+        "frac": round(it_bytes * it_max / t64 / 1e9 / 8000.0, 4), "iteration_bytes": it_bytes,
+        "ms_per_iteration": round(t64 * 1e3 / it_max, 4),
+        "bytes_model": "per FP64 iteration (launches run to the slowest column): 12*N*k*8 + 2*N*8 + nnz*12 + (N+1)*4"}
     log(f"batching/multi-RHS: FP64 {t64 * 1e3:.0f} ms, refined {tr * 1e3:.0f} ms, 64 single solves "
         f"{tseq * 1e3:.0f} ms, SpMM {best_ms:.3f} ms")
     del dm, db, dx, pm, B
@@ -643,8 +652,19 @@ def bench_batching(args):
         sdm.solve(sb, sx, ps)
         tsmp += time.perf_counter() - t
     est_seq = tsmp * 512 / len(sample)
+    # algorithmic bytes: every system moves, per iteration, 12 B per stored entry and per row 4 B of
+    # row length, 24 B for the p update, 16 B of SpMV vectors and 64 B for the fused x/r/z pass
+    rows_s = np.diff(np.asarray(sys_row, np.int64))
+    nnz_s = np.asarray(bm.index, np.int64)[np.asarray(sys_row[1:], np.int64)] - \
+        np.asarray(bm.index, np.int64)[np.asarray(sys_row[:-1], np.int64)]
+    it_s = np.asarray(iters, np.int64)
+    batch_bytes = int(np.sum(it_s * (nnz_s * 12 + rows_s * (4 + 24 + 16 + 64))))
     res["batch_512_poisson"] = {
         "workload": f"512 independent 7-point Poisson systems, edges U{{10..40}}, {bm.n} DOF total, DIAG CG to 1e-8",
+        "roofline": {"bound": "hbm", "achieved": round(batch_bytes / (ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(batch_bytes / (ms * 1e-3) / 1e9 / 8000.0, 4),
+                     "bytes": batch_bytes,
+                     "bytes_model": "sum over systems of iterations x (12 B per entry + 108 B per row)"},
         "ms": round(ms, 3), "all_converged": bool(np.all(status == 0)),
         "iters_min": int(iters.min()), "iters_max": int(iters.max()),
         "config": tune["configs"][tune["best"]], "autotune_ms": [round(v, 3) for v in tune["ms"]],
