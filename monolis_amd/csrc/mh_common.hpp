@@ -122,7 +122,9 @@ __device__ __forceinline__ void store_partials(const double (&v)[NV], double* pa
 }
 
 // Fixed-order sum of ntiles x NV partials by one workgroup of NT threads;
-// valid in every thread.  A pass covers NT * PER tiles (4096 for NV <= 2) and
+// valid in every thread.  A pass covers NT * PER tiles (4096 for NV <= 2 with
+// 256 threads, 8192 with 1024: the 8192 slices' partials of a scalar SpMV on
+// 2 M rows in one pass) and
 // issues all of its loads before the first add (out-of-range slots read the
 // last tile and add 0), so a sum of a few thousand tiles costs about one
 // memory latency: kernels that take the previous kernel's partials in their
@@ -130,7 +132,7 @@ __device__ __forceinline__ void store_partials(const double (&v)[NV], double* pa
 // derivation of one scalar uses the same NT.
 template <int NV, int NT = kBlock>
 __device__ __forceinline__ void sum_partials(const double* partials, int ntiles, double (&total)[NV]) {
-  constexpr int PER = NV <= 2 ? 16 * kBlock / NT : 8 * kBlock / NT;
+  constexpr int PER = NV <= 2 ? (NT > kBlock ? 8 : 16) : (NT > kBlock ? 4 : 8);
   double acc[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) acc[k] = 0.0;

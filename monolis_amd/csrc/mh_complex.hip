@@ -107,7 +107,8 @@ __device__ __forceinline__ bool zfinalize_check(ZState* st, double* hist, int it
 // geometry of real-path variant 0: contiguous slice ranges at 8 waves per CU.
 // Measured on the 100^3 BCSR3 pattern (26.5 M complex 3x3 blocks) it is the
 // fastest of unroll 1..3, the pipelined loop and wpc 0..24 (664-689 us,
-// 73-76 % of HBM peak; unroll 2 705-735 us; the pipelined loop 1.2-2.1 ms).
+// 73-76 % of HBM peak; unroll 2 705-735 us; the pipelined loop 1.2-2.1 ms;
+// loading the next slot's column ids during the current slot 698 vs 676 us).
 constexpr int kZSellGeom = 0;
 template <int B>
 struct ZUnroll {
