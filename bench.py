@@ -274,6 +274,18 @@ def main() -> int:
 
     # --- CG FP64 vs FP32-refined on the same system -------------------------------
     if not args.no_solve:
+        # the solvers run the SpMV variant that is fastest inside a block-Jacobi CG iteration
+        # (timed over whole iterations), which need not be the back-to-back winner above
+        stune = {"fp64": dm.autotune_solver(fp32=False, iters=20, reps=2),
+                 "fp32": dm.autotune_solver(fp32=True, iters=20, reps=2)}
+        log(f"solver-context autotune: FP64 variant {stune['fp64']['best']} "
+            f"({min(stune['fp64']['ms_per_iter']):.4f} ms/iteration), FP32 variant {stune['fp32']['best']} "
+            f"({min(stune['fp32']['ms_per_iter']):.4f} ms/iteration)")
+        tune["solver64"], tune["solver32"] = stune["fp64"]["best"], stune["fp32"]["best"]
+        out["autotune"]["solver_variant_fp64"] = stune["fp64"]["best"]
+        out["autotune"]["solver_variant_fp32"] = stune["fp32"]["best"]
+        out["autotune"]["solver_ms_per_iter_fp64"] = [round(v, 4) for v in stune["fp64"]["ms_per_iter"]]
+        out["autotune"]["solver_ms_per_iter_fp32"] = [round(v, 4) for v in stune["fp32"]["ms_per_iter"]]
         db = api.DeviceBuffer.from_numpy(b)
         x64 = api.DeviceBuffer(m.npndof)
         prm = api.Prm().quiet()
@@ -467,8 +479,8 @@ def bench_precision(m, xt, args, tune):
     b = W.rhs_from_solution(ms_, xt)
     log(f"built D A D twin in {time.time() - t0:.1f}s")
     dm = api.DeviceMatrix(ms_)
-    dm.set_variant(tune["best64"])
-    dm.set_variant(tune["best32"], fp32=True)
+    dm.set_variant(tune.get("solver64", tune["best64"]))
+    dm.set_variant(tune.get("solver32", tune["best32"]), fp32=True)
     db = api.DeviceBuffer.from_numpy(b)
     x64, xr = api.DeviceBuffer(ms_.npndof), api.DeviceBuffer(ms_.npndof)
     p64 = _prm(tol=1e-10)

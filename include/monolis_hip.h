@@ -194,6 +194,13 @@ int monolis_hip_time_matvec_each(monolis_hip_matrix* M, int fp32, const void* x,
  * only. */
 int monolis_hip_num_variants(void);
 int monolis_hip_matrix_autotune(monolis_hip_matrix* M, int fp32, int reps, int* best, double* ms);
+/* Solver-context variant tuning (no counterpart in monolis): times `reps`
+ * solves of exactly `iters` iterations of `method` / `precond` (tolerance 0)
+ * per SpMV variant in one precision (fp32 = 0 or 1, the FP32 copy that the
+ * refined solve's inner iterations use) and keeps the fastest on the handle;
+ * ms (if not NULL) gets monolis_hip_num_variants() milliseconds per iteration. */
+int monolis_hip_matrix_autotune_solver(monolis_hip_matrix* M, int fp32, int method, int precond, int iters,
+                                       int reps, int* best, double* ms);
 /* Forces a variant for one precision (fp32 = 0 or 1); an untuned handle uses
  * variant 4 (one wavefront per slice, 3 slots per step). */
 int monolis_hip_matrix_set_variant(monolis_hip_matrix* M, int fp32, int variant);

@@ -72,6 +72,8 @@ SIGNATURES = [
     ("monolis_hip_stream_read", c_int, [c_void_p, ctypes.c_longlong, c_int, c_double_p]),
     ("monolis_hip_num_variants", c_int, []),
     ("monolis_hip_matrix_autotune", c_int, [c_void_p, c_int, c_int, c_int_p, c_double_p]),
+    ("monolis_hip_matrix_autotune_solver", c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int_p,
+                                                   c_double_p]),
     ("monolis_hip_matrix_set_variant", c_int, [c_void_p, c_int, c_int]),
     ("monolis_hip_batch_create", c_int, [ctypes.POINTER(c_void_p), c_int, c_void_p, c_int, c_int,
                                          c_void_p, c_void_p, c_void_p]),

@@ -418,6 +418,19 @@ class DeviceMatrix:
         return {"best64": best[0], "best32": best[1] if fp32 else None,
                 "ms64": list(ms)[:nv], "ms32": list(ms)[nv:] if fp32 else None}
 
+    def autotune_solver(self, fp32: bool = False, method: int = 1, precond: int = 1, iters: int = 20,
+                        reps: int = 2) -> dict:
+        """Times `reps` solves of `iters` iterations per SpMV variant and keeps the fastest
+        (ms per iteration in the result)."""
+        L = _lib.load()
+        nv = L.monolis_hip_num_variants()
+        best = ctypes.c_int()
+        ms = (ctypes.c_double * nv)()
+        _lib.check(L.monolis_hip_matrix_autotune_solver(ctypes.c_void_p(self.h), int(fp32), int(method), int(precond),
+                                                        int(iters), int(reps), ctypes.byref(best), ms),
+                   "autotune_solver")
+        return {"best": best.value, "ms_per_iter": list(ms)}
+
     def set_variant(self, variant: int, fp32: bool = False) -> None:
         _lib.check(_lib.load().monolis_hip_matrix_set_variant(ctypes.c_void_p(self.h), int(fp32), int(variant)),
                    "set_variant")

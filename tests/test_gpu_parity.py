@@ -137,6 +137,40 @@ def test_matvec_every_kernel_variant_is_bit_identical(gen):
     assert np.array_equal(dy.numpy(), outs[0])
 
 
+def test_autotune_solver_keeps_solution():
+    """monolis_hip_matrix_autotune_solver (no counterpart in monolis): per-variant
+    per-iteration times of a fixed-length block-Jacobi CG, one valid variant kept per
+    precision, and the solve after tuning is unchanged (the variants compute
+    bit-identical SpMVs): same iterations, bit-identical x, refined solve to tol."""
+    from monolis_amd import _lib
+    m = W.stencil27_bcsr3(12, seed=6)
+    b = oracle.matvec(m, W.uniform_vector(m.nndof, seed=7))
+    dm = api.DeviceMatrix(m)
+    db = api.DeviceBuffer.from_numpy(b)
+    x0 = api.DeviceBuffer(m.npndof)
+    p0 = api.Prm().quiet()
+    dm.solve(db, x0, p0)
+    nv = _lib.load().monolis_hip_num_variants()
+    t64 = dm.autotune_solver(fp32=False, iters=4, reps=1)
+    t32 = dm.autotune_solver(fp32=True, iters=4, reps=1)
+    for t in (t64, t32):
+        assert 0 <= t["best"] < nv and len(t["ms_per_iter"]) == nv
+        assert all(v > 0 for v in t["ms_per_iter"]), t
+        assert t["ms_per_iter"][t["best"]] == min(t["ms_per_iter"])
+    x1 = api.DeviceBuffer(m.npndof)
+    p1 = api.Prm().quiet()
+    dm.solve(db, x1, p1)
+    assert int(p1.Iarray[4]) == int(p0.Iarray[4]) and p1.Iarray[4] > 0
+    assert np.array_equal(x1.numpy(), x0.numpy())
+    xr = api.DeviceBuffer(m.npndof)
+    pr = api.Prm().quiet()
+    dm.solve_refined(db, xr, pr, inner_tol=1e-4)
+    r = b - oracle.matvec(m, xr.numpy()[:m.nndof])
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-8
+    with pytest.raises(RuntimeError):
+        dm.autotune_solver(iters=0)
+
+
 def _with_far_zero_block(m, far):
     """Copy of m with one extra stored block (row 0, column `far`) holding zeros:
     the same operator, but the column span exceeds the 16-bit delta range."""
