@@ -290,9 +290,11 @@ def test_solve_chain_matches_oracle(ndof, precond, method):
     x, mon = _solve_flat(m, b, method, precond, tol=1e-10)
     xo, st, I, R = oracle.solve(m, b, method=method, precond=precond, tol=1e-10)
     assert api.monolis_get_error_tag(mon) == 0 and st == 0
+    # exact: the oracle's count, or (where the oracle's own count moves with its
+    # summation order, e.g. ndof 3 BiCGSTAB: 25-26) a count inside that spread
     lo, hi = _oracle_iter_spread(m, b, method=method, precond=precond, tol=1e-10)
     it = api.monolis_get_converge_iter(mon)
-    assert _iters_close(it, int(I[4])) or lo - 1 <= it <= hi + 1, (it, int(I[4]), lo, hi)
+    assert it == int(I[4]) or lo <= it <= hi, (it, int(I[4]), lo, hi)
     assert np.allclose(x, 1.0, rtol=0, atol=1e-8)
 
 
