@@ -358,6 +358,21 @@ int monolis_hip_matrix_create_V_C(monolis_hip_matrix_C** out, int n, int np, int
 int monolis_hip_matrix_destroy_C(monolis_hip_matrix_C* M);
 int monolis_hip_matrix_set_values_C(monolis_hip_matrix_C* M, const double _Complex* A);
 /* info[0..7] as monolis_hip_matrix_info, value bytes at 16 per entry (info[7] = 0). */
+/* Complex device assembly, the counterparts of the real calls above on a
+ * complex handle: the pattern of a simple mesh (monolis_get_nonzero_pattern_by_simple_mesh_C,
+ * wrapper/matrix/monolis_spmat_nzpattern_c.h, src/matrix/spmat_nzpattern.f90), element
+ * matrices (monolis_add_matrix_to_sparse_matrix_C, src/matrix/spmat_handler_util.f90:501-600,
+ * bit for bit one call per element in element order), Dirichlet conditions
+ * (monolis_set_Dirichlet_bc_C, src/matrix/spmat_handler_util.f90:681-749; b -= val * A with
+ * the complex product (ar*br - ai*bi, ar*bi + ai*br)) and read-back. */
+int monolis_hip_matrix_create_by_simple_mesh_C(monolis_hip_matrix_C** out, int n_node, int n_base, int n_dof,
+                                               int n_elem, const int* elem);
+int monolis_hip_matrix_get_pattern_C(const monolis_hip_matrix_C* M, int* index, int* item);
+int monolis_hip_matrix_get_values_C(const monolis_hip_matrix_C* M, double _Complex* A);
+int monolis_hip_matrix_add_elements_C(monolis_hip_matrix_C* M, int n_elem, int n_base, const int* elem,
+                                      const double _Complex* Ke);
+int monolis_hip_set_Dirichlet_bc_C(monolis_hip_matrix_C* M, double _Complex* b, int n_bc, const int* node_id,
+                                   const int* n_dof_bc, const double _Complex* val);
 int monolis_hip_matrix_info_C(const monolis_hip_matrix_C* M, long long* info);
 /* y = A x (monolis_matvec_product_main_C, src/linalg/matvec.f90:192-227). */
 int monolis_hip_matvec_C(monolis_hip_matrix_C* M, const double _Complex* x, double _Complex* y);

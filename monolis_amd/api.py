@@ -539,6 +539,55 @@ class DeviceMatrixC:
         return ms.value
 
 
+    # device assembly (monolis_add_matrix_to_sparse_matrix_C, monolis_set_Dirichlet_bc_C, 0-based)
+    def values(self) -> np.ndarray:
+        """The device values in monolis's A layout (complex128)."""
+        n = self.mat.nz * self.mat.ndof * self.mat.ndof
+        A = np.zeros(max(n, 1), np.complex128)
+        _lib.check(_lib.load().monolis_hip_matrix_get_values_C(ctypes.c_void_p(self.h), _p(A)), "get_values_C")
+        return A[:n]
+
+    def add_elements(self, elems, Ke) -> None:
+        """monolis_add_matrix_to_sparse_matrix_C for every element; Ke: complex
+        (n_elem, n_base*b, n_base*b) element matrices."""
+        e = np.ascontiguousarray(elems, np.int32)
+        k = np.ascontiguousarray(Ke, np.complex128)
+        _lib.check(_lib.load().monolis_hip_matrix_add_elements_C(ctypes.c_void_p(self.h), int(e.shape[0]),
+                                                                  int(e.shape[1]), _p(e), _p(k)), "add_elements_C")
+
+    def set_dirichlet(self, b: np.ndarray, node_id, n_dof_bc, val) -> np.ndarray:
+        """monolis_set_Dirichlet_bc_C for every listed condition; b (complex128, N*b) in place."""
+        nd = np.ascontiguousarray(node_id, np.int32)
+        dd = np.ascontiguousarray(n_dof_bc, np.int32)
+        vv = np.ascontiguousarray(val, np.complex128)
+        assert b.dtype == np.complex128 and b.flags.c_contiguous
+        _lib.check(_lib.load().monolis_hip_set_Dirichlet_bc_C(ctypes.c_void_p(self.h), _p(b), int(len(nd)), _p(nd),
+                                                               _p(dd), _p(vv)), "set_Dirichlet_bc_C")
+        return b
+
+
+def device_matrix_by_simple_mesh_C(n_node: int, n_dof: int, elems) -> "DeviceMatrixC":
+    """monolis_get_nonzero_pattern_by_simple_mesh_C on the GPU: a DeviceMatrixC with
+    the mesh's pattern and zero values."""
+    L = _lib.load()
+    e = np.ascontiguousarray(elems, np.int32)
+    n_elem, n_base = (e.shape[0], e.shape[1]) if e.ndim == 2 else (0, 1)
+    h = ctypes.c_void_p()
+    _lib.check(L.monolis_hip_matrix_create_by_simple_mesh_C(ctypes.byref(h), int(n_node), int(n_base), int(n_dof),
+                                                            int(n_elem), _p(e)), "create_by_simple_mesh_C")
+    info = (ctypes.c_longlong * 8)()
+    _lib.check(L.monolis_hip_matrix_info_C(h, info), "info_C")
+    nz = int(info[2])
+    index = np.zeros(n_node + 1, np.int32)
+    item = np.zeros(max(nz, 1), np.int32)
+    _lib.check(L.monolis_hip_matrix_get_pattern_C(h, _p(index), _p(item)), "get_pattern_C")
+    mat = Matrix(n_node, n_node, n_dof, index, item[:nz], np.zeros(nz * n_dof * n_dof, np.complex128))
+    mat.ensure_dof_index()
+    dm = DeviceMatrixC.__new__(DeviceMatrixC)
+    dm.h, dm.mat, dm.m, dm.vec = h.value, mat, mat.nndof, mat.npndof
+    return dm
+
+
 def inner_product_device_C(x: DeviceBuffer, y: DeviceBuffer, m: Optional[int] = None) -> complex:
     """Unconjugated (x, y) of two complex device buffers."""
     n = x.count if m is None else m

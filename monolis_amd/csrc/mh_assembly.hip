@@ -12,6 +12,11 @@
 //   * Dirichlet conditions (monolis_set_Dirichlet_bc_main_R,
 //     src/matrix/spmat_handler_util.f90:604-677) for a whole list in one pass;
 //   * value and pattern read-back in monolis's layout.
+// The value kernels are shared by real and complex matrices (complex element
+// matrices: monolis_add_matrix_to_sparse_matrix_main_C, spmat_handler_util.f90:
+// 501-600, the same loop on complex(kdouble); complex conditions:
+// monolis_set_Dirichlet_bc_main_C, :681-749, whose B update multiplies two
+// complex numbers as (ar*br - ai*bi, ar*bi + ai*br) without contraction).
 #include <algorithm>
 #include <cstdlib>
 #include <vector>
@@ -132,6 +137,7 @@ __global__ void k_elem_targets(int ne, int n_base, const int* __restrict__ elem,
 }
 
 // SELL position of block `blk`, component e (row found by bisection).
+template <typename V>
 __device__ __forceinline__ size_t sell_pos(int blk, int e, int bb, int n, const int* __restrict__ rowptr,
                                            const int* __restrict__ slice_ptr) {
   int lo = 0, hi = n - 1;  // first row r with rowptr[r + 1] > blk
@@ -142,15 +148,16 @@ __device__ __forceinline__ size_t sell_pos(int blk, int e, int bb, int n, const 
   }
   const int r = lo;
   const size_t slot = (size_t)slice_ptr[r / kWave] + (size_t)(blk - rowptr[r]);
-  return sell_vpos<double>(slot, e, bb, r % kWave);
+  return sell_vpos<V>(slot, e, bb, r % kWave);
 }
 
 // Thread (i, component): at the head i of each run of equal block keys, adds
 // the component of every pair in the run, in order, onto the stored value.
+template <typename V>
 __global__ void k_elem_accumulate(long long m, const unsigned long long* __restrict__ keys,
-                                  const int* __restrict__ ids, const double* __restrict__ Ke, int n_base, int b,
+                                  const int* __restrict__ ids, const V* __restrict__ Ke, int n_base, int b,
                                   int n, const int* __restrict__ rowptr, const int* __restrict__ slice_ptr,
-                                  double* __restrict__ val, unsigned long long none) {
+                                  V* __restrict__ val, unsigned long long none) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int bb = b * b;
   if (t >= m * bb) return;
@@ -161,8 +168,8 @@ __global__ void k_elem_accumulate(long long m, const unsigned long long* __restr
   const int k1 = ee / b, k2 = ee % b;
   const long long per = (long long)n_base * n_base;
   const long long nzm = (long long)n_base * b;
-  const size_t pos = sell_pos((int)key, ee, bb, n, rowptr, slice_ptr);
-  double acc = val[pos];
+  const size_t pos = sell_pos<V>((int)key, ee, bb, n, rowptr, slice_ptr);
+  V acc = val[pos];
   for (long long j = i; j < m && keys[j] == key; ++j) {
     const long long pid = ids[j];
     const long long e = pid / per;
@@ -173,8 +180,9 @@ __global__ void k_elem_accumulate(long long m, const unsigned long long* __restr
   val[pos] = acc;
 }
 
+template <typename V>
 __global__ void k_bc_mark(int nbc, const int* __restrict__ node, const int* __restrict__ dof,
-                          const double* __restrict__ v, int b, int* __restrict__ isbc, double* __restrict__ bcval,
+                          const V* __restrict__ v, int b, int* __restrict__ isbc, V* __restrict__ bcval,
                           int* __restrict__ dup) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nbc) return;
@@ -190,12 +198,23 @@ __global__ void k_bc_check_diag(int nbc, const int* __restrict__ node, const int
   if (find_block(rowptr, item, node[i], node[i]) < 0) atomicOr(bad, 1);
 }
 
+// val * A of a condition column, as monolis multiplies (no fused multiply-add).
+__device__ __forceinline__ double bc_prod(double v, double a) {
+#pragma clang fp contract(off)
+  return v * a;
+}
+__device__ __forceinline__ zd bc_prod(zd v, zd a) {
+#pragma clang fp contract(off)
+  return zd(v.re * a.re - v.im * a.im, v.re * a.im + v.im * a.re);
+}
+
 // One thread per block row: rows of condition DOFs become unit rows with
 // rhs = val; other rows move val * A(r, c) of condition columns to the
 // right-hand side (blocks in column order, then component order) and zero them.
+template <typename V>
 __global__ void k_bc_apply(int n, int b, const int* __restrict__ slice_ptr, const int* __restrict__ rowptr,
-                           const int* __restrict__ col, double* __restrict__ val, const int* __restrict__ isbc,
-                           const double* __restrict__ bcval, double* __restrict__ rhs) {
+                           const int* __restrict__ col, V* __restrict__ val, const int* __restrict__ isbc,
+                           const V* __restrict__ bcval, V* __restrict__ rhs) {
 #pragma clang fp contract(off)
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
@@ -205,26 +224,27 @@ __global__ void k_bc_apply(int n, int b, const int* __restrict__ slice_ptr, cons
   const int bb = b * b;
   for (int k1 = 0; k1 < b; ++k1) {
     const bool rowbc = isbc[r * b + k1] != 0;
-    double acc = rhs[(size_t)r * b + k1];
+    V acc = rhs[(size_t)r * b + k1];
     for (int k = 0; k < len; ++k) {
       const size_t slot = (size_t)k0 + k;
       const int c = col[slot * kWave + lane];
       for (int k2 = 0; k2 < b; ++k2) {
         const bool colbc = isbc[c * b + k2] != 0;
-        const size_t pos = sell_vpos<double>(slot, k1 * b + k2, bb, lane);
+        const size_t pos = sell_vpos<V>(slot, k1 * b + k2, bb, lane);
         if (colbc && !rowbc) {
-          const double t = bcval[c * b + k2] * val[pos];
-          acc = acc - t;  // B(n) = B(n) - val*A(m)  (spmat_handler_util.f90:636)
+          const V t = bc_prod(bcval[c * b + k2], val[pos]);
+          acc = acc - t;  // B(n) = B(n) - val*A(m)  (spmat_handler_util.f90:636, :723)
         }
-        if (rowbc || colbc) val[pos] = (rowbc && c == r && k1 == k2) ? 1.0 : 0.0;
+        if (rowbc || colbc) val[pos] = V((rowbc && c == r && k1 == k2) ? 1.0 : 0.0);
       }
     }
     rhs[(size_t)r * b + k1] = rowbc ? bcval[r * b + k1] : acc;
   }
 }
 
+template <typename V>
 __global__ void k_sell_gather(int n, int b, const int* __restrict__ slice_ptr, const int* __restrict__ rowptr,
-                              const double* __restrict__ val, double* __restrict__ A) {
+                              const V* __restrict__ val, V* __restrict__ A) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int s = r / kWave, lane = r % kWave;
@@ -233,12 +253,12 @@ __global__ void k_sell_gather(int n, int b, const int* __restrict__ slice_ptr, c
   const int bb = b * b;
   for (int k = 0; k < len; ++k)
     for (int e = 0; e < bb; ++e)
-      A[((size_t)j0 + k) * bb + e] = val[sell_vpos<double>((size_t)k0 + k, e, bb, lane)];
+      A[((size_t)j0 + k) * bb + e] = val[sell_vpos<V>((size_t)k0 + k, e, bb, lane)];
 }
 
-int require_assemblable(const Matrix& M, const char* what) {
-  if (M.vb || M.is_complex) {
-    set_error("%s: real fixed-block matrices only", what);
+int require_assemblable(const Matrix& M, const char* what, bool cplx) {
+  if (M.vb || M.is_complex != cplx) {
+    set_error("%s: %s fixed-block matrices only", what, cplx ? "complex" : "real");
     return -2;
   }
   if (M.np != M.n) {
@@ -252,6 +272,17 @@ void mark_values_changed(Matrix* M) {
   M->diag_ready = M->diag32_ready = M->diagz_ready = false;
   M->val32_ready = false;  // the FP32 copy is rebuilt on its next use
   M->transpose_stale = true;
+}
+
+template <typename V>
+V* values_of(Matrix* M);
+template <>
+double* values_of<double>(Matrix* M) {
+  return M->val64.p;
+}
+template <>
+zd* values_of<zd>(Matrix* M) {
+  return M->valz.p;
 }
 
 }  // namespace
@@ -310,8 +341,9 @@ int pattern_by_simple_mesh(int n_node, int n_base, int n_elem, const int* elem, 
   return 0;
 }
 
-int matrix_add_elements(Matrix* M, int n_elem, int n_base, const int* elem, const double* Ke, hipStream_t s) {
-  MH_TRY(require_assemblable(*M, "element assembly"));
+template <typename V>
+static int add_elements_T(Matrix* M, int n_elem, int n_base, const int* elem, const V* Ke, hipStream_t s) {
+  MH_TRY(require_assemblable(*M, "element assembly", sizeof(V) == sizeof(zd)));
   if (n_elem < 0 || n_base < 1) {
     set_error("invalid element sizes n_elem=%d n_base=%d", n_elem, n_base);
     return -2;
@@ -352,7 +384,7 @@ int matrix_add_elements(Matrix* M, int n_elem, int n_base, const int* elem, cons
   const long long cap = (long long)chunk * pairs_elem;
   DevArray<unsigned long long> k0, k1;
   DevArray<int> i0, i1;
-  DevArray<double> kt;
+  DevArray<V> kt;
   MH_TRY(k0.alloc((size_t)cap));
   MH_TRY(k1.alloc((size_t)cap));
   MH_TRY(i0.alloc((size_t)cap));
@@ -368,11 +400,11 @@ int matrix_add_elements(Matrix* M, int n_elem, int n_base, const int* elem, cons
   for (int e0 = 0; e0 < n_elem; e0 += chunk) {
     const int ne = std::min(chunk, n_elem - e0);
     const long long m = (long long)ne * pairs_elem;
-    const double* kc = nullptr;
+    const V* kc = nullptr;
     if (ke_dev) {
       kc = Ke + (size_t)e0 * per_elem;
     } else {
-      MH_TRY(upload(kt.p, Ke + (size_t)e0 * per_elem, (size_t)ne * per_elem * sizeof(double), s));
+      MH_TRY(upload(kt.p, Ke + (size_t)e0 * per_elem, (size_t)ne * per_elem * sizeof(V), s));
       kc = kt.p;
     }
     k_elem_targets<<<grid_for(m), kBlock, 0, s>>>(ne, n_base, de + (size_t)e0 * n_base, M->rowptr.p, M->csr_item.p,
@@ -387,8 +419,8 @@ int matrix_add_elements(Matrix* M, int n_elem, int n_base, const int* elem, cons
     }
     size_t tb = tmp_bytes;
     MH_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0.p, k1.p, i0.p, i1.p, (int)m, 0, end_bit, s));
-    k_elem_accumulate<<<grid_for(m * bb), kBlock, 0, s>>>(m, k1.p, i1.p, kc, n_base, b, M->n, M->rowptr.p,
-                                                           M->slice_ptr.p, M->val64.p, none);
+    k_elem_accumulate<V><<<grid_for(m * bb), kBlock, 0, s>>>(m, k1.p, i1.p, kc, n_base, b, M->n, M->rowptr.p,
+                                                              M->slice_ptr.p, values_of<V>(M), none);
     MH_HIP(hipGetLastError());
   }
   MH_HIP(hipStreamSynchronize(s));
@@ -396,9 +428,16 @@ int matrix_add_elements(Matrix* M, int n_elem, int n_base, const int* elem, cons
   return 0;
 }
 
-int matrix_dirichlet(Matrix* M, double* rhs, int n_bc, const int* node, const int* dof, const double* val,
-                     hipStream_t s) {
-  MH_TRY(require_assemblable(*M, "Dirichlet conditions"));
+int matrix_add_elements(Matrix* M, int n_elem, int n_base, const int* elem, const double* Ke, hipStream_t s) {
+  return add_elements_T<double>(M, n_elem, n_base, elem, Ke, s);
+}
+int matrix_add_elements_C(Matrix* M, int n_elem, int n_base, const int* elem, const zd* Ke, hipStream_t s) {
+  return add_elements_T<zd>(M, n_elem, n_base, elem, Ke, s);
+}
+
+template <typename V>
+static int dirichlet_T(Matrix* M, V* rhs, int n_bc, const int* node, const int* dof, const V* val, hipStream_t s) {
+  MH_TRY(require_assemblable(*M, "Dirichlet conditions", sizeof(V) == sizeof(zd)));
   if (n_bc < 0) {
     set_error("negative number of conditions");
     return -2;
@@ -406,9 +445,9 @@ int matrix_dirichlet(Matrix* M, double* rhs, int n_bc, const int* node, const in
   if (n_bc == 0 || M->n == 0) return 0;
   const int b = M->b;
   DevArray<int> nt, dt;
-  DevArray<double> vt;
+  DevArray<V> vt;
   const int *dn = nullptr, *dd = nullptr;
-  const double* dv = nullptr;
+  const V* dv = nullptr;
   MH_TRY(stage(node, (size_t)n_bc, nt, &dn, s));
   MH_TRY(stage(dof, (size_t)n_bc, dt, &dd, s));
   MH_TRY(stage(val, (size_t)n_bc, vt, &dv, s));
@@ -429,12 +468,12 @@ int matrix_dirichlet(Matrix* M, double* rhs, int n_bc, const int* node, const in
   k_bc_check_diag<<<grid_for(n_bc), kBlock, 0, s>>>(n_bc, dn, M->rowptr.p, M->csr_item.p, flag.p);
   MH_HIP(hipGetLastError());
   DevArray<int> isbc;
-  DevArray<double> bcval;
+  DevArray<V> bcval;
   const size_t m = (size_t)M->n * b;
   MH_TRY(isbc.alloc(m));
   MH_TRY(bcval.alloc(m));
   MH_HIP(hipMemsetAsync(isbc.p, 0, m * sizeof(int), s));
-  k_bc_mark<<<grid_for(n_bc), kBlock, 0, s>>>(n_bc, dn, dd, dv, b, isbc.p, bcval.p, flag.p + 1);
+  k_bc_mark<V><<<grid_for(n_bc), kBlock, 0, s>>>(n_bc, dn, dd, dv, b, isbc.p, bcval.p, flag.p + 1);
   MH_HIP(hipGetLastError());
   MH_HIP(hipMemcpyAsync(bad, flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
   MH_HIP(hipStreamSynchronize(s));
@@ -447,46 +486,59 @@ int matrix_dirichlet(Matrix* M, double* rhs, int n_bc, const int* node, const in
     set_error("the same DOF is listed twice among the Dirichlet conditions");
     return -2;
   }
-  DevArray<double> rtmp;
-  double* dr = rhs;
+  DevArray<V> rtmp;
+  V* dr = rhs;
   const bool rhs_dev = is_device_pointer(rhs);
   if (!rhs_dev) {
     MH_TRY(rtmp.alloc(m));
-    MH_TRY(upload(rtmp.p, rhs, m * sizeof(double), s));
+    MH_TRY(upload(rtmp.p, rhs, m * sizeof(V), s));
     dr = rtmp.p;
   }
-  k_bc_apply<<<grid_for(M->n), kBlock, 0, s>>>(M->n, b, M->slice_ptr.p, M->rowptr.p, M->col.p, M->val64.p, isbc.p,
-                                               bcval.p, dr);
+  k_bc_apply<V><<<grid_for(M->n), kBlock, 0, s>>>(M->n, b, M->slice_ptr.p, M->rowptr.p, M->col.p, values_of<V>(M),
+                                                  isbc.p, bcval.p, dr);
   MH_HIP(hipGetLastError());
-  if (!rhs_dev) MH_HIP(hipMemcpyAsync(rhs, dr, m * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (!rhs_dev) MH_HIP(hipMemcpyAsync(rhs, dr, m * sizeof(V), hipMemcpyDeviceToHost, s));
   MH_HIP(hipStreamSynchronize(s));
   mark_values_changed(M);
   return 0;
 }
 
-int matrix_get_values(const Matrix& M, double* A, hipStream_t s) {
-  if (M.vb || M.is_complex) {
-    set_error("value read-back: real fixed-block matrices only");
+int matrix_dirichlet(Matrix* M, double* rhs, int n_bc, const int* node, const int* dof, const double* val,
+                     hipStream_t s) {
+  return dirichlet_T<double>(M, rhs, n_bc, node, dof, val, s);
+}
+int matrix_dirichlet_C(Matrix* M, zd* rhs, int n_bc, const int* node, const int* dof, const zd* val, hipStream_t s) {
+  return dirichlet_T<zd>(M, rhs, n_bc, node, dof, val, s);
+}
+
+template <typename V>
+static int get_values_T(const Matrix& M, V* A, hipStream_t s) {
+  if (M.vb || M.is_complex != (sizeof(V) == sizeof(zd))) {
+    set_error("value read-back: %s fixed-block matrices only", sizeof(V) == sizeof(zd) ? "complex" : "real");
     return -2;
   }
   const size_t nv = (size_t)M.nz * M.b * M.b;
   if (nv == 0) return 0;
-  DevArray<double> tmp;
-  double* dA = A;
+  DevArray<V> tmp;
+  V* dA = A;
   const bool dev = is_device_pointer(A);
   if (!dev) {
     MH_TRY(tmp.alloc(nv));
     dA = tmp.p;
-    if (M.n < M.np) MH_HIP(hipMemsetAsync(dA, 0, nv * sizeof(double), s));
+    if (M.n < M.np) MH_HIP(hipMemsetAsync(dA, 0, nv * sizeof(V), s));
   }
   if (M.n > 0) {
-    k_sell_gather<<<grid_for(M.n), kBlock, 0, s>>>(M.n, M.b, M.slice_ptr.p, M.rowptr.p, M.val64.p, dA);
+    k_sell_gather<V><<<grid_for(M.n), kBlock, 0, s>>>(M.n, M.b, M.slice_ptr.p, M.rowptr.p,
+                                                      values_of<V>(const_cast<Matrix*>(&M)), dA);
     MH_HIP(hipGetLastError());
   }
-  if (!dev) MH_HIP(hipMemcpyAsync(A, dA, nv * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (!dev) MH_HIP(hipMemcpyAsync(A, dA, nv * sizeof(V), hipMemcpyDeviceToHost, s));
   MH_HIP(hipStreamSynchronize(s));
   return 0;
 }
+
+int matrix_get_values(const Matrix& M, double* A, hipStream_t s) { return get_values_T<double>(M, A, s); }
+int matrix_get_values_C(const Matrix& M, zd* A, hipStream_t s) { return get_values_T<zd>(M, A, s); }
 
 int matrix_get_pattern(const Matrix& M, int* index, int* item, hipStream_t s) {
   if (index) MH_HIP(hipMemcpyAsync(index, M.rowptr.p, sizeof(int) * ((size_t)M.np + 1), hipMemcpyDefault, s));

@@ -1449,6 +1449,62 @@ int monolis_hip_matrix_set_values_C(monolis_hip_matrix_C* H, const double _Compl
   return 0;
 }
 
+int monolis_hip_matrix_create_by_simple_mesh_C(monolis_hip_matrix_C** out, int n_node, int n_base, int n_dof,
+                                               int n_elem, const int* elem) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (!out) {
+    mh::set_error("null output handle");
+    return ST_ARGS;
+  }
+  *out = nullptr;
+  if (n_dof < 1 || n_dof > mh::kMaxB) {
+    mh::set_error("block size %d outside 1..%d", n_dof, mh::kMaxB);
+    return ST_ARGS;
+  }
+  mh::DevArray<int> index, item;
+  int nz = 0;
+  MH_TRY(mh::pattern_by_simple_mesh(n_node, n_base, n_elem, elem, index, item, &nz, c->stream));
+  // values start at zero (monolis_alloc_nonzero_pattern_mat_val_C)
+  mh::DevArray<double> zeros;
+  const size_t nd = (size_t)nz * n_dof * n_dof * 2;
+  MH_TRY(zeros.alloc(nd ? nd : 1));
+  MH_HIP(hipMemsetAsync(zeros.p, 0, sizeof(double) * (nd ? nd : 1), c->stream));
+  Matrix* M = nullptr;
+  MH_TRY(mh::matrix_create_complex(&M, n_node, n_node, nz, n_dof, index.p, item.p, zeros.p, c->stream));
+  auto* H = new monolis_hip_matrix_C();
+  H->M = M;
+  *out = H;
+  return 0;
+}
+
+int monolis_hip_matrix_get_pattern_C(const monolis_hip_matrix_C* H, int* index, int* item) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  return mh::matrix_get_pattern(*H->M, index, item, c->stream);
+}
+
+int monolis_hip_matrix_get_values_C(const monolis_hip_matrix_C* H, double _Complex* A) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  return mh::matrix_get_values_C(*H->M, reinterpret_cast<mh::zd*>(A), c->stream);
+}
+
+int monolis_hip_matrix_add_elements_C(monolis_hip_matrix_C* H, int n_elem, int n_base, const int* elem,
+                                      const double _Complex* Ke) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  return mh::matrix_add_elements_C(H->M, n_elem, n_base, elem, reinterpret_cast<const mh::zd*>(Ke), c->stream);
+}
+
+int monolis_hip_set_Dirichlet_bc_C(monolis_hip_matrix_C* H, double _Complex* b, int n_bc, const int* node_id,
+                                   const int* n_dof_bc, const double _Complex* val) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  return mh::matrix_dirichlet_C(H->M, reinterpret_cast<mh::zd*>(b), n_bc, node_id, n_dof_bc,
+                                reinterpret_cast<const mh::zd*>(val), c->stream);
+}
+
 int monolis_hip_matrix_info_C(const monolis_hip_matrix_C* H, long long* info) {
   const Matrix* M = H->M;
   info[0] = M->n;

@@ -103,6 +103,10 @@ def lib():
         L.orc_dirichlet.restype = c_int
         L.orc_dirichlet_list.argtypes = [c_int, c_int, _i32p, _i32p, _f64p, _f64p, c_int, _i32p, _i32p, _f64p]
         L.orc_dirichlet_list.restype = c_int
+        L.orc_dirichlet_C.argtypes = [c_int, c_int, _i32p, _i32p, _f64p, _f64p, c_int, c_int, c_double, c_double]
+        L.orc_dirichlet_C.restype = c_int
+        L.orc_dirichlet_list_C.argtypes = [c_int, c_int, _i32p, _i32p, _f64p, _f64p, c_int, _i32p, _i32p, _f64p]
+        L.orc_dirichlet_list_C.restype = c_int
         _lib = L
     return _lib
 
@@ -388,6 +392,21 @@ def add_elements(mat, elems, Ke):
     if st:
         raise KeyError("element couples nodes outside the nonzero pattern")
     return mat
+
+
+def dirichlet_C(index, item, A, n, ndof, B, node, dof, val):
+    """Complex conditions applied one by one (monolis_set_Dirichlet_bc_C) on a complex
+    BCSR matrix: A (nz*ndof^2 complex128) and B (n*ndof complex128) updated in place."""
+    node = np.ascontiguousarray(node, np.int32)
+    dof = np.ascontiguousarray(dof, np.int32)
+    v = np.ascontiguousarray(np.asarray(val, np.complex128)).view(np.float64)
+    Af = A.view(np.float64)
+    Bf = B.view(np.float64)
+    st = lib().orc_dirichlet_list_C(n, ndof, np.ascontiguousarray(index, np.int32),
+                                    np.ascontiguousarray(item, np.int32), Af, Bf, len(node), node, dof, v)
+    if st:
+        raise KeyError("Dirichlet condition on a node without a diagonal block")
+    return A, B
 
 
 def dirichlet(mat, B, node, dof, val):

@@ -9,7 +9,10 @@
  * (src_test/matrix/spmat_nzpattern_test.f90:24-73 pattern,
  * src_test/matrix/spmat_handler_test.f90:18-133 element add and Dirichlet
  * values), dense brute force (pair sets, dense element sums, the solution of
- * the reduced system after Dirichlet conditions).
+ * the reduced system after Dirichlet conditions).  The complex conditions:
+ * the reference's unit test (spmat_handler_test.f90:236-248) and the closed
+ * forms of the complex product (real values reduce to the real conditions;
+ * values i*v move the real update into the imaginary parts).
  */
 #include <stdlib.h>
 #include <string.h>
@@ -182,5 +185,57 @@ int orc_dirichlet_list(int n, int b, const int* index, const int* item, double* 
                        const int* node, const int* dof, const double* val) {
   for (int i = 0; i < n_bc; ++i)
     if (orc_dirichlet(n, b, index, item, A, B, node[i], dof[i], val[i])) return -1;
+  return 0;
+}
+
+/* monolis_set_Dirichlet_bc_main_C (src/matrix/spmat_handler_util.f90:681-749):
+ * orc_dirichlet on complex values (A and B interleaved re, im).  The right-hand
+ * side update B(n) = B(n) - val*A(m) (:723) takes Fortran's complex product
+ * (vr*ar - vi*ai, vr*ai + vi*ar), written out so that no contraction applies;
+ * the condition row is zeroed, its diagonal set to (1, 0) and B(dof) = val. */
+int orc_dirichlet_C(int n, int b, const int* index, const int* item, double* A, double* B, int node_id, int ndof_bc,
+                    double vr, double vi) {
+  for (int jn = 0; jn < n; ++jn) {
+    for (int j = index[jn]; j < index[jn + 1]; ++j) {
+      if (item[j] - 1 != node_id) continue;
+      for (int k = 0; k < b; ++k) {
+        const long long m = (long long)j * b * b + (long long)b * k + ndof_bc;
+        const long long nn = (long long)jn * b + k;
+        const double ar = A[2 * m], ai = A[2 * m + 1];
+        const double pr = vr * ar, pi_ = vi * ai, qr = vr * ai, qi = vi * ar;
+        const double tr = pr - pi_, ti = qr + qi;
+        B[2 * nn] = B[2 * nn] - tr;
+        B[2 * nn + 1] = B[2 * nn + 1] - ti;
+        A[2 * m] = 0.0;
+        A[2 * m + 1] = 0.0;
+      }
+    }
+  }
+  int is_add = 0;
+  for (int j = index[node_id]; j < index[node_id + 1]; ++j) {
+    const int jn = item[j] - 1;
+    for (int k = 0; k < b; ++k) {
+      const long long m = (long long)j * b * b + (long long)b * ndof_bc + k;
+      A[2 * m] = 0.0;
+      A[2 * m + 1] = 0.0;
+    }
+    if (jn == node_id) {
+      const long long m = (long long)j * b * b + (long long)(b + 1) * ndof_bc;
+      A[2 * m] = 1.0;
+      A[2 * m + 1] = 0.0;
+      is_add = 1;
+    }
+  }
+  if (!is_add) return -1;
+  B[2 * ((long long)node_id * b + ndof_bc)] = vr;
+  B[2 * ((long long)node_id * b + ndof_bc) + 1] = vi;
+  return 0;
+}
+
+/* The complex conditions of a list applied one after the other (val: 2 * n_bc). */
+int orc_dirichlet_list_C(int n, int b, const int* index, const int* item, double* A, double* B, int n_bc,
+                         const int* node, const int* dof, const double* val) {
+  for (int i = 0; i < n_bc; ++i)
+    if (orc_dirichlet_C(n, b, index, item, A, B, node[i], dof[i], val[2 * i], val[2 * i + 1])) return -1;
   return 0;
 }

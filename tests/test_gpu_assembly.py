@@ -186,3 +186,109 @@ def test_assemble_condition_solve_pipeline():
     x = dx.numpy()
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
     assert not x[bottom[0] * b:(bottom[-1] + 1) * b].any()
+
+
+# --- complex matrices (monolis_*_C) ------------------------------------------------------
+def _complex_elements(n_elem, nzm, seed):
+    """Complex symmetric element matrices: SPD real part + i * 0.3 * SPD."""
+    return element_matrices(n_elem, nzm, seed=seed) + 0.3j * element_matrices(n_elem, nzm, seed=seed + 100)
+
+
+@pytest.mark.parametrize("b", [1, 2, 3])
+def test_complex_element_assembly_bit_identical(b):
+    """monolis_add_matrix_to_sparse_matrix_main_C (spmat_handler_util.f90:501-600) adds
+    complex values in the real routine's order, so the real and imaginary parts are
+    each the real assembly of Ke.real and Ke.imag (the oracle's), bit for bit."""
+    n_node, elems = hex_mesh(5, 4, 4)
+    Ke = element_matrices(len(elems), 8 * b, seed=b, spd=False) + 1j * element_matrices(len(elems), 8 * b,
+                                                                                          seed=b + 50, spd=False)
+    dm = api.device_matrix_by_simple_mesh_C(n_node, b, elems)
+    index, item = oracle.pattern_by_simple_mesh(n_node, elems)
+    assert np.array_equal(dm.mat.index, index) and np.array_equal(dm.mat.item, item)
+    half = len(elems) // 2
+    dm.add_elements(elems[:half], Ke[:half])
+    dm.add_elements(elems[half:], Ke[half:])
+    mr, mi = _oracle_mat(n_node, b, elems), _oracle_mat(n_node, b, elems)
+    oracle.add_elements(mr, elems, np.ascontiguousarray(Ke.real))
+    oracle.add_elements(mi, elems, np.ascontiguousarray(Ke.imag))
+    A = dm.values()
+    assert np.array_equal(A.real, mr.A) and np.array_equal(A.imag, mi.A)
+    # the assembled device matrix multiplies as the oracle's complex SpMV
+    x = np.random.default_rng(1).uniform(-1, 1, n_node * b) + 1j * np.random.default_rng(2).uniform(-1, 1, n_node * b)
+    mo = mm.Matrix(n_node, n_node, b, index, item, A.copy()).ensure_dof_index()
+    dx, dy = api.DeviceBuffer.from_numpy(x), api.DeviceBuffer(n_node * b, np.complex128)
+    dm.matvec(dx, dy)
+    ref = oracle.matvec_C(mo, x)
+    assert np.max(np.abs(dy.numpy() - ref)) <= 1e-12 * np.max(np.abs(ref))
+    with pytest.raises(_lib.MonolisHipError, match="not in the nonzero pattern"):
+        dm.add_elements(np.array([[0, n_node - 1]], np.int32), np.ones((1, 2 * b, 2 * b), np.complex128))
+
+
+def test_complex_dirichlet_reference_unit_test():
+    """src_test/matrix/spmat_handler_test.f90:236-248 on the device: A = (2, 2), B = 0,
+    condition on node 2 (1-based), DOF 1, value (1, 0)."""
+    dm = api.device_matrix_by_simple_mesh_C(4, 2, [[0, 1], [1, 2], [2, 3]])
+    dm.set_values(np.full(10 * 4, 2.0 + 2.0j))
+    B = np.zeros(8, np.complex128)
+    dm.set_dirichlet(B, [1], [0], [1.0 + 0.0j])
+    assert B.tolist() == [-2 - 2j, -2 - 2j, 1 + 0j, -2 - 2j, -2 - 2j, -2 - 2j, 0j, 0j]
+    A = np.full(10 * 4, 2.0 + 2.0j)
+    oracle.dirichlet_C(dm.mat.index, dm.mat.item, A, 4, 2, np.zeros(8, np.complex128), [1], [0], [1.0 + 0.0j])
+    assert np.array_equal(dm.values(), A)
+
+
+@pytest.mark.parametrize("b", [1, 3])
+def test_complex_dirichlet_list_bit_identical_to_sequential(b):
+    n_node, elems = hex_mesh(7)
+    Ke = _complex_elements(len(elems), 8 * b, seed=5)
+    dm = api.device_matrix_by_simple_mesh_C(n_node, b, elems)
+    dm.add_elements(elems, Ke)
+    A = dm.values().copy()
+    rng = np.random.default_rng(9)
+    f = rng.uniform(-1, 1, n_node * b) + 1j * rng.uniform(-1, 1, n_node * b)
+    nodes = np.sort(rng.choice(n_node, 40, replace=False)).astype(np.int32)
+    nodes = np.repeat(nodes, b)
+    dofs = np.tile(np.arange(b, dtype=np.int32), 40)
+    keep = rng.random(len(nodes)) < 0.7
+    nodes, dofs = nodes[keep], dofs[keep]
+    val = rng.uniform(-1, 1, len(nodes)) + 1j * rng.uniform(-1, 1, len(nodes))
+    Bg = f.copy()
+    dm.set_dirichlet(Bg, nodes, dofs, val)
+    Bo = f.copy()
+    oracle.dirichlet_C(dm.mat.index, dm.mat.item, A, n_node, b, Bo, nodes, dofs, val)
+    assert np.array_equal(dm.values(), A)
+    assert np.array_equal(Bg, Bo)
+    with pytest.raises(_lib.MonolisHipError, match="twice"):
+        dm.set_dirichlet(np.zeros(n_node * b, np.complex128), [0, 0], [0, 0], [1.0, 2.0])
+
+
+def test_complex_assemble_condition_cocg_pipeline():
+    """Complex symmetric element matrices, Dirichlet conditions and COCG on the device,
+    against the same pipeline through the oracle (assembly restated as two real
+    assemblies, complex conditions, the oracle's COCG)."""
+    b = 3
+    n_node, elems = hex_mesh(8)
+    Ke = _complex_elements(len(elems), 8 * b, seed=21)
+    dm = api.device_matrix_by_simple_mesh_C(n_node, b, elems)
+    dm.add_elements(elems, Ke)
+    bottom = np.arange(9 * 9, dtype=np.int32)
+    nodes, dofs = np.repeat(bottom, b), np.tile(np.arange(b, dtype=np.int32), len(bottom))
+    rng = np.random.default_rng(2)
+    f = rng.uniform(-1, 1, n_node * b) + 1j * rng.uniform(-1, 1, n_node * b)
+    A0 = dm.values().copy()
+    Bg = f.copy()
+    dm.set_dirichlet(Bg, nodes, dofs, np.zeros(len(nodes), np.complex128))
+    db, dx = api.DeviceBuffer.from_numpy(Bg), api.DeviceBuffer(n_node * b, np.complex128)
+    prm = api.Prm().quiet()
+    prm.Iarray[1], prm.Iarray[2], prm.Iarray[3] = api.MONOLIS_ITER_COCG, 1, 2000   # COCG + block-Jacobi
+    prm.Rarray[1] = 1e-10
+    dm.solve(db, dx, prm)
+    Bo = f.copy()
+    oracle.dirichlet_C(dm.mat.index, dm.mat.item, A0, n_node, b, Bo, nodes, dofs, np.zeros(len(nodes), np.complex128))
+    mo = mm.Matrix(n_node, n_node, b, dm.mat.index, dm.mat.item, A0).ensure_dof_index()
+    xo, st, I, R = oracle.solve_C(mo, Bo, tol=1e-10, precond=1, max_iter=2000)
+    assert prm.Iarray[5] == 0 and st == 0
+    assert abs(int(prm.Iarray[4]) - int(I[4])) <= max(1, int(0.02 * I[4]))
+    x = dx.numpy()
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert not x[bottom[0] * b:(bottom[-1] + 1) * b].any()

@@ -105,3 +105,53 @@ def test_dirichlet_solution_is_the_reduced_solve(b):
     Ad = m.dense()
     assert np.array_equal(Ad[fixed][:, fixed], np.eye(len(fixed)))
     assert not Ad[np.ix_(free, fixed)].any() and not Ad[np.ix_(fixed, free)].any()
+
+
+# --- complex conditions (monolis_set_Dirichlet_bc_C) -------------------------------------
+def test_dirichlet_C_reference_unit_test():
+    """src_test/matrix/spmat_handler_test.f90:236-248: A = (2, 2) everywhere, B = 0,
+    condition on node 2 (1-based), DOF 1, value (1, 0)."""
+    m = _mat(4, 2, [[0, 1], [1, 2], [2, 3]])
+    A = np.full(len(m.A), 2.0 + 2.0j)
+    B = np.zeros(8, np.complex128)
+    oracle.dirichlet_C(m.index, m.item, A, 4, 2, B, [1], [0], [1.0 + 0.0j])
+    assert B.tolist() == [-2 - 2j, -2 - 2j, 1 + 0j, -2 - 2j, -2 - 2j, -2 - 2j, 0j, 0j]
+
+
+@pytest.mark.parametrize("b", [1, 2, 3])
+def test_dirichlet_C_reduces_to_the_real_conditions(b):
+    """Closed forms of the complex product: with real A, B and values the complex
+    conditions equal the real ones (imaginary parts stay 0); with values i*v the
+    real parts of B stay as they are and the imaginary parts receive the real update
+    of a zero right-hand side with values v (B.im - v*A), bit for bit."""
+    n_node, elems = hex_mesh(3)
+    m = _mat(n_node, b, elems)
+    oracle.add_elements(m, elems, element_matrices(len(elems), 8 * b, seed=11))
+    rng = np.random.default_rng(5)
+    f = rng.uniform(-1, 1, n_node * b)
+    nodes = np.arange(1, n_node, 4, dtype=np.int32)
+    dof = ((nodes // 4) % b).astype(np.int32)
+    v = rng.uniform(-1, 1, len(nodes))
+    # real values
+    mr = _mat(n_node, b, elems)
+    mr.A[:] = m.A
+    Br = f.copy()
+    oracle.dirichlet(mr, Br, nodes, dof, v)
+    A = m.A.astype(np.complex128)
+    B = f.astype(np.complex128)
+    oracle.dirichlet_C(m.index, m.item, A, n_node, b, B, nodes, dof, v.astype(np.complex128))
+    assert np.array_equal(B.real, Br) and not B.imag.any()
+    assert np.array_equal(A.real, mr.A) and not A.imag.any()
+    # imaginary values
+    mi = _mat(n_node, b, elems)
+    mi.A[:] = m.A
+    Bi = np.zeros(n_node * b)
+    oracle.dirichlet(mi, Bi, nodes, dof, v)
+    A = m.A.astype(np.complex128)
+    B = f.astype(np.complex128)
+    oracle.dirichlet_C(m.index, m.item, A, n_node, b, B, nodes, dof, 1j * v)
+    fixed = nodes * b + dof
+    free = np.setdiff1d(np.arange(n_node * b), fixed)
+    assert np.array_equal(B.real[free], f[free]) and np.array_equal(B.imag[free], Bi[free])
+    assert np.array_equal(B[fixed], 1j * v)
+    assert np.array_equal(A.real, mi.A) and not A.imag.any()
