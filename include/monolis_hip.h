@@ -311,7 +311,8 @@ int monolis_hip_adjoint_R(monolis_hip_matrix* M, const double* x, const double* 
  * at zero.  elem holds n_elem * n_base node ids (element-major). */
 int monolis_hip_matrix_create_by_simple_mesh(monolis_hip_matrix** out, int n_node, int n_base, int n_dof, int n_elem,
                                              const int* elem);
-/* index (np + 1, 0-based offsets) and item (nz, 1-based columns) of the handle; host or device. */
+/* index (np + 1, 0-based offsets) and item (nz, 1-based columns) of the handle (variable
+ * blocks: the block pattern); host or device. */
 int monolis_hip_matrix_get_pattern(const monolis_hip_matrix* M, int* index, int* item);
 /* The values in monolis's layout (nz * n_dof^2, row-major blocks); host or device. */
 int monolis_hip_matrix_get_values_R(const monolis_hip_matrix* M, double* A);
@@ -337,6 +338,16 @@ int monolis_hip_matrix_add_elements_R(monolis_hip_matrix* M, int n_elem, int n_b
  * block (monolis stops there), is an error. */
 int monolis_hip_set_Dirichlet_bc_R(monolis_hip_matrix* M, double* b, int n_bc, const int* node_id,
                                    const int* n_dof_bc, const double* val);
+/* Variable block sizes (monolis n_dof = -1): monolis_get_nonzero_pattern_by_simple_mesh_V
+ * (src/matrix/spmat_nzpattern.f90:53-87) with n_dof_list[n_node] (host or device) block
+ * sizes of 1..16; the matrix starts at zero.  On such a handle add_elements_R takes
+ * element e's matrix as (sum_a n_dof_list[elem[e*n_base + a]])^2 row-major values,
+ * the elements' matrices back to back; set_Dirichlet_bc_R takes DOFs below the node's
+ * block size; get_values_R returns monolis's variable-block layout (n_dof_index2[nz]
+ * values).  The complex handle has the same (create_by_simple_mesh_V_C with the _C
+ * calls). */
+int monolis_hip_matrix_create_by_simple_mesh_V(monolis_hip_matrix** out, int n_node, int n_base,
+                                               const int* n_dof_list, int n_elem, const int* elem);
 
 /* Device-resident complex matrix (the complex counterpart of monolis_hip_matrix;
  * same BCSR arrays, A holds nz * n_dof^2 double _Complex values).  A separate
@@ -367,6 +378,8 @@ int monolis_hip_matrix_set_values_C(monolis_hip_matrix_C* M, const double _Compl
  * the complex product (ar*br - ai*bi, ar*bi + ai*br)) and read-back. */
 int monolis_hip_matrix_create_by_simple_mesh_C(monolis_hip_matrix_C** out, int n_node, int n_base, int n_dof,
                                                int n_elem, const int* elem);
+int monolis_hip_matrix_create_by_simple_mesh_V_C(monolis_hip_matrix_C** out, int n_node, int n_base,
+                                                 const int* n_dof_list, int n_elem, const int* elem);
 int monolis_hip_matrix_get_pattern_C(const monolis_hip_matrix_C* M, int* index, int* item);
 int monolis_hip_matrix_get_values_C(const monolis_hip_matrix_C* M, double _Complex* A);
 int monolis_hip_matrix_add_elements_C(monolis_hip_matrix_C* M, int n_elem, int n_base, const int* elem,

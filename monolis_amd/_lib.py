@@ -49,6 +49,10 @@ SIGNATURES = [
     ("monolis_hip_matrix_create_by_simple_mesh_C", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_int, c_int,
                                                            c_void_p]),
     ("monolis_hip_matrix_get_pattern_C", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("monolis_hip_matrix_create_by_simple_mesh_V", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_void_p, c_int,
+                                                           c_void_p]),
+    ("monolis_hip_matrix_create_by_simple_mesh_V_C", c_int, [ctypes.POINTER(c_void_p), c_int, c_int, c_void_p,
+                                                             c_int, c_void_p]),
     ("monolis_hip_matrix_get_values_C", c_int, [c_void_p, c_void_p]),
     ("monolis_hip_matrix_add_elements_C", c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     ("monolis_hip_set_Dirichlet_bc_C", c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),

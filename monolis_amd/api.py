@@ -335,9 +335,10 @@ class DeviceMatrix:
     # device-side scalar access (monolis_{set,add,get}_scalar_*_sparse_matrix_R, 0-based)
     def values(self) -> np.ndarray:
         """The device values in monolis's A layout."""
-        A = np.zeros(max(self.mat.nz * self.mat.ndof * self.mat.ndof, 1), np.float64)
+        n = self.mat.nzndof2
+        A = np.zeros(max(n, 1), np.float64)
         _lib.check(_lib.load().monolis_hip_matrix_get_values_R(ctypes.c_void_p(self.h), _p(A)), "get_values")
-        return A[: self.mat.nz * self.mat.ndof * self.mat.ndof]
+        return A[:n]
 
     def add_elements(self, elems, Ke) -> None:
         """monolis_add_matrix_to_sparse_matrix_R for every element (rows of `elems`,
@@ -542,7 +543,7 @@ class DeviceMatrixC:
     # device assembly (monolis_add_matrix_to_sparse_matrix_C, monolis_set_Dirichlet_bc_C, 0-based)
     def values(self) -> np.ndarray:
         """The device values in monolis's A layout (complex128)."""
-        n = self.mat.nz * self.mat.ndof * self.mat.ndof
+        n = self.mat.nzndof2
         A = np.zeros(max(n, 1), np.complex128)
         _lib.check(_lib.load().monolis_hip_matrix_get_values_C(ctypes.c_void_p(self.h), _p(A)), "get_values_C")
         return A[:n]
@@ -564,6 +565,28 @@ class DeviceMatrixC:
         _lib.check(_lib.load().monolis_hip_set_Dirichlet_bc_C(ctypes.c_void_p(self.h), _p(b), int(len(nd)), _p(nd),
                                                                _p(dd), _p(vv)), "set_Dirichlet_bc_C")
         return b
+
+
+def device_matrix_by_simple_mesh_V(n_node: int, n_dof_list, elems, complex_values: bool = False):
+    """monolis_get_nonzero_pattern_by_simple_mesh_V(_C) on the GPU: a DeviceMatrix
+    (DeviceMatrixC with complex_values) with the mesh's pattern, a block size per
+    node and zero values."""
+    from .matrix import nonzero_pattern_by_simple_mesh_V
+    L = _lib.load()
+    e = np.ascontiguousarray(elems, np.int32)
+    n_elem, n_base = (e.shape[0], e.shape[1]) if e.ndim == 2 else (0, 1)
+    lst = np.ascontiguousarray(n_dof_list, np.int32)
+    h = ctypes.c_void_p()
+    fn = L.monolis_hip_matrix_create_by_simple_mesh_V_C if complex_values else L.monolis_hip_matrix_create_by_simple_mesh_V
+    _lib.check(fn(ctypes.byref(h), int(n_node), int(n_base), _p(lst), int(n_elem), _p(e)), "create_by_simple_mesh_V")
+    mat = nonzero_pattern_by_simple_mesh_V(n_node, lst, e)   # host restatement: index, item, dof arrays
+    if complex_values:
+        mat.A = mat.A.astype(np.complex128)
+        dm = DeviceMatrixC.__new__(DeviceMatrixC)
+    else:
+        dm = DeviceMatrix.__new__(DeviceMatrix)
+    dm.h, dm.mat, dm.m, dm.vec = h.value, mat, mat.nndof, mat.npndof
+    return dm
 
 
 def device_matrix_by_simple_mesh_C(n_node: int, n_dof: int, elems) -> "DeviceMatrixC":

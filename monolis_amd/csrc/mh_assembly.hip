@@ -256,6 +256,120 @@ __global__ void k_sell_gather(int n, int b, const int* __restrict__ slice_ptr, c
       A[((size_t)j0 + k) * bb + e] = val[sell_vpos<V>((size_t)k0 + k, e, bb, lane)];
 }
 
+// ---- variable block sizes (monolis n_dof = -1): values in monolis's layout,
+// block (i, j) at index2[e] with n_i x n_j row-major components ----------------
+// Element e's matrix is (sum_a n(elem[e][a]))^2 row-major; sz[e] is its size.
+__global__ void k_v_elem_sizes(int ne, int n_base, const int* __restrict__ elem, const int* __restrict__ list,
+                               long long* __restrict__ sz) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  long long d = 0;
+  for (int a = 0; a < n_base; ++a) d += list[elem[(size_t)e * n_base + a]];
+  sz[e] = d * d;
+}
+
+// Block row of block position blk (bisection over the block CSR offsets).
+__device__ __forceinline__ int block_row(int blk, int n, const int* __restrict__ index) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = lo + (hi - lo) / 2;
+    if (index[mid + 1] > blk) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// k_elem_accumulate for variable blocks: thread (i, component k1*maxb + k2) at
+// the head i of a run of equal block keys adds that component of every pair in
+// the run, in order, onto monolis's A (entry (k1, k2) of block (r, c) at
+// index2[blk] + n_c*k1 + k2; element entry (off_a + k1, off_c + k2), off_a the
+// sum of the block sizes of the element's nodes before a).
+template <typename V>
+__global__ void k_v_elem_accumulate(long long m, const unsigned long long* __restrict__ keys,
+                                    const int* __restrict__ ids, const V* __restrict__ Ke,
+                                    const long long* __restrict__ koff, const int* __restrict__ elem, int n_base,
+                                    int maxb, int n, const int* __restrict__ list, const int* __restrict__ index,
+                                    const int* __restrict__ item, const int* __restrict__ index2, V* __restrict__ A,
+                                    unsigned long long none) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int mm = maxb * maxb;
+  if (t >= m * mm) return;
+  const long long i = t / mm;
+  const int comp = (int)(t % mm);
+  const unsigned long long key = keys[i];
+  if (key == none || (i > 0 && keys[i - 1] == key)) return;
+  const int blk = (int)key;
+  const int r = block_row(blk, n, index), c = item[blk] - 1;
+  const int n1 = list[r], n2 = list[c];
+  const int k1 = comp / maxb, k2 = comp % maxb;
+  if (k1 >= n1 || k2 >= n2) return;
+  const long long pos = (long long)index2[blk] + (long long)n2 * k1 + k2;
+  const long long per = (long long)n_base * n_base;
+  V acc = A[pos];
+  for (long long j = i; j < m && keys[j] == key; ++j) {
+    const long long pid = ids[j];
+    const long long e = pid / per;
+    const int q = (int)(pid % per);
+    const int a = q / n_base, cc = q % n_base;
+    long long S = 0, ro = 0, co = 0;
+    for (int u = 0; u < n_base; ++u) {
+      const int d = list[elem[e * n_base + u]];
+      if (u < a) ro += d;
+      if (u < cc) co += d;
+      S += d;
+    }
+    acc = acc + Ke[koff[e] + (ro + k1) * S + co + k2];
+  }
+  A[pos] = acc;
+}
+
+template <typename V>
+__global__ void k_v_bc_mark(int nbc, const int* __restrict__ node, const int* __restrict__ dof,
+                            const V* __restrict__ v, const int* __restrict__ dof_index, const int* __restrict__ list,
+                            int* __restrict__ isbc, V* __restrict__ bcval, int* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbc) return;
+  if (dof[i] >= list[node[i]]) {  // monolis_stop_by_submatrix_access
+    atomicOr(flags + 2, 1);
+    return;
+  }
+  const int d = dof_index[node[i]] + dof[i];
+  if (atomicAdd(&isbc[d], 1) != 0) atomicOr(flags + 1, 1);
+  bcval[d] = v[i];
+}
+
+// k_bc_apply on monolis's variable-block layout (one thread per block row).
+template <typename V>
+__global__ void k_v_bc_apply(int n, const int* __restrict__ list, const int* __restrict__ dof_index,
+                             const int* __restrict__ index, const int* __restrict__ item,
+                             const int* __restrict__ index2, V* __restrict__ A, const int* __restrict__ isbc,
+                             const V* __restrict__ bcval, V* __restrict__ rhs) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int n1 = list[r];
+  for (int k1 = 0; k1 < n1; ++k1) {
+    const int d = dof_index[r] + k1;
+    const bool rowbc = isbc[d] != 0;
+    V acc = rhs[d];
+    for (int e = index[r]; e < index[r + 1]; ++e) {
+      const int c = item[e] - 1;
+      const int n2 = list[c];
+      for (int k2 = 0; k2 < n2; ++k2) {
+        const int dc = dof_index[c] + k2;
+        const bool colbc = isbc[dc] != 0;
+        const long long pos = (long long)index2[e] + (long long)n2 * k1 + k2;
+        if (colbc && !rowbc) {
+          const V t = bc_prod(bcval[dc], A[pos]);
+          acc = acc - t;
+        }
+        if (rowbc || colbc) A[pos] = V((rowbc && c == r && k1 == k2) ? 1.0 : 0.0);
+      }
+    }
+    rhs[d] = rowbc ? bcval[d] : acc;
+  }
+}
+
 int require_assemblable(const Matrix& M, const char* what, bool cplx) {
   if (M.vb || M.is_complex != cplx) {
     set_error("%s: %s fixed-block matrices only", what, cplx ? "complex" : "real");
@@ -272,6 +386,33 @@ void mark_values_changed(Matrix* M) {
   M->diag_ready = M->diag32_ready = M->diagz_ready = false;
   M->val32_ready = false;  // the FP32 copy is rebuilt on its next use
   M->transpose_stale = true;
+}
+
+template <typename V>
+DevArray<V>& vb_values(VBlock& Vb);
+template <>
+DevArray<double>& vb_values<double>(VBlock& Vb) {
+  return Vb.A64;
+}
+template <>
+DevArray<zd>& vb_values<zd>(VBlock& Vb) {
+  return Vb.Az;
+}
+
+// Re-expands a variable-block matrix from its kept monolis-layout values.
+int vb_reexpand(Matrix* M, const double* A, hipStream_t s) { return matrix_set_values_vblock(M, A, s); }
+int vb_reexpand(Matrix* M, const zd* A, hipStream_t s) { return matrix_set_values_vblock_complex(M, A, s); }
+
+int require_vblock(const Matrix& M, const char* what, bool cplx) {
+  if (!M.vb || M.is_complex != cplx) {
+    set_error("%s: %s matrices only", what, cplx ? "complex" : "real");
+    return -2;
+  }
+  if (M.vb->npb != M.vb->nb) {
+    set_error("%s: square local matrices only (n == np)", what);
+    return -2;
+  }
+  return 0;
 }
 
 template <typename V>
@@ -428,10 +569,110 @@ static int add_elements_T(Matrix* M, int n_elem, int n_base, const int* elem, co
   return 0;
 }
 
+// Element assembly on a variable-block matrix: monolis's layout values kept in
+// the VBlock are updated (sorted node pairs, sums in element order, as for fixed
+// blocks), then the scalar expansion and the diag_V factors are rebuilt.
+template <typename V>
+static int add_elements_V_T(Matrix* M, int n_elem, int n_base, const int* elem, const V* Ke, hipStream_t s) {
+  MH_TRY(require_vblock(*M, "element assembly", sizeof(V) == sizeof(zd)));
+  VBlock& Vb = *M->vb;
+  if (n_elem < 0 || n_base < 1) {
+    set_error("invalid element sizes n_elem=%d n_base=%d", n_elem, n_base);
+    return -2;
+  }
+  if (n_elem == 0 || Vb.nb == 0) return 0;
+  const int n = Vb.nb;
+  const long long pairs_elem = (long long)n_base * n_base;
+  long long max_pairs = 1LL << 24;
+  if (const char* env = getenv("MONOLIS_HIP_ASSEMBLY_CHUNK_PAIRS")) {
+    const long long v = atoll(env);
+    if (v > 0) max_pairs = v;
+  }
+  const int chunk = (int)std::max<long long>(1, std::min<long long>(n_elem, max_pairs / pairs_elem));
+  DevArray<int> etmp;
+  const int* de = nullptr;
+  MH_TRY(stage(elem, (size_t)n_elem * n_base, etmp, &de, s));
+  DevArray<int> flag;
+  MH_TRY(flag.alloc(1));
+  MH_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+  k_check_range<<<std::min(grid_for((long long)n_elem * n_base), 65536), kBlock, 0, s>>>((long long)n_elem * n_base,
+                                                                                        de, n, flag.p);
+  MH_HIP(hipGetLastError());
+  int bad = 0;
+  MH_TRY(read_flag(flag.p, &bad, s));
+  if (bad) {
+    set_error("element connectivity holds a node id outside 0..%d", n - 1);
+    return -2;
+  }
+  // element offsets into Ke: exclusive sum of (sum of the nodes' block sizes)^2
+  DevArray<long long> sz, koff;
+  MH_TRY(sz.alloc((size_t)n_elem + 1));
+  MH_TRY(koff.alloc((size_t)n_elem + 1));
+  MH_HIP(hipMemsetAsync(sz.p + n_elem, 0, sizeof(long long), s));
+  k_v_elem_sizes<<<grid_for(n_elem), kBlock, 0, s>>>(n_elem, n_base, de, Vb.list.p, sz.p);
+  MH_HIP(hipGetLastError());
+  size_t scan_bytes = 0;
+  MH_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, sz.p, koff.p, n_elem + 1, s));
+  DevArray<unsigned char> scan_tmp;
+  MH_TRY(scan_tmp.alloc(scan_bytes ? scan_bytes : 1));
+  MH_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, sz.p, koff.p, n_elem + 1, s));
+  long long total = 0;
+  MH_HIP(hipMemcpyAsync(&total, koff.p + n_elem, sizeof(long long), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  DevArray<V> kt;
+  const V* dk = Ke;
+  if (!is_device_pointer(Ke)) {
+    MH_TRY(kt.alloc((size_t)total + 1));
+    MH_TRY(upload(kt.p, Ke, (size_t)total * sizeof(V), s));
+    dk = kt.p;
+  }
+  const long long cap = (long long)chunk * pairs_elem;
+  DevArray<unsigned long long> k0, k1;
+  DevArray<int> i0, i1;
+  MH_TRY(k0.alloc((size_t)cap));
+  MH_TRY(k1.alloc((size_t)cap));
+  MH_TRY(i0.alloc((size_t)cap));
+  MH_TRY(i1.alloc((size_t)cap));
+  const int end_bit = bits_for((unsigned long long)Vb.nzb) + 1;
+  const unsigned long long none = 1ull << (end_bit - 1);
+  size_t tmp_bytes = 0;
+  MH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0.p, k1.p, i0.p, i1.p, (int)cap, 0, end_bit, s));
+  DevArray<unsigned char> tmp;
+  MH_TRY(tmp.alloc(tmp_bytes ? tmp_bytes : 1));
+  V* A = vb_values<V>(Vb).p;
+  const int mm = Vb.maxb * Vb.maxb;
+  for (int e0 = 0; e0 < n_elem; e0 += chunk) {
+    const int ne = std::min(chunk, n_elem - e0);
+    const long long m = (long long)ne * pairs_elem;
+    k_elem_targets<<<grid_for(m), kBlock, 0, s>>>(ne, n_base, de + (size_t)e0 * n_base, Vb.index.p, Vb.item.p, k0.p,
+                                                   i0.p, flag.p, none);
+    MH_HIP(hipGetLastError());
+    MH_TRY(read_flag(flag.p, &bad, s));
+    if (bad) {
+      set_error("element %d..%d couples nodes whose block is not in the nonzero pattern", e0, e0 + ne - 1);
+      MH_TRY(vb_reexpand(M, A, s));  // earlier chunks stay added, as monolis stops there
+      mark_values_changed(M);
+      return -2;
+    }
+    size_t tb = tmp_bytes;
+    MH_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, k0.p, k1.p, i0.p, i1.p, (int)m, 0, end_bit, s));
+    k_v_elem_accumulate<V><<<grid_for(m * mm), kBlock, 0, s>>>(m, k1.p, i1.p, dk, koff.p + e0,
+                                                                de + (size_t)e0 * n_base, n_base, Vb.maxb, n,
+                                                                Vb.list.p, Vb.index.p, Vb.item.p, Vb.index2.p, A,
+                                                                none);
+    MH_HIP(hipGetLastError());
+  }
+  MH_TRY(vb_reexpand(M, A, s));
+  mark_values_changed(M);
+  return 0;
+}
+
 int matrix_add_elements(Matrix* M, int n_elem, int n_base, const int* elem, const double* Ke, hipStream_t s) {
+  if (M->vb) return add_elements_V_T<double>(M, n_elem, n_base, elem, Ke, s);
   return add_elements_T<double>(M, n_elem, n_base, elem, Ke, s);
 }
 int matrix_add_elements_C(Matrix* M, int n_elem, int n_base, const int* elem, const zd* Ke, hipStream_t s) {
+  if (M->vb) return add_elements_V_T<zd>(M, n_elem, n_base, elem, Ke, s);
   return add_elements_T<zd>(M, n_elem, n_base, elem, Ke, s);
 }
 
@@ -503,16 +744,99 @@ static int dirichlet_T(Matrix* M, V* rhs, int n_bc, const int* node, const int* 
   return 0;
 }
 
+// Dirichlet conditions on a variable-block matrix (monolis's layout values kept
+// in the VBlock, then the expansion and the diag_V factors are rebuilt).
+template <typename V>
+static int dirichlet_V_T(Matrix* M, V* rhs, int n_bc, const int* node, const int* dof, const V* val, hipStream_t s) {
+  MH_TRY(require_vblock(*M, "Dirichlet conditions", sizeof(V) == sizeof(zd)));
+  VBlock& Vb = *M->vb;
+  if (n_bc < 0) {
+    set_error("negative number of conditions");
+    return -2;
+  }
+  if (n_bc == 0 || Vb.nb == 0) return 0;
+  DevArray<int> nt, dt;
+  DevArray<V> vt;
+  const int *dn = nullptr, *dd = nullptr;
+  const V* dv = nullptr;
+  MH_TRY(stage(node, (size_t)n_bc, nt, &dn, s));
+  MH_TRY(stage(dof, (size_t)n_bc, dt, &dd, s));
+  MH_TRY(stage(val, (size_t)n_bc, vt, &dv, s));
+  DevArray<int> flag;
+  MH_TRY(flag.alloc(3));
+  MH_HIP(hipMemsetAsync(flag.p, 0, 3 * sizeof(int), s));
+  k_check_range<<<std::min(grid_for(n_bc), 65536), kBlock, 0, s>>>(n_bc, dn, Vb.nb, flag.p);
+  k_check_range<<<std::min(grid_for(n_bc), 65536), kBlock, 0, s>>>(n_bc, dd, Vb.maxb, flag.p + 1);
+  MH_HIP(hipGetLastError());
+  int bad[3] = {0, 0, 0};
+  MH_HIP(hipMemcpyAsync(bad, flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  if (bad[0] || bad[1]) {
+    set_error("Dirichlet condition outside the matrix (%s)", bad[0] ? "node id" : "DOF index");
+    return -2;
+  }
+  MH_HIP(hipMemsetAsync(flag.p, 0, 3 * sizeof(int), s));
+  k_bc_check_diag<<<grid_for(n_bc), kBlock, 0, s>>>(n_bc, dn, Vb.index.p, Vb.item.p, flag.p);
+  MH_HIP(hipGetLastError());
+  const size_t m = (size_t)M->n;  // scalar rows = DOFs
+  DevArray<int> isbc;
+  DevArray<V> bcval;
+  MH_TRY(isbc.alloc(m));
+  MH_TRY(bcval.alloc(m));
+  MH_HIP(hipMemsetAsync(isbc.p, 0, m * sizeof(int), s));
+  k_v_bc_mark<V><<<grid_for(n_bc), kBlock, 0, s>>>(n_bc, dn, dd, dv, Vb.dof_index.p, Vb.list.p, isbc.p, bcval.p,
+                                                   flag.p);
+  MH_HIP(hipGetLastError());
+  MH_HIP(hipMemcpyAsync(bad, flag.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  if (bad[0]) {
+    set_error("Dirichlet condition on a node without a diagonal block");
+    return -2;
+  }
+  if (bad[2]) {  // monolis_stop_by_submatrix_access
+    set_error("Dirichlet condition on a DOF beyond its node's block size");
+    return -2;
+  }
+  if (bad[1]) {
+    set_error("the same DOF is listed twice among the Dirichlet conditions");
+    return -2;
+  }
+  DevArray<V> rtmp;
+  V* dr = rhs;
+  const bool rhs_dev = is_device_pointer(rhs);
+  if (!rhs_dev) {
+    MH_TRY(rtmp.alloc(m));
+    MH_TRY(upload(rtmp.p, rhs, m * sizeof(V), s));
+    dr = rtmp.p;
+  }
+  V* A = vb_values<V>(Vb).p;
+  k_v_bc_apply<V><<<grid_for(Vb.nb), kBlock, 0, s>>>(Vb.nb, Vb.list.p, Vb.dof_index.p, Vb.index.p, Vb.item.p,
+                                                     Vb.index2.p, A, isbc.p, bcval.p, dr);
+  MH_HIP(hipGetLastError());
+  if (!rhs_dev) MH_HIP(hipMemcpyAsync(rhs, dr, m * sizeof(V), hipMemcpyDeviceToHost, s));
+  MH_TRY(vb_reexpand(M, A, s));
+  mark_values_changed(M);
+  return 0;
+}
+
 int matrix_dirichlet(Matrix* M, double* rhs, int n_bc, const int* node, const int* dof, const double* val,
                      hipStream_t s) {
+  if (M->vb) return dirichlet_V_T<double>(M, rhs, n_bc, node, dof, val, s);
   return dirichlet_T<double>(M, rhs, n_bc, node, dof, val, s);
 }
 int matrix_dirichlet_C(Matrix* M, zd* rhs, int n_bc, const int* node, const int* dof, const zd* val, hipStream_t s) {
+  if (M->vb) return dirichlet_V_T<zd>(M, rhs, n_bc, node, dof, val, s);
   return dirichlet_T<zd>(M, rhs, n_bc, node, dof, val, s);
 }
 
 template <typename V>
 static int get_values_T(const Matrix& M, V* A, hipStream_t s) {
+  if (M.vb && M.is_complex == (sizeof(V) == sizeof(zd))) {  // monolis's layout, kept in the VBlock
+    const size_t nv = (size_t)M.vb->nzs;
+    if (nv) MH_HIP(hipMemcpyAsync(A, vb_values<V>(*M.vb).p, nv * sizeof(V), hipMemcpyDefault, s));
+    MH_HIP(hipStreamSynchronize(s));
+    return 0;
+  }
   if (M.vb || M.is_complex != (sizeof(V) == sizeof(zd))) {
     set_error("value read-back: %s fixed-block matrices only", sizeof(V) == sizeof(zd) ? "complex" : "real");
     return -2;
@@ -541,6 +865,13 @@ int matrix_get_values(const Matrix& M, double* A, hipStream_t s) { return get_va
 int matrix_get_values_C(const Matrix& M, zd* A, hipStream_t s) { return get_values_T<zd>(M, A, s); }
 
 int matrix_get_pattern(const Matrix& M, int* index, int* item, hipStream_t s) {
+  if (M.vb) {  // the block pattern, not the scalar expansion's
+    const VBlock& V = *M.vb;
+    if (index) MH_HIP(hipMemcpyAsync(index, V.index.p, sizeof(int) * ((size_t)V.npb + 1), hipMemcpyDefault, s));
+    if (item && V.nzb > 0) MH_HIP(hipMemcpyAsync(item, V.item.p, sizeof(int) * (size_t)V.nzb, hipMemcpyDefault, s));
+    MH_HIP(hipStreamSynchronize(s));
+    return 0;
+  }
   if (index) MH_HIP(hipMemcpyAsync(index, M.rowptr.p, sizeof(int) * ((size_t)M.np + 1), hipMemcpyDefault, s));
   if (item && M.nz > 0) MH_HIP(hipMemcpyAsync(item, M.csr_item.p, sizeof(int) * (size_t)M.nz, hipMemcpyDefault, s));
   MH_HIP(hipStreamSynchronize(s));

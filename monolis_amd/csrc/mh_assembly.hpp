@@ -31,7 +31,8 @@ int matrix_dirichlet_C(Matrix* M, zd* b, int n_bc, const int* node, const int* d
 // A in monolis's layout (nz * b^2, row-major blocks, CSR order); host or device.
 int matrix_get_values(const Matrix& M, double* A, hipStream_t s);
 int matrix_get_values_C(const Matrix& M, zd* A, hipStream_t s);
-// index (np + 1) and item (nz, 1-based) as stored; host or device.
+// index (np + 1) and item (nz, 1-based) as stored (variable blocks: the block
+// pattern); host or device.
 int matrix_get_pattern(const Matrix& M, int* index, int* item, hipStream_t s);
 
 }  // namespace mh

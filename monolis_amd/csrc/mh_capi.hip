@@ -297,6 +297,53 @@ __global__ void __launch_bounds__(256) k_stream_read(long long nv, const V* __re
 
 }  // namespace
 
+// Pattern of a simple mesh with per-node block sizes (monolis_get_nonzero_pattern_by_simple_mesh_V,
+// src/matrix/spmat_nzpattern.f90:53-87): the same nodal graph; n_dof_index / n_dof_index2
+// follow from the block sizes; values start at zero.
+template <typename V>
+int create_by_simple_mesh_V(mh::Matrix** out, int n_node, int n_base, const int* n_dof_list, int n_elem,
+                            const int* elem, hipStream_t s) {
+  *out = nullptr;
+  if (n_node < 0 || !n_dof_list) {
+    mh::set_error("invalid node count or null n_dof_list");
+    return ST_ARGS;
+  }
+  mh::DevArray<int> index, item;
+  int nz = 0;
+  MH_TRY(mh::pattern_by_simple_mesh(n_node, n_base, n_elem, elem, index, item, &nz, s));
+  std::vector<int> idx((size_t)n_node + 1), itm((size_t)nz + 1), list((size_t)n_node + 1);
+  MH_HIP(hipMemcpyAsync(idx.data(), index.p, sizeof(int) * ((size_t)n_node + 1), hipMemcpyDeviceToHost, s));
+  if (nz) MH_HIP(hipMemcpyAsync(itm.data(), item.p, sizeof(int) * (size_t)nz, hipMemcpyDeviceToHost, s));
+  if (n_node) MH_HIP(hipMemcpyAsync(list.data(), n_dof_list, sizeof(int) * (size_t)n_node, hipMemcpyDefault, s));
+  MH_HIP(hipStreamSynchronize(s));
+  std::vector<int> dofi((size_t)n_node + 1, 0), idx2((size_t)nz + 1, 0);
+  for (int i = 0; i < n_node; ++i) {
+    if (list[i] < 1 || list[i] > mh::kMaxVBlock) {
+      mh::set_error("block size %d of node %d outside 1..%d", list[i], i, mh::kMaxVBlock);
+      return ST_ARGS;
+    }
+    dofi[i + 1] = dofi[i] + list[i];
+  }
+  long long acc = 0;
+  for (int i = 0; i < n_node; ++i)
+    for (int e = idx[i]; e < idx[i + 1]; ++e) {
+      idx2[e] = (int)acc;
+      acc += (long long)list[i] * list[itm[e] - 1];
+      if (acc >= (1LL << 31)) {
+        mh::set_error("variable-block values exceed 2^31 entries");
+        return ST_ARGS;
+      }
+    }
+  idx2[nz] = (int)acc;
+  std::vector<V> zeros((size_t)acc + 1, V(0.0));
+  if constexpr (sizeof(V) == sizeof(mh::zd))
+    return mh::matrix_create_vblock_complex(out, n_node, n_node, nz, list.data(), dofi.data(), idx2.data(),
+                                            idx.data(), itm.data(), zeros.data(), s);
+  else
+    return mh::matrix_create_vblock(out, n_node, n_node, nz, list.data(), dofi.data(), idx2.data(), idx.data(),
+                                    itm.data(), zeros.data(), s);
+}
+
 // Solver-context tuning of the SpMV variant: a variant's time inside a Krylov
 // loop (between the row kernels that rewrite its input and consume its output)
 // does not follow its back-to-back time (100^3 BCSR3, block-Jacobi CG: variant
@@ -1349,6 +1396,40 @@ int monolis_hip_matrix_create_by_simple_mesh(monolis_hip_matrix** out, int n_nod
   Matrix* M = nullptr;
   MH_TRY(mh::matrix_create(&M, n_node, n_node, nz, n_dof, index.p, item.p, zeros.p, c->stream));
   auto* H = new monolis_hip_matrix();
+  H->M = M;
+  *out = H;
+  return 0;
+}
+
+int monolis_hip_matrix_create_by_simple_mesh_V(monolis_hip_matrix** out, int n_node, int n_base,
+                                               const int* n_dof_list, int n_elem, const int* elem) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (!out) {
+    mh::set_error("null output handle");
+    return ST_ARGS;
+  }
+  *out = nullptr;
+  Matrix* M = nullptr;
+  MH_TRY(create_by_simple_mesh_V<double>(&M, n_node, n_base, n_dof_list, n_elem, elem, c->stream));
+  auto* H = new monolis_hip_matrix();
+  H->M = M;
+  *out = H;
+  return 0;
+}
+
+int monolis_hip_matrix_create_by_simple_mesh_V_C(monolis_hip_matrix_C** out, int n_node, int n_base,
+                                                 const int* n_dof_list, int n_elem, const int* elem) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (!out) {
+    mh::set_error("null output handle");
+    return ST_ARGS;
+  }
+  *out = nullptr;
+  Matrix* M = nullptr;
+  MH_TRY(create_by_simple_mesh_V<mh::zd>(&M, n_node, n_base, n_dof_list, n_elem, elem, c->stream));
+  auto* H = new monolis_hip_matrix_C();
   H->M = M;
   *out = H;
   return 0;

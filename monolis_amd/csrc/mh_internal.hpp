@@ -111,6 +111,8 @@ struct VBlock {
   DevArray<float> D32;
   bool d32_ready = false;
   DevArray<zd> Dz;                         // complex matrices: diag_V_C factors
+  DevArray<double> A64;                    // the values in monolis's layout (nzs), kept for device assembly
+  DevArray<zd> Az;                         // complex matrices: the same
   int zbad = -1;                           // first block row whose complex factor hit a zero pivot, or -1
 };
 

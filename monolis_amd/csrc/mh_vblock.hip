@@ -139,6 +139,17 @@ int to_host(std::vector<U>& dst, const U* src, size_t count, hipStream_t s) {
   return 0;
 }
 
+template <typename V>
+DevArray<V>& vkeep(VBlock& Vb);
+template <>
+DevArray<double>& vkeep<double>(VBlock& Vb) {
+  return Vb.A64;
+}
+template <>
+DevArray<zd>& vkeep<zd>(VBlock& Vb) {
+  return Vb.Az;
+}
+
 // Values of the expansion (device) from monolis's A (host or device), and the
 // diag_V factors from the same A.
 template <typename V>
@@ -151,6 +162,11 @@ int fill_vblock_values(Matrix* M, const V* A, hipStream_t s, DevArray<int>* item
     MH_TRY(tmp.alloc((size_t)Vb.nzs + 1));
     MH_TRY(upload(tmp.p, A, sizeof(V) * (size_t)Vb.nzs, s));
     dA = tmp.p;
+  }
+  {  // the monolis-layout copy that device assembly updates
+    DevArray<V>& keep = vkeep<V>(Vb);
+    if (keep.n < (size_t)Vb.nzs + 1) MH_TRY(keep.alloc((size_t)Vb.nzs + 1));
+    if (dA != keep.p && Vb.nzs > 0) MH_HIP(hipMemcpyAsync(keep.p, dA, sizeof(V) * (size_t)Vb.nzs, hipMemcpyDeviceToDevice, s));
   }
   DevArray<int> bad;
   if constexpr (cplx) {

@@ -292,3 +292,116 @@ def test_complex_assemble_condition_cocg_pipeline():
     x = dx.numpy()
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
     assert not x[bottom[0] * b:(bottom[-1] + 1) * b].any()
+
+
+# --- variable block sizes (monolis n_dof = -1) -------------------------------------------
+def _v_elements(elems, lst, seed, cplx=False, spd=False):
+    """Element matrices of a variable-block mesh: element e is (sum of its nodes'
+    block sizes)^2, row-major; returned as a list and as the concatenation."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for e in elems:
+        S = int(lst[e].sum())
+        R = rng.uniform(-1, 1, (S, S))
+        K = R @ R.T / S + 0.05 * np.eye(S) if spd else R
+        if cplx:
+            K = K + 0.3j * (R.T @ R / S if spd else rng.uniform(-1, 1, (S, S)))
+        out.append(K)
+    return out, np.concatenate([k.reshape(-1) for k in out])
+
+
+def _v_dense_sum(n_dofs, dofi, lst, elems, Ks, dtype):
+    D = np.zeros((n_dofs, n_dofs), dtype)
+    for e, K in zip(elems, Ks):  # in element order, entry by entry (np.add.at is unbuffered)
+        d = np.concatenate([np.arange(dofi[v], dofi[v] + lst[v]) for v in e])
+        np.add.at(D, (np.repeat(d, len(d)), np.tile(d, len(d))), K.reshape(-1))
+    return D
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_vblock_element_assembly_matches_sequential_dense_sum(cplx):
+    """monolis_add_matrix_to_sparse_matrix_R/_C on a variable-block matrix
+    (spmat_handler_util.f90:398-600 through n_dof_index / n_dof_index2): the device
+    values equal the dense element-by-element sums, bit for bit."""
+    n_node, elems = hex_mesh(4, 3, 3)
+    lst = np.random.default_rng(4).integers(1, 4, n_node).astype(np.int32)
+    dm = api.device_matrix_by_simple_mesh_V(n_node, lst, elems, complex_values=cplx)
+    index = np.zeros(n_node + 1, np.int32)
+    item = np.zeros(len(dm.mat.item), np.int32)
+    get = _lib.load().monolis_hip_matrix_get_pattern_C if cplx else _lib.load().monolis_hip_matrix_get_pattern
+    assert get(ctypes.c_void_p(dm.h), index.ctypes.data_as(ctypes.c_void_p), item.ctypes.data_as(ctypes.c_void_p)) == 0
+    assert np.array_equal(index, dm.mat.index) and np.array_equal(item, dm.mat.item)
+    Ks, flat = _v_elements(elems, lst, seed=3, cplx=cplx)
+    half = len(elems) // 2
+    cut = sum(k.size for k in Ks[:half])
+    dm.add_elements(elems[:half], flat[:cut])
+    dm.add_elements(elems[half:], flat[cut:])
+    dofi = dm.mat.n_dof_index
+    D = _v_dense_sum(int(dofi[-1]), dofi, lst, elems, Ks, np.complex128 if cplx else np.float64)
+    dm.mat.A = dm.values()
+    assert np.array_equal(dm.mat.dense(), D)
+    # the expanded device matrix multiplies as the dense sum
+    rng = np.random.default_rng(6)
+    x = rng.uniform(-1, 1, D.shape[0]) + (1j * rng.uniform(-1, 1, D.shape[0]) if cplx else 0)
+    dt = np.complex128 if cplx else np.float64
+    dx, dy = api.DeviceBuffer.from_numpy(x.astype(dt)), api.DeviceBuffer(D.shape[0], dt)
+    dm.matvec(dx, dy)
+    ref = D @ x
+    assert np.max(np.abs(dy.numpy() - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_vblock_uniform_sizes_equal_fixed_block_assembly_and_conditions():
+    """With every block size b the variable-block layout is the fixed-block one:
+    element sums and Dirichlet conditions bit-identical to the fixed-block device path."""
+    b = 2
+    n_node, elems = hex_mesh(5, 4, 3)
+    Ke = element_matrices(len(elems), 8 * b, seed=8, spd=False)
+    dv = api.device_matrix_by_simple_mesh_V(n_node, np.full(n_node, b, np.int32), elems)
+    dfx = api.device_matrix_by_simple_mesh(n_node, b, elems)
+    dv.add_elements(elems, Ke.reshape(-1))
+    dfx.add_elements(elems, Ke)
+    assert np.array_equal(dv.values(), dfx.values())
+    rng = np.random.default_rng(1)
+    f = rng.uniform(-1, 1, n_node * b)
+    nodes = np.sort(rng.choice(n_node, 20, replace=False)).astype(np.int32)
+    dofs = (nodes % b).astype(np.int32)
+    val = rng.uniform(-1, 1, len(nodes))
+    Bv, Bf = f.copy(), f.copy()
+    dv.set_dirichlet(Bv, nodes, dofs, val)
+    dfx.set_dirichlet(Bf, nodes, dofs, val)
+    assert np.array_equal(dv.values(), dfx.values()) and np.array_equal(Bv, Bf)
+
+
+def test_vblock_dirichlet_and_cg_solve_are_the_reduced_solve():
+    """Variable blocks: conditions on a list of (node, DOF), then the device CG (with
+    diag_V) on the conditioned system; its solution is that of the reduced system
+    (dense brute force), the condition DOFs hold their values."""
+    n_node, elems = hex_mesh(6)
+    lst = np.random.default_rng(12).integers(1, 4, n_node).astype(np.int32)
+    dm = api.device_matrix_by_simple_mesh_V(n_node, lst, elems)
+    Ks, flat = _v_elements(elems, lst, seed=13, spd=True)
+    dm.add_elements(elems, flat)
+    dofi = dm.mat.n_dof_index
+    m = int(dofi[-1])
+    A0 = _v_dense_sum(m, dofi, lst, elems, Ks, np.float64)
+    nodes = np.arange(0, n_node, 3, dtype=np.int32)
+    dofs = np.array([(v // 3) % lst[v] for v in nodes], np.int32)
+    rng = np.random.default_rng(14)
+    val = rng.uniform(-1, 1, len(nodes))
+    f = rng.uniform(-1, 1, m)
+    Bg = f.copy()
+    dm.set_dirichlet(Bg, nodes, dofs, val)
+    db, dx = api.DeviceBuffer.from_numpy(Bg), api.DeviceBuffer(m)
+    prm = api.Prm().quiet()
+    prm.Rarray[1] = 1e-10
+    dm.solve(db, dx, prm)
+    assert prm.Iarray[5] == 0
+    x = dx.numpy()
+    fixed = dofi[nodes] + dofs
+    free = np.setdiff1d(np.arange(m), fixed)
+    xr = np.linalg.solve(A0[np.ix_(free, free)], f[free] - A0[np.ix_(free, fixed)] @ val)
+    np.testing.assert_allclose(x[fixed], val, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(x[free], xr, rtol=1e-7, atol=1e-7)
+    with pytest.raises(_lib.MonolisHipError, match="block size"):
+        big = int(np.argmax(lst == 1))
+        dm.set_dirichlet(np.zeros(m), [big], [1], [0.0])
