@@ -140,8 +140,11 @@ def test_matvec_every_kernel_variant_is_bit_identical(gen):
 def test_autotune_solver_keeps_solution():
     """monolis_hip_matrix_autotune_solver (no counterpart in monolis): per-variant
     per-iteration times of a fixed-length block-Jacobi CG, one valid variant kept per
-    precision, and the solve after tuning is unchanged (the variants compute
-    bit-identical SpMVs): same iterations, bit-identical x, refined solve to tol."""
+    precision, and the solve after tuning agrees with the one before.  The variants'
+    SpMV products are bit-identical, but the (p, q) partial sums come one per
+    workgroup and the variants' grids differ, so the dots (and x) may differ in the
+    last bits: the iteration count within the usual tolerance, x to 1e-8, and the
+    refined solve to tol."""
     from monolis_amd import _lib
     m = W.stencil27_bcsr3(12, seed=6)
     b = oracle.matvec(m, W.uniform_vector(m.nndof, seed=7))
@@ -160,8 +163,8 @@ def test_autotune_solver_keeps_solution():
     x1 = api.DeviceBuffer(m.npndof)
     p1 = api.Prm().quiet()
     dm.solve(db, x1, p1)
-    assert int(p1.Iarray[4]) == int(p0.Iarray[4]) and p1.Iarray[4] > 0
-    assert np.array_equal(x1.numpy(), x0.numpy())
+    assert p1.Iarray[4] > 0 and _iters_close(int(p1.Iarray[4]), int(p0.Iarray[4]))
+    assert np.linalg.norm(x1.numpy() - x0.numpy()) <= 1e-8 * np.linalg.norm(x0.numpy())
     xr = api.DeviceBuffer(m.npndof)
     pr = api.Prm().quiet()
     dm.solve_refined(db, xr, pr, inner_tol=1e-4)
