@@ -217,14 +217,27 @@ int monolis_hip_batch_create(monolis_hip_batch** out, int nsys, const int* sys_r
                              const int* index, const int* item, const double* A);
 int monolis_hip_batch_destroy(monolis_hip_batch* B);
 /* b, x: device vectors of length n.  iters / resid / status: host arrays of
- * nsys entries (status 0 converged, 1 not converged, 2 zero right-hand side). */
+ * nsys entries (status 0 converged, 1 not converged, 2 zero right-hand side,
+ * 3 abandoned: a team of workgroups gave up waiting for a member, in which case
+ * the call returns an error). */
 int monolis_hip_batch_solve(monolis_hip_batch* B, const double* b, double* x, int* Iarray, double* Rarray,
                             int* iters, double* resid, int* status);
-/* Autotuner of the persistent launch: workgroup size x slots per step.  Runs
- * the solve `reps` times per configuration, keeps the fastest; ms (if not
- * NULL) gets monolis_hip_batch_num_configs() average times. */
+/* Autotuner of the persistent launch: workgroup size x slots per step x team
+ * split.  Runs the solve `reps` times per configuration, keeps the fastest; ms
+ * (if not NULL) gets monolis_hip_batch_num_configs() average times. */
 int monolis_hip_batch_num_configs(void);
 int monolis_hip_batch_config(int cfg, int* threads, int* unroll);
+/* Selects configuration cfg for the following solves (the autotuner's choice
+ * otherwise; 0 before any tuning). */
+int monolis_hip_batch_set_config(monolis_hip_batch* B, int cfg);
+/* Team split of configuration cfg (0: one workgroup per system; s > 0: systems
+ * whose estimated work exceeds s/4 of one workgroup's balanced share are solved
+ * by a team of up to 8 workgroups) and the number of slots (one per workgroup
+ * task) and teams its schedule for B has. */
+int monolis_hip_batch_schedule(monolis_hip_batch* B, int cfg, int* split, int* nslot, int* nteam);
+/* Longest time in seconds (default 5, at most 600) that one team barrier waits
+ * for a member before the launch gives up on the system (status 3). */
+int monolis_hip_batch_set_wait_limit(monolis_hip_batch* B, double seconds);
 int monolis_hip_batch_autotune(monolis_hip_batch* B, const double* b, double* x, int* Iarray, double* Rarray,
                                int reps, int* best, double* ms);
 /* Average time of `reps` graph replays of the batched solve (ms). */

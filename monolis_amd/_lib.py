@@ -92,6 +92,9 @@ SIGNATURES = [
                                         c_void_p]),
     ("monolis_hip_batch_num_configs", c_int, []),
     ("monolis_hip_batch_config", c_int, [c_int, c_int_p, c_int_p]),
+    ("monolis_hip_batch_set_config", c_int, [c_void_p, c_int]),
+    ("monolis_hip_batch_schedule", c_int, [c_void_p, c_int, c_int_p, c_int_p, c_int_p]),
+    ("monolis_hip_batch_set_wait_limit", c_int, [c_void_p, c_double]),
     ("monolis_hip_batch_autotune", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int_p,
                                            c_double_p]),
     ("monolis_hip_batch_time", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_double_p]),

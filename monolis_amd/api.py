@@ -671,8 +671,22 @@ class DeviceBatch:
         for c in range(nc):
             t, u = ctypes.c_int(), ctypes.c_int()
             L.monolis_hip_batch_config(c, ctypes.byref(t), ctypes.byref(u))
-            cfgs.append({"threads": t.value, "unroll": u.value})
+            cfgs.append({"threads": t.value, "unroll": u.value, **self.schedule(c)})
         return {"best": best.value, "ms": list(ms), "configs": cfgs}
+
+    def set_config(self, cfg: int):
+        _lib.check(_lib.load().monolis_hip_batch_set_config(ctypes.c_void_p(self.h), int(cfg)), "batch_set_config")
+
+    def schedule(self, cfg: int) -> dict:
+        """Team split of configuration cfg and its schedule's slot / team counts."""
+        sp, ns, nt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().monolis_hip_batch_schedule(ctypes.c_void_p(self.h), int(cfg), ctypes.byref(sp),
+                                                          ctypes.byref(ns), ctypes.byref(nt)), "batch_schedule")
+        return {"split": sp.value, "slots": ns.value, "teams": nt.value}
+
+    def set_wait_limit(self, seconds: float):
+        _lib.check(_lib.load().monolis_hip_batch_set_wait_limit(ctypes.c_void_p(self.h), float(seconds)),
+                   "batch_set_wait_limit")
 
     def time(self, b: DeviceBuffer, x: DeviceBuffer, prm: Prm, reps: int = 5) -> float:
         ms = ctypes.c_double()

@@ -8,11 +8,14 @@
 // monolis_solve_R were called once per system.
 //
 // Layout: a system-aligned SELL-64 -- every system starts on a fresh 64-row
-// slice, so a workgroup walks only its own slices.  One persistent kernel: each
-// workgroup takes systems wg, wg + G, ... (largest first) and runs the whole CG
-// loop with workgroup barriers only; there is no cross-workgroup waiting, so
-// every workgroup exits after its last system.  The launch (init kernel +
-// solver kernel) is captured once in a hipGraph and replayed.
+// slice, so a workgroup walks only its own slices.  Two persistent kernels:
+//  * k_batch_cg: each workgroup takes systems wg, wg + G, ... (largest first) and
+//    runs the whole CG loop with workgroup barriers only;
+//  * k_batch_sched: the largest systems are split over teams of up to 8
+//    workgroups (contiguous slice ranges, team barriers in global memory with a
+//    bounded wait), so that one system no longer sets the critical path.
+// The launch (init kernel + solver kernel) is captured once in a hipGraph and
+// replayed.
 #include <algorithm>
 #include <memory>
 #include <cmath>
@@ -27,6 +30,8 @@ namespace mh {
 namespace {
 
 constexpr int kIterRR = 200;
+constexpr int kTeamMax = 8;         // workgroups per team at most
+constexpr int kSpinMax = 1 << 24;   // polls of one team wait at most (a bound that needs no clock)
 
 // workgroup sum over NT threads (fixed order), result returned to every thread
 template <int NT, int NV>
@@ -66,11 +71,12 @@ struct BatchView {
 struct BatchOut {
   int* iters;
   double* resid;
-  int* status;   // 0 converged, 1 not converged, 2 zero rhs
+  int* status;   // 0 converged, 1 not converged, 2 zero rhs, 3 a team wait gave up
 };
 
-template <int U>
-__device__ __forceinline__ double sell_row_scalar(const BatchView& V, int s, int lane, int len, const double* x) {
+// One SELL row times a vector whose entry c is gather(c).
+template <int U, class G>
+__device__ __forceinline__ double sell_row_g(const BatchView& V, int s, int lane, int len, G&& gather) {
   SellView<double> M;
   M.nrows = 0x7fffffff;
   M.nslices = 0;
@@ -80,23 +86,27 @@ __device__ __forceinline__ double sell_row_scalar(const BatchView& V, int s, int
   M.val = V.val;
   M.col16 = nullptr;
   double acc[1] = {0.0};
-  auto xg = [&](int c, double(&xv)[1]) { xv[0] = x[c]; };
+  auto xg = [&](int c, double(&xv)[1]) { xv[0] = gather(c); };
   sell_row<double, double, 1, U, true>(M, s, lane, len, xg, acc);
   return acc[0];
 }
 
-// The persistent solver.  NT threads per workgroup; U slots per step.
+template <int U>
+__device__ __forceinline__ double sell_row_scalar(const BatchView& V, int s, int lane, int len, const double* x) {
+  return sell_row_g<U>(V, s, lane, len, [&](int c) { return x[c]; });
+}
+
+
+// One system solved by one workgroup (NT threads, U slots per step).
 template <int NT, int U>
-__global__ void __launch_bounds__(NT) k_batch_cg(BatchView V, const double* __restrict__ b, double* __restrict__ x,
-                                                 double* __restrict__ r, double* __restrict__ z,
-                                                 double* __restrict__ p, double* __restrict__ q, BatchOut out,
-                                                 int max_iter, double tol, int init_x, int precond) {
-  __shared__ double red[NT / kWave * 2];
+__device__ __forceinline__ void cg_solo(const BatchView& V, int sys, const double* __restrict__ b,
+                                        double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                        double* __restrict__ p, double* __restrict__ q, const BatchOut& out,
+                                        int max_iter, double tol, int init_x, int precond, double* red) {
   constexpr int NW = NT / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   const int w = threadIdx.x / kWave;
-  for (int k = blockIdx.x; k < V.nsys; k += gridDim.x) {
-    const int sys = V.order[k];
+  {
     const int s0 = V.sys_slice[sys], s1 = V.sys_slice[sys + 1];
     const int row0 = V.sys_row[sys], nrow = V.sys_row[sys + 1] - row0;
     // rows of this system: local row l = (slice - s0) * 64 + lane; global row0 + l
@@ -131,7 +141,7 @@ __global__ void __launch_bounds__(NT) k_batch_cg(BatchView V, const double* __re
         out.status[sys] = 2;
       }
       __syncthreads();
-      continue;
+      return;
     }
     // z = M^-1 r ; rho = (r, z) ; p = 0
     double rho = 0.0;
@@ -221,13 +231,271 @@ __global__ void __launch_bounds__(NT) k_batch_cg(BatchView V, const double* __re
   }
 }
 
-__global__ void k_batch_reset(int nsys, int* iters, double* resid, int* status) {
+// The persistent solver, one workgroup per system.
+template <int NT, int U>
+__global__ void __launch_bounds__(NT) k_batch_cg(BatchView V, const double* __restrict__ b, double* __restrict__ x,
+                                                 double* __restrict__ r, double* __restrict__ z,
+                                                 double* __restrict__ p, double* __restrict__ q, BatchOut out,
+                                                 int max_iter, double tol, int init_x, int precond) {
+  __shared__ double red[NT / kWave * 2];
+  for (int k = blockIdx.x; k < V.nsys; k += gridDim.x)
+    cg_solo<NT, U>(V, V.order[k], b, x, r, z, p, q, out, max_iter, tol, init_x, precond, red);
+}
+
+// Team schedule.  Workgroups take slots in order from one counter; the slots of
+// a team are consecutive, so at most one team is incomplete at any time and a
+// member waiting for a teammate that has not started waits only for running
+// workgroups to finish their own complete teams -- nothing assumes that the
+// whole grid is resident.  Every wait is bounded (wall clock and poll count):
+// one that gives up raises the abort flag, after which every team wait returns
+// at once and the systems concerned report status 3.
+struct TeamView {
+  int nslot;
+  const int* slot;     // 4 ints per slot: system, rank in its team, team size, team index
+  int* ctl;            // [0] next slot, [1] abort flag, [2 + t] arrivals at team t's barriers
+  double* part;        // per team: 2 alternating sets x kTeamMax members x 2 partial sums
+  long long limit;     // wall-clock ticks one wait may last
+};
+
+// Team barrier: arrival `target / T` of a team of T workgroups.  After the
+// workgroup barrier one thread releases the workgroup's stores at device scope
+// (they precede it through the barrier), counts the arrival, polls the counter
+// and acquires, then the workgroup barrier again -- one cache write-back and one
+// invalidation per member and barrier, so loads after it see what the other
+// members stored before it, whichever XCD they run on.
+__device__ __forceinline__ bool team_wait(int* cnt, int target, int* abort_flag, long long limit, int* flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    const long long t0 = (long long)wall_clock64();
+    for (int k = 0;; ++k) {
+      if (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || k >= kSpinMax ||
+          (long long)wall_clock64() - t0 > limit) {
+        __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *flag = ok;
+  }
+  __syncthreads();
+  return *flag != 0;
+}
+
+// Member `rank` of a team of T workgroups solving one system: it owns slices
+// [s0 + ns*rank/T, s0 + ns*(rank+1)/T) and their rows; the SpMV reads the
+// other members' p (and x) after a team barrier; each dot product is a
+// workgroup sum per member, then the T member sums added in rank order (the
+// same order in every member, so all members take the same branches).
+template <int NT, int U>
+__device__ __forceinline__ void cg_team(const BatchView& V, const TeamView& S, int sys, int rank, int T, int team,
+                                        const double* __restrict__ b, double* __restrict__ x,
+                                        double* __restrict__ r, double* __restrict__ z, double* __restrict__ p,
+                                        double* __restrict__ p2, double* __restrict__ q, const BatchOut& out,
+                                        int max_iter, double tol, int init_x, int precond, double* red, int* flag) {
+  constexpr int NW = NT / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const int s0 = V.sys_slice[sys], ns = V.sys_slice[sys + 1] - s0;
+  const int a0 = s0 + (int)((long long)ns * rank / T), a1 = s0 + (int)((long long)ns * (rank + 1) / T);
+  const int row0 = V.sys_row[sys], nrow = V.sys_row[sys + 1] - row0;
+  const int m0 = (a0 - s0) * kWave, m1 = min((a1 - s0) * kWave, nrow);  // this member's local rows
+  int* cnt = S.ctl + 2 + team;
+  double* part = S.part + (size_t)team * (2 * kTeamMax * 2);
+  int nbar = 0, nred = 0;
+  auto wait = [&]() -> bool {
+    ++nbar;
+    return team_wait(cnt, nbar * T, S.ctl + 1, S.limit, flag);
+  };
+  // team-wide sums; consecutive reductions alternate between two partial sets,
+  // and a set is rewritten only after the next reduction's barrier, which every
+  // member passes after reading it
+  auto tsum = [&](double(&v)[2]) -> bool {
+    wg_sum<NT, 2>(v, red);
+    double* ps = part + (nred & 1) * (kTeamMax * 2);
+    ++nred;
+    if (threadIdx.x == 0) {
+      ps[2 * rank] = v[0];
+      ps[2 * rank + 1] = v[1];
+    }
+    if (!wait()) return false;
+    double t0 = 0.0, t1 = 0.0;
+    for (int m = 0; m < T; ++m) {
+      t0 += ps[2 * m];
+      t1 += ps[2 * m + 1];
+    }
+    v[0] = t0;
+    v[1] = t1;
+    return true;
+  };
+  auto fail = [&]() {
+    if (threadIdx.x == 0) __hip_atomic_fetch_max(out.status + sys, 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  auto finish = [&](int iters, double resid, int st) {
+    if (rank == 0 && threadIdx.x == 0) {
+      out.iters[sys] = iters;
+      out.resid[sys] = resid;
+    }
+    if (threadIdx.x == 0) __hip_atomic_fetch_max(out.status + sys, st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  auto for_rows = [&](auto&& f) {
+    for (int s = a0 + w; s < a1; s += NW) {
+      const int l = (s - s0) * kWave + lane;
+      f(s, l, l < nrow);
+    }
+  };
+  if (init_x) {
+    for (int l = m0 + threadIdx.x; l < m1; l += NT) x[row0 + l] = 0.0;
+    if (!wait()) return fail();
+  }
+  // r = b - A x ; B2 = (r, r)
+  double acc2[2] = {0.0, 0.0};
+  for_rows([&](int s, int l, bool valid) {
+    const int len = valid ? V.rowlen[s * kWave + lane] : 0;
+    const double ax = sell_row_scalar<U>(V, s, lane, len, x);
+    if (valid) {
+      const double rv = b[row0 + l] - ax;
+      r[row0 + l] = rv;
+      acc2[0] += rv * rv;
+    }
+  });
+  if (!tsum(acc2)) return fail();
+  const double B2 = acc2[0];
+  if (B2 == 0.0) {  // mat_converge.f90:40-48
+    for (int l = m0 + threadIdx.x; l < m1; l += NT) x[row0 + l] = 0.0;
+    return finish(0, 0.0, 2);
+  }
+  // z = M^-1 r ; rho = (r, z) ; p = 0
+  double rho = 0.0;
+  {
+    double a[2] = {0.0, 0.0};
+    for (int l = m0 + threadIdx.x; l < m1; l += NT) {
+      const int g = row0 + l;
+      const double zv = precond ? V.dinv[g] * r[g] : r[g];
+      z[g] = zv;
+      p[g] = 0.0;
+      a[0] += r[g] * zv;
+    }
+    if (!tsum(a)) return fail();
+    rho = a[0];
+  }
+  double beta = 0.0, resid = 1.0;
+  int it = 1, conv = 0;
+  // p alternates between two buffers: iteration it writes pn and reads po
+  // (zero at it = 1).  The other members' rows of the new p are not waited for:
+  // the SpMV forms them from their z and previous p, complete since the last
+  // barrier, exactly as their owners compute them.
+  const int g0 = row0 + m0, g1 = row0 + m1;
+  for (; it <= max_iter; ++it) {
+    const double* po = (it & 1) ? p : p2;
+    double* pn = (it & 1) ? p2 : p;
+    // p = z + beta p   (CG.f90:80-84), this member's rows
+    for (int l = m0 + threadIdx.x; l < m1; l += NT) pn[row0 + l] = fma(beta, po[row0 + l], z[row0 + l]);
+    __syncthreads();
+    // q = A p ; (p, q)
+    double a[2] = {0.0, 0.0};
+    for_rows([&](int s, int l, bool valid) {
+      const int len = valid ? V.rowlen[s * kWave + lane] : 0;
+      const double qv = sell_row_g<U>(V, s, lane, len, [&](int c) {
+        double v;
+        if (c >= g0 && c < g1)
+          v = pn[c];
+        else
+          v = fma(beta, po[c], z[c]);
+        return v;
+      });
+      if (valid) {
+        q[row0 + l] = qv;
+        a[0] += pn[row0 + l] * qv;
+      }
+    });
+    if (!tsum(a)) return fail();
+    const double alpha = rho / a[0];
+    const bool rr = (it % kIterRR) == 0;
+    double c[2] = {0.0, 0.0};
+    if (!rr) {
+      for (int l = m0 + threadIdx.x; l < m1; l += NT) {
+        const int g = row0 + l;
+        x[g] = fma(alpha, pn[g], x[g]);
+        const double rv = fma(-alpha, q[g], r[g]);
+        r[g] = rv;
+        const double zv = precond ? V.dinv[g] * rv : rv;
+        z[g] = zv;
+        c[0] += rv * rv;
+        c[1] += rv * zv;
+      }
+    } else {
+      for (int l = m0 + threadIdx.x; l < m1; l += NT) {
+        const int g = row0 + l;
+        x[g] = fma(alpha, pn[g], x[g]);
+      }
+      if (!wait()) return fail();
+      for_rows([&](int s, int l, bool valid) {
+        const int len = valid ? V.rowlen[s * kWave + lane] : 0;
+        const double ax = sell_row_scalar<U>(V, s, lane, len, x);
+        if (valid) r[row0 + l] = b[row0 + l] - ax;
+      });
+      __syncthreads();
+      for (int l = m0 + threadIdx.x; l < m1; l += NT) {
+        const int g = row0 + l;
+        const double rv = r[g];
+        const double zv = precond ? V.dinv[g] * rv : rv;
+        z[g] = zv;
+        c[0] += rv * rv;
+        c[1] += rv * zv;
+      }
+    }
+    if (!tsum(c)) return fail();
+    resid = sqrt(c[0] / B2);
+    if (resid < tol) {
+      conv = 1;
+      break;
+    }
+    beta = c[1] / rho;
+    rho = c[1];
+  }
+  finish(conv ? it : max_iter, resid, conv ? 0 : 1);
+}
+
+template <int NT, int U>
+__global__ void __launch_bounds__(NT) k_batch_sched(BatchView V, TeamView S, const double* __restrict__ b,
+                                                    double* __restrict__ x, double* __restrict__ r,
+                                                    double* __restrict__ z, double* __restrict__ p,
+                                                    double* __restrict__ p2, double* __restrict__ q, BatchOut out,
+                                                    int max_iter, double tol, int init_x, int precond) {
+  __shared__ double red[NT / kWave * 2];
+  __shared__ int sh[2];
+  for (;;) {
+    if (threadIdx.x == 0) sh[0] = __hip_atomic_fetch_add(S.ctl, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int k = sh[0];
+    if (k >= S.nslot) break;
+    const int sys = S.slot[4 * k], rank = S.slot[4 * k + 1], T = S.slot[4 * k + 2], team = S.slot[4 * k + 3];
+    if (T == 1)
+      cg_solo<NT, U>(V, sys, b, x, r, z, p, q, out, max_iter, tol, init_x, precond, red);
+    else
+      cg_team<NT, U>(V, S, sys, rank, T, team, b, x, r, z, p, p2, q, out, max_iter, tol, init_x, precond, red,
+                     sh + 1);
+    __syncthreads();
+  }
+}
+
+// Per-launch reset (a kernel, so that the captured graph holds no memset
+// nodes): results to "not run", the slot counter, abort flag and team counters
+// to zero.
+__global__ void k_batch_reset(int nsys, int* iters, double* resid, int* status, int* ctl, int nctl) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nsys) {
     iters[i] = -1;
     resid[i] = 0.0;
     status[i] = -1;
   }
+  if (i < nctl) ctl[i] = 0;
 }
 
 // system-aligned SELL build: one thread per SELL row
@@ -270,23 +538,160 @@ __global__ void k_batch_fill(int nsys_rows_total, const int* __restrict__ sell_s
   }
 }
 
-#define MH_BATCH_CFG(cfg, ...)                                                  \
-  switch (cfg) {                                                                \
-    case 1: { constexpr int NT = 256; constexpr int UU = 2; __VA_ARGS__; } break;  \
-    case 2: { constexpr int NT = 512; constexpr int UU = 2; __VA_ARGS__; } break;  \
-    case 3: { constexpr int NT = 1024; constexpr int UU = 2; __VA_ARGS__; } break; \
-    case 4: { constexpr int NT = 512; constexpr int UU = 4; __VA_ARGS__; } break;  \
-    case 5: { constexpr int NT = 1024; constexpr int UU = 4; __VA_ARGS__; } break; \
-    case 6: { constexpr int NT = 512; constexpr int UU = 1; __VA_ARGS__; } break;  \
-    default: { constexpr int NT = 256; constexpr int UU = 4; __VA_ARGS__; } break; \
+// Launch configurations: threads per workgroup, slots per SpMV step, team split
+// (0: k_batch_cg; s > 0: k_batch_sched with teams for systems above s/4 of a
+// workgroup's balanced share of the estimated work).
+#define MH_BATCH_CASE(c, nt, u, ts, ...) \
+  case c: {                              \
+    constexpr int NT = nt;               \
+    constexpr int UU = u;                \
+    constexpr int TS = ts;               \
+    __VA_ARGS__;                         \
+  } break;
+#define MH_BATCH_CFG(cfg, ...)                       \
+  switch (cfg) {                                     \
+    MH_BATCH_CASE(1, 256, 2, 0, __VA_ARGS__)         \
+    MH_BATCH_CASE(2, 512, 2, 0, __VA_ARGS__)         \
+    MH_BATCH_CASE(3, 1024, 2, 0, __VA_ARGS__)        \
+    MH_BATCH_CASE(4, 512, 4, 0, __VA_ARGS__)         \
+    MH_BATCH_CASE(5, 1024, 4, 0, __VA_ARGS__)        \
+    MH_BATCH_CASE(6, 512, 1, 0, __VA_ARGS__)         \
+    MH_BATCH_CASE(7, 1024, 4, 4, __VA_ARGS__)        \
+    MH_BATCH_CASE(8, 1024, 4, 2, __VA_ARGS__)        \
+    MH_BATCH_CASE(9, 1024, 2, 2, __VA_ARGS__)        \
+    MH_BATCH_CASE(10, 512, 4, 2, __VA_ARGS__)        \
+    MH_BATCH_CASE(11, 1024, 2, 4, __VA_ARGS__)       \
+    MH_BATCH_CASE(12, 1024, 2, 6, __VA_ARGS__)       \
+    default: {                                       \
+      constexpr int NT = 256;                        \
+      constexpr int UU = 4;                          \
+      constexpr int TS = 0;                          \
+      __VA_ARGS__;                                   \
+    } break;                                         \
   }
+
+// Team schedule of a configuration (NT threads per workgroup, `resident`
+// workgroups at once, split factor ts): the estimated work of a system is
+// (entries + 6 rows) x rows^(1/3) (the SpMV and vector bytes times an iteration
+// count that grows like a 3-D mesh's edge); systems above ts/4 of the balanced
+// share get ceil(work / that) members, at most 8 and at most one slice per wave
+// and member; team tasks come first (while their slots fit the resident
+// workgroups, all teams start together), each group ordered by work per
+// member, largest first.
+int build_sched(const Batch& B, int NT, int resident, int ts, BatchSched& S, hipStream_t s) {
+  const int nsys = B.nsys;
+  std::vector<double> w((size_t)nsys);
+  double W = 0.0;
+  for (int k = 0; k < nsys; ++k) {
+    const double rows = (double)B.h_rows[k];
+    w[k] = ((double)B.h_nnz[k] + 6.0 * rows) * std::cbrt(std::max(rows, 1.0));
+    W += w[k];
+  }
+  const double share = W / std::max(resident, 1) * ts / 4.0;
+  struct Task {
+    int sys, T;
+    double d;
+  };
+  std::vector<Task> tasks((size_t)nsys);
+  for (int k = 0; k < nsys; ++k) {
+    const int tmax = std::min(kTeamMax, B.h_slices[k] / (NT / kWave));
+    int T = 1;
+    if (tmax > 1 && share > 0.0 && w[k] > share) T = std::min(tmax, (int)std::ceil(w[k] / share));
+    tasks[k] = Task{k, T, w[k] / T};
+  }
+  std::stable_sort(tasks.begin(), tasks.end(), [](const Task& a, const Task& b) {
+    if ((a.T > 1) != (b.T > 1)) return a.T > 1;
+    return a.d > b.d;
+  });
+  std::vector<int> slot;
+  int nteam = 0;
+  for (const Task& t : tasks) {
+    const int team = t.T > 1 ? nteam++ : -1;
+    for (int m = 0; m < t.T; ++m) {
+      slot.push_back(t.sys);
+      slot.push_back(m);
+      slot.push_back(t.T);
+      slot.push_back(team);
+    }
+  }
+  S.nslot = (int)(slot.size() / 4);
+  S.nteam = nteam;
+  S.grid = std::max(1, std::min(S.nslot, resident));
+  MH_TRY(S.slot.alloc(slot.size()));
+  MH_TRY(S.ctl.alloc((size_t)2 + nteam));
+  MH_TRY(S.part.alloc((size_t)std::max(nteam, 1) * 2 * kTeamMax * 2));
+  MH_TRY(upload(S.slot.p, slot.data(), sizeof(int) * slot.size(), s));
+  MH_HIP(hipStreamSynchronize(s));  // `slot` is freed on return
+  int dev = 0, khz = 0;
+  MH_HIP(hipGetDevice(&dev));
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || khz <= 0) khz = 100000;
+  S.ticks_per_s = (double)khz * 1000.0;
+  return 0;
+}
 
 }  // namespace
 
-int batch_num_configs() { return 7; }
+int batch_num_configs() { return 13; }
 
 void batch_config_info(int cfg, int* threads, int* unroll) {
   MH_BATCH_CFG(cfg, *threads = NT; *unroll = UU);
+}
+
+int batch_config_split(int cfg) {
+  int ts = 0;
+  MH_BATCH_CFG(cfg, ts = TS);
+  return ts;
+}
+
+// The team schedule of configuration cfg, built on first use (outside any
+// stream capture: it uploads synchronously).  Its control words are zeroed by
+// the reset kernel of every launch.
+static int ensure_sched(Batch& B, int cfg, BatchSched** out, hipStream_t s) {
+  *out = nullptr;
+  if ((int)B.sched.size() < batch_num_configs()) B.sched.resize(batch_num_configs());
+  if (!B.sched[cfg]) {
+    std::unique_ptr<BatchSched> S(new BatchSched());
+    int rc = 0;
+    MH_BATCH_CFG(cfg, {
+      if constexpr (TS > 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_sched<NT, UU>, NT, 0) != hipSuccess ||
+            per_cu < 1)
+          per_cu = 1;
+        rc = build_sched(B, NT, per_cu * cu_count_batch(), TS, *S, s);
+      }
+    });
+    if (rc) return rc;
+    B.sched[cfg] = std::move(S);
+  }
+  *out = B.sched[cfg].get();
+  return 0;
+}
+
+int batch_schedule_info(Batch& B, int cfg, int* split, int* nslot, int* nteam, hipStream_t s) {
+  if (cfg < 0 || cfg >= batch_num_configs()) {
+    set_error("batch config %d out of range", cfg);
+    return -2;
+  }
+  *split = batch_config_split(cfg);
+  *nslot = B.nsys;
+  *nteam = 0;
+  if (*split == 0) return 0;
+  BatchSched* S = nullptr;
+  MH_TRY(ensure_sched(B, cfg, &S, s));
+  *nslot = S->nslot;
+  *nteam = S->nteam;
+  return 0;
+}
+
+int batch_aborted(Batch& B, int cfg, int* aborted, hipStream_t s) {
+  *aborted = 0;
+  if (cfg < 0 || cfg >= (int)B.sched.size() || !B.sched[cfg] || batch_config_split(cfg) == 0) return 0;
+  int ctl[2] = {0, 0};
+  MH_HIP(hipMemcpyAsync(ctl, B.sched[cfg]->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, s));
+  MH_HIP(hipStreamSynchronize(s));
+  *aborted = ctl[1] != 0;
+  return 0;
 }
 
 Batch::~Batch() {
@@ -318,8 +723,13 @@ int batch_create(Batch** out, int nsys, const int* sys_row_h, int n, int nz, con
   B->n = n;
   B->nz = nz;
   std::vector<int> sys_slice((size_t)nsys + 1, 0), width, sell_sys;
+  B->h_rows.resize((size_t)nsys);
+  B->h_nnz.resize((size_t)nsys);
+  B->h_slices.resize((size_t)nsys);
   for (int k = 0; k < nsys; ++k) {
     const int r0 = sys_row_h[k], r1 = sys_row_h[k + 1];
+    B->h_rows[k] = r1 - r0;
+    B->h_nnz[k] = hidx[r1] - hidx[r0];
     for (int i = r0; i < r1; ++i) {
       if (hidx[i + 1] < hidx[i]) {
         set_error("batch CSR offsets decrease at row %d", i);
@@ -332,6 +742,7 @@ int batch_create(Batch** out, int nsys, const int* sys_row_h, int n, int nz, con
         }
     }
     const int ns = (r1 - r0 + kWave - 1) / kWave;
+    B->h_slices[k] = ns;
     sys_slice[k + 1] = sys_slice[k] + ns;
     for (int q = 0; q < ns; ++q) {
       int wmax = 0;
@@ -370,7 +781,7 @@ int batch_create(Batch** out, int nsys, const int* sys_row_h, int n, int nz, con
   MH_TRY(B->val.alloc(nslots * kWave));
   MH_TRY(B->rowlen.alloc((size_t)nslices * kWave));
   MH_TRY(B->dinv.alloc((size_t)n));
-  MH_TRY(B->work.alloc((size_t)n * 4));
+  MH_TRY(B->work.alloc((size_t)n * 5));  // r, z, p, q and the second p buffer of the team solver
   MH_TRY(B->iters.alloc((size_t)nsys));
   MH_TRY(B->resid.alloc((size_t)nsys));
   MH_TRY(B->status.alloc((size_t)nsys));
@@ -394,21 +805,36 @@ int batch_solve(Batch& B, const BatchParams& P, const double* b, double* x, int 
   double* z = r + B.n;
   double* p = z + B.n;
   double* q = p + B.n;
-  int threads = 256, unroll = 2;
-  batch_config_info(cfg, &threads, &unroll);
-  // one workgroup per system while they fit two per CU, else grid-stride over systems
+  double* p2 = q + B.n;
+  if (cfg < 0 || cfg >= batch_num_configs()) {
+    set_error("batch config %d out of range", cfg);
+    return -2;
+  }
+  BatchSched* S = nullptr;
+  if (batch_config_split(cfg) > 0) MH_TRY(ensure_sched(B, cfg, &S, s));
+  // k_batch_cg: one workgroup per system while they fit two per CU, else
+  // grid-stride over systems
   int grid = std::min(B.nsys, cu_count_batch() * 2);
   if (grid < 1) grid = 1;
   auto enqueue = [&]() -> int {
-    k_batch_reset<<<(B.nsys + 255) / 256, 256, 0, s>>>(B.nsys, B.iters.p, B.resid.p, B.status.p);
-    MH_BATCH_CFG(cfg, k_batch_cg<NT, UU><<<grid, NT, 0, s>>>(V, b, x, r, z, p, q, O, P.max_iter, P.tol, P.init_x,
-                                                             P.precond));
+    const int nctl = S ? 2 + S->nteam : 0;
+    k_batch_reset<<<(std::max(B.nsys, nctl) + 255) / 256, 256, 0, s>>>(B.nsys, B.iters.p, B.resid.p, B.status.p,
+                                                                         S ? S->ctl.p : nullptr, nctl);
+    MH_BATCH_CFG(cfg, {
+      if constexpr (TS > 0) {
+        const TeamView TV{S->nslot, S->slot.p, S->ctl.p, S->part.p, (long long)(S->ticks_per_s * B.wait_s)};
+        k_batch_sched<NT, UU><<<S->grid, NT, 0, s>>>(V, TV, b, x, r, z, p, p2, q, O, P.max_iter, P.tol,
+                                                     P.init_x, P.precond);
+      } else {
+        k_batch_cg<NT, UU><<<grid, NT, 0, s>>>(V, b, x, r, z, p, q, O, P.max_iter, P.tol, P.init_x, P.precond);
+      }
+    });
     MH_HIP(hipGetLastError());
     return 0;
   };
   if (!use_graph) return enqueue();
   const bool same = B.graph_exec && B.g_b == b && B.g_x == x && B.g_cfg == cfg && B.g_max_iter == P.max_iter &&
-                    B.g_tol == P.tol && B.g_init_x == P.init_x && B.g_precond == P.precond;
+                    B.g_tol == P.tol && B.g_init_x == P.init_x && B.g_precond == P.precond && B.g_wait == B.wait_s;
   if (!same) {
     if (B.graph_exec) {
       (void)hipGraphExecDestroy(B.graph_exec);
@@ -438,6 +864,7 @@ int batch_solve(Batch& B, const BatchParams& P, const double* b, double* x, int 
     B.g_tol = P.tol;
     B.g_init_x = P.init_x;
     B.g_precond = P.precond;
+    B.g_wait = B.wait_s;
   }
   MH_HIP(hipGraphLaunch(B.graph_exec, s));
   return 0;

@@ -858,6 +858,35 @@ int monolis_hip_batch_config(int cfg, int* threads, int* unroll) {
   return 0;
 }
 
+int monolis_hip_batch_set_config(monolis_hip_batch* H, int cfg) {
+  if (!H || cfg < 0 || cfg >= mh::batch_num_configs()) {
+    mh::set_error("batch config %d out of range", cfg);
+    return ST_ARGS;
+  }
+  H->B->cfg = cfg;
+  return 0;
+}
+
+int monolis_hip_batch_schedule(monolis_hip_batch* H, int cfg, int* split, int* nslot, int* nteam) {
+  Context* c = nullptr;
+  MH_TRY(mh::context(&c));
+  if (!H) {
+    mh::set_error("null batch");
+    return ST_ARGS;
+  }
+  const int rc = mh::batch_schedule_info(*H->B, cfg, split, nslot, nteam, c->stream);
+  return rc == -2 ? ST_ARGS : rc ? ST_HIP : 0;
+}
+
+int monolis_hip_batch_set_wait_limit(monolis_hip_batch* H, double seconds) {
+  if (!H || !(seconds >= 0.0) || seconds > 600.0) {
+    mh::set_error("team wait limit must lie in [0, 600] s");
+    return ST_ARGS;
+  }
+  H->B->wait_s = seconds;
+  return 0;
+}
+
 int monolis_hip_batch_solve(monolis_hip_batch* H, const double* b, double* x, int* Iarray, double* Rarray, int* iters,
                             double* resid, int* status) {
   Context* c = nullptr;
@@ -872,6 +901,12 @@ int monolis_hip_batch_solve(monolis_hip_batch* H, const double* b, double* x, in
   if (resid) MH_HIP(hipMemcpy(resid, B.resid.p, sizeof(double) * B.nsys, hipMemcpyDeviceToHost));
   if (status) MH_HIP(hipMemcpy(status, B.status.p, sizeof(int) * B.nsys, hipMemcpyDeviceToHost));
   Rarray[R_TIME_SOL] = now() - t0;
+  int aborted = 0;
+  MH_TRY(mh::batch_aborted(B, B.cfg, &aborted, c->stream));
+  if (aborted) {
+    mh::set_error("batched solve: a team of workgroups gave up waiting for a member (systems with status 3)");
+    return ST_HIP;
+  }
   return 0;
 }
 
@@ -896,6 +931,12 @@ int monolis_hip_batch_time(monolis_hip_batch* H, const double* b, double* x, int
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   *ms = reps > 0 ? (double)t / reps : 0.0;
+  int aborted = 0;
+  MH_TRY(mh::batch_aborted(B, B.cfg, &aborted, c->stream));
+  if (aborted) {
+    mh::set_error("batched solve: a team of workgroups gave up waiting for a member");
+    return ST_HIP;
+  }
   return 0;
 }
 
@@ -922,7 +963,9 @@ int monolis_hip_batch_autotune(monolis_hip_batch* H, const double* b, double* x,
     MH_HIP(hipEventElapsedTime(&t, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    const double avg = reps > 0 ? (double)t / reps : 0.0;
+    int aborted = 0;
+    MH_TRY(mh::batch_aborted(B, cfg, &aborted, c->stream));
+    const double avg = aborted ? 1e30 : reps > 0 ? (double)t / reps : 0.0;  // a configuration that gave up is never chosen
     if (ms) ms[cfg] = avg;
     if (avg < best_ms) {
       best_ms = avg;

@@ -152,12 +152,16 @@ def rhs_from_solution(mat: Matrix, x_true: np.ndarray) -> np.ndarray:
     return np.asarray(mat.to_scipy() @ x_true, dtype=np.float64)
 
 
-def poisson_batch(count: int = 512, lo: int = 10, hi: int = 40, seed: int = 0):
-    """`count` independent 7-point Poisson cubes with edges U{lo..hi}, packed as
-    one block-diagonal CSR.  Returns (Matrix, row_ptr) where system s owns rows
-    row_ptr[s]..row_ptr[s+1]-1."""
-    rng = np.random.default_rng(seed)
-    edges = rng.integers(lo, hi + 1, size=count)
+def poisson_batch(count: int = 512, lo: int = 10, hi: int = 40, seed: int = 0, edges=None):
+    """`count` independent 7-point Poisson cubes with edges U{lo..hi} (or the
+    given `edges`), packed as one block-diagonal CSR.  Returns (Matrix, row_ptr)
+    where system s owns rows row_ptr[s]..row_ptr[s+1]-1."""
+    if edges is None:
+        rng = np.random.default_rng(seed)
+        edges = rng.integers(lo, hi + 1, size=count)
+    else:
+        edges = np.asarray(edges, dtype=np.int64)
+        count, lo, hi = len(edges), int(edges.min()), int(edges.max())
     cache = {}
     idx_parts, item_parts, a_parts = [], [], []
     row_ptr = np.zeros(count + 1, dtype=np.int64)

@@ -3,6 +3,10 @@ launch) against one oracle solve per system.
 
 Bars: status identical, iteration count within +-2 % (+-1 below 50) of the
 oracle's solve of that system alone, ||x_k - x_k,oracle|| / ||x_k,oracle|| <= 1e-6.
+The team configurations (the largest systems split over several workgroups)
+are held to the same bars, on batches whose largest system gets a team, one
+with a zero right-hand side and one past the residual replacement at
+iteration 200.
 """
 import numpy as np
 import pytest
@@ -76,3 +80,81 @@ def test_batch_configs_graph_replay_and_autotune():
     it2, r2, s2 = B.solve(db, dx, prm)          # tuned configuration
     assert np.all(s2 == 0) and np.all(np.abs(it2 - it0) <= 1)
     assert np.linalg.norm(dx.numpy() - x0) / np.linalg.norm(x0) <= 1e-9
+
+
+def _check_against_oracle(mat, sys_row, b, x, iters, resid, status, precond=1, zero=()):
+    for k in range(len(sys_row) - 1):
+        xk = x[sys_row[k]:sys_row[k + 1]]
+        if k in zero:
+            assert status[k] == 2 and np.all(xk == 0.0), k
+            continue
+        sk = _system(mat, sys_row, k)
+        xo, st, I, R = oracle.solve(sk, b[sys_row[k]:sys_row[k + 1]], precond=precond)
+        assert status[k] == 0 and st == 0, (k, status[k])
+        assert abs(int(iters[k]) - int(I[4])) <= max(1, int(0.02 * I[4])), (k, iters[k], I[4])
+        assert np.linalg.norm(xk - xo) / np.linalg.norm(xo) <= 1e-6, k
+        assert resid[k] < 1e-8
+
+
+@pytest.mark.parametrize("cfg", [7, 8, 9, 10, 11, 12])
+def test_batch_team_configs_match_individual_solves(cfg):
+    mat, sys_row = W.poisson_batch(count=24, lo=4, hi=20, seed=3)
+    sizes = np.diff(sys_row)
+    big = int(np.argmax(sizes))
+    zero = int(np.argsort(sizes)[-2])               # the second largest system: zero right-hand side
+    b = W.uniform_vector(mat.n, seed=4)
+    b[sys_row[zero]:sys_row[zero + 1]] = 0.0
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(mat.n)
+    B = api.DeviceBatch(mat, sys_row)
+    sch = B.schedule(cfg)
+    assert sch["split"] > 0 and sch["teams"] >= 1 and sch["slots"] > B.nsys, sch
+    B.set_config(cfg)
+    iters, resid, status = B.solve(db, dx, _prm())
+    x = dx.numpy()
+    print(f"cfg {cfg}: {sch}, largest system {sizes[big]} rows, {iters[big]} iterations", flush=True)
+    _check_against_oracle(mat, sys_row, b, x, iters, resid, status, zero=(zero,))
+    it1, r1, s1 = B.solve(db, dx, _prm())        # graph replay: same bits
+    assert np.array_equal(x, dx.numpy()) and np.array_equal(iters, it1) and np.array_equal(status, s1)
+
+
+@pytest.mark.parametrize("cfg", [7, 10])
+def test_batch_team_residual_replacement(cfg):
+    # one 56^3 system (about 210 DIAG-CG iterations) among small ones: its team
+    # passes the true-residual replacement at iteration 200
+    edges = [56] + [5, 9, 12, 7, 14, 6, 11, 8]
+    mat, sys_row = W.poisson_batch(edges=edges)
+    b = W.uniform_vector(mat.n, seed=21)
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(mat.n)
+    B = api.DeviceBatch(mat, sys_row)
+    sch = B.schedule(cfg)
+    assert sch["teams"] >= 1, sch
+    B.set_config(cfg)
+    iters, resid, status = B.solve(db, dx, _prm())
+    assert iters[0] > 200, iters[0]
+    _check_against_oracle(mat, sys_row, b, dx.numpy(), iters, resid, status)
+
+
+def test_batch_team_wait_limit_gives_up_cleanly():
+    # a zero wait limit makes any team member that arrives before its teammates
+    # give up at once (every wait still ends): the call reports an error or, if
+    # no member ever had to wait, succeeds; afterwards the default limit solves
+    # the batch correctly (the abort flag and counters are reset per launch)
+    mat, sys_row = W.poisson_batch(count=24, lo=4, hi=20, seed=3)
+    b = W.uniform_vector(mat.n, seed=4)
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(mat.n)
+    B = api.DeviceBatch(mat, sys_row)
+    B.set_config(10)
+    B.set_wait_limit(0.0)
+    try:
+        B.solve(db, dx, _prm())
+        gave_up = False
+    except RuntimeError as e:
+        gave_up = True
+        assert "gave up" in str(e)
+    B.set_wait_limit(5.0)
+    iters, resid, status = B.solve(db, dx, _prm())
+    print(f"zero wait limit gave up: {gave_up}", flush=True)
+    _check_against_oracle(mat, sys_row, b, dx.numpy(), iters, resid, status)
