@@ -158,3 +158,33 @@ def test_batch_team_wait_limit_gives_up_cleanly():
     iters, resid, status = B.solve(db, dx, _prm())
     print(f"zero wait limit gave up: {gave_up}", flush=True)
     _check_against_oracle(mat, sys_row, b, dx.numpy(), iters, resid, status)
+
+
+def test_batch_long_range_coupling():
+    # a 40,000-row system whose first and last rows couple 39,999 rows apart (a
+    # SELL slice whose columns span the whole system) beside Poisson systems,
+    # one workgroup per system and with teams
+    n = 40000
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j, v in ((i - 1, -1.0), (i, 4.0), (i + 1, -1.0)):
+            if 0 <= j < n:
+                rows.append(i); cols.append(j); vals.append(v)
+    rows += [0, n - 1]; cols += [n - 1, 0]; vals += [-1.0, -1.0]
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = np.array(rows)[order], np.array(cols)[order], np.array(vals)[order]
+    pm, prow = W.poisson_batch(edges=[12, 7, 20])
+    index = np.concatenate([np.searchsorted(rows, np.arange(n + 1)).astype(np.int64),
+                            pm.index[1:].astype(np.int64) + len(rows)]).astype(np.int32)
+    item = np.concatenate([cols + 1, pm.item.astype(np.int64) + n]).astype(np.int32)
+    A = np.concatenate([vals, pm.A])
+    mat = Matrix(n + pm.n, n + pm.n, 1, index, item, A).ensure_dof_index()
+    sys_row = np.concatenate([[0], np.asarray(prow, np.int64) + n])
+    b = W.uniform_vector(mat.n, seed=5)
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(mat.n)
+    for cfg in (5, 11):
+        B = api.DeviceBatch(mat, sys_row)
+        B.set_config(cfg)
+        iters, resid, status = B.solve(db, dx, _prm())
+        _check_against_oracle(mat, sys_row, b, dx.numpy(), iters, resid, status)
