@@ -223,8 +223,10 @@ int monolis_hip_batch_destroy(monolis_hip_batch* B);
 int monolis_hip_batch_solve(monolis_hip_batch* B, const double* b, double* x, int* Iarray, double* Rarray,
                             int* iters, double* resid, int* status);
 /* Autotuner of the persistent launch: workgroup size x slots per step x team
- * split.  Runs the solve `reps` times per configuration, keeps the fastest; ms
- * (if not NULL) gets monolis_hip_batch_num_configs() average times. */
+ * split x vector streams (configurations 13 and 14 load and store the vectors
+ * of the elementwise passes non-temporally).  Runs the solve `reps` times per
+ * configuration, keeps the fastest; ms (if not NULL) gets
+ * monolis_hip_batch_num_configs() average times. */
 int monolis_hip_batch_num_configs(void);
 int monolis_hip_batch_config(int cfg, int* threads, int* unroll);
 /* Selects configuration cfg for the following solves (the autotuner's choice

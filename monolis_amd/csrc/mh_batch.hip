@@ -97,8 +97,21 @@ __device__ __forceinline__ double sell_row_scalar(const BatchView& V, int s, int
 }
 
 
+// Vector loads and stores of the elementwise passes: plain, or non-temporal
+// when SV (a batch whose vectors exceed what L2 / MALL keep between passes).
+template <bool SV>
+__device__ __forceinline__ double ldv(const double* p) {
+  if constexpr (SV) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool SV>
+__device__ __forceinline__ void stv(double* p, double v) {
+  if constexpr (SV) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // One system solved by one workgroup (NT threads, U slots per step).
-template <int NT, int U>
+template <int NT, int U, bool SV>
 __device__ __forceinline__ void cg_solo(const BatchView& V, int sys, const double* __restrict__ b,
                                         double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
                                         double* __restrict__ p, double* __restrict__ q, const BatchOut& out,
@@ -161,7 +174,8 @@ __device__ __forceinline__ void cg_solo(const BatchView& V, int sys, const doubl
     int it = 1, conv = 0;
     for (; it <= max_iter; ++it) {
       // p = z + beta p   (CG.f90:80-84)
-      for (int l = threadIdx.x; l < nrow; l += NT) p[row0 + l] = fma(beta, p[row0 + l], z[row0 + l]);
+      for (int l = threadIdx.x; l < nrow; l += NT)
+        stv<false>(p + row0 + l, fma(beta, ldv<SV>(p + row0 + l), ldv<SV>(z + row0 + l)));
       __syncthreads();
       // q = A p ; (p, q)
       double a[2] = {0.0, 0.0};
@@ -184,11 +198,11 @@ __device__ __forceinline__ void cg_solo(const BatchView& V, int sys, const doubl
       if (!rr) {
         for (int l = threadIdx.x; l < nrow; l += NT) {
           const int g = row0 + l;
-          x[g] = fma(alpha, p[g], x[g]);
-          const double rv = fma(-alpha, q[g], r[g]);
-          r[g] = rv;
-          const double zv = precond ? V.dinv[g] * rv : rv;
-          z[g] = zv;
+          stv<SV>(x + g, fma(alpha, ldv<SV>(p + g), ldv<SV>(x + g)));
+          const double rv = fma(-alpha, ldv<SV>(q + g), ldv<SV>(r + g));
+          stv<SV>(r + g, rv);
+          const double zv = precond ? ldv<SV>(V.dinv + g) * rv : rv;
+          stv<SV>(z + g, zv);
           c[0] += rv * rv;
           c[1] += rv * zv;
         }
@@ -232,14 +246,14 @@ __device__ __forceinline__ void cg_solo(const BatchView& V, int sys, const doubl
 }
 
 // The persistent solver, one workgroup per system.
-template <int NT, int U>
+template <int NT, int U, bool SV>
 __global__ void __launch_bounds__(NT) k_batch_cg(BatchView V, const double* __restrict__ b, double* __restrict__ x,
                                                  double* __restrict__ r, double* __restrict__ z,
                                                  double* __restrict__ p, double* __restrict__ q, BatchOut out,
                                                  int max_iter, double tol, int init_x, int precond) {
   __shared__ double red[NT / kWave * 2];
   for (int k = blockIdx.x; k < V.nsys; k += gridDim.x)
-    cg_solo<NT, U>(V, V.order[k], b, x, r, z, p, q, out, max_iter, tol, init_x, precond, red);
+    cg_solo<NT, U, SV>(V, V.order[k], b, x, r, z, p, q, out, max_iter, tol, init_x, precond, red);
 }
 
 // Team schedule.  Workgroups take slots in order from one counter; the slots of
@@ -292,7 +306,7 @@ __device__ __forceinline__ bool team_wait(int* cnt, int target, int* abort_flag,
 // other members' p (and x) after a team barrier; each dot product is a
 // workgroup sum per member, then the T member sums added in rank order (the
 // same order in every member, so all members take the same branches).
-template <int NT, int U>
+template <int NT, int U, bool SV>
 __device__ __forceinline__ void cg_team(const BatchView& V, const TeamView& S, int sys, int rank, int T, int team,
                                         const double* __restrict__ b, double* __restrict__ x,
                                         double* __restrict__ r, double* __restrict__ z, double* __restrict__ p,
@@ -395,7 +409,8 @@ __device__ __forceinline__ void cg_team(const BatchView& V, const TeamView& S, i
     const double* po = (it & 1) ? p : p2;
     double* pn = (it & 1) ? p2 : p;
     // p = z + beta p   (CG.f90:80-84), this member's rows
-    for (int l = m0 + threadIdx.x; l < m1; l += NT) pn[row0 + l] = fma(beta, po[row0 + l], z[row0 + l]);
+    for (int l = m0 + threadIdx.x; l < m1; l += NT)
+      stv<false>(pn + row0 + l, fma(beta, ldv<SV>(po + row0 + l), ldv<SV>(z + row0 + l)));
     __syncthreads();
     // q = A p ; (p, q)
     double a[2] = {0.0, 0.0};
@@ -421,11 +436,11 @@ __device__ __forceinline__ void cg_team(const BatchView& V, const TeamView& S, i
     if (!rr) {
       for (int l = m0 + threadIdx.x; l < m1; l += NT) {
         const int g = row0 + l;
-        x[g] = fma(alpha, pn[g], x[g]);
-        const double rv = fma(-alpha, q[g], r[g]);
-        r[g] = rv;
-        const double zv = precond ? V.dinv[g] * rv : rv;
-        z[g] = zv;
+        stv<SV>(x + g, fma(alpha, ldv<SV>(pn + g), ldv<SV>(x + g)));
+        const double rv = fma(-alpha, ldv<SV>(q + g), ldv<SV>(r + g));
+        stv<SV>(r + g, rv);
+        const double zv = precond ? ldv<SV>(V.dinv + g) * rv : rv;
+        stv<SV>(z + g, zv);
         c[0] += rv * rv;
         c[1] += rv * zv;
       }
@@ -462,7 +477,7 @@ __device__ __forceinline__ void cg_team(const BatchView& V, const TeamView& S, i
   finish(conv ? it : max_iter, resid, conv ? 0 : 1);
 }
 
-template <int NT, int U>
+template <int NT, int U, bool SV>
 __global__ void __launch_bounds__(NT) k_batch_sched(BatchView V, TeamView S, const double* __restrict__ b,
                                                     double* __restrict__ x, double* __restrict__ r,
                                                     double* __restrict__ z, double* __restrict__ p,
@@ -477,9 +492,9 @@ __global__ void __launch_bounds__(NT) k_batch_sched(BatchView V, TeamView S, con
     if (k >= S.nslot) break;
     const int sys = S.slot[4 * k], rank = S.slot[4 * k + 1], T = S.slot[4 * k + 2], team = S.slot[4 * k + 3];
     if (T == 1)
-      cg_solo<NT, U>(V, sys, b, x, r, z, p, q, out, max_iter, tol, init_x, precond, red);
+      cg_solo<NT, U, SV>(V, sys, b, x, r, z, p, q, out, max_iter, tol, init_x, precond, red);
     else
-      cg_team<NT, U>(V, S, sys, rank, T, team, b, x, r, z, p, p2, q, out, max_iter, tol, init_x, precond, red,
+      cg_team<NT, U, SV>(V, S, sys, rank, T, team, b, x, r, z, p, p2, q, out, max_iter, tol, init_x, precond, red,
                      sh + 1);
     __syncthreads();
   }
@@ -540,32 +555,38 @@ __global__ void k_batch_fill(int nsys_rows_total, const int* __restrict__ sell_s
 
 // Launch configurations: threads per workgroup, slots per SpMV step, team split
 // (0: k_batch_cg; s > 0: k_batch_sched with teams for systems above s/4 of a
-// workgroup's balanced share of the estimated work).
-#define MH_BATCH_CASE(c, nt, u, ts, ...) \
-  case c: {                              \
-    constexpr int NT = nt;               \
-    constexpr int UU = u;                \
-    constexpr int TS = ts;               \
-    __VA_ARGS__;                         \
+// workgroup's balanced share of the estimated work), non-temporal vector
+// streams (measured: 10 % faster per iteration when 256 40^3 systems fill the
+// GPU, slower when the active vectors fit in L2 / MALL).
+#define MH_BATCH_CASE(c, nt, u, ts, sv, ...) \
+  case c: {                                  \
+    constexpr int NT = nt;                   \
+    constexpr int UU = u;                    \
+    constexpr int TS = ts;                   \
+    constexpr bool SV = sv;                  \
+    __VA_ARGS__;                             \
   } break;
 #define MH_BATCH_CFG(cfg, ...)                       \
   switch (cfg) {                                     \
-    MH_BATCH_CASE(1, 256, 2, 0, __VA_ARGS__)         \
-    MH_BATCH_CASE(2, 512, 2, 0, __VA_ARGS__)         \
-    MH_BATCH_CASE(3, 1024, 2, 0, __VA_ARGS__)        \
-    MH_BATCH_CASE(4, 512, 4, 0, __VA_ARGS__)         \
-    MH_BATCH_CASE(5, 1024, 4, 0, __VA_ARGS__)        \
-    MH_BATCH_CASE(6, 512, 1, 0, __VA_ARGS__)         \
-    MH_BATCH_CASE(7, 1024, 4, 4, __VA_ARGS__)        \
-    MH_BATCH_CASE(8, 1024, 4, 2, __VA_ARGS__)        \
-    MH_BATCH_CASE(9, 1024, 2, 2, __VA_ARGS__)        \
-    MH_BATCH_CASE(10, 512, 4, 2, __VA_ARGS__)        \
-    MH_BATCH_CASE(11, 1024, 2, 4, __VA_ARGS__)       \
-    MH_BATCH_CASE(12, 1024, 2, 6, __VA_ARGS__)       \
+    MH_BATCH_CASE(1, 256, 2, 0, false, __VA_ARGS__)  \
+    MH_BATCH_CASE(2, 512, 2, 0, false, __VA_ARGS__)  \
+    MH_BATCH_CASE(3, 1024, 2, 0, false, __VA_ARGS__) \
+    MH_BATCH_CASE(4, 512, 4, 0, false, __VA_ARGS__)  \
+    MH_BATCH_CASE(5, 1024, 4, 0, false, __VA_ARGS__) \
+    MH_BATCH_CASE(6, 512, 1, 0, false, __VA_ARGS__)  \
+    MH_BATCH_CASE(7, 1024, 4, 4, false, __VA_ARGS__) \
+    MH_BATCH_CASE(8, 1024, 4, 2, false, __VA_ARGS__) \
+    MH_BATCH_CASE(9, 1024, 2, 2, false, __VA_ARGS__) \
+    MH_BATCH_CASE(10, 512, 4, 2, false, __VA_ARGS__) \
+    MH_BATCH_CASE(11, 1024, 2, 4, false, __VA_ARGS__) \
+    MH_BATCH_CASE(12, 1024, 2, 6, false, __VA_ARGS__) \
+    MH_BATCH_CASE(13, 1024, 2, 0, true, __VA_ARGS__) \
+    MH_BATCH_CASE(14, 1024, 2, 4, true, __VA_ARGS__) \
     default: {                                       \
       constexpr int NT = 256;                        \
       constexpr int UU = 4;                          \
       constexpr int TS = 0;                          \
+      constexpr bool SV = false;                     \
       __VA_ARGS__;                                   \
     } break;                                         \
   }
@@ -631,7 +652,7 @@ int build_sched(const Batch& B, int NT, int resident, int ts, BatchSched& S, hip
 
 }  // namespace
 
-int batch_num_configs() { return 13; }
+int batch_num_configs() { return 15; }
 
 void batch_config_info(int cfg, int* threads, int* unroll) {
   MH_BATCH_CFG(cfg, *threads = NT; *unroll = UU);
@@ -655,7 +676,7 @@ static int ensure_sched(Batch& B, int cfg, BatchSched** out, hipStream_t s) {
     MH_BATCH_CFG(cfg, {
       if constexpr (TS > 0) {
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_sched<NT, UU>, NT, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_sched<NT, UU, SV>, NT, 0) != hipSuccess ||
             per_cu < 1)
           per_cu = 1;
         rc = build_sched(B, NT, per_cu * cu_count_batch(), TS, *S, s);
@@ -823,10 +844,10 @@ int batch_solve(Batch& B, const BatchParams& P, const double* b, double* x, int 
     MH_BATCH_CFG(cfg, {
       if constexpr (TS > 0) {
         const TeamView TV{S->nslot, S->slot.p, S->ctl.p, S->part.p, (long long)(S->ticks_per_s * B.wait_s)};
-        k_batch_sched<NT, UU><<<S->grid, NT, 0, s>>>(V, TV, b, x, r, z, p, p2, q, O, P.max_iter, P.tol,
+        k_batch_sched<NT, UU, SV><<<S->grid, NT, 0, s>>>(V, TV, b, x, r, z, p, p2, q, O, P.max_iter, P.tol,
                                                      P.init_x, P.precond);
       } else {
-        k_batch_cg<NT, UU><<<grid, NT, 0, s>>>(V, b, x, r, z, p, q, O, P.max_iter, P.tol, P.init_x, P.precond);
+        k_batch_cg<NT, UU, SV><<<grid, NT, 0, s>>>(V, b, x, r, z, p, q, O, P.max_iter, P.tol, P.init_x, P.precond);
       }
     });
     MH_HIP(hipGetLastError());

@@ -96,7 +96,21 @@ def _check_against_oracle(mat, sys_row, b, x, iters, resid, status, precond=1, z
         assert resid[k] < 1e-8
 
 
-@pytest.mark.parametrize("cfg", [7, 8, 9, 10, 11, 12])
+@pytest.mark.parametrize("cfg", [3, 5, 13])
+def test_batch_one_workgroup_configs_match_individual_solves(cfg):
+    # 1024-thread workgroups, plain and non-temporal vector streams
+    mat, sys_row = W.poisson_batch(count=24, lo=4, hi=20, seed=3)
+    b = W.uniform_vector(mat.n, seed=4)
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(mat.n)
+    B = api.DeviceBatch(mat, sys_row)
+    assert B.schedule(cfg)["split"] == 0
+    B.set_config(cfg)
+    iters, resid, status = B.solve(db, dx, _prm())
+    _check_against_oracle(mat, sys_row, b, dx.numpy(), iters, resid, status)
+
+
+@pytest.mark.parametrize("cfg", [7, 8, 9, 10, 11, 12, 14])
 def test_batch_team_configs_match_individual_solves(cfg):
     mat, sys_row = W.poisson_batch(count=24, lo=4, hi=20, seed=3)
     sizes = np.diff(sys_row)
