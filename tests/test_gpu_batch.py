@@ -202,3 +202,32 @@ def test_batch_long_range_coupling():
         B.set_config(cfg)
         iters, resid, status = B.solve(db, dx, _prm())
         _check_against_oracle(mat, sys_row, b, dx.numpy(), iters, resid, status)
+
+
+@pytest.mark.parametrize("cfg", [11, 14])
+def test_batch_team_no_preconditioner_and_iteration_limit(cfg):
+    mat, sys_row = W.poisson_batch(count=24, lo=4, hi=20, seed=3)
+    b = W.uniform_vector(mat.n, seed=4)
+    db = api.DeviceBuffer.from_numpy(b)
+    dx = api.DeviceBuffer(mat.n)
+    B = api.DeviceBatch(mat, sys_row)
+    assert B.schedule(cfg)["teams"] >= 1
+    B.set_config(cfg)
+    iters, resid, status = B.solve(db, dx, _prm(precond=0))
+    _check_against_oracle(mat, sys_row, b, dx.numpy(), iters, resid, status, precond=0)
+    # an iteration limit of 12: the systems that need more stop with status 1,
+    # 12 iterations and the oracle's iterate after 12 iterations
+    iters, resid, status = B.solve(db, dx, _prm(max_iter=12))
+    x = dx.numpy()
+    limited = 0
+    for k in range(B.nsys):
+        sk = _system(mat, sys_row, k)
+        xo, st, I, R = oracle.solve(sk, b[sys_row[k]:sys_row[k + 1]], precond=1, max_iter=12)
+        xk = x[sys_row[k]:sys_row[k + 1]]
+        if st == 0:
+            assert status[k] == 0, k
+        else:
+            limited += 1
+            assert status[k] == 1 and iters[k] == 12, (k, status[k], iters[k])
+        assert np.linalg.norm(xk - xo) / np.linalg.norm(xo) <= 1e-6, k
+    assert limited > 0
