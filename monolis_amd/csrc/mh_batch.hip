@@ -269,6 +269,7 @@ struct TeamView {
   int* ctl;            // [0] next slot, [1] abort flag, [2 + t] arrivals at team t's barriers
   double* part;        // per team: 2 alternating sets x kTeamMax members x 2 partial sums
   long long limit;     // wall-clock ticks one wait may last
+  int* sticky;         // set with the abort flag, cleared only by the host (survives later launches)
 };
 
 // Team barrier: arrival `target / T` of a team of T workgroups.  After the
@@ -277,7 +278,8 @@ struct TeamView {
 // and acquires, then the workgroup barrier again -- one cache write-back and one
 // invalidation per member and barrier, so loads after it see what the other
 // members stored before it, whichever XCD they run on.
-__device__ __forceinline__ bool team_wait(int* cnt, int target, int* abort_flag, long long limit, int* flag) {
+__device__ __forceinline__ bool team_wait(int* cnt, int target, int* abort_flag, int* sticky, long long limit,
+                                          int* flag) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -289,6 +291,7 @@ __device__ __forceinline__ bool team_wait(int* cnt, int target, int* abort_flag,
       if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || k >= kSpinMax ||
           (long long)wall_clock64() - t0 > limit) {
         __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(sticky, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = 0;
         break;
       }
@@ -324,7 +327,7 @@ __device__ __forceinline__ void cg_team(const BatchView& V, const TeamView& S, i
   int nbar = 0, nred = 0;
   auto wait = [&]() -> bool {
     ++nbar;
-    return team_wait(cnt, nbar * T, S.ctl + 1, S.limit, flag);
+    return team_wait(cnt, nbar * T, S.ctl + 1, S.sticky, S.limit, flag);
   };
   // team-wide sums; consecutive reductions alternate between two partial sets,
   // and a set is rewritten only after the next reduction's barrier, which every
@@ -641,7 +644,9 @@ int build_sched(const Batch& B, int NT, int resident, int ts, BatchSched& S, hip
   MH_TRY(S.slot.alloc(slot.size()));
   MH_TRY(S.ctl.alloc((size_t)2 + nteam));
   MH_TRY(S.part.alloc((size_t)std::max(nteam, 1) * 2 * kTeamMax * 2));
+  MH_TRY(S.sticky.alloc(1));
   MH_TRY(upload(S.slot.p, slot.data(), sizeof(int) * slot.size(), s));
+  MH_HIP(hipMemsetAsync(S.sticky.p, 0, sizeof(int), s));
   MH_HIP(hipStreamSynchronize(s));  // `slot` is freed on return
   int dev = 0, khz = 0;
   MH_HIP(hipGetDevice(&dev));
@@ -708,10 +713,16 @@ int batch_schedule_info(Batch& B, int cfg, int* split, int* nslot, int* nteam, h
 int batch_aborted(Batch& B, int cfg, int* aborted, hipStream_t s) {
   *aborted = 0;
   if (cfg < 0 || cfg >= (int)B.sched.size() || !B.sched[cfg] || batch_config_split(cfg) == 0) return 0;
-  int ctl[2] = {0, 0};
-  MH_HIP(hipMemcpyAsync(ctl, B.sched[cfg]->ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, s));
+  // the sticky flag: set by any launch since the last check, then cleared
+  int flag = 0;
+  BatchSched& S = *B.sched[cfg];
+  MH_HIP(hipMemcpyAsync(&flag, S.sticky.p, sizeof(int), hipMemcpyDeviceToHost, s));
   MH_HIP(hipStreamSynchronize(s));
-  *aborted = ctl[1] != 0;
+  if (flag) {
+    MH_HIP(hipMemsetAsync(S.sticky.p, 0, sizeof(int), s));
+    MH_HIP(hipStreamSynchronize(s));
+  }
+  *aborted = flag != 0;
   return 0;
 }
 
@@ -843,7 +854,8 @@ int batch_solve(Batch& B, const BatchParams& P, const double* b, double* x, int 
                                                                          S ? S->ctl.p : nullptr, nctl);
     MH_BATCH_CFG(cfg, {
       if constexpr (TS > 0) {
-        const TeamView TV{S->nslot, S->slot.p, S->ctl.p, S->part.p, (long long)(S->ticks_per_s * B.wait_s)};
+        const TeamView TV{S->nslot, S->slot.p, S->ctl.p, S->part.p, (long long)(S->ticks_per_s * B.wait_s),
+                          S->sticky.p};
         k_batch_sched<NT, UU, SV><<<S->grid, NT, 0, s>>>(V, TV, b, x, r, z, p, p2, q, O, P.max_iter, P.tol,
                                                      P.init_x, P.precond);
       } else {

@@ -27,6 +27,7 @@ struct BatchSched {
   DevArray<int> slot;       // 4 ints per slot
   DevArray<int> ctl;        // [0] next slot, [1] abort flag, [2 ..] team arrival counters
   DevArray<double> part;    // per team: 2 x 8 x 2 partial sums
+  DevArray<int> sticky;     // a team wait gave up in some launch since the host last looked
 };
 
 struct Batch {
@@ -58,8 +59,8 @@ int batch_create(Batch** out, int nsys, const int* sys_row_host, int n, int nz, 
                  const double* A, hipStream_t s);
 // Enqueues one batched solve (graph replay when use_graph); results in B.iters/resid/status.
 int batch_solve(Batch& B, const BatchParams& P, const double* b, double* x, int cfg, int use_graph, hipStream_t s);
-// After a launch of configuration cfg has completed: 1 when one of its team waits
-// gave up (the affected systems carry status 3), else 0.
+// After launches of configuration cfg have completed: 1 when a team wait gave up
+// in any of them since the last call (the affected systems carry status 3), else 0.
 int batch_aborted(Batch& B, int cfg, int* aborted, hipStream_t s);
 // Team split of cfg and the slot / team counts of its schedule for B (built on
 // first use; one slot per system and no teams for split 0).

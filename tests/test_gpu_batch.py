@@ -168,9 +168,17 @@ def test_batch_team_wait_limit_gives_up_cleanly():
     except RuntimeError as e:
         gave_up = True
         assert "gave up" in str(e)
+    # timed replays report a give-up in any replay, not only the last one
+    try:
+        B.time(db, dx, _prm(), reps=3)
+        timed_gave_up = False
+    except RuntimeError as e:
+        timed_gave_up = True
+        assert "gave up" in str(e)
     B.set_wait_limit(5.0)
+    B.time(db, dx, _prm(), reps=2)                # no give-up left over from the runs above
     iters, resid, status = B.solve(db, dx, _prm())
-    print(f"zero wait limit gave up: {gave_up}", flush=True)
+    print(f"zero wait limit gave up: solve {gave_up}, timed replays {timed_gave_up}", flush=True)
     _check_against_oracle(mat, sys_row, b, dx.numpy(), iters, resid, status)
 
 
