@@ -582,7 +582,7 @@ __global__ void k_batch_fill(int nsys_rows_total, const int* __restrict__ sell_s
     MH_BATCH_CASE(9, 1024, 2, 2, false, __VA_ARGS__) \
     MH_BATCH_CASE(10, 512, 4, 2, false, __VA_ARGS__) \
     MH_BATCH_CASE(11, 1024, 2, 4, false, __VA_ARGS__) \
-    MH_BATCH_CASE(12, 1024, 2, 6, false, __VA_ARGS__) \
+    MH_BATCH_CASE(12, 1024, 2, 3, false, __VA_ARGS__) \
     MH_BATCH_CASE(13, 1024, 2, 0, true, __VA_ARGS__) \
     MH_BATCH_CASE(14, 1024, 2, 4, true, __VA_ARGS__) \
     default: {                                       \
@@ -599,9 +599,9 @@ __global__ void k_batch_fill(int nsys_rows_total, const int* __restrict__ sell_s
 // (entries + 6 rows) x rows^(1/3) (the SpMV and vector bytes times an iteration
 // count that grows like a 3-D mesh's edge); systems above ts/4 of the balanced
 // share get ceil(work / that) members, at most 8 and at most one slice per wave
-// and member; team tasks come first (while their slots fit the resident
-// workgroups, all teams start together), each group ordered by work per
-// member, largest first.
+// and member; tasks are ordered by work per member, largest first (teams and
+// single workgroups interleaved: putting every team first left the largest
+// single-workgroup systems for the end).
 int build_sched(const Batch& B, int NT, int resident, int ts, BatchSched& S, hipStream_t s) {
   const int nsys = B.nsys;
   std::vector<double> w((size_t)nsys);
@@ -623,10 +623,7 @@ int build_sched(const Batch& B, int NT, int resident, int ts, BatchSched& S, hip
     if (tmax > 1 && share > 0.0 && w[k] > share) T = std::min(tmax, (int)std::ceil(w[k] / share));
     tasks[k] = Task{k, T, w[k] / T};
   }
-  std::stable_sort(tasks.begin(), tasks.end(), [](const Task& a, const Task& b) {
-    if ((a.T > 1) != (b.T > 1)) return a.T > 1;
-    return a.d > b.d;
-  });
+  std::stable_sort(tasks.begin(), tasks.end(), [](const Task& a, const Task& b) { return a.d > b.d; });
   std::vector<int> slot;
   int nteam = 0;
   for (const Task& t : tasks) {
