@@ -234,7 +234,7 @@ int monolis_hip_batch_config(int cfg, int* threads, int* unroll);
 int monolis_hip_batch_set_config(monolis_hip_batch* B, int cfg);
 /* Team split of configuration cfg (0: one workgroup per system; s > 0: systems
  * whose estimated work exceeds s/4 of one workgroup's balanced share are solved
- * by a team of up to 8 workgroups) and the number of slots (one per workgroup
+ * by a team of up to 32 workgroups) and the number of slots (one per workgroup
  * task) and teams its schedule for B has. */
 int monolis_hip_batch_schedule(monolis_hip_batch* B, int cfg, int* split, int* nslot, int* nteam);
 /* Longest time in seconds (default 5, at most 600) that one team barrier waits

@@ -11,7 +11,7 @@
 // slice, so a workgroup walks only its own slices.  Two persistent kernels:
 //  * k_batch_cg: each workgroup takes systems wg, wg + G, ... (largest first) and
 //    runs the whole CG loop with workgroup barriers only;
-//  * k_batch_sched: the largest systems are split over teams of up to 8
+//  * k_batch_sched: the largest systems are split over teams of up to 32
 //    workgroups (contiguous slice ranges, team barriers in global memory with a
 //    bounded wait), so that one system no longer sets the critical path.
 // The launch (init kernel + solver kernel) is captured once in a hipGraph and
@@ -30,7 +30,7 @@ namespace mh {
 namespace {
 
 constexpr int kIterRR = 200;
-constexpr int kTeamMax = 8;         // workgroups per team at most
+constexpr int kTeamMax = 32;        // workgroups per team at most
 constexpr int kSpinMax = 1 << 24;   // polls of one team wait at most (a bound that needs no clock)
 
 // workgroup sum over NT threads (fixed order), result returned to every thread
@@ -598,8 +598,8 @@ __global__ void k_batch_fill(int nsys_rows_total, const int* __restrict__ sell_s
 // workgroups at once, split factor ts): the estimated work of a system is
 // (entries + 6 rows) x rows^(1/3) (the SpMV and vector bytes times an iteration
 // count that grows like a 3-D mesh's edge); systems above ts/4 of the balanced
-// share get ceil(work / that) members, at most 8 and at most one slice per wave
-// and member; tasks are ordered by work per member, largest first (teams and
+// share get ceil(work / that) members, at most 32, at most the system's share of
+// the resident workgroups and at most one slice per wave and member; tasks are ordered by work per member, largest first (teams and
 // single workgroups interleaved: putting every team first left the largest
 // single-workgroup systems for the end).
 int build_sched(const Batch& B, int NT, int resident, int ts, BatchSched& S, hipStream_t s) {
@@ -618,7 +618,10 @@ int build_sched(const Batch& B, int NT, int resident, int ts, BatchSched& S, hip
   };
   std::vector<Task> tasks((size_t)nsys);
   for (int k = 0; k < nsys; ++k) {
-    const int tmax = std::min(kTeamMax, B.h_slices[k] / (NT / kWave));
+    // at most its proportional part of the resident workgroups (rounded up):
+    // teams that together ask for more than are resident would wait for one another
+    const int tfair = (int)std::ceil((double)resident * w[k] / W);
+    const int tmax = std::min(std::min(kTeamMax, tfair), B.h_slices[k] / (NT / kWave));
     int T = 1;
     if (tmax > 1 && share > 0.0 && w[k] > share) T = std::min(tmax, (int)std::ceil(w[k] / share));
     tasks[k] = Task{k, T, w[k] / T};

@@ -26,7 +26,7 @@ struct BatchSched {
   double ticks_per_s = 0;   // wall-clock counter rate
   DevArray<int> slot;       // 4 ints per slot
   DevArray<int> ctl;        // [0] next slot, [1] abort flag, [2 ..] team arrival counters
-  DevArray<double> part;    // per team: 2 x 8 x 2 partial sums
+  DevArray<double> part;    // per team: 2 x 32 x 2 partial sums
   DevArray<int> sticky;     // a team wait gave up in some launch since the host last looked
 };
 
