@@ -93,6 +93,9 @@ def main() -> int:
                     help="only the 128^3 convection-diffusion FP64 / refined BiCGSTAB solves (profiling runs)")
     ap.add_argument("--inner-tol-sweep", action="store_true",
                     help="refined solves of the convection-diffusion and headline systems over inner tolerances")
+    ap.add_argument("--batch-teams", action="store_true",
+                    help="batched CG of k identical 40^3 Poisson systems, k = 1 .. 256: one workgroup per "
+                         "system against team configurations (JSON line), then exit")
     ap.add_argument("--cg-variant-sweep", action="store_true",
                     help="time the headline FP64 CG once per SpMV kernel variant (JSON line), then exit")
     ap.add_argument("--trace-summary", metavar="CSV",
@@ -167,6 +170,10 @@ def main() -> int:
 
     if args.convdiff_only or args.inner_tol_sweep:
         print(json.dumps(bench_refine_sweep(args)), flush=True)
+        return 0
+
+    if args.batch_teams:
+        print(json.dumps({"batch_teams": bench_batch_teams(args)}), flush=True)
         return 0
 
     if args.copy_only:
@@ -686,6 +693,34 @@ def bench_batching(args):
     }
     log(f"batching/512 systems: {ms:.2f} ms per graph replay (sequential est {est_seq * 1e3:.0f} ms)")
     return res
+
+
+def bench_batch_teams(args):
+    """k identical 40^3 Poisson systems (DIAG CG to 1e-8) in one batch: the
+    one-workgroup configuration (3) against team configurations 7 (split 4,
+    4 slots per step) and 12 (split 3): microseconds per iteration of the batch."""
+    from monolis_amd import api, workloads as W
+    rows = []
+    for k in (1, 4, 16, 32, 64, 128, 256):
+        bm, sys_row = W.poisson_batch(edges=[40] * k)
+        dbat = api.DeviceBatch(bm, sys_row)
+        db = api.DeviceBuffer.from_numpy(W.uniform_vector(bm.n, seed=args.seed + 9))
+        dx = api.DeviceBuffer(bm.n)
+        row = {"systems": k}
+        for cfg in (3, 7, 12):
+            sch = dbat.schedule(cfg)
+            dbat.set_config(cfg)
+            it, _, st = dbat.solve(db, dx, _prm(tol=1e-8))
+            ms = dbat.time(db, dx, _prm(tol=1e-8), reps=3)
+            row[f"cfg{cfg}"] = {"members_per_system": sch["slots"] // k, "ms": round(ms, 3),
+                                "us_per_iteration": round(ms * 1e3 / int(it.max()), 1),
+                                "converged": bool((st == 0).all())}
+        log(f"batch teams: {k} x 40^3: " + ", ".join(
+            f"cfg {c} T={row[f'cfg{c}']['members_per_system']} {row[f'cfg{c}']['us_per_iteration']} us/it"
+            for c in (3, 7, 12)))
+        rows.append(row)
+        del dbat, db, dx
+    return rows
 
 
 def bench_differentiation(args):
