@@ -697,8 +697,9 @@ def bench_batching(args):
 
 def bench_batch_teams(args):
     """k identical 40^3 Poisson systems (DIAG CG to 1e-8) in one batch: the
-    one-workgroup configuration (3) against team configurations 7 (split 4,
-    4 slots per step) and 12 (split 3): microseconds per iteration of the batch."""
+    one-workgroup configurations 3 (plain vector streams) and 13 (non-temporal)
+    against team configurations 7 (split 4, 4 slots per step) and 12 (split 3):
+    microseconds per iteration of the batch."""
     from monolis_amd import api, workloads as W
     rows = []
     for k in (1, 4, 16, 32, 64, 128, 256):
@@ -707,7 +708,7 @@ def bench_batch_teams(args):
         db = api.DeviceBuffer.from_numpy(W.uniform_vector(bm.n, seed=args.seed + 9))
         dx = api.DeviceBuffer(bm.n)
         row = {"systems": k}
-        for cfg in (3, 7, 12):
+        for cfg in (3, 13, 7, 12):
             sch = dbat.schedule(cfg)
             dbat.set_config(cfg)
             it, _, st = dbat.solve(db, dx, _prm(tol=1e-8))
@@ -717,7 +718,7 @@ def bench_batch_teams(args):
                                 "converged": bool((st == 0).all())}
         log(f"batch teams: {k} x 40^3: " + ", ".join(
             f"cfg {c} T={row[f'cfg{c}']['members_per_system']} {row[f'cfg{c}']['us_per_iteration']} us/it"
-            for c in (3, 7, 12)))
+            for c in (3, 13, 7, 12)))
         rows.append(row)
         del dbat, db, dx
     return rows
